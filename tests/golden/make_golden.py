@@ -1,0 +1,82 @@
+"""Writes the golden vectors under tests/golden/ from the REFERENCE engine.
+
+Runs oracle/_ref/ref_<app> (the reference's own edgeMap/vertexMap, functors
+and partitioned_graph, compiled from /root/reference by oracle/Makefile) on
+small graphs and stores inputs and outputs as .npz fixtures.  Run in the
+build container, where /root/reference exists:
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Inputs: the reference's own 'sixnode' graph (its only shipped fixture, kept
+here as data) and seeded R-MAT graphs from the shared generator.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+import oracle  # noqa: E402
+
+# sixnode (reference file 'sixnode'): AdjacencyGraph, n=6, m=15
+SIX_OFF = np.array([0, 2, 4, 7, 10, 14, 15], np.uint32)
+SIX_IDS = np.array([1, 2, 2, 4, 1, 4, 5, 0, 2, 4, 0, 1, 2, 5, 1], np.uint32)
+
+
+def int_weights(seed, m):
+    # integer edge weights 1..19 (the reference's SPMV takes intE weights)
+    return np.array([1 + oracle.splitmix64(seed ^ e) % 19 for e in range(m)], np.int32)
+
+
+def case(name, off, ids, symmetric, P, root, apps, wseed=None, x=None, store=None):
+    out = {"off": off, "ids": ids, "symmetric": np.int64(symmetric), "P": np.int64(P),
+           "root": np.int64(root)}
+    for part_vertex in (False, True):
+        tag = "vertex" if part_vertex else "edge"
+        for app in apps:
+            w = int_weights(wseed, len(ids)) if app == "spmv" else None
+            res, iters, parts = oracle.run_ref(app, off, ids, w=w, symmetric=symmetric, P=P,
+                                               part_vertex=part_vertex, root=root, x=x,
+                                               with_partitions=True, threads=1)
+            out[f"{app}_{tag}"] = res
+            out[f"{app}_{tag}_iters"] = np.int64(iters)
+            if app == "spmv":
+                out["spmv_w"] = w
+                if x is not None:
+                    out["spmv_x"] = x
+            coo_starts, csc_starts, lists = parts
+            out[f"coo_starts_{tag}"] = coo_starts
+            out[f"csc_starts_{tag}"] = csc_starts
+            if lists:
+                out[f"coo_count_{tag}"] = np.array([len(l) for l in lists], np.int64)
+                out[f"coo_edges_{tag}"] = (np.concatenate(lists) if lists else
+                                           np.zeros((0, 2), np.uint32))
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def main():
+    if not oracle.ref_available("pagerank"):
+        sys.exit("oracle/_ref not built (needs /root/reference): make -C oracle")
+    apps_dir = ["pagerank", "bfs", "cc", "spmv"]
+    case("sixnode_p2", SIX_OFF, SIX_IDS, False, 2, 0, apps_dir, wseed=7)
+    # P=3 is not used: the greedy overruns its arrays on sixnode (UB in the reference)
+    case("sixnode_p6", SIX_OFF, SIX_IDS, False, 6, 0, apps_dir, wseed=7)
+    so, si = oracle.csr_from_edges(6, np.repeat(np.arange(6), np.diff(SIX_OFF)), SIX_IDS,
+                                   symmetric=True)
+    case("sixnode_sym_p2", so, si, True, 2, 0, ["bfs", "cc"])
+    off, ids = oracle.rmat_csr(10, 16, 1)
+    x = oracle.uniform_pm1(3, len(off) - 1)
+    case("rmat10_dir_p16", off, ids, False, 16, 100, apps_dir, wseed=3, x=x)
+    off, ids = oracle.rmat_csr(11, 16, 2, symmetric=True)
+    case("rmat11_sym_p32", off, ids, True, 32, 100, ["bfs", "cc"])
+    off, ids = oracle.rmat_csr(12, 8, 5)
+    case("rmat12_dir_p64", off, ids, False, 64, 100, ["pagerank", "bfs"])
+
+
+if __name__ == "__main__":
+    main()
