@@ -1,0 +1,442 @@
+"""graphgrind_amd -- MI355X-native GraphGrind traversal engine (host mirror).
+
+Python view of the C ABI in include/graphgrind_amd.h.  Names follow the
+reference's app-facing API (graphgrind/ligra-numa.h, graph-numa.h):
+
+    Graph            partitioned_graph: CSR + in-edge twin + partitions on HBM
+    Frontier         partitioned_vertices (create / bits / dense / sparse)
+    edge_map         edgeMap(GA, frontier, functor, threshold, remDups)
+    vertex_map       vertexMap(part, frontier, functor) / map_vertexL
+    pagerank, bfs, components, spmv   the apps' Compute() loops
+
+Errors come back as GGError carrying the C status and message (the
+reference abort()s in the same places).  The compute path is the HIP
+library only: importing this package fails loudly when
+libgraphgrind_amd.so is missing (build it with __graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgraphgrind_amd.so")
+
+GG_OK = 0
+ERR_NAMES = {-1: "INVALID", -2: "HIP", -3: "OOM", -4: "SIZE", -5: "PARTITION",
+             -6: "UNSUPPORTED", -7: "STATE"}
+F64, F32, U32, I32, U8 = 0, 1, 2, 3, 4
+NP_OF = {F64: np.float64, F32: np.float32, U32: np.uint32, I32: np.int32, U8: np.uint8}
+DT_OF = {np.dtype(np.float64): F64, np.dtype(np.float32): F32, np.dtype(np.uint32): U32,
+         np.dtype(np.int32): I32, np.dtype(np.uint8): U8}
+OP_PAGERANK, OP_BFS, OP_CC, OP_SPMV = 0, 1, 2, 3
+VOP_PAGERANK, VOP_RESET, VOP_CC_COPY, VOP_ABSDIFF = 0, 1, 2, 3
+FRONTIER_EMPTY, FRONTIER_SINGLE, FRONTIER_ALL = 0, 1, 2
+PATH_EMPTY, PATH_SPARSE, PATH_CSC, PATH_COO = -1, 0, 1, 2
+PATH_NAMES = {-1: "empty", 0: "sparse", 1: "csc", 2: "coo"}
+
+
+class GGError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"GG_ERR_{ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Op(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("dtype", C.c_int32), ("a", C.c_void_p), ("b", C.c_void_p)]
+
+
+class VOp(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("dtype", C.c_int32), ("a", C.c_void_p), ("b", C.c_void_p),
+                ("c0", C.c_double), ("c1", C.c_double)]
+
+
+class RunInfo(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("paths", C.c_int32 * 64), ("edges_scanned", C.c_uint64),
+                ("ms", C.c_double)]
+
+    def path_list(self):
+        return [PATH_NAMES[self.paths[i]] for i in range(min(self.iters, 64))]
+
+
+_lib = None
+
+# Every entry point of include/graphgrind_amd.h with (restype, argtypes).
+_VP, _U64P, _U32P, _I32P, _F32P, _U8P = (C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                         C.POINTER(C.c_uint8))
+_IP = C.POINTER(C.c_int)
+SIGNATURES = {
+    "gg_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "gg_ctx_destroy": (C.c_int, [_VP]),
+    "gg_ctx_sync": (C.c_int, [_VP]),
+    "gg_last_error": (C.c_char_p, []),
+    "gg_version": (C.c_char_p, []),
+    "gg_malloc": (C.c_int, [_VP, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "gg_free": (C.c_int, [_VP, _VP]),
+    "gg_memcpy_h2d": (C.c_int, [_VP, _VP, _VP, C.c_uint64]),
+    "gg_memcpy_d2h": (C.c_int, [_VP, _VP, _VP, C.c_uint64]),
+    "gg_memcpy_d2d": (C.c_int, [_VP, _VP, _VP, C.c_uint64]),
+    "gg_fill": (C.c_int, [_VP, _VP, C.c_int, C.c_uint64, C.c_double]),
+    "gg_iota": (C.c_int, [_VP, _VP, C.c_int, C.c_uint64]),
+    "gg_graph_create_csr": (C.c_int, [_VP, C.c_uint32, C.c_uint64, _VP, _VP, _VP, C.c_int,
+                                      C.POINTER(C.c_void_p)]),
+    "gg_graph_create_rmat": (C.c_int, [_VP, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
+                                       C.POINTER(C.c_void_p)]),
+    "gg_graph_destroy": (C.c_int, [_VP]),
+    "gg_graph_info": (C.c_int, [_VP, _U32P, _U64P, _IP, _IP]),
+    "gg_graph_copy_csr": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "gg_graph_copy_csc": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "gg_graph_degrees": (C.c_int, [_VP, C.c_int, _VP]),
+    "gg_graph_set_weights": (C.c_int, [_VP, _VP]),
+    "gg_graph_partition": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "gg_graph_partition_starts": (C.c_int, [_VP, C.c_int, _VP, _IP]),
+    "gg_graph_coo": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "gg_frontier_create": (C.c_int, [_VP, C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "gg_frontier_from_host": (C.c_int, [_VP, _VP, _VP, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "gg_frontier_destroy": (C.c_int, [_VP]),
+    "gg_frontier_info": (C.c_int, [_VP, _U64P, _U64P, _IP, _IP, _U32P]),
+    "gg_frontier_to_dense": (C.c_int, [_VP]),
+    "gg_frontier_to_sparse": (C.c_int, [_VP]),
+    "gg_frontier_copy_dense": (C.c_int, [_VP, _VP]),
+    "gg_frontier_copy_sparse": (C.c_int, [_VP, _VP, C.c_uint64, _U64P]),
+    "gg_edgemap": (C.c_int, [_VP, _VP, C.POINTER(Op), C.c_int64, C.c_int, C.POINTER(C.c_void_p),
+                             _IP]),
+    "gg_vertexmap": (C.c_int, [_VP, _VP, C.POINTER(VOp)]),
+    "gg_vertexfilter": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_void_p)]),
+    "gg_reduce_sum": (C.c_int, [_VP, _VP, C.c_int, C.c_uint64, C.POINTER(C.c_double)]),
+    "gg_pagerank": (C.c_int, [_VP, C.c_int, C.c_int, C.c_double, C.c_double, _VP,
+                              C.POINTER(RunInfo)]),
+    "gg_bfs": (C.c_int, [_VP, C.c_uint32, C.c_int64, _VP, _VP, C.POINTER(RunInfo)]),
+    "gg_cc": (C.c_int, [_VP, _VP, C.POINTER(RunInfo)]),
+    "gg_spmv": (C.c_int, [_VP, C.c_int, _VP, _VP, C.POINTER(RunInfo)]),
+    "gg_last_kernel_time": (C.c_int, [_VP, C.POINTER(C.c_double), _IP]),
+}
+
+
+def lib():
+    """The loaded HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: the HIP extension is not built "
+                              "(run __graft_entry__.build() or python -m graphgrind_amd.build_lib)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != GG_OK:
+        raise GGError(rc, lib().gg_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+# ----------------------------------------------------------------- memory --
+
+class Context:
+    """One GPU and one HIP stream (replaces the Cilk/NUMA runtime setup)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().gg_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().gg_ctx_destroy(self.h)
+            self.h = None
+
+    def sync(self):
+        check(lib().gg_ctx_sync(self.h))
+
+    def kernel_time(self):
+        ms = C.c_double()
+        k = C.c_int()
+        check(lib().gg_last_kernel_time(self.h, C.byref(ms), C.byref(k)))
+        return ms.value, k.value
+
+
+class DeviceArray:
+    """mmap_ptr<T>: an HBM allocation of 'count' elements."""
+
+    def __init__(self, ctx: Context, count: int, dtype):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().gg_malloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.size, a.dtype)
+        check(lib().gg_memcpy_h2d(ctx.h, d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def to_host(self):
+        out = np.empty(self.count, self.dtype)
+        check(lib().gg_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def fill(self, value):
+        check(lib().gg_fill(self.ctx.h, self.ptr, DT_OF[self.dtype], self.count, float(value)))
+        return self
+
+    def iota(self):
+        check(lib().gg_iota(self.ctx.h, self.ptr, DT_OF[self.dtype], self.count))
+        return self
+
+    def free(self):
+        if self.ptr:
+            lib().gg_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ graph --
+
+class Graph:
+    """partitioned_graph on the device: CSR, in-edge twin, partitions."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.h = handle
+        n = C.c_uint32()
+        m = C.c_uint64()
+        s = C.c_int()
+        w = C.c_int()
+        check(lib().gg_graph_info(self.h, C.byref(n), C.byref(m), C.byref(s), C.byref(w)))
+        self.n, self.m, self.symmetric, self.weighted = n.value, m.value, bool(s.value), bool(w.value)
+        self.P = 0
+
+    @classmethod
+    def from_csr(cls, ctx, off, ids, w=None, symmetric=False):
+        off = np.ascontiguousarray(off, np.uint32)
+        ids = np.ascontiguousarray(ids, np.uint32)
+        wv = None if w is None else np.ascontiguousarray(w, np.float32)
+        h = C.c_void_p()
+        check(lib().gg_graph_create_csr(ctx.h, len(off) - 1, len(ids), off.ctypes.data,
+                                        ids.ctypes.data if len(ids) else None, _ptr(wv),
+                                        int(symmetric), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def rmat(cls, ctx, scale, edge_factor=16, seed=1, symmetric=False, weighted=False):
+        h = C.c_void_p()
+        check(lib().gg_graph_create_rmat(ctx.h, scale, edge_factor, seed, int(symmetric),
+                                         int(weighted), C.byref(h)))
+        return cls(ctx, h)
+
+    def close(self):
+        if self.h:
+            lib().gg_graph_destroy(self.h)
+            self.h = None
+
+    def partition(self, P=384, by_vertex=False, relabel=False, hilbert=True):
+        """-c P, -v edge|vertex, -o, EDGES_HILBERT (ligra-numa.h:1104-1124)."""
+        check(lib().gg_graph_partition(self.h, P, int(by_vertex), int(relabel), int(hilbert)))
+        self.P = P
+        self.part_vertex = by_vertex
+        return self
+
+    def starts(self, which=0):
+        s = np.zeros(self.P + 1, np.uint64)
+        u = C.c_int()
+        check(lib().gg_graph_partition_starts(self.h, which, s.ctypes.data, C.byref(u)))
+        return s, bool(u.value)
+
+    def coo(self):
+        src = np.empty(max(self.m, 1), np.uint32)
+        dst = np.empty(max(self.m, 1), np.uint32)
+        po = np.zeros(self.P + 1, np.uint64)
+        check(lib().gg_graph_coo(self.h, src.ctypes.data, dst.ctypes.data, po.ctypes.data))
+        return src[:self.m], dst[:self.m], po
+
+    def csr(self):
+        off = np.empty(self.n + 1, np.uint32)
+        ids = np.empty(max(self.m, 1), np.uint32)
+        w = np.empty(max(self.m, 1), np.float32) if self.weighted else None
+        check(lib().gg_graph_copy_csr(self.h, off.ctypes.data, ids.ctypes.data, _ptr(w)))
+        return off, ids[:self.m], (None if w is None else w[:self.m])
+
+    def csc(self):
+        off = np.empty(self.n + 1, np.uint32)
+        ids = np.empty(max(self.m, 1), np.uint32)
+        w = np.empty(max(self.m, 1), np.float32) if self.weighted else None
+        check(lib().gg_graph_copy_csc(self.h, off.ctypes.data, ids.ctypes.data, _ptr(w)))
+        return off, ids[:self.m], (None if w is None else w[:self.m])
+
+    def degrees(self, in_edges=False):
+        d = np.empty(self.n, np.uint32)
+        check(lib().gg_graph_degrees(self.h, int(in_edges), d.ctypes.data))
+        return d
+
+    def set_weights(self, w_csr):
+        w = np.ascontiguousarray(w_csr, np.float32)
+        check(lib().gg_graph_set_weights(self.h, w.ctypes.data))
+        self.weighted = True
+
+
+class Frontier:
+    """partitioned_vertices (ligra-numa.h:91-272)."""
+
+    def __init__(self, graph: Graph, handle):
+        self.g = graph
+        self.h = handle
+
+    @classmethod
+    def create(cls, g, kind, v=0):
+        h = C.c_void_p()
+        check(lib().gg_frontier_create(g.h, kind, v, C.byref(h)))
+        return cls(g, h)
+
+    @classmethod
+    def single(cls, g, v):
+        return cls.create(g, FRONTIER_SINGLE, v)
+
+    @classmethod
+    def bits(cls, g):
+        return cls.create(g, FRONTIER_ALL)
+
+    @classmethod
+    def empty(cls, g):
+        return cls.create(g, FRONTIER_EMPTY)
+
+    @classmethod
+    def from_dense(cls, g, flags):
+        f = np.ascontiguousarray(flags, np.uint8)
+        h = C.c_void_p()
+        check(lib().gg_frontier_from_host(g.h, f.ctypes.data, None, len(f), C.byref(h)))
+        return cls(g, h)
+
+    @classmethod
+    def from_sparse(cls, g, ids):
+        a = np.ascontiguousarray(ids, np.uint32)
+        h = C.c_void_p()
+        check(lib().gg_frontier_from_host(g.h, None, a.ctypes.data if len(a) else None, len(a),
+                                          C.byref(h)))
+        return cls(g, h)
+
+    def info(self):
+        dm = C.c_uint64()
+        oe = C.c_uint64()
+        dense = C.c_int()
+        bit = C.c_int()
+        n = C.c_uint32()
+        check(lib().gg_frontier_info(self.h, C.byref(dm), C.byref(oe), C.byref(dense),
+                                     C.byref(bit), C.byref(n)))
+        return {"d_m": dm.value, "out_edges": oe.value, "dense": bool(dense.value),
+                "bit": bool(bit.value), "n": n.value}
+
+    def numNonzeros(self):
+        return self.info()["d_m"]
+
+    def isEmpty(self):
+        return self.numNonzeros() == 0
+
+    def to_dense(self):
+        check(lib().gg_frontier_to_dense(self.h))
+
+    def to_sparse(self):
+        check(lib().gg_frontier_to_sparse(self.h))
+
+    def dense(self):
+        out = np.zeros(self.g.n, np.uint8)
+        check(lib().gg_frontier_copy_dense(self.h, out.ctypes.data))
+        return out
+
+    def sparse(self):
+        cap = self.g.n + self.info()["d_m"] + 1
+        out = np.empty(cap, np.uint32)
+        cnt = C.c_uint64()
+        check(lib().gg_frontier_copy_sparse(self.h, out.ctypes.data, cap, C.byref(cnt)))
+        return out[:cnt.value]
+
+    def close(self):
+        if self.h:
+            lib().gg_frontier_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def edge_map(g: Graph, frontier: Frontier, kind, a: DeviceArray, b: DeviceArray = None,
+             threshold=-1, remdups=False):
+    """edgeMap(GA, frontier, F, threshold, DENSE, remDups) -> (output, path)."""
+    dtype = DT_OF[a.dtype] if kind in (OP_PAGERANK, OP_SPMV) else DT_OF[a.dtype]
+    op = Op(kind, dtype, a.ptr, None if b is None else b.ptr)
+    h = C.c_void_p()
+    path = C.c_int()
+    check(lib().gg_edgemap(g.h, frontier.h, C.byref(op), threshold, int(remdups), C.byref(h),
+                           C.byref(path)))
+    return Frontier(g, h), path.value
+
+
+def vertex_map(g: Graph, frontier, kind, a: DeviceArray, b: DeviceArray = None, c0=0.0, c1=0.0):
+    """vertexMap(part, frontier, F); frontier None = map_vertexL over all."""
+    op = VOp(kind, DT_OF[a.dtype], a.ptr, None if b is None else b.ptr, c0, c1)
+    check(lib().gg_vertexmap(g.h, None if frontier is None else frontier.h, C.byref(op)))
+
+
+def reduce_sum(ctx: Context, a: DeviceArray):
+    out = C.c_double()
+    check(lib().gg_reduce_sum(ctx.h, a.ptr, DT_OF[a.dtype], a.count, C.byref(out)))
+    return out.value
+
+
+# ------------------------------------------------------------------- apps --
+
+def pagerank(g: Graph, dtype=np.float64, max_iter=10, damping=0.85, eps=1e-7, out=None):
+    """app/PageRank.C Compute(): returns (ranks DeviceArray, RunInfo)."""
+    r = out if out is not None else DeviceArray(g.ctx, g.n, dtype)
+    info = RunInfo()
+    check(lib().gg_pagerank(g.h, DT_OF[np.dtype(dtype)], max_iter, damping, eps, r.ptr,
+                            C.byref(info)))
+    return r, info
+
+
+def bfs(g: Graph, root, threshold=-1, levels=True, parents=None, level_out=None):
+    """app/BFS.C Compute(): returns (parents, levels|None, RunInfo)."""
+    p = parents if parents is not None else DeviceArray(g.ctx, g.n, np.int32)
+    lv = level_out if level_out is not None else (DeviceArray(g.ctx, g.n, np.int32) if levels else None)
+    info = RunInfo()
+    check(lib().gg_bfs(g.h, root, threshold, p.ptr, None if lv is None else lv.ptr, C.byref(info)))
+    return p, lv, info
+
+
+def components(g: Graph, out=None):
+    """app/Components.C Compute(): returns (labels, RunInfo)."""
+    lab = out if out is not None else DeviceArray(g.ctx, g.n, np.uint32)
+    info = RunInfo()
+    check(lib().gg_cc(g.h, lab.ptr, C.byref(info)))
+    return lab, info
+
+
+def spmv(g: Graph, x: DeviceArray, y: DeviceArray = None):
+    """y[d] = sum_{s->d} w * x[s] over the in-edge twin (app/SPMV.C)."""
+    yy = y if y is not None else DeviceArray(g.ctx, g.n, x.dtype)
+    info = RunInfo()
+    check(lib().gg_spmv(g.h, DT_OF[x.dtype], x.ptr, yy.ptr, C.byref(info)))
+    return yy, info

@@ -1,0 +1,447 @@
+// gg_apps.hip -- the four apps' Compute() loops on the device.
+//
+// gg_pagerank runs PageRank.C's frontier-gated iteration (app/PageRank.C:
+// 102-151) with edgeMap, both vertexMaps, the L1 reduction and the reset
+// fused into one destination-owned pull per iteration:
+//   x[d]  += sum over active in-neighbours s of c[s]/outdeg(s)   (PR_F)
+//   x[d]   = 0.85*x[d] + 0.15/n            if d in F               (PR_Vertex_F)
+//   L1    += |c[d] - x[d]|;  c'[d] = d in F ? 0 : |c[d]-x[d]|       (136-142)
+//   next   = {d : some active in-neighbour}; contributions for the next
+//            iteration written in the same epilogue
+// The three rank buffers rotate (c, x, c') -> (x, c', free) exactly as
+// swap(p_curr, p_next) does, with the reset copy kept apart so no kernel
+// reads an array it writes.  A device flag carries the L1 < eps break.
+#include <cstring>
+
+#include "gg_kernels.h"
+
+namespace gg {
+
+template <class V>
+struct OpPRFused {
+    using acc_t = V;
+    const V *contrib_in;
+    V *contrib_out;
+    const V *c;
+    V *x;
+    V *y;
+    const uint8_t *Fin;
+    int bit;
+    uint8_t *Fout;
+    const uint32_t *outdeg;
+    V damping, added;
+    const int *done;
+    __device__ bool skip() const { return *done != 0; }
+    __device__ V identity() const { return -V(0); }
+    __device__ V combine(V a, V b) const { return a + b; }
+    __device__ V gather(uint32_t, uint32_t s) const { return contrib_in[s]; }
+    __device__ void finish(uint32_t d, V acc, LocalStats &st) const
+    {
+        V xv = x[d];
+        const bool any = !signbit(acc);
+        if (any) xv = xv + acc;
+        const bool inF = bit || Fin[d];
+        if (inF) xv = damping * xv + added;
+        const V diff = fabs(c[d] - xv);
+        st.l1 += (double)diff;
+        y[d] = inF ? V(0) : diff;
+        x[d] = xv;
+        Fout[d] = any;
+        const uint32_t od = outdeg[d];
+        contrib_out[d] = (any && od) ? xv / (V)od : -V(0);
+        if (any) {
+            st.active++;
+            st.out_edges += od;
+        }
+    }
+};
+
+template <class V>
+__global__ void k_pr_init(V *c, V *x, V *contrib, const uint32_t *__restrict__ outdeg, uint32_t n, V one_over_n)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    c[v] = one_over_n;
+    x[v] = V(0);
+    const uint32_t od = outdeg[v];
+    contrib[v] = od ? one_over_n / (V)od : -V(0);
+}
+
+// Per-iteration bookkeeping in one block: L1 over the tile partials in index
+// order, the break flag, and the frontier statistics of this iteration.
+__global__ __launch_bounds__(kBlock) void k_pr_check(const double *__restrict__ l1_part, uint32_t np,
+                                                     const double *__restrict__ l1_fix, uint32_t nf,
+                                                     double eps, int iter, int *done, int *done_iter,
+                                                     unsigned long long *counters,
+                                                     unsigned long long *stats, double *l1_hist)
+{
+    if (*done) return;
+    double s = 0;
+    for (uint32_t i = threadIdx.x; i < np; i += kBlock) s += l1_part[i];
+    for (uint32_t i = threadIdx.x; i < nf; i += kBlock) s += l1_fix[i];
+    s = wave_sum_f64(s);
+    __shared__ double w[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < kBlock / 64; k++) t += w[k];
+        l1_hist[iter - 1] = t;
+        stats[2 * (iter - 1)] = counters[CNT_ACTIVE];
+        stats[2 * (iter - 1) + 1] = counters[CNT_OUT];
+        counters[CNT_ACTIVE] = 0;
+        counters[CNT_OUT] = 0;
+        if (t < eps) {
+            *done = 1;
+            *done_iter = iter;
+        }
+    }
+}
+
+struct PRBuffers {
+    Buf R[3], K[2], F[2], l1p, l1f, flags, stats, l1h;
+    void free_all()
+    {
+        Buf *all[] = {&R[0], &R[1], &R[2], &K[0], &K[1], &F[0], &F[1], &l1p, &l1f, &flags, &stats, &l1h};
+        for (Buf *b : all) buf_free(*b);
+    }
+};
+
+template <class V>
+static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, void *ranks, gg_run_info *info,
+                          bool *needs_generic)
+{
+    Ctx *c = g->ctx;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    PullSched &s = g->pull;
+    const uint32_t nfix = blocks_for(s.tiles, kBlock);
+    PRBuffers B;
+    int rc = GG_OK;
+    for (int i = 0; i < 3 && !rc; i++) rc = buf_alloc(c, B.R[i], sizeof(V) * (uint64_t)n);
+    for (int i = 0; i < 2 && !rc; i++) rc = buf_alloc(c, B.K[i], sizeof(V) * (uint64_t)n);
+    for (int i = 0; i < 2 && !rc; i++) rc = buf_alloc(c, B.F[i], (uint64_t)n + 64);
+    if (!rc) rc = buf_alloc(c, B.l1p, 8ull * s.tiles);
+    if (!rc) rc = buf_alloc(c, B.l1f, 8ull * nfix);
+    if (!rc) rc = buf_alloc(c, B.flags, 64);
+    if (!rc) rc = buf_alloc(c, B.stats, 16ull * (max_iter + 1));
+    if (!rc) rc = buf_alloc(c, B.l1h, 8ull * (max_iter + 1));
+    if (rc) { B.free_all(); return rc; }
+    int *done = (int *)B.flags.p;
+    int *done_iter = done + 1;
+    unsigned long long *cnt = (unsigned long long *)c->counters.p;
+    const V one_over_n = (V)(1 / (double)n);
+    const V added = (V)((1 - damping) * (1 / (double)n)); // PageRank.C:80
+    const int nev = 2 * max_iter + 2;
+    std::vector<hipEvent_t> ev(nev);
+    for (auto &e : ev) (void)hipEventCreate(&e);
+    // timed region: init + iterations (the reference times Compute() whole)
+    GG_HIP(hipEventRecord(ev[0], c->stream));
+    fill_bytes(c, B.flags.p, 64, 0);
+    fill_bytes(c, c->counters.p, 64, 0);
+    k_pr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)B.R[0].p, (V *)B.R[1].p, (V *)B.K[0].p,
+                                                                   (const uint32_t *)g->outdeg.p, n, one_over_n);
+    for (int it = 1; it <= max_iter; it++) {
+        OpPRFused<V> op;
+        op.c = (const V *)B.R[(it - 1) % 3].p;
+        op.x = (V *)B.R[it % 3].p;
+        op.y = (V *)B.R[(it + 1) % 3].p;
+        op.contrib_in = (const V *)B.K[(it - 1) % 2].p;
+        op.contrib_out = (V *)B.K[it % 2].p;
+        op.bit = it == 1;
+        op.Fin = it == 1 ? nullptr : (const uint8_t *)B.F[(it - 2) % 2].p;
+        op.Fout = (uint8_t *)B.F[(it - 1) % 2].p;
+        op.outdeg = (const uint32_t *)g->outdeg.p;
+        op.damping = (V)damping;
+        op.added = added;
+        op.done = done;
+        rc = launch_pull(c, g, op, cnt, (double *)B.l1p.p, (double *)B.l1f.p, ev[2 * it], ev[2 * it + 1]);
+        if (rc) break;
+        k_pr_check<<<1, kBlock, 0, c->stream>>>((const double *)B.l1p.p, s.tiles, (const double *)B.l1f.p, nfix,
+                                                 eps, it, done, done_iter, cnt,
+                                                 (unsigned long long *)B.stats.p, (double *)B.l1h.p);
+    }
+    GG_HIP(hipEventRecord(ev[1], c->stream));
+    GG_HIP(hipGetLastError());
+    if (!rc) rc = sync(c);
+    int hflags[2] = {0, 0};
+    std::vector<unsigned long long> st(2ull * (max_iter + 1));
+    if (!rc) rc = gg_memcpy_d2h(abi(c), hflags, B.flags.p, 8);
+    if (!rc) rc = gg_memcpy_d2h(abi(c), st.data(), B.stats.p, 16ull * max_iter);
+    int iters = hflags[0] ? hflags[1] : max_iter;
+    if (!rc) {
+        float ms = 0, kms = 0, tot = 0;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        for (int it = 1; it <= iters; it++) {
+            (void)hipEventElapsedTime(&kms, ev[2 * it], ev[2 * it + 1]);
+            tot += kms;
+        }
+        c->last_kernel_ms = tot;
+        c->last_kernel_launches = iters;
+        if (info) {
+            info->iters = iters;
+            info->ms = ms;
+            info->edges_scanned = (uint64_t)iters * m;
+            for (int it = 1; it <= iters && it <= 64; it++) info->paths[it - 1] = GG_PATH_COO;
+        }
+        // The fused loop takes the dense branch every iteration.  Check the
+        // decisions edgeMap would have made (ligra-numa.h:947,953): iteration
+        // it > 1 reads the statistics of iteration it-1's output.
+        const int64_t thr = (int64_t)(m / 20), dthr = (int64_t)(m / 2);
+        for (int it = 2; it <= iters; it++) {
+            const uint64_t dm = st[2 * (it - 2)], oe = st[2 * (it - 2) + 1];
+            if (oe == 0 || (int64_t)(dm + oe) <= thr) *needs_generic = true;
+            if (info && it <= 64)
+                info->paths[it - 1] = (int64_t)(dm + oe) > dthr && g->P && !g->part_vertex ? GG_PATH_COO
+                                                                                           : GG_PATH_CSC;
+        }
+        if (info && iters >= 1)
+            info->paths[0] = g->P && !g->part_vertex ? GG_PATH_COO : GG_PATH_CSC;
+        if (!*needs_generic)
+            rc = gg_memcpy_d2d(abi(c), ranks, B.R[iters % 3].p, sizeof(V) * (uint64_t)n);
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    B.free_all();
+    return rc;
+}
+
+// PageRank.C's loop through the generic edgeMap/vertexMap entry points; used
+// when a frontier turns small enough for the sparse branch, whose duplicate
+// outputs the fused loop does not model.
+template <class V>
+static int pagerank_generic(Graph *g, int max_iter, double damping, double eps, void *ranks, gg_run_info *info)
+{
+    Ctx *c = g->ctx;
+    const uint32_t n = g->n;
+    const int dt = sizeof(V) == 8 ? GG_F64 : GG_F32;
+    Buf pc, pn, red;
+    GG_TRY(buf_alloc(c, pc, sizeof(V) * (uint64_t)n));
+    GG_TRY(buf_alloc(c, pn, sizeof(V) * (uint64_t)n));
+    GG_TRY(buf_alloc(c, red, 8));
+    GG_TRY(gg_fill(abi(c), pc.p, dt, n, 1 / (double)n));
+    GG_TRY(gg_fill(abi(c), pn.p, dt, n, 0));
+    Frontier *F = new Frontier();
+    frontier_alloc(g, F);
+    F->bit = true;
+    F->has_dense = true;
+    F->d_m = n;
+    F->out_edges = g->m;
+    int count = 0, rc = GG_OK;
+    bool broke = false;
+    while (count < max_iter) {
+        count++;
+        gg_op_t op = {GG_OP_PAGERANK, dt, pc.p, pn.p};
+        Frontier *out = nullptr;
+        int path = 0;
+        if ((rc = edgemap_core(g, F, &op, (int64_t)(g->m / 20), 0, &out, &path, nullptr))) break;
+        if (info && count <= 64) info->paths[count - 1] = path;
+        gg_vop_t v1 = {GG_VOP_PAGERANK, dt, pc.p, pn.p, damping, (1 - damping) * (1 / (double)n)};
+        if ((rc = vertexmap_core(g, F, &v1))) break;
+        gg_vop_t v2 = {GG_VOP_ABSDIFF, dt, pc.p, pn.p, 0, 0};
+        if ((rc = vertexmap_core(g, nullptr, &v2))) break;
+        double L1 = 0;
+        if ((rc = reduce_sum_core(c, pc.p, dt, n, (double *)red.p))) break;
+        if ((rc = gg_memcpy_d2h(abi(c), &L1, red.p, 8))) break;
+        if (L1 < eps) {
+            broke = true;
+            frontier_delete(out);
+            break;
+        }
+        gg_vop_t v3 = {GG_VOP_RESET, dt, pc.p, nullptr, 0, 0};
+        if ((rc = vertexmap_core(g, F, &v3))) break;
+        std::swap(pc, pn);
+        frontier_delete(F);
+        F = out;
+        if (F->n == 0 && count < max_iter) {
+            rc = fail(GG_ERR_SIZE, "PageRank: edgeMap returned empty() and the next call has mismatched sizes "
+                                   "(ligra-numa.h:937-942)");
+            break;
+        }
+    }
+    if (!rc) rc = gg_memcpy_d2d(abi(c), ranks, broke ? pn.p : pc.p, sizeof(V) * (uint64_t)n);
+    if (info) {
+        info->iters = count;
+        info->edges_scanned = (uint64_t)count * g->m;
+    }
+    frontier_delete(F);
+    buf_free(pc);
+    buf_free(pn);
+    buf_free(red);
+    return rc;
+}
+
+} // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_pagerank(gg_graph_t g, int dtype, int max_iter, double damping, double eps, void *ranks_dev,
+                           gg_run_info *info)
+{
+    if (!g || !ranks_dev) return fail(GG_ERR_INVALID, "gg_pagerank: NULL argument");
+    if (max_iter < 1 || max_iter > 100000) return fail(GG_ERR_INVALID, "gg_pagerank: max_iter out of range");
+    if (dtype != GG_F64 && dtype != GG_F32) return fail(GG_ERR_INVALID, "gg_pagerank: dtype must be F64 or F32");
+    GG_HIP(hipSetDevice(g->ctx->device));
+    if (info) std::memset(info, 0, sizeof(*info));
+    bool generic = false;
+    int rc = dtype == GG_F64 ? pagerank_fused<double>(g, max_iter, damping, eps, ranks_dev, info, &generic)
+                             : pagerank_fused<float>(g, max_iter, damping, eps, ranks_dev, info, &generic);
+    if (rc || !generic) return rc;
+    return dtype == GG_F64 ? pagerank_generic<double>(g, max_iter, damping, eps, ranks_dev, info)
+                           : pagerank_generic<float>(g, max_iter, damping, eps, ranks_dev, info);
+}
+
+extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *parents, int32_t *levels,
+                      gg_run_info *info)
+{
+    if (!g || !parents) return fail(GG_ERR_INVALID, "gg_bfs: NULL argument");
+    if (root >= g->n) return fail(GG_ERR_INVALID, "gg_bfs: root out of range");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    if (info) std::memset(info, 0, sizeof(*info));
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    GG_HIP(hipEventRecord(e0, c->stream));
+    // app/BFS.C:86-92
+    GG_TRY(fill_u32(c, parents, g->n, 0xffffffffu));
+    if (levels) GG_TRY(fill_u32(c, levels, g->n, 0xffffffffu));
+    int32_t r = (int32_t)root, zero = 0;
+    GG_HIP(hipMemcpyAsync(parents + root, &r, 4, hipMemcpyHostToDevice, c->stream));
+    if (levels) GG_HIP(hipMemcpyAsync(levels + root, &zero, 4, hipMemcpyHostToDevice, c->stream));
+    // partitioned_vertices::create(n, root, outdeg(root)) (ligra-numa.h:103-116)
+    Frontier *F = new Frontier();
+    frontier_alloc(g, F);
+    uint32_t od = 0;
+    int rc0 = buf_alloc(c, F->list, 8);
+    if (!rc0 && hipMemcpyAsync(F->list.p, &root, 4, hipMemcpyHostToDevice, c->stream)) rc0 = fail(GG_ERR_HIP, "root upload");
+    if (!rc0 && hipMemcpyAsync(&od, (uint32_t *)g->outdeg.p + root, 4, hipMemcpyDeviceToHost, c->stream))
+        rc0 = fail(GG_ERR_HIP, "outdeg readback");
+    if (!rc0) rc0 = sync(c);
+    if (rc0) { frontier_delete(F); return rc0; }
+    F->d_m = 1;
+    F->out_edges = od;
+    F->sparse_valid = true;
+    EdgeMapExtra ex;
+    ex.levels = levels;
+    int lv = 0, rc = GG_OK;
+    while (F->d_m != 0) {
+        ex.level = lv + 1;
+        gg_op_t op = {GG_OP_BFS, GG_I32, parents, nullptr};
+        Frontier *out = nullptr;
+        int path = 0;
+        if ((rc = edgemap_core(g, F, &op, threshold, 0, &out, &path, &ex))) break;
+        if (info && lv < 64) info->paths[lv] = path;
+        lv++;
+        frontier_delete(F);
+        F = out;
+    }
+    GG_HIP(hipEventRecord(e1, c->stream));
+    if (!rc) rc = sync(c);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    frontier_delete(F);
+    if (info) {
+        info->iters = lv;
+        info->ms = ms;
+        info->edges_scanned = ex.scanned;
+    }
+    return rc;
+}
+
+extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
+{
+    if (!g || !labels) return fail(GG_ERR_INVALID, "gg_cc: NULL argument");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    if (info) std::memset(info, 0, sizeof(*info));
+    Buf prev;
+    GG_TRY(buf_alloc(c, prev, 4ull * g->n));
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    GG_HIP(hipEventRecord(e0, c->stream));
+    GG_TRY(gg_iota(abi(c), labels, GG_U32, g->n)); // Components.C:113
+    Frontier *F = new Frontier();
+    frontier_alloc(g, F);
+    F->bit = true;
+    F->has_dense = true;
+    F->d_m = g->n;
+    F->out_edges = g->m;
+    EdgeMapExtra ex;
+    int rounds = 0, rc = GG_OK;
+    while (F->d_m != 0) {
+        gg_vop_t cp = {GG_VOP_CC_COPY, GG_U32, labels, prev.p, 0, 0};
+        if ((rc = vertexmap_core(g, F, &cp))) break;
+        gg_op_t op = {GG_OP_CC, GG_U32, labels, prev.p};
+        Frontier *out = nullptr;
+        int path = 0;
+        if ((rc = edgemap_core(g, F, &op, -1, 0, &out, &path, &ex))) break;
+        if (info && rounds < 64) info->paths[rounds] = path;
+        rounds++;
+        frontier_delete(F);
+        F = out;
+    }
+    GG_HIP(hipEventRecord(e1, c->stream));
+    if (!rc) rc = sync(c);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    frontier_delete(F);
+    buf_free(prev);
+    if (info) {
+        info->iters = rounds;
+        info->ms = ms;
+        info->edges_scanned = ex.scanned;
+    }
+    return rc;
+}
+
+namespace gg {
+template <class V>
+struct OpSpmvY {
+    using acc_t = V;
+    const V *x;
+    const float *tw;
+    V *y;
+    __device__ bool skip() const { return false; }
+    __device__ V identity() const { return V(0); }
+    __device__ V combine(V a, V b) const { return a + b; }
+    __device__ V gather(uint32_t e, uint32_t s) const { return tw ? x[s] * (V)tw[e] : x[s]; }
+    __device__ void finish(uint32_t d, V acc, LocalStats &) const { y[d] = acc; }
+};
+} // namespace gg
+
+extern "C" int gg_spmv(gg_graph_t g, int dtype, const void *x, void *y, gg_run_info *info)
+{
+    if (!g || !x || !y) return fail(GG_ERR_INVALID, "gg_spmv: NULL argument");
+    if (dtype != GG_F64 && dtype != GG_F32) return fail(GG_ERR_INVALID, "gg_spmv: dtype must be F64 or F32");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    if (info) std::memset(info, 0, sizeof(*info));
+    const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    GG_HIP(hipEventRecord(e0, c->stream));
+    int rc = dtype == GG_F64
+                 ? launch_pull(c, g, OpSpmvY<double>{(const double *)x, tw, (double *)y}, nullptr, nullptr, nullptr)
+                 : launch_pull(c, g, OpSpmvY<float>{(const float *)x, tw, (float *)y}, nullptr, nullptr, nullptr);
+    GG_HIP(hipEventRecord(e1, c->stream));
+    if (!rc) rc = sync(c);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->last_kernel_ms = ms;
+    c->last_kernel_launches = 1;
+    if (info) {
+        info->iters = 1;
+        info->ms = ms;
+        info->edges_scanned = g->m;
+        info->paths[0] = GG_PATH_CSC;
+    }
+    return rc;
+}

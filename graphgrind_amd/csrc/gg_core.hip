@@ -1,0 +1,1123 @@
+// gg_core.hip -- context, device memory, graph construction and the device
+// partitioner of libgraphgrind_amd (MI355X / gfx950).
+//
+// Graph construction replaces the reference's load path (IO.h:136-361,
+// wholeGraph graph-numa.h:500-557) and partitioned_graph constructor
+// (graph-numa.h:642-685) with device kernels: the in-edge twin comes from a
+// stable device radix sort, the partition greedy runs on the device as P
+// binary searches over the degree prefix sum, and COO partitions are
+// Hilbert-ordered by a stable segmented radix sort of Hilbert keys.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "gg_internal.h"
+
+namespace gg {
+
+static thread_local std::string t_err = "no error";
+
+void set_error(const std::string &msg) { t_err = msg; }
+
+int fail(int code, const std::string &msg)
+{
+    t_err = msg;
+    return code;
+}
+
+int buf_alloc(Ctx *c, Buf &b, uint64_t bytes)
+{
+    (void)c;
+    buf_free(b);
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(GG_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return GG_OK;
+}
+
+int buf_ensure(Ctx *c, Buf &b, uint64_t bytes)
+{
+    if (b.p && b.bytes >= bytes) return GG_OK;
+    return buf_alloc(c, b, bytes);
+}
+
+void buf_free(Buf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+int sync(Ctx *c)
+{
+    GG_HIP(hipStreamSynchronize(c->stream));
+    return GG_OK;
+}
+
+// ---- small kernels -----------------------------------------------------------
+
+__global__ void k_fill_u8(uint8_t *p, uint64_t n, uint8_t v)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+template <class T>
+__global__ void k_fill(T *p, uint64_t n, T v)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+template <class T>
+__global__ void k_iota(T *p, uint64_t n)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = (T)i;
+}
+
+static uint32_t grid_for(uint64_t n)
+{
+    uint64_t b = (n + kBlock - 1) / kBlock;
+    if (b > 65536) b = 65536; // grid-stride beyond this
+    return (uint32_t)(b ? b : 1);
+}
+
+int fill_bytes(Ctx *c, void *p, uint64_t bytes, uint8_t v)
+{
+    if (!bytes) return GG_OK;
+    k_fill_u8<<<grid_for(bytes), kBlock, 0, c->stream>>>((uint8_t *)p, bytes, v);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+int fill_u32(Ctx *c, void *p, uint64_t count, uint32_t v)
+{
+    if (!count) return GG_OK;
+    k_fill<uint32_t><<<grid_for(count), kBlock, 0, c->stream>>>((uint32_t *)p, count, v);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+// 32 flag bytes -> one bitmap word; one thread per word, coalesced reads.
+__global__ void k_bitmap_from_bytes(const uint8_t *__restrict__ bytes, uint32_t *__restrict__ bm,
+                                    uint32_t n)
+{
+    uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nw = (n + 31) / 32;
+    if (w >= nw) return;
+    uint32_t base = w * 32, word = 0;
+    if (base + 32 <= n) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(bytes + base);
+        uint4 a = p[0], b = p[1];
+        uint32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        for (int k = 0; k < 8; k++)
+            for (int j = 0; j < 4; j++)
+                if ((v[k] >> (8 * j)) & 0xff) word |= 1u << (k * 4 + j);
+    } else {
+        for (uint32_t j = 0; base + j < n; j++)
+            if (bytes[base + j]) word |= 1u << j;
+    }
+    bm[w] = word;
+}
+
+int bitmap_from_bytes(Ctx *c, const uint8_t *bytes, uint32_t *bitmap, uint32_t n)
+{
+    uint32_t nw = (n + 31) / 32;
+    if (!nw) return GG_OK;
+    // 16-byte loads need 16-byte alignment of every 32-byte group
+    k_bitmap_from_bytes<<<blocks_for(nw, kBlock), kBlock, 0, c->stream>>>(bytes, bitmap, n);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+// ---- merge-path schedule -------------------------------------------------------
+
+// Largest x in [max(diag-edges,0), min(diag,rows)] with off[x] <= diag-x,
+// i.e. the row coordinate where diagonal 'diag' crosses the merge path of
+// row ends off[1..rows] against edge indices 0..edges-1.
+__device__ __forceinline__ uint32_t merge_path_row(const uint32_t *off, uint32_t rows,
+                                                   uint64_t edges, uint64_t diag)
+{
+    uint64_t lo = diag > edges ? diag - edges : 0;
+    uint64_t hi = diag < rows ? diag : rows;
+    while (lo < hi) {
+        uint64_t pivot = (lo + hi) >> 1;
+        if ((uint64_t)off[pivot + 1] <= diag - pivot - 1) lo = pivot + 1;
+        else hi = pivot;
+    }
+    return (uint32_t)lo;
+}
+
+__global__ void k_sched(const uint32_t *__restrict__ off, uint32_t rows, uint64_t edges,
+                        uint32_t tiles, uint32_t *srow, uint32_t *sedge)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > tiles) return;
+    uint64_t total = (uint64_t)rows + edges;
+    uint64_t diag = (uint64_t)t * kPullIPB;
+    if (diag > total) diag = total;
+    uint32_t r = merge_path_row(off, rows, edges, diag);
+    srow[t] = r;
+    sedge[t] = (uint32_t)(diag - r);
+}
+
+int sched_build(Ctx *c, PullSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges)
+{
+    s.rows = rows;
+    s.edges = edges;
+    s.tiles = blocks_for((uint64_t)rows + edges, kPullIPB);
+    GG_TRY(buf_ensure(c, s.start_row, 4ull * (s.tiles + 1)));
+    GG_TRY(buf_ensure(c, s.start_edge, 4ull * (s.tiles + 1)));
+    GG_TRY(buf_ensure(c, s.carry_row, 4ull * s.tiles));
+    GG_TRY(buf_ensure(c, s.carry_val, 8ull * s.tiles));
+    GG_TRY(buf_ensure(c, s.head_val, 8ull * s.tiles));
+    k_sched<<<blocks_for(s.tiles + 1, kBlock), kBlock, 0, c->stream>>>(
+        d_off, rows, edges, s.tiles, (uint32_t *)s.start_row.p, (uint32_t *)s.start_edge.p);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+} // namespace gg
+
+using namespace gg;
+
+// =============================================================================
+// C ABI: context and memory
+// =============================================================================
+
+extern "C" const char *gg_last_error(void) { return t_err.c_str(); }
+
+extern "C" const char *gg_version(void) { return "graphgrind_amd 0.1 (gfx950)"; }
+
+extern "C" int gg_ctx_create(int device, gg_ctx_t *out)
+{
+    if (!out) return fail(GG_ERR_INVALID, "gg_ctx_create: out is NULL");
+    int count = 0;
+    GG_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(GG_ERR_INVALID, "gg_ctx_create: no device " + std::to_string(device));
+    GG_HIP(hipSetDevice(device));
+    gg_ctx_s *c = new gg_ctx_s();
+    c->device = device;
+    // default-priority stream
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(GG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    (void)hipEventCreate(&c->ev[0]);
+    (void)hipEventCreate(&c->ev[1]);
+    int rc = buf_alloc(c, c->counters, 4096);
+    if (rc) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return GG_OK;
+}
+
+extern "C" int gg_ctx_destroy(gg_ctx_t c)
+{
+    if (!c) return fail(GG_ERR_INVALID, "gg_ctx_destroy: NULL");
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    buf_free(c->counters);
+    buf_free(c->tmp);
+    buf_free(c->host_pin);
+    (void)hipEventDestroy(c->ev[0]);
+    (void)hipEventDestroy(c->ev[1]);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GG_OK;
+}
+
+extern "C" int gg_ctx_sync(gg_ctx_t c)
+{
+    if (!c) return fail(GG_ERR_INVALID, "gg_ctx_sync: NULL");
+    return sync(c);
+}
+
+extern "C" int gg_malloc(gg_ctx_t c, uint64_t bytes, void **dev)
+{
+    if (!c || !dev) return fail(GG_ERR_INVALID, "gg_malloc: NULL argument");
+    GG_HIP(hipSetDevice(c->device));
+    hipError_t e = hipMalloc(dev, bytes ? bytes : 8);
+    if (e != hipSuccess) {
+        *dev = nullptr;
+        return fail(GG_ERR_OOM, "gg_malloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    return GG_OK;
+}
+
+extern "C" int gg_free(gg_ctx_t c, void *dev)
+{
+    if (!c) return fail(GG_ERR_INVALID, "gg_free: NULL ctx");
+    if (!dev) return GG_OK;
+    GG_HIP(hipStreamSynchronize(c->stream));
+    GG_HIP(hipFree(dev));
+    return GG_OK;
+}
+
+extern "C" int gg_memcpy_h2d(gg_ctx_t c, void *dev, const void *host, uint64_t bytes)
+{
+    if (!c || (bytes && (!dev || !host))) return fail(GG_ERR_INVALID, "gg_memcpy_h2d: NULL argument");
+    if (!bytes) return GG_OK;
+    GG_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+extern "C" int gg_memcpy_d2h(gg_ctx_t c, void *host, const void *dev, uint64_t bytes)
+{
+    if (!c || (bytes && (!dev || !host))) return fail(GG_ERR_INVALID, "gg_memcpy_d2h: NULL argument");
+    if (!bytes) return GG_OK;
+    GG_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+extern "C" int gg_memcpy_d2d(gg_ctx_t c, void *dst, const void *src, uint64_t bytes)
+{
+    if (!c || (bytes && (!dst || !src))) return fail(GG_ERR_INVALID, "gg_memcpy_d2d: NULL argument");
+    if (!bytes) return GG_OK;
+    GG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return sync(c);
+}
+
+extern "C" int gg_fill(gg_ctx_t c, void *dev, int dtype, uint64_t count, double value)
+{
+    if (!c || (count && !dev)) return fail(GG_ERR_INVALID, "gg_fill: NULL argument");
+    if (!count) return GG_OK;
+    uint32_t g = grid_for(count);
+    switch (dtype) {
+    case GG_F64: k_fill<double><<<g, kBlock, 0, c->stream>>>((double *)dev, count, value); break;
+    case GG_F32: k_fill<float><<<g, kBlock, 0, c->stream>>>((float *)dev, count, (float)value); break;
+    case GG_U32: k_fill<uint32_t><<<g, kBlock, 0, c->stream>>>((uint32_t *)dev, count, (uint32_t)(int64_t)value); break;
+    case GG_I32: k_fill<int32_t><<<g, kBlock, 0, c->stream>>>((int32_t *)dev, count, (int32_t)value); break;
+    case GG_U8: k_fill_u8<<<g, kBlock, 0, c->stream>>>((uint8_t *)dev, count, (uint8_t)value); break;
+    default: return fail(GG_ERR_INVALID, "gg_fill: bad dtype");
+    }
+    GG_HIP(hipGetLastError());
+    return sync(c);
+}
+
+extern "C" int gg_iota(gg_ctx_t c, void *dev, int dtype, uint64_t count)
+{
+    if (!c || (count && !dev)) return fail(GG_ERR_INVALID, "gg_iota: NULL argument");
+    if (!count) return GG_OK;
+    uint32_t g = grid_for(count);
+    switch (dtype) {
+    case GG_U32: k_iota<uint32_t><<<g, kBlock, 0, c->stream>>>((uint32_t *)dev, count); break;
+    case GG_I32: k_iota<int32_t><<<g, kBlock, 0, c->stream>>>((int32_t *)dev, count); break;
+    case GG_F64: k_iota<double><<<g, kBlock, 0, c->stream>>>((double *)dev, count); break;
+    case GG_F32: k_iota<float><<<g, kBlock, 0, c->stream>>>((float *)dev, count); break;
+    default: return fail(GG_ERR_INVALID, "gg_iota: bad dtype");
+    }
+    GG_HIP(hipGetLastError());
+    return sync(c);
+}
+
+// =============================================================================
+// Graph construction
+// =============================================================================
+
+namespace gg {
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+// One raw R-MAT edge per thread; self loops become the sentinel key.
+// key = (src << scale) | dst, sentinel = 1 << (2*scale).
+__global__ void k_rmat_keys(int scale, uint64_t nraw, uint64_t seed, int symmetric,
+                            uint64_t *__restrict__ keys)
+{
+    const uint32_t TA = 2448131359u, TAB = TA + 816043786u, TABC = TAB + 816043786u;
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t sentinel = 1ULL << (2 * scale);
+    for (; e < nraw; e += stride) {
+        uint32_t s = 0, d = 0;
+        for (int l = scale - 1; l >= 0; l--) {
+            uint32_t r = (uint32_t)(splitmix64((seed << 40) ^ (e << 6) ^ (uint64_t)l) >> 32);
+            uint32_t sb = r >= TAB;
+            uint32_t db = (r >= TA && r < TAB) || r >= TABC;
+            s |= sb << l;
+            d |= db << l;
+        }
+        keys[e] = (s == d) ? sentinel : (((uint64_t)s << scale) | d);
+        if (symmetric) keys[nraw + e] = (s == d) ? sentinel : (((uint64_t)d << scale) | s);
+    }
+}
+
+__global__ void k_decode_keys(const uint64_t *__restrict__ keys, uint64_t m, int shift,
+                              uint32_t *__restrict__ ids, uint32_t *__restrict__ deg)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t mask = (1ULL << shift) - 1;
+    for (; i < m; i += stride) {
+        uint64_t k = keys[i];
+        ids[i] = (uint32_t)(k & mask);
+        atomicAdd(&deg[k >> shift], 1u);
+    }
+}
+
+__global__ void k_diff(const uint32_t *__restrict__ off, uint32_t *__restrict__ deg, uint32_t n)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) deg[v] = off[v + 1] - off[v];
+}
+
+// In-edge sort keys: (dst << bits) | src for CSR edge e, value e.
+__global__ void k_csc_keys(const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
+                           uint32_t n, uint64_t m, int bits, uint64_t *__restrict__ keys,
+                           uint32_t *__restrict__ vals)
+{
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; e < m; e += stride) {
+        // source = last v with off[v] <= e
+        uint32_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        keys[e] = ((uint64_t)ids[e] << bits) | lo;
+        vals[e] = (uint32_t)e;
+    }
+}
+
+__global__ void k_csc_decode(const uint64_t *__restrict__ keys, uint64_t m, int bits,
+                             uint32_t *__restrict__ tids, uint32_t *__restrict__ indeg)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t mask = (1ULL << bits) - 1;
+    for (; i < m; i += stride) {
+        uint64_t k = keys[i];
+        tids[i] = (uint32_t)(k & mask);
+        atomicAdd(&indeg[k >> bits], 1u);
+    }
+}
+
+__global__ void k_gather_f32(const float *__restrict__ src, const uint32_t *__restrict__ idx,
+                             float *__restrict__ dst, uint64_t m)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < m; i += stride) dst[i] = src[idx[i]];
+}
+
+__global__ void k_rmat_weights(uint64_t seed, float *__restrict__ w, uint64_t m)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < m; i += stride) {
+        uint32_t k = (uint32_t)(splitmix64((seed << 40) ^ (i << 6) ^ 63u) >> 40);
+        w[i] = (float)(k + 1u) * (1.0f / 16777216.0f);
+    }
+}
+
+static int bits_for(uint64_t n)
+{
+    int b = 1;
+    while ((1ULL << b) < n) b++;
+    return b;
+}
+
+// deg[i] for i < n, 0 at i = n: input of the offsets scan
+struct PadDeg {
+    const uint32_t *deg;
+    uint32_t n;
+    __host__ __device__ uint32_t operator()(uint32_t i) const { return i < n ? deg[i] : 0u; }
+};
+
+// exclusive scan of deg[0..n) into off[0..n], off[n] = total
+static int scan_offsets(Ctx *c, const uint32_t *deg, uint32_t *off, uint32_t n)
+{
+    size_t tb = 0;
+    auto in = rocprim::make_transform_iterator(rocprim::make_counting_iterator<uint32_t>(0),
+                                               PadDeg{deg, n});
+    GG_HIP(rocprim::exclusive_scan(nullptr, tb, in, off, 0u, (size_t)n + 1,
+                                   rocprim::plus<uint32_t>(), c->stream));
+    GG_TRY(buf_ensure(c, c->tmp, tb + 16));
+    GG_HIP(rocprim::exclusive_scan(c->tmp.p, tb, in, off, 0u, (size_t)n + 1,
+                                   rocprim::plus<uint32_t>(), c->stream));
+    return GG_OK;
+}
+
+// Builds outdeg, the in-edge twin (toff/tids/perm/tw/indeg) and the pull
+// schedule from a graph whose CSR (off/ids[/w]) is already on the device.
+static int build_twin(Graph *g)
+{
+    Ctx *c = g->ctx;
+    uint32_t n = g->n;
+    uint64_t m = g->m;
+    GG_TRY(buf_alloc(c, g->outdeg, 4ull * n));
+    GG_TRY(buf_alloc(c, g->indeg, 4ull * n));
+    GG_TRY(buf_alloc(c, g->toff, 4ull * (n + 1)));
+    GG_TRY(buf_alloc(c, g->tids, 4ull * (m ? m : 1)));
+    GG_TRY(buf_alloc(c, g->perm, 4ull * (m ? m : 1)));
+    if (n) k_diff<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((uint32_t *)g->off.p,
+                                                                    (uint32_t *)g->outdeg.p, n);
+    GG_TRY(fill_u32(c, g->indeg.p, n, 0));
+    if (m) {
+        int bits = bits_for(n);
+        Buf k0, k1, v1;
+        GG_TRY(buf_alloc(c, k0, 8 * m));
+        GG_TRY(buf_alloc(c, k1, 8 * m));
+        GG_TRY(buf_alloc(c, v1, 4 * m));
+        uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+        k_csc_keys<<<grid, kBlock, 0, c->stream>>>((uint32_t *)g->off.p, (uint32_t *)g->ids.p, n,
+                                                    m, bits, (uint64_t *)k0.p, (uint32_t *)v1.p);
+        GG_HIP(hipGetLastError());
+        size_t tb = 0;
+        GG_HIP(rocprim::radix_sort_pairs(nullptr, tb, (uint64_t *)k0.p, (uint64_t *)k1.p,
+                                         (uint32_t *)v1.p, (uint32_t *)g->perm.p, (size_t)m, 0,
+                                         (unsigned)(2 * bits), c->stream));
+        GG_TRY(buf_ensure(c, c->tmp, tb + 16));
+        GG_HIP(rocprim::radix_sort_pairs(c->tmp.p, tb, (uint64_t *)k0.p, (uint64_t *)k1.p,
+                                         (uint32_t *)v1.p, (uint32_t *)g->perm.p, (size_t)m, 0,
+                                         (unsigned)(2 * bits), c->stream));
+        k_csc_decode<<<grid, kBlock, 0, c->stream>>>((uint64_t *)k1.p, m, bits,
+                                                      (uint32_t *)g->tids.p, (uint32_t *)g->indeg.p);
+        GG_HIP(hipGetLastError());
+        GG_TRY(sync(c));
+        buf_free(k0);
+        buf_free(k1);
+        buf_free(v1);
+        if (g->weighted) {
+            GG_TRY(buf_alloc(c, g->tw, 4 * m));
+            k_gather_f32<<<grid, kBlock, 0, c->stream>>>((float *)g->w.p, (uint32_t *)g->perm.p,
+                                                          (float *)g->tw.p, m);
+            GG_HIP(hipGetLastError());
+        }
+    }
+    GG_TRY(scan_offsets(c, (uint32_t *)g->indeg.p, (uint32_t *)g->toff.p, n));
+    GG_TRY(sched_build(c, g->pull, (uint32_t *)g->toff.p, n, m));
+    return sync(c);
+}
+
+static void graph_free(Graph *g)
+{
+    Buf *bufs[] = {&g->off, &g->ids, &g->w, &g->toff, &g->tids, &g->tw, &g->perm, &g->outdeg,
+                   &g->indeg, &g->comp, &g->coo_src, &g->coo_dst, &g->coo_w, &g->pull.start_row,
+                   &g->pull.start_edge, &g->pull.carry_row, &g->pull.carry_val, &g->pull.head_val};
+    for (Buf *b : bufs) buf_free(*b);
+}
+
+} // namespace gg
+
+extern "C" int gg_graph_create_csr(gg_ctx_t c, uint32_t n, uint64_t m, const uint32_t *off,
+                                   const uint32_t *ids, const float *w, int symmetric,
+                                   gg_graph_t *out)
+{
+    if (!c || !out || !off || (m && !ids)) return fail(GG_ERR_INVALID, "gg_graph_create_csr: NULL argument");
+    if (n == 0) return fail(GG_ERR_INVALID, "gg_graph_create_csr: n must be > 0");
+    if (m > 0xffffffffULL) return fail(GG_ERR_INVALID, "gg_graph_create_csr: m must be < 2^32");
+    // the reference's loader checks header counts (IO.h:150-161); check the CSR itself
+    if (off[0] != 0 || off[n] != m) return fail(GG_ERR_SIZE, "gg_graph_create_csr: offsets do not span [0, m]");
+    for (uint32_t v = 0; v < n; v++)
+        if (off[v + 1] < off[v]) return fail(GG_ERR_INVALID, "gg_graph_create_csr: offsets decrease at " + std::to_string(v));
+    for (uint64_t e = 0; e < m; e++)
+        if (ids[e] >= n) return fail(GG_ERR_INVALID, "gg_graph_create_csr: id out of range at edge " + std::to_string(e));
+    GG_HIP(hipSetDevice(c->device));
+    gg_graph_s *g = new gg_graph_s();
+    g->ctx = c;
+    g->n = n;
+    g->m = m;
+    g->symmetric = symmetric != 0;
+    g->weighted = w != nullptr;
+    int rc = buf_alloc(c, g->off, 4ull * (n + 1));
+    if (!rc) rc = buf_alloc(c, g->ids, 4ull * (m ? m : 1));
+    if (!rc && w) rc = buf_alloc(c, g->w, 4ull * (m ? m : 1));
+    if (!rc && hipMemcpyAsync(g->off.p, off, 4ull * (n + 1), hipMemcpyHostToDevice, c->stream)) rc = fail(GG_ERR_HIP, "upload offsets");
+    if (!rc && m && hipMemcpyAsync(g->ids.p, ids, 4ull * m, hipMemcpyHostToDevice, c->stream)) rc = fail(GG_ERR_HIP, "upload ids");
+    if (!rc && w && m && hipMemcpyAsync(g->w.p, w, 4ull * m, hipMemcpyHostToDevice, c->stream)) rc = fail(GG_ERR_HIP, "upload weights");
+    if (!rc) rc = build_twin(g);
+    if (rc) {
+        graph_free(g);
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return GG_OK;
+}
+
+extern "C" int gg_graph_create_rmat(gg_ctx_t c, int scale, int edge_factor, uint64_t seed,
+                                    int symmetric, int weighted, gg_graph_t *out)
+{
+    if (!c || !out) return fail(GG_ERR_INVALID, "gg_graph_create_rmat: NULL argument");
+    if (scale < 1 || scale > 30 || edge_factor < 1 || edge_factor > 1024)
+        return fail(GG_ERR_INVALID, "gg_graph_create_rmat: scale in [1,30], edge_factor in [1,1024]");
+    GG_HIP(hipSetDevice(c->device));
+    uint64_t n = 1ULL << scale;
+    uint64_t nraw = (uint64_t)edge_factor << scale;
+    uint64_t nkeys = symmetric ? 2 * nraw : nraw;
+    if (nkeys > 0xffffffffULL) return fail(GG_ERR_INVALID, "gg_graph_create_rmat: too many edges for uint32 offsets");
+    gg_graph_s *g = new gg_graph_s();
+    g->ctx = c;
+    g->n = (uint32_t)n;
+    g->symmetric = symmetric != 0;
+    g->weighted = weighted != 0;
+    Buf k0, k1, cnt;
+    auto cleanup = [&](int rc) {
+        buf_free(k0);
+        buf_free(k1);
+        buf_free(cnt);
+        if (rc) {
+            graph_free(g);
+            delete g;
+        }
+        return rc;
+    };
+    int rc = buf_alloc(c, k0, 8 * nkeys);
+    if (!rc) rc = buf_alloc(c, k1, 8 * nkeys);
+    if (!rc) rc = buf_alloc(c, cnt, 8);
+    if (rc) return cleanup(rc);
+    uint32_t grid = (uint32_t)std::min<uint64_t>((nraw + kBlock - 1) / kBlock, 1u << 20);
+    k_rmat_keys<<<grid, kBlock, 0, c->stream>>>(scale, nraw, seed, symmetric, (uint64_t *)k0.p);
+    if (hipGetLastError() != hipSuccess) return cleanup(fail(GG_ERR_HIP, "k_rmat_keys launch"));
+    size_t tb = 0;
+    unsigned end_bit = (unsigned)(2 * scale + 1);
+    if (rocprim::radix_sort_keys(nullptr, tb, (uint64_t *)k0.p, (uint64_t *)k1.p, (size_t)nkeys, 0,
+                                 end_bit, c->stream) != hipSuccess)
+        return cleanup(fail(GG_ERR_HIP, "radix_sort_keys size query"));
+    if ((rc = buf_ensure(c, c->tmp, tb + 16))) return cleanup(rc);
+    if (rocprim::radix_sort_keys(c->tmp.p, tb, (uint64_t *)k0.p, (uint64_t *)k1.p, (size_t)nkeys, 0,
+                                 end_bit, c->stream) != hipSuccess)
+        return cleanup(fail(GG_ERR_HIP, "radix_sort_keys"));
+    // dedupe into k0
+    tb = 0;
+    if (rocprim::unique(nullptr, tb, (uint64_t *)k1.p, (uint64_t *)k0.p, (uint64_t *)cnt.p,
+                        (size_t)nkeys, rocprim::equal_to<uint64_t>(), c->stream) != hipSuccess)
+        return cleanup(fail(GG_ERR_HIP, "unique size query"));
+    if ((rc = buf_ensure(c, c->tmp, tb + 16))) return cleanup(rc);
+    if (rocprim::unique(c->tmp.p, tb, (uint64_t *)k1.p, (uint64_t *)k0.p, (uint64_t *)cnt.p,
+                        (size_t)nkeys, rocprim::equal_to<uint64_t>(), c->stream) != hipSuccess)
+        return cleanup(fail(GG_ERR_HIP, "unique"));
+    uint64_t uniq = 0, last = 0;
+    if (hipMemcpyAsync(&uniq, cnt.p, 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return cleanup(fail(GG_ERR_HIP, "count readback"));
+    if ((rc = sync(c))) return cleanup(rc);
+    if (uniq &&
+        hipMemcpyAsync(&last, (uint64_t *)k0.p + (uniq - 1), 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return cleanup(fail(GG_ERR_HIP, "last key readback"));
+    if ((rc = sync(c))) return cleanup(rc);
+    uint64_t m = (uniq && last == (1ULL << (2 * scale))) ? uniq - 1 : uniq;
+    g->m = m;
+    Buf deg;
+    rc = buf_alloc(c, g->off, 4 * (n + 1));
+    if (!rc) rc = buf_alloc(c, g->ids, 4 * (m ? m : 1));
+    if (!rc) rc = buf_alloc(c, deg, 4 * n);
+    if (!rc) rc = fill_u32(c, deg.p, n, 0);
+    if (rc) { buf_free(deg); return cleanup(rc); }
+    if (m) {
+        uint32_t g2 = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+        k_decode_keys<<<g2, kBlock, 0, c->stream>>>((uint64_t *)k0.p, m, scale, (uint32_t *)g->ids.p,
+                                                     (uint32_t *)deg.p);
+    }
+    rc = scan_offsets(c, (uint32_t *)deg.p, (uint32_t *)g->off.p, (uint32_t)n);
+    if (!rc) rc = sync(c);
+    buf_free(deg);
+    buf_free(k0);
+    buf_free(k1);
+    if (rc) return cleanup(rc);
+    if (weighted) {
+        rc = buf_alloc(c, g->w, 4 * (m ? m : 1));
+        if (rc) return cleanup(rc);
+        if (m) {
+            uint32_t g2 = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+            k_rmat_weights<<<g2, kBlock, 0, c->stream>>>(seed, (float *)g->w.p, m);
+        }
+    }
+    rc = build_twin(g);
+    if (rc) return cleanup(rc);
+    cleanup(0);
+    *out = g;
+    return GG_OK;
+}
+
+extern "C" int gg_graph_destroy(gg_graph_t g)
+{
+    if (!g) return fail(GG_ERR_INVALID, "gg_graph_destroy: NULL");
+    (void)hipStreamSynchronize(g->ctx->stream);
+    graph_free(g);
+    delete g;
+    return GG_OK;
+}
+
+extern "C" int gg_graph_info(gg_graph_t g, uint32_t *n, uint64_t *m, int *symmetric, int *weighted)
+{
+    if (!g) return fail(GG_ERR_INVALID, "gg_graph_info: NULL");
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (symmetric) *symmetric = g->symmetric;
+    if (weighted) *weighted = g->weighted;
+    return GG_OK;
+}
+
+static int copy_out(Ctx *c, void *host, const Buf &b, uint64_t bytes)
+{
+    if (!host || !bytes) return GG_OK;
+    GG_HIP(hipMemcpyAsync(host, b.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    return GG_OK;
+}
+
+extern "C" int gg_graph_copy_csr(gg_graph_t g, uint32_t *off, uint32_t *ids, float *w)
+{
+    if (!g) return fail(GG_ERR_INVALID, "gg_graph_copy_csr: NULL");
+    if (w && !g->weighted) return fail(GG_ERR_STATE, "gg_graph_copy_csr: graph has no weights");
+    GG_TRY(copy_out(g->ctx, off, g->off, 4ull * (g->n + 1)));
+    GG_TRY(copy_out(g->ctx, ids, g->ids, 4ull * g->m));
+    if (w) GG_TRY(copy_out(g->ctx, w, g->w, 4ull * g->m));
+    return sync(g->ctx);
+}
+
+extern "C" int gg_graph_copy_csc(gg_graph_t g, uint32_t *toff, uint32_t *tids, float *tw)
+{
+    if (!g) return fail(GG_ERR_INVALID, "gg_graph_copy_csc: NULL");
+    if (tw && !g->weighted) return fail(GG_ERR_STATE, "gg_graph_copy_csc: graph has no weights");
+    GG_TRY(copy_out(g->ctx, toff, g->toff, 4ull * (g->n + 1)));
+    GG_TRY(copy_out(g->ctx, tids, g->tids, 4ull * g->m));
+    if (tw) GG_TRY(copy_out(g->ctx, tw, g->tw, 4ull * g->m));
+    return sync(g->ctx);
+}
+
+extern "C" int gg_graph_degrees(gg_graph_t g, int in_edges, uint32_t *out)
+{
+    if (!g || !out) return fail(GG_ERR_INVALID, "gg_graph_degrees: NULL");
+    GG_TRY(copy_out(g->ctx, out, in_edges ? g->indeg : g->outdeg, 4ull * g->n));
+    return sync(g->ctx);
+}
+
+extern "C" int gg_graph_set_weights(gg_graph_t g, const float *w_csr)
+{
+    if (!g || !w_csr) return fail(GG_ERR_INVALID, "gg_graph_set_weights: NULL");
+    Ctx *c = g->ctx;
+    uint64_t m = g->m;
+    if (!g->weighted) {
+        GG_TRY(buf_alloc(c, g->w, 4 * (m ? m : 1)));
+        GG_TRY(buf_alloc(c, g->tw, 4 * (m ? m : 1)));
+        g->weighted = true;
+    }
+    if (!m) return GG_OK;
+    GG_HIP(hipMemcpyAsync(g->w.p, w_csr, 4 * m, hipMemcpyHostToDevice, c->stream));
+    uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+    k_gather_f32<<<grid, kBlock, 0, c->stream>>>((float *)g->w.p, (uint32_t *)g->perm.p,
+                                                  (float *)g->tw.p, m);
+    GG_HIP(hipGetLastError());
+    if (g->coo_w.p) {
+        // COO weights follow the COO order; rebuild them from the partition
+        // by re-partitioning (rare path)
+        int rc = gg_graph_partition(g, g->P, g->part_vertex, g->relabel, g->hilbert);
+        if (rc) return rc;
+    }
+    return sync(c);
+}
+
+// =============================================================================
+// Device partitioner (graph-numa.h:1177-1357) and COO build (1116-1143)
+// =============================================================================
+
+namespace gg {
+
+// Edge-balanced greedy of partitionByDegree / cscpartitionByDegree on a
+// prefix array pre[0..count] (pre[i+1]-pre[i] = degree of item i).  With
+// E(s,i) = pre[i+1]-pre[s], a partition starting at s closes after
+//   i1 = first i with E(s,i) >= avg                     (graph-numa.h:1337)
+//   i2 = first i <= count-2 with E(s,i+1) > 1.1*avg,    (look-ahead, 1335)
+//        when E(s,i2) < avg (then i2 < i1)
+// whichever comes first; the look-ahead has no P guard, so firing it in the
+// last partition is the reference's overrun.  The degree past the end
+// (degrees[i+1] at i = count-1) is taken as 0.  One thread runs the P
+// sequential cuts, each a pair of binary searches.
+__global__ void k_partition_greedy(const uint32_t *__restrict__ pre, uint64_t count, uint64_t m,
+                                   int P, uint64_t *__restrict__ starts, int *__restrict__ status)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int64_t avg = (int64_t)(m / (uint64_t)P);
+    const double lim = 1.1 * (double)avg;
+    uint64_t s = 0;
+    int cuts = 0;
+    int st = 0;
+    starts[0] = 0;
+    for (int c = 0; c < P; c++) {
+        if (s >= count) { // nothing left: the remaining partitions are empty
+            for (int k = c; k < P; k++) starts[k + 1] = count;
+            break;
+        }
+        const uint64_t base = pre[s];
+        // i2: first j in [s+2, count] with pre[j]-base > lim, i2 = j-2
+        uint64_t lo = s + 2, hi = count + 1;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi) >> 1;
+            if ((double)(int64_t)((uint64_t)pre[mid] - base) > lim) hi = mid;
+            else lo = mid + 1;
+        }
+        uint64_t i2 = lo <= count ? lo - 2 : count; // count = none
+        bool fires = i2 < count && (int64_t)((uint64_t)pre[i2 + 1] - base) < avg;
+        if (c == P - 1) {
+            if (fires) st = -1; // greedy overruns its P partitions
+            starts[P] = count;
+            break;
+        }
+        // i1: first j in [s+1, count] with pre[j]-base >= avg, i1 = j-1
+        lo = s + 1;
+        hi = count + 1;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi) >> 1;
+            if ((int64_t)((uint64_t)pre[mid] - base) >= avg) hi = mid;
+            else lo = mid + 1;
+        }
+        uint64_t i1 = lo <= count ? lo - 1 : count;
+        uint64_t cut = fires ? i2 : i1;
+        if (cut >= count) { // the rest stays in this partition
+            starts[c + 1] = count;
+            for (int k = c + 1; k < P; k++) starts[k + 1] = count;
+            break;
+        }
+        cuts++;
+        s = cut + 1;
+        starts[c + 1] = s;
+        if (s >= count) {
+            for (int k = c + 1; k < P; k++) starts[k + 1] = count;
+            break;
+        }
+    }
+    // assert(counter+1 == P) at 1353: counter = number of increments
+    if (st == 0 && cuts + 1 != P) st = 1;
+    *status = st;
+}
+
+// Relabel ('-o') rule: cut after the first P-1 positions j with
+// deg[j+1] > deg[j] (graph-numa.h:1188-1193, 1298-1303).
+__global__ void k_partition_relabel(const uint32_t *__restrict__ pre, uint64_t count, int P,
+                                    uint64_t *__restrict__ starts, int *__restrict__ status)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int c = 0;
+    starts[0] = 0;
+    for (uint64_t j = 0; j + 1 < count && c < P - 1; j++) {
+        uint32_t dj = pre[j + 1] - pre[j], dn = pre[j + 2] - pre[j + 1];
+        if (dn > dj) starts[++c] = j + 1;
+    }
+    int used = c;
+    for (int k = c + 1; k <= P; k++) starts[k] = count;
+    *status = (used + 1 != P) ? 1 : 0;
+}
+
+// Prefix of in-degrees over the compressed in-vertex list: cpre[k] =
+// toff[comp[k]], cpre[nc] = m.
+__global__ void k_compress_flags(const uint32_t *__restrict__ indeg, uint32_t n,
+                                 uint32_t *__restrict__ flag)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) flag[v] = indeg[v] != 0;
+}
+
+__global__ void k_compress_scatter(const uint32_t *__restrict__ indeg, const uint32_t *__restrict__ pos,
+                                   const uint32_t *__restrict__ toff, uint32_t n,
+                                   uint32_t *__restrict__ comp, uint32_t *__restrict__ cpre)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n && indeg[v]) {
+        comp[pos[v]] = v;
+        cpre[pos[v]] = toff[v];
+    }
+}
+
+__device__ __forceinline__ uint64_t hilbert_xy2d(int64_t side, int64_t x, int64_t y)
+{
+    // graph-numa.h:375-386 and rot 403-414, signed 64-bit as intT = long
+    int64_t d = 0;
+    for (int64_t s = side / 2; s > 0; s /= 2) {
+        int64_t rx = (x & s) > 0;
+        int64_t ry = (y & s) > 0;
+        d += s * s * ((3 * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = s - 1 - x;
+                y = s - 1 - y;
+            }
+            int64_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return (uint64_t)d;
+}
+
+// COO keys for the in-edges of destination partitions: edge k of the in-edge
+// twin belongs to partition p (eoff[p] <= k < eoff[p+1]); its key is the
+// Hilbert index of (src, dst) on the side x side grid with side =
+// roundUpPow2(#edges of p) (HilbertEdgeSort(num_edges), graph-numa.h:358,487),
+// or (src,dst) packed for CSR order (CSRDestSort, 338-350).
+__global__ void k_coo_keys(const uint32_t *__restrict__ toff, const uint32_t *__restrict__ tids,
+                           uint32_t n, uint64_t m, const uint64_t *__restrict__ eoff,
+                           const int64_t *__restrict__ side, int P, int hilbert,
+                           uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                           uint32_t *__restrict__ edst)
+{
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; k < m; k += stride) {
+        uint32_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (toff[mid] <= k) lo = mid;
+            else hi = mid;
+        }
+        uint32_t d = lo;
+        int plo = 0, phi = P;
+        while (phi - plo > 1) {
+            int mid = (plo + phi) >> 1;
+            if (eoff[mid] <= k) plo = mid;
+            else phi = mid;
+        }
+        uint32_t s = tids[k];
+        keys[k] = hilbert ? hilbert_xy2d(side[plo], s, d) : (((uint64_t)s << 32) | d);
+        vals[k] = (uint32_t)k;
+        edst[k] = d;
+    }
+}
+
+__global__ void k_coo_emit(const uint32_t *__restrict__ order, const uint32_t *__restrict__ tids,
+                           const uint32_t *__restrict__ edst, const float *__restrict__ tw,
+                           uint64_t m, uint32_t *__restrict__ csrc, uint32_t *__restrict__ cdst,
+                           float *__restrict__ cw)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < m; i += stride) {
+        uint32_t k = order[i];
+        csrc[i] = tids[k];
+        cdst[i] = edst[k];
+        if (cw) cw[i] = tw[k];
+    }
+}
+
+static uint64_t round_up_pow2(uint64_t v)
+{
+    --v;
+    for (unsigned sh = 1; sh < 64; sh <<= 1) v |= v >> sh;
+    return ++v;
+}
+
+// Runs the partition sizing on a device prefix array.  Returns GG_OK,
+// sets *underran; GG_ERR_PARTITION on overrun.
+static int run_sizing(Ctx *c, const uint32_t *pre, uint64_t count, uint64_t m, int P, bool by_vertex,
+                      bool relabel, std::vector<uint64_t> &starts, bool *underran)
+{
+    starts.assign((size_t)P + 1, 0);
+    *underran = false;
+    if (relabel) {
+        Buf ds, st;
+        GG_TRY(buf_alloc(c, ds, 8ull * (P + 1)));
+        GG_TRY(buf_alloc(c, st, 8));
+        k_partition_relabel<<<1, 64, 0, c->stream>>>(pre, count, P, (uint64_t *)ds.p, (int *)st.p);
+        GG_HIP(hipGetLastError());
+        int status = 0;
+        GG_HIP(hipMemcpyAsync(starts.data(), ds.p, 8ull * (P + 1), hipMemcpyDeviceToHost, c->stream));
+        GG_HIP(hipMemcpyAsync(&status, st.p, 4, hipMemcpyDeviceToHost, c->stream));
+        GG_TRY(sync(c));
+        buf_free(ds);
+        buf_free(st);
+        *underran = (status == 1) && !by_vertex;
+        return GG_OK;
+    }
+    if (by_vertex) {
+        // partitionByVertex / cscpartitionByVertex: count/P each, rest last
+        uint64_t chunk = count / (uint64_t)P;
+        for (int p = 0; p < P; p++) starts[p] = chunk * p;
+        starts[P] = count;
+        return GG_OK;
+    }
+    Buf ds, st;
+    GG_TRY(buf_alloc(c, ds, 8ull * (P + 1)));
+    GG_TRY(buf_alloc(c, st, 8));
+    k_partition_greedy<<<1, 64, 0, c->stream>>>(pre, count, m, P, (uint64_t *)ds.p, (int *)st.p);
+    GG_HIP(hipGetLastError());
+    int status = 0;
+    GG_HIP(hipMemcpyAsync(starts.data(), ds.p, 8ull * (P + 1), hipMemcpyDeviceToHost, c->stream));
+    GG_HIP(hipMemcpyAsync(&status, st.p, 4, hipMemcpyDeviceToHost, c->stream));
+    GG_TRY(sync(c));
+    buf_free(ds);
+    buf_free(st);
+    if (status < 0)
+        return fail(GG_ERR_PARTITION, "partition greedy overruns " + std::to_string(P) +
+                                          " partitions (graph-numa.h:1335 has no bound check)");
+    *underran = status == 1;
+    return GG_OK;
+}
+
+} // namespace gg
+
+extern "C" int gg_graph_partition(gg_graph_t g, int P, int by_vertex, int relabel, int hilbert)
+{
+    if (!g) return fail(GG_ERR_INVALID, "gg_graph_partition: NULL");
+    if (P < 1 || (uint64_t)P > g->n) return fail(GG_ERR_INVALID, "gg_graph_partition: need 1 <= P <= n");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    // COO partitioner over all vertices, in-degree (graph-numa.h:651-655)
+    std::vector<uint64_t> coo, csc;
+    bool u1 = false, u2 = false;
+    GG_TRY(run_sizing(c, (uint32_t *)g->toff.p, n, m, P, by_vertex, relabel, coo, &u1));
+    // compressed in-vertex list and its prefix (PartitionByDest 1085-1107)
+    Buf flag, pos, cpre;
+    GG_TRY(buf_alloc(c, flag, 4ull * n));
+    GG_TRY(buf_alloc(c, pos, 4ull * (n + 1)));
+    k_compress_flags<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((uint32_t *)g->indeg.p, n,
+                                                                       (uint32_t *)flag.p);
+    GG_TRY(scan_offsets(c, (uint32_t *)flag.p, (uint32_t *)pos.p, n));
+    uint32_t nc = 0;
+    GG_HIP(hipMemcpyAsync(&nc, (uint32_t *)pos.p + n, 4, hipMemcpyDeviceToHost, c->stream));
+    GG_TRY(sync(c));
+    g->ncomp = nc;
+    GG_TRY(buf_alloc(c, g->comp, 4ull * (nc ? nc : 1)));
+    GG_TRY(buf_alloc(c, cpre, 4ull * (nc + 1)));
+    k_compress_scatter<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
+        (uint32_t *)g->indeg.p, (uint32_t *)pos.p, (uint32_t *)g->toff.p, n, (uint32_t *)g->comp.p,
+        (uint32_t *)cpre.p);
+    uint32_t mm = (uint32_t)m;
+    GG_HIP(hipMemcpyAsync((uint32_t *)cpre.p + nc, &mm, 4, hipMemcpyHostToDevice, c->stream));
+    GG_TRY(sync(c));
+    int rc = run_sizing(c, (uint32_t *)cpre.p, nc, m, P, by_vertex, relabel, csc, &u2);
+    buf_free(flag);
+    buf_free(pos);
+    buf_free(cpre);
+    if (rc) return rc;
+    g->P = P;
+    g->part_vertex = by_vertex != 0;
+    g->relabel = relabel != 0;
+    g->hilbert = hilbert != 0;
+    g->coo_starts = coo;
+    g->csc_starts = csc;
+    g->coo_underran = u1;
+    g->csc_underran = u2;
+    // COO lists (edge mode only, graph-numa.h:661-675)
+    buf_free(g->coo_src);
+    buf_free(g->coo_dst);
+    buf_free(g->coo_w);
+    g->coo_eoff.assign((size_t)P + 1, 0);
+    if (by_vertex || m == 0) return GG_OK;
+    std::vector<uint32_t> toff_h((size_t)n + 1);
+    GG_HIP(hipMemcpyAsync(toff_h.data(), g->toff.p, 4ull * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    GG_TRY(sync(c));
+    std::vector<int64_t> side((size_t)P);
+    std::vector<uint32_t> seg((size_t)P + 1);
+    int maxbits = 1;
+    for (int p = 0; p <= P; p++) {
+        g->coo_eoff[p] = toff_h[coo[p]];
+        seg[p] = (uint32_t)g->coo_eoff[p];
+    }
+    for (int p = 0; p < P; p++) {
+        uint64_t cnt = g->coo_eoff[p + 1] - g->coo_eoff[p];
+        side[p] = (int64_t)round_up_pow2(cnt);
+        int b = 0;
+        while (b < 62 && (1ULL << b) < (uint64_t)side[p]) b++;
+        maxbits = std::max(maxbits, 2 * b);
+    }
+    Buf deoff, dside, dseg, k0, k1, v0, v1, edst;
+    auto cleanup = [&]() {
+        Buf *bs[] = {&deoff, &dside, &dseg, &k0, &k1, &v0, &v1, &edst};
+        for (Buf *b : bs) buf_free(*b);
+    };
+    rc = buf_alloc(c, deoff, 8ull * (P + 1));
+    if (!rc) rc = buf_alloc(c, dside, 8ull * P);
+    if (!rc) rc = buf_alloc(c, dseg, 4ull * (P + 1));
+    if (!rc) rc = buf_alloc(c, k0, 8 * m);
+    if (!rc) rc = buf_alloc(c, k1, 8 * m);
+    if (!rc) rc = buf_alloc(c, v0, 4 * m);
+    if (!rc) rc = buf_alloc(c, v1, 4 * m);
+    if (!rc) rc = buf_alloc(c, edst, 4 * m);
+    if (!rc) rc = buf_alloc(c, g->coo_src, 4 * m);
+    if (!rc) rc = buf_alloc(c, g->coo_dst, 4 * m);
+    if (!rc && g->weighted) rc = buf_alloc(c, g->coo_w, 4 * m);
+    if (rc) { cleanup(); return rc; }
+    if (hipMemcpyAsync(deoff.p, g->coo_eoff.data(), 8ull * (P + 1), hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dside.p, side.data(), 8ull * P, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dseg.p, seg.data(), 4ull * (P + 1), hipMemcpyHostToDevice, c->stream)) {
+        cleanup();
+        return fail(GG_ERR_HIP, "partition upload");
+    }
+    uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+    k_coo_keys<<<grid, kBlock, 0, c->stream>>>((uint32_t *)g->toff.p, (uint32_t *)g->tids.p, n, m,
+                                                (uint64_t *)deoff.p, (int64_t *)dside.p, P, hilbert,
+                                                (uint64_t *)k0.p, (uint32_t *)v0.p, (uint32_t *)edst.p);
+    size_t tb = 0;
+    unsigned end_bit = hilbert ? (unsigned)std::min(64, maxbits) : 64u;
+    hipError_t e = rocprim::segmented_radix_sort_pairs(
+        nullptr, tb, (uint64_t *)k0.p, (uint64_t *)k1.p, (uint32_t *)v0.p, (uint32_t *)v1.p,
+        (unsigned)m, (unsigned)P, (uint32_t *)dseg.p, (uint32_t *)dseg.p + 1, 0, end_bit, c->stream);
+    if (e == hipSuccess) rc = buf_ensure(c, c->tmp, tb + 16);
+    if (e == hipSuccess && !rc)
+        e = rocprim::segmented_radix_sort_pairs(c->tmp.p, tb, (uint64_t *)k0.p, (uint64_t *)k1.p,
+                                                (uint32_t *)v0.p, (uint32_t *)v1.p, (unsigned)m,
+                                                (unsigned)P, (uint32_t *)dseg.p,
+                                                (uint32_t *)dseg.p + 1, 0, end_bit, c->stream);
+    if (e != hipSuccess || rc) {
+        cleanup();
+        return rc ? rc : fail(GG_ERR_HIP, std::string("segmented sort: ") + hipGetErrorString(e));
+    }
+    k_coo_emit<<<grid, kBlock, 0, c->stream>>>((uint32_t *)v1.p, (uint32_t *)g->tids.p,
+                                                (uint32_t *)edst.p, (float *)g->tw.p, m,
+                                                (uint32_t *)g->coo_src.p, (uint32_t *)g->coo_dst.p,
+                                                (float *)g->coo_w.p);
+    e = hipGetLastError();
+    rc = sync(c);
+    cleanup();
+    if (e != hipSuccess) return fail(GG_ERR_HIP, "k_coo_emit");
+    return rc;
+}
+
+extern "C" int gg_graph_partition_starts(gg_graph_t g, int which, uint64_t *starts, int *underran)
+{
+    if (!g || !starts) return fail(GG_ERR_INVALID, "gg_graph_partition_starts: NULL");
+    if (!g->P) return fail(GG_ERR_STATE, "gg_graph_partition_starts: graph not partitioned");
+    const std::vector<uint64_t> &s = which == 0 ? g->coo_starts : g->csc_starts;
+    std::memcpy(starts, s.data(), 8 * s.size());
+    if (underran) *underran = which == 0 ? g->coo_underran : g->csc_underran;
+    return GG_OK;
+}
+
+extern "C" int gg_graph_coo(gg_graph_t g, uint32_t *src, uint32_t *dst, uint64_t *part_off)
+{
+    if (!g) return fail(GG_ERR_INVALID, "gg_graph_coo: NULL");
+    if (!g->P) return fail(GG_ERR_STATE, "gg_graph_coo: graph not partitioned");
+    if (g->part_vertex) return fail(GG_ERR_STATE, "gg_graph_coo: '-v vertex' builds no COO lists");
+    if (part_off) std::memcpy(part_off, g->coo_eoff.data(), 8 * g->coo_eoff.size());
+    GG_TRY(copy_out(g->ctx, src, g->coo_src, 4 * g->m));
+    GG_TRY(copy_out(g->ctx, dst, g->coo_dst, 4 * g->m));
+    return sync(g->ctx);
+}
+
+extern "C" int gg_last_kernel_time(gg_ctx_t c, double *avg_ms, int *launches)
+{
+    if (!c) return fail(GG_ERR_INVALID, "gg_last_kernel_time: NULL");
+    if (avg_ms) *avg_ms = c->last_kernel_launches ? c->last_kernel_ms / c->last_kernel_launches : 0;
+    if (launches) *launches = c->last_kernel_launches;
+    return GG_OK;
+}

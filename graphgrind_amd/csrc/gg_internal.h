@@ -1,0 +1,143 @@
+// Internal declarations shared by the HIP translation units of libgraphgrind_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "graphgrind_amd.h"
+
+namespace gg {
+
+// ---- error plumbing --------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define GG_HIP(expr)                                                               \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess)                                                      \
+            return ::gg::fail(_e == hipErrorOutOfMemory ? GG_ERR_OOM : GG_ERR_HIP, \
+                              std::string(#expr) + ": " + hipGetErrorString(_e));  \
+    } while (0)
+
+#define GG_TRY(expr)              \
+    do {                          \
+        int _rc = (expr);         \
+        if (_rc != GG_OK) return _rc; \
+    } while (0)
+
+// Work-group size of every kernel in the library (4 wave64s).
+constexpr int kBlock = 256;
+// Items (rows + edges) per work-group of the merge-path pull.
+constexpr int kPullIPT = 8;
+constexpr int kPullIPB = kBlock * kPullIPT;
+
+inline uint32_t blocks_for(uint64_t items, uint64_t per_block)
+{
+    uint64_t b = (items + per_block - 1) / per_block;
+    return (uint32_t)(b ? b : 1);
+}
+
+// ---- device buffer owned by a context --------------------------------------
+struct Buf {
+    void *p = nullptr;
+    uint64_t bytes = 0;
+};
+
+struct Ctx;
+int buf_alloc(Ctx *c, Buf &b, uint64_t bytes);
+int buf_ensure(Ctx *c, Buf &b, uint64_t bytes); // grow-only
+void buf_free(Buf &b);
+
+// ---- merge-path schedule of one CSR-like array ------------------------------
+// For offsets off[0..rows] and rows+edges items cut into kPullIPB-item tiles:
+// tile t starts at diagonal t*kPullIPB at (row start_row[t], edge start_edge[t]).
+struct PullSched {
+    uint32_t rows = 0;
+    uint64_t edges = 0;
+    uint32_t tiles = 0;
+    Buf start_row;  // uint32[tiles+1]
+    Buf start_edge; // uint32[tiles+1]
+    Buf carry_row;  // uint32[tiles]  row in progress at tile end
+    Buf carry_val;  // 8 bytes [tiles] partial of that row
+    Buf head_val;   // 8 bytes [tiles] partial of the first row if it began earlier
+};
+
+// ---- context ----------------------------------------------------------------
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    Buf counters;   // small scratch: uint64 counters/flags (256 B)
+    Buf host_pin;   // unused placeholder for pinned staging
+    Buf tmp;        // grow-only temporary (sort space etc.)
+    double last_kernel_ms = 0;
+    int last_kernel_launches = 0;
+};
+
+// ---- graph ------------------------------------------------------------------
+struct Graph {
+    Ctx *ctx = nullptr;
+    uint32_t n = 0;
+    uint64_t m = 0;
+    bool symmetric = false;
+    bool weighted = false;
+    // CSR (out-edges) and in-edge twin
+    Buf off, ids, w;     // uint32[n+1], uint32[m], float[m]
+    Buf toff, tids, tw;  // uint32[n+1], uint32[m], float[m]
+    Buf perm;            // uint32[m]: CSC position -> CSR position
+    Buf outdeg, indeg;   // uint32[n]
+    // partitions
+    int P = 0;
+    bool part_vertex = false, hilbert = true, relabel = false;
+    bool coo_underran = false, csc_underran = false;
+    std::vector<uint64_t> coo_starts, csc_starts; // host copies (P+1)
+    std::vector<uint64_t> coo_eoff;               // host (P+1)
+    Buf comp;           // uint32[ncomp] compressed in-vertices
+    uint64_t ncomp = 0;
+    Buf coo_src, coo_dst; // uint32[m] (edge mode only)
+    Buf coo_w;            // float[m] when weighted
+    // merge-path schedule over the in-edge twin
+    PullSched pull;
+};
+
+// ---- frontier ---------------------------------------------------------------
+struct Frontier {
+    Graph *g = nullptr;
+    uint32_t n = 0;      // numRows (0 for the empty() frontier)
+    bool bit = false;    // all active, flags not read
+    bool has_dense = false;
+    bool dense_valid = false;  // bytes + bitmap hold the set
+    bool sparse_valid = false; // list holds the set
+    uint64_t d_m = 0;
+    uint64_t out_edges = 0;
+    Buf bytes;  // uint8[n]
+    Buf bitmap; // uint32[(n+31)/32]
+    Buf list;   // uint32[cap]
+};
+
+// ---- shared device helpers (defined in gg_core.hip) ------------------------
+int sched_build(Ctx *c, PullSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges);
+int fill_bytes(Ctx *c, void *p, uint64_t bytes, uint8_t v);
+int fill_u32(Ctx *c, void *p, uint64_t count, uint32_t v);
+int bitmap_from_bytes(Ctx *c, const uint8_t *bytes, uint32_t *bitmap, uint32_t n);
+int sync(Ctx *c);
+
+// Frontier helpers (gg_traverse.hip)
+int frontier_alloc(Graph *g, Frontier *f);
+int frontier_make_dense(Frontier *f);
+int frontier_make_sparse(Frontier *f);
+int frontier_stats_dense(Frontier *f);
+
+} // namespace gg
+
+struct gg_ctx_s : gg::Ctx {};
+struct gg_graph_s : gg::Graph {};
+struct gg_frontier_s : gg::Frontier {};
+
+namespace gg {
+// Contexts are always allocated as gg_ctx_s, so the downcast is exact.
+inline gg_ctx_t abi(Ctx *c) { return static_cast<gg_ctx_t>(c); }
+} // namespace gg

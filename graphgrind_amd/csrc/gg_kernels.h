@@ -1,0 +1,247 @@
+// gg_kernels.h -- device kernel templates shared by the traversal and app
+// translation units.  Included by .hip files only.
+#pragma once
+#include "gg_internal.h"
+
+namespace gg {
+
+// Per-thread statistics of an edgeMap output frontier (GoutDegree,
+// ligra-numa.h:275-288): active count, their out-degree sum, plus an FP
+// residual used by the fused PageRank loop.
+struct LocalStats {
+    unsigned long long active = 0;
+    unsigned long long out_edges = 0;
+    double l1 = 0.0;
+};
+
+// counters layout (unsigned long long): [0] active, [1] out_edges,
+// [2] edges scanned, [3] list cursor
+enum { CNT_ACTIVE = 0, CNT_OUT = 1, CNT_SCANNED = 2, CNT_CURSOR = 3 };
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Block reduction of LocalStats in a fixed tree; thread 0 adds the counts to
+// the global counters and returns the block's l1 (meaningful in thread 0).
+__device__ __forceinline__ double block_stats(const LocalStats &st, unsigned long long *counters)
+{
+    __shared__ unsigned long long s_a[kBlock / 64], s_o[kBlock / 64];
+    __shared__ double s_l[kBlock / 64];
+    unsigned long long a = wave_sum_u64(st.active);
+    unsigned long long o = wave_sum_u64(st.out_edges);
+    double l = wave_sum_f64(st.l1);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_a[w] = a;
+        s_o[w] = o;
+        s_l[w] = l;
+    }
+    __syncthreads();
+    double tot = 0;
+    if (threadIdx.x == 0) {
+        unsigned long long ta = 0, to = 0;
+        for (int k = 0; k < kBlock / 64; k++) {
+            ta += s_a[k];
+            to += s_o[k];
+            tot += s_l[k];
+        }
+        if (counters) {
+            if (ta) atomicAdd(&counters[CNT_ACTIVE], ta);
+            if (to) atomicAdd(&counters[CNT_OUT], to);
+        }
+    }
+    return tot;
+}
+
+__device__ __forceinline__ bool bit_test(const uint32_t *bm, uint32_t v)
+{
+    return (bm[v >> 5] >> (v & 31)) & 1u;
+}
+
+// ---------------------------------------------------------------------------
+// Merge-path pull over an in-edge (CSC-like) array: every destination row is
+// owned by exactly one work-group, so per-row results have a fixed
+// combination order (no floating-point atomics).  Tile t covers kPullIPB
+// consecutive items of the merge of row ends and edges.  Gathered values
+// are staged in LDS, each thread walks kPullIPT items, and a block-wide
+// segmented scan joins rows that cross thread boundaries.  Rows crossing
+// tile boundaries are finished by k_pull_fixup from per-tile carries.
+//
+// Op: acc_t, identity(), combine(earlier, later), gather(edge, src),
+//     finish(row, acc, LocalStats&).
+// ---------------------------------------------------------------------------
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_pull(Op op, const uint32_t *__restrict__ toff,
+                                                 const uint32_t *__restrict__ tids,
+                                                 const uint32_t *__restrict__ srow,
+                                                 const uint32_t *__restrict__ sedge,
+                                                 uint32_t *__restrict__ carry_row,
+                                                 typename Op::acc_t *__restrict__ carry_val,
+                                                 typename Op::acc_t *__restrict__ head_val,
+                                                 unsigned long long *__restrict__ counters,
+                                                 double *__restrict__ l1_part)
+{
+    using A = typename Op::acc_t;
+    __shared__ A s_val[kPullIPB];
+    __shared__ uint32_t s_end[kPullIPB];
+    __shared__ uint32_t s_key[kBlock];
+    __shared__ A s_sv[kBlock];
+    if (op.skip()) return;
+    const uint32_t tid = threadIdx.x, t = blockIdx.x;
+    const uint32_t r0 = srow[t], r1 = srow[t + 1];
+    const uint32_t e0 = sedge[t], e1 = sedge[t + 1];
+    const uint32_t nr = r1 - r0, ne = e1 - e0;
+    for (uint32_t i = tid; i < nr; i += kBlock) s_end[i] = toff[r0 + i + 1] - e0;
+    for (uint32_t j = tid; j < ne; j += kBlock) {
+        const uint32_t e = e0 + j;
+        s_val[j] = op.gather(e, tids[e]);
+    }
+    const bool head_open = toff[r0] < e0; // first row has edges in earlier tiles
+    __syncthreads();
+    const uint32_t tot = nr + ne;
+    const uint32_t diag = min(tid * (uint32_t)kPullIPT, tot);
+    const uint32_t dend = min(diag + (uint32_t)kPullIPT, tot);
+    uint32_t lo = diag > ne ? diag - ne : 0u, hi = min(diag, nr);
+    while (lo < hi) {
+        const uint32_t p = (lo + hi) >> 1;
+        if (s_end[p] <= diag - p - 1) lo = p + 1;
+        else hi = p;
+    }
+    const uint32_t ri0 = lo, ej0 = diag - lo;
+    // pass A: carry-out of this thread
+    uint32_t ri = ri0, ej = ej0;
+    A acc = op.identity();
+    for (uint32_t d = diag; d < dend; d++) {
+        if (ri < nr && ej >= s_end[ri]) {
+            acc = op.identity();
+            ri++;
+        } else {
+            acc = op.combine(acc, s_val[ej]);
+            ej++;
+        }
+    }
+    s_key[tid] = ri;
+    s_sv[tid] = acc;
+    __syncthreads();
+    // inclusive segmented scan over threads (keys are non-decreasing)
+    for (uint32_t off = 1; off < kBlock; off <<= 1) {
+        const bool take = tid >= off;
+        uint32_t k2 = 0;
+        A v2 = op.identity();
+        if (take) {
+            k2 = s_key[tid - off];
+            v2 = s_sv[tid - off];
+        }
+        __syncthreads();
+        if (take && k2 == s_key[tid]) s_sv[tid] = op.combine(v2, s_sv[tid]);
+        __syncthreads();
+    }
+    A run = op.identity();
+    if (tid > 0 && s_key[tid - 1] == ri0) run = s_sv[tid - 1];
+    // pass B: finish rows that end in this thread's items
+    LocalStats st;
+    ri = ri0;
+    ej = ej0;
+    for (uint32_t d = diag; d < dend; d++) {
+        if (ri < nr && ej >= s_end[ri]) {
+            if (ri == 0 && head_open) head_val[t] = run;
+            else op.finish(r0 + ri, run, st);
+            run = op.identity();
+            ri++;
+        } else {
+            run = op.combine(run, s_val[ej]);
+            ej++;
+        }
+    }
+    if (diag < dend && dend == tot) {
+        carry_row[t] = r0 + ri;
+        carry_val[t] = run;
+    }
+    double l1 = block_stats(st, counters);
+    if (l1_part && tid == 0) l1_part[t] = l1;
+}
+
+// Finishes rows that began in an earlier tile: row r0 of tile t when its
+// end lies in t.  Partials are combined in tile order.
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_pull_fixup(Op op, const uint32_t *__restrict__ toff,
+                                                       const uint32_t *__restrict__ srow,
+                                                       const uint32_t *__restrict__ sedge,
+                                                       const uint32_t *__restrict__ carry_row,
+                                                       const typename Op::acc_t *__restrict__ carry_val,
+                                                       const typename Op::acc_t *__restrict__ head_val,
+                                                       uint32_t tiles,
+                                                       unsigned long long *__restrict__ counters,
+                                                       double *__restrict__ l1_fix)
+{
+    using A = typename Op::acc_t;
+    if (op.skip()) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    LocalStats st;
+    if (t < tiles) {
+        const uint32_t r0 = srow[t], r1 = srow[t + 1], e0 = sedge[t];
+        if (r1 > r0 && toff[r0] < e0) {
+            uint32_t tb = t;
+            while (tb > 0 && carry_row[tb - 1] == r0) tb--;
+            A acc = op.identity();
+            for (uint32_t k = tb; k < t; k++) acc = op.combine(acc, carry_val[k]);
+            acc = op.combine(acc, head_val[t]);
+            op.finish(r0, acc, st);
+        }
+    }
+    double l1 = block_stats(st, counters);
+    if (l1_fix && threadIdx.x == 0) l1_fix[blockIdx.x] = l1;
+}
+
+template <class Op>
+int launch_pull(Ctx *c, Graph *g, const Op &op, unsigned long long *counters, double *l1_part,
+                double *l1_fix, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr)
+{
+    using A = typename Op::acc_t;
+    static_assert(sizeof(A) <= 8, "carry buffers hold 8-byte partials");
+    PullSched &s = g->pull;
+    if (ev_a) GG_HIP(hipEventRecord(ev_a, c->stream));
+    k_pull<Op><<<s.tiles, kBlock, 0, c->stream>>>(
+        op, (const uint32_t *)g->toff.p, (const uint32_t *)g->tids.p, (const uint32_t *)s.start_row.p,
+        (const uint32_t *)s.start_edge.p, (uint32_t *)s.carry_row.p, (A *)s.carry_val.p,
+        (A *)s.head_val.p, counters, l1_part);
+    if (ev_b) GG_HIP(hipEventRecord(ev_b, c->stream));
+    k_pull_fixup<Op><<<blocks_for(s.tiles, kBlock), kBlock, 0, c->stream>>>(
+        op, (const uint32_t *)g->toff.p, (const uint32_t *)s.start_row.p,
+        (const uint32_t *)s.start_edge.p, (const uint32_t *)s.carry_row.p, (const A *)s.carry_val.p,
+        (const A *)s.head_val.p, s.tiles, counters, l1_fix);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+// BFS level bookkeeping carried through edgemap_core by the fused loops.
+struct EdgeMapExtra {
+    int32_t *levels = nullptr;
+    int32_t level = 0;
+    unsigned long long scanned = 0;
+};
+
+int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, int remdups,
+                 Frontier **pout, int *path, EdgeMapExtra *ex);
+int vertexmap_core(Graph *g, Frontier *f, const gg_vop_t *op);
+int reduce_sum_core(Ctx *c, const void *dev, int dtype, uint64_t count, double *dev_out);
+
+inline void frontier_delete(Frontier *f)
+{
+    if (!f) return;
+    buf_free(f->bytes);
+    buf_free(f->bitmap);
+    buf_free(f->list);
+    delete f;
+}
+
+} // namespace gg

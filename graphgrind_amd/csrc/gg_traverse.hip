@@ -1,0 +1,1169 @@
+// gg_traverse.hip -- frontier objects, edgeMap and vertexMap of
+// libgraphgrind_amd.
+//
+// edgeMap (ligra-numa.h:919-1019) keeps the reference's three-way decision
+// on (d_m + out_edges) against m/20 and m/2 and runs one of:
+//   sparse push   load-balanced over the frontier's out-edges, CAS /
+//                 atomicMin / atomicAdd as the app's updateAtomic, output
+//                 compacted with wave ballots (edgeMapSparseWithG 868-916)
+//   dense CSC     merge-path pull over the in-edge twin, destination-owned
+//                 (edgeMapDenseCSC 549-729); BFS uses a bottom-up kernel
+//                 with the early exit of 711-712
+//   dense COO     Hilbert-ordered COO partitions streamed edge-parallel for
+//                 the order-free min operators (BFS, CC); the sum operators
+//                 (PageRank, SpMV) use the destination-owned pull so their
+//                 floating-point sums keep a fixed order (edgeMapDense
+//                 803-866 under PART96)
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
+
+#include <cstring>
+
+#include "gg_kernels.h"
+
+namespace gg {
+
+// ============================================================================
+// Operators of the generic edgeMap
+// ============================================================================
+
+// PR_F (app/PageRank.C:28-70): contributions p_curr[s]/outdeg(s) come from
+// a prepass; an inactive source holds -0.0 (ranks are >= +0), the sum
+// starts at -0.0, so the sign of a row's sum says whether any source of
+// the frontier reached it.
+template <class V>
+struct OpPRSum {
+    using acc_t = V;
+    const V *contrib;
+    V *p_next;
+    uint8_t *next;
+    const uint32_t *outdeg;
+    __device__ bool skip() const { return false; }
+    __device__ V identity() const { return -V(0); }
+    __device__ V combine(V a, V b) const { return a + b; }
+    __device__ V gather(uint32_t, uint32_t s) const { return contrib[s]; }
+    __device__ void finish(uint32_t d, V acc, LocalStats &st) const
+    {
+        const bool any = !signbit(acc);
+        if (any) {
+            p_next[d] = p_next[d] + acc;
+            st.active++;
+            st.out_edges += outdeg[d];
+        }
+        next[d] = any;
+    }
+};
+
+// SPMV_F (app/SPMV.C:7-48) with an all-active frontier (SPMV.C:81).
+template <class V>
+struct OpSpmvBit {
+    using acc_t = V;
+    const V *x;
+    const float *tw; // in-edge order, NULL = unit weights
+    V *p_next;
+    uint8_t *next;
+    const uint32_t *outdeg, *indeg;
+    __device__ bool skip() const { return false; }
+    __device__ V identity() const { return V(0); }
+    __device__ V combine(V a, V b) const { return a + b; }
+    __device__ V gather(uint32_t e, uint32_t s) const { return tw ? x[s] * (V)tw[e] : x[s]; }
+    __device__ void finish(uint32_t d, V acc, LocalStats &st) const
+    {
+        const bool any = indeg[d] != 0;
+        if (any) {
+            p_next[d] = p_next[d] + acc;
+            st.active++;
+            st.out_edges += outdeg[d];
+        }
+        next[d] = any;
+    }
+};
+
+// SPMV_F with a frontier: the source membership travels with the value.
+struct SpmvAcc32 {
+    float v;
+    uint32_t c;
+};
+template <class V>
+struct OpSpmvFrontier {
+    using acc_t = SpmvAcc32; // 8-byte partials; values are float here
+    const V *x;
+    const float *tw;
+    const uint32_t *bm;
+    V *p_next;
+    uint8_t *next;
+    const uint32_t *outdeg;
+    __device__ bool skip() const { return false; }
+    __device__ acc_t identity() const { return {0.0f, 0u}; }
+    __device__ acc_t combine(acc_t a, acc_t b) const { return {a.v + b.v, a.c + b.c}; }
+    __device__ acc_t gather(uint32_t e, uint32_t s) const
+    {
+        if (!bit_test(bm, s)) return {0.0f, 0u};
+        return {(float)(tw ? x[s] * (V)tw[e] : x[s]), 1u};
+    }
+    __device__ void finish(uint32_t d, acc_t acc, LocalStats &st) const
+    {
+        const bool any = acc.c != 0;
+        if (any) {
+            p_next[d] = p_next[d] + (V)acc.v;
+            st.active++;
+            st.out_edges += outdeg[d];
+        }
+        next[d] = any;
+    }
+};
+
+// CC_F cache path (app/Components.C:57-79): Jacobi over prevIDs of the
+// frontier's sources; g[s] = prevIDs[s] for active s, else UINT_MAX.
+struct OpCCMin {
+    using acc_t = uint32_t;
+    const uint32_t *g;
+    uint32_t *ids;
+    const uint32_t *prev;
+    uint8_t *next;
+    const uint32_t *outdeg;
+    __device__ bool skip() const { return false; }
+    __device__ uint32_t identity() const { return 0xffffffffu; }
+    __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a < b ? a : b; }
+    __device__ uint32_t gather(uint32_t, uint32_t s) const { return g[s]; }
+    __device__ void finish(uint32_t d, uint32_t acc, LocalStats &st) const
+    {
+        const uint32_t cur = ids[d];
+        const bool dec = acc < cur;
+        const bool ch = dec && cur == prev[d];
+        if (dec) ids[d] = acc;
+        if (ch) {
+            st.active++;
+            st.out_edges += outdeg[d];
+        }
+        next[d] = ch;
+    }
+};
+
+// ============================================================================
+// Prepass / vertex kernels
+// ============================================================================
+
+template <class V>
+__global__ void k_pr_contrib(const V *__restrict__ p_curr, const uint32_t *__restrict__ outdeg,
+                             const uint8_t *__restrict__ F, int bit, V *__restrict__ contrib, uint32_t n)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint32_t od = outdeg[v];
+    contrib[v] = ((bit || F[v]) && od) ? p_curr[v] / (V)od : -V(0);
+}
+
+__global__ void k_cc_gate(const uint32_t *__restrict__ prev, const uint8_t *__restrict__ F, int bit,
+                          uint32_t *__restrict__ g, uint32_t n)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) g[v] = (bit || F[v]) ? prev[v] : 0xffffffffu;
+}
+
+// BFS bottom-up (edgeMapDenseCSC, bit=false, no cache: ligra-numa.h:706-713):
+// an unvisited vertex scans its in-list (ascending source) and takes the
+// first frontier source as parent, then stops because cond() turns false.
+__global__ __launch_bounds__(kBlock) void k_bfs_pull(const uint32_t *__restrict__ toff,
+                                                     const uint32_t *__restrict__ tids,
+                                                     const uint32_t *__restrict__ bm, int bit,
+                                                     int32_t *__restrict__ parents,
+                                                     int32_t *__restrict__ levels, int32_t level,
+                                                     uint8_t *__restrict__ next,
+                                                     const uint32_t *__restrict__ outdeg, uint32_t n,
+                                                     unsigned long long *__restrict__ counters)
+{
+    LocalStats st;
+    unsigned long long scanned = 0;
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < n) {
+        uint8_t nx = 0;
+        if (parents[d] == -1) {
+            const uint32_t b = toff[d], e = toff[d + 1];
+            for (uint32_t j = b; j < e; j++) {
+                const uint32_t s = tids[j];
+                scanned++;
+                if (bit || bit_test(bm, s)) {
+                    parents[d] = (int32_t)s;
+                    if (levels) levels[d] = level;
+                    nx = 1;
+                    st.active++;
+                    st.out_edges += outdeg[d];
+                    break;
+                }
+            }
+        }
+        next[d] = nx;
+    }
+    unsigned long long sc = wave_sum_u64(scanned);
+    if ((threadIdx.x & 63) == 0 && sc) atomicAdd(&counters[CNT_SCANNED], sc);
+    block_stats(st, counters);
+}
+
+// Dense COO for BFS_F (edgeMapDense with PART96, ligra-numa.h:838-863):
+// one thread per COO edge; CAS elects one parent per destination.
+__global__ void k_bfs_coo(const uint32_t *__restrict__ csrc, const uint32_t *__restrict__ cdst,
+                          uint64_t m, const uint32_t *__restrict__ bm, int bit,
+                          int32_t *__restrict__ parents, int32_t *__restrict__ levels, int32_t level,
+                          uint8_t *__restrict__ next)
+{
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; k < m; k += stride) {
+        const uint32_t s = csrc[k], d = cdst[k];
+        if (parents[d] != -1) continue; // cond(d)
+        if (!bit && !bit_test(bm, s)) continue;
+        if (atomicCAS(&parents[d], -1, (int32_t)s) == -1) {
+            next[d] = 1;
+            if (levels) levels[d] = level;
+        }
+    }
+}
+
+// Dense COO for CC_F: atomicMin of the source's prevID (Jacobi; the
+// reference reads IDs[s] here, which only changes the round count).
+__global__ void k_cc_coo(const uint32_t *__restrict__ csrc, const uint32_t *__restrict__ cdst,
+                         uint64_t m, const uint8_t *__restrict__ F, int bit,
+                         uint32_t *__restrict__ ids, const uint32_t *__restrict__ prev,
+                         uint8_t *__restrict__ next)
+{
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; k < m; k += stride) {
+        const uint32_t s = csrc[k], d = cdst[k];
+        if (!bit && !F[s]) continue;
+        const uint32_t v = prev[s];
+        if (v >= ids[d]) continue;
+        const uint32_t old = atomicMin(&ids[d], v);
+        if (v < old && old == prev[d]) next[d] = 1;
+    }
+}
+
+// Count and out-degree sum of a dense flag array (GoutDegree + reduce,
+// ligra-numa.h:985-988).
+__global__ __launch_bounds__(kBlock) void k_dense_stats(const uint8_t *__restrict__ F,
+                                                        const uint32_t *__restrict__ outdeg,
+                                                        uint32_t n, unsigned long long *counters)
+{
+    LocalStats st;
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (; v < n; v += stride)
+        if (F[v]) {
+            st.active++;
+            st.out_edges += outdeg[v];
+        }
+    block_stats(st, counters);
+}
+
+// ---- sparse push (edgeMapSparseWithG, ligra-numa.h:868-916) -----------------
+
+enum { PUSH_PR = 0, PUSH_BFS = 1, PUSH_CC = 2, PUSH_SPMV = 3 };
+
+struct PushArgs {
+    const uint32_t *list;   // frontier ids
+    const uint32_t *offs;   // exclusive scan of their out-degrees (k+1)
+    uint32_t k;
+    uint64_t total;
+    const uint32_t *off, *ids, *outdeg;
+    const float *w;         // CSR weights or NULL
+    int kind, f32;
+    void *a, *b;            // op arrays
+    int32_t *levels;
+    int32_t level;
+    uint32_t *out;          // output list (capacity total)
+    unsigned long long *counters;
+};
+
+constexpr int kPushTile = kBlock * 4;
+constexpr int kPushLds = 2048;
+
+__device__ __forceinline__ uint32_t upper_idx(const uint32_t *a, uint32_t lo, uint32_t hi, uint64_t e)
+{
+    // last i in [lo, hi) with a[i] <= e
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)a[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_push(PushArgs A)
+{
+    __shared__ uint32_t s_off[kPushLds];
+    __shared__ uint32_t s_lo, s_hi;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kPushTile;
+    const uint64_t t1 = min(t0 + (uint64_t)kPushTile, A.total);
+    if (threadIdx.x == 0) {
+        s_lo = upper_idx(A.offs, 0, A.k, t0);
+        s_hi = upper_idx(A.offs, 0, A.k, t1 - 1) + 1;
+    }
+    __syncthreads();
+    const uint32_t lo = s_lo, hi = s_hi;
+    const bool in_lds = hi - lo <= (uint32_t)kPushLds;
+    if (in_lds)
+        for (uint32_t i = threadIdx.x; i < hi - lo; i += kBlock) s_off[i] = A.offs[lo + i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    for (int it = 0; it < kPushTile / kBlock; it++) {
+        const uint64_t e = t0 + (uint64_t)it * kBlock + threadIdx.x;
+        bool emit = false;
+        uint32_t ngh = 0;
+        if (e < t1) {
+            uint32_t kk;
+            if (in_lds) kk = lo + upper_idx(s_off, 0, hi - lo, e);
+            else kk = upper_idx(A.offs, lo, hi, e);
+            const uint32_t v = A.list[kk];
+            const uint32_t pos = A.off[v] + (uint32_t)(e - A.offs[kk]);
+            ngh = A.ids[pos];
+            switch (A.kind) {
+            case PUSH_BFS: {
+                int32_t *par = (int32_t *)A.a;
+                if (par[ngh] == -1 && atomicCAS(&par[ngh], -1, (int32_t)v) == -1) {
+                    emit = true;
+                    if (A.levels) A.levels[ngh] = A.level;
+                }
+                break;
+            }
+            case PUSH_CC: {
+                uint32_t *ids = (uint32_t *)A.a;
+                const uint32_t *prev = (const uint32_t *)A.b;
+                const uint32_t val = prev[v];
+                if (val < ids[ngh]) {
+                    const uint32_t old = atomicMin(&ids[ngh], val);
+                    emit = val < old && old == prev[ngh];
+                }
+                break;
+            }
+            case PUSH_PR:
+                if (A.f32) {
+                    const float *pc = (const float *)A.a;
+                    atomicAdd(&((float *)A.b)[ngh], pc[v] / (float)A.outdeg[v]);
+                } else {
+                    const double *pc = (const double *)A.a;
+                    atomicAdd(&((double *)A.b)[ngh], pc[v] / (double)A.outdeg[v]);
+                }
+                emit = true;
+                break;
+            default: { // SPMV
+                const float wv = A.w ? A.w[pos] : 1.0f;
+                if (A.f32) atomicAdd(&((float *)A.b)[ngh], ((const float *)A.a)[v] * wv);
+                else atomicAdd(&((double *)A.b)[ngh], ((const double *)A.a)[v] * (double)wv);
+                emit = true;
+                break;
+            }
+            }
+        }
+        // wave-ballot compaction of the emitted destinations
+        const uint64_t mask = __ballot(emit);
+        const uint32_t cnt = (uint32_t)__popcll(mask);
+        unsigned long long od = emit ? A.outdeg[ngh] : 0;
+        od = wave_sum_u64(od);
+        unsigned long long base = 0;
+        if (lane == 0 && cnt) {
+            base = atomicAdd(&A.counters[CNT_CURSOR], (unsigned long long)cnt);
+            atomicAdd(&A.counters[CNT_ACTIVE], (unsigned long long)cnt);
+            atomicAdd(&A.counters[CNT_OUT], od);
+        }
+        base = __shfl(base, 0, 64);
+        if (emit) {
+            const uint64_t below = mask & ((1ULL << lane) - 1ULL);
+            A.out[base + __popcll(below)] = ngh;
+        }
+    }
+}
+
+// remDuplicates (ligra-numa.h:317-348): the lowest slot of each id wins.
+__global__ void k_dedup_mark(const uint32_t *__restrict__ list, uint64_t k, uint32_t *__restrict__ flags)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) atomicMin(&flags[list[i]], (uint32_t)i);
+}
+
+__global__ void k_dedup_keep(const uint32_t *__restrict__ list, uint64_t k, uint32_t *__restrict__ flags,
+                             const uint32_t *__restrict__ outdeg, uint32_t *__restrict__ out,
+                             unsigned long long *__restrict__ counters)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool keep = false;
+    uint32_t v = 0;
+    if (i < k) {
+        v = list[i];
+        keep = flags[v] == (uint32_t)i;
+    }
+    __syncthreads();
+    if (keep) flags[v] = 0xffffffffu;
+    const int lane = threadIdx.x & 63;
+    const uint64_t mask = __ballot(keep);
+    const uint32_t cnt = (uint32_t)__popcll(mask);
+    unsigned long long od = wave_sum_u64(keep ? outdeg[v] : 0ull);
+    unsigned long long base = 0;
+    if (lane == 0 && cnt) {
+        base = atomicAdd(&counters[CNT_CURSOR], (unsigned long long)cnt);
+        atomicAdd(&counters[CNT_ACTIVE], (unsigned long long)cnt);
+        atomicAdd(&counters[CNT_OUT], od);
+    }
+    base = __shfl(base, 0, 64);
+    if (keep) out[base + __popcll(mask & ((1ULL << lane) - 1ULL))] = v;
+}
+
+// ---- frontier conversions --------------------------------------------------
+
+__global__ void k_scatter_flags(const uint32_t *__restrict__ list, uint64_t k, uint8_t *__restrict__ F)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) F[list[i]] = 1;
+}
+
+// Ordered compaction (packIndex, utils.h:404): block b handles 4096 flags.
+constexpr int kPackPer = kBlock * 16;
+__global__ __launch_bounds__(kBlock) void k_pack_count(const uint8_t *__restrict__ F, uint32_t n,
+                                                       uint32_t *__restrict__ bcount)
+{
+    const uint32_t b0 = blockIdx.x * kPackPer;
+    uint32_t c = 0;
+    for (int i = 0; i < 16; i++) {
+        const uint32_t v = b0 + i * kBlock + threadIdx.x;
+        if (v < n && F[v]) c++;
+    }
+    __shared__ uint32_t s[kBlock];
+    s[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bcount[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_write(const uint8_t *__restrict__ F, uint32_t n,
+                                                       const uint32_t *__restrict__ bbase,
+                                                       uint32_t *__restrict__ out)
+{
+    // thread t owns flags [b0 + 16t, b0 + 16t + 16) so the output keeps id order
+    const uint32_t b0 = blockIdx.x * kPackPer + threadIdx.x * 16;
+    uint32_t c = 0;
+    for (int i = 0; i < 16; i++) {
+        const uint32_t v = b0 + i;
+        if (v < n && F[v]) c++;
+    }
+    __shared__ uint32_t s[kBlock];
+    s[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        uint32_t add = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t pos = bbase[blockIdx.x] + s[threadIdx.x] - c;
+    for (int i = 0; i < 16; i++) {
+        const uint32_t v = b0 + i;
+        if (v < n && F[v]) out[pos++] = v;
+    }
+}
+
+// ---- vertexMap kernels ---------------------------------------------------------
+
+template <class V>
+__device__ __forceinline__ void vop_apply(int kind, uint32_t i, void *a, void *b, double c0, double c1)
+{
+    V *pa = (V *)a;
+    V *pb = (V *)b;
+    switch (kind) {
+    case GG_VOP_PAGERANK: pb[i] = (V)c0 * pb[i] + (V)c1; break;
+    case GG_VOP_RESET: pa[i] = V(0); break;
+    case GG_VOP_ABSDIFF: pa[i] = fabs(pa[i] - pb[i]); break;
+    default: break;
+    }
+}
+
+template <class V>
+__global__ void k_vmap(int kind, int mode, const uint8_t *__restrict__ F,
+                       const uint32_t *__restrict__ list, uint64_t count, void *a, void *b, double c0,
+                       double c1)
+{
+    // mode 0: all vertices, 1: dense flags, 2: sparse list
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) {
+        uint32_t v;
+        if (mode == 2) v = list[i];
+        else {
+            v = (uint32_t)i;
+            if (mode == 1 && !F[v]) continue;
+        }
+        if (kind == GG_VOP_CC_COPY) ((uint32_t *)b)[v] = ((const uint32_t *)a)[v];
+        else vop_apply<V>(kind, v, a, b, c0, c1);
+    }
+}
+
+// vertexFilter (ligra-numa.h:1053-1082): out[i] = active(i) && keep[i]
+__global__ void k_vfilter(int mode, const uint8_t *__restrict__ F, const uint8_t *__restrict__ keep,
+                          uint8_t *__restrict__ out, uint32_t n)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) out[v] = (mode == 0 || F[v]) && keep[v];
+}
+
+// Fixed-order sum: block b sums a fixed strided slice, partials reduced by
+// one block in index order.
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_sum_part(const V *__restrict__ a, uint64_t n, double *part)
+{
+    double s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kBlock)
+        s += (double)a[i];
+    s = wave_sum_f64(s);
+    __shared__ double w[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < kBlock / 64; k++) t += w[k];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_final(const double *__restrict__ part, uint32_t np,
+                                                      double *out)
+{
+    double s = 0;
+    for (uint32_t i = threadIdx.x; i < np; i += kBlock) s += part[i];
+    s = wave_sum_f64(s);
+    __shared__ double w[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < kBlock / 64; k++) t += w[k];
+        *out = t;
+    }
+}
+
+// ============================================================================
+// Frontier helpers
+// ============================================================================
+
+int frontier_alloc(Graph *g, Frontier *f)
+{
+    f->g = g;
+    f->n = g->n;
+    return GG_OK;
+}
+
+static int ensure_bytes(Frontier *f)
+{
+    return buf_ensure(f->g->ctx, f->bytes, ((uint64_t)f->n + 64) & ~63ull);
+}
+
+static int ensure_bitmap(Frontier *f)
+{
+    return buf_ensure(f->g->ctx, f->bitmap, 4ull * ((f->n + 31) / 32 + 1));
+}
+
+// Dense flags (+ bitmap) from whatever the frontier holds.
+int frontier_make_dense(Frontier *f)
+{
+    Ctx *c = f->g->ctx;
+    if (f->bit) {
+        GG_TRY(ensure_bytes(f));
+        GG_TRY(fill_bytes(c, f->bytes.p, f->n, 1));
+        GG_TRY(ensure_bitmap(f));
+        GG_TRY(bitmap_from_bytes(c, (uint8_t *)f->bytes.p, (uint32_t *)f->bitmap.p, f->n));
+        f->dense_valid = true;
+        return GG_OK;
+    }
+    if (f->dense_valid) return GG_OK;
+    if (!f->sparse_valid) return fail(GG_ERR_STATE, "frontier has no representation");
+    GG_TRY(ensure_bytes(f));
+    GG_TRY(fill_bytes(c, f->bytes.p, f->n, 0));
+    if (f->d_m)
+        k_scatter_flags<<<blocks_for(f->d_m, kBlock), kBlock, 0, c->stream>>>(
+            (const uint32_t *)f->list.p, f->d_m, (uint8_t *)f->bytes.p);
+    GG_TRY(ensure_bitmap(f));
+    GG_TRY(bitmap_from_bytes(c, (uint8_t *)f->bytes.p, (uint32_t *)f->bitmap.p, f->n));
+    GG_HIP(hipGetLastError());
+    f->dense_valid = true;
+    return GG_OK;
+}
+
+int frontier_make_sparse(Frontier *f)
+{
+    Ctx *c = f->g->ctx;
+    if (f->sparse_valid) return GG_OK;
+    if (f->bit) {
+        // bits() frontier: its flags are not set in the reference (toSparse
+        // on it would read uninitialised memory); list every vertex
+        GG_TRY(buf_ensure(c, f->list, 4ull * f->n + 4));
+        GG_TRY(gg_iota(abi(c), f->list.p, GG_U32, f->n));
+        f->sparse_valid = true;
+        return GG_OK;
+    }
+    if (!f->dense_valid) return fail(GG_ERR_STATE, "frontier has no representation");
+    uint32_t nb = blocks_for(f->n, kPackPer);
+    Buf cnt, base;
+    GG_TRY(buf_alloc(c, cnt, 4ull * (nb + 1)));
+    GG_TRY(buf_alloc(c, base, 4ull * (nb + 1)));
+    k_pack_count<<<nb, kBlock, 0, c->stream>>>((const uint8_t *)f->bytes.p, f->n, (uint32_t *)cnt.p);
+    size_t tb = 0;
+    GG_HIP(rocprim::exclusive_scan(nullptr, tb, (uint32_t *)cnt.p, (uint32_t *)base.p, 0u, (size_t)nb,
+                                   rocprim::plus<uint32_t>(), c->stream));
+    GG_TRY(buf_ensure(c, c->tmp, tb + 16));
+    GG_HIP(rocprim::exclusive_scan(c->tmp.p, tb, (uint32_t *)cnt.p, (uint32_t *)base.p, 0u, (size_t)nb,
+                                   rocprim::plus<uint32_t>(), c->stream));
+    // capacity n whatever d_m says, so a stale count cannot overrun the list
+    GG_TRY(buf_ensure(c, f->list, 4ull * f->n + 4));
+    k_pack_write<<<nb, kBlock, 0, c->stream>>>((const uint8_t *)f->bytes.p, f->n,
+                                                (const uint32_t *)base.p, (uint32_t *)f->list.p);
+    GG_HIP(hipGetLastError());
+    uint32_t last[2] = {0, 0};
+    GG_HIP(hipMemcpyAsync(&last[0], (uint32_t *)base.p + (nb - 1), 4, hipMemcpyDeviceToHost, c->stream));
+    GG_HIP(hipMemcpyAsync(&last[1], (uint32_t *)cnt.p + (nb - 1), 4, hipMemcpyDeviceToHost, c->stream));
+    GG_TRY(sync(c));
+    buf_free(cnt);
+    buf_free(base);
+    if ((uint64_t)last[0] + last[1] != f->d_m) // toSparse's "bad stored value of m" (ligra-numa.h:251-255)
+        return fail(GG_ERR_STATE, "toSparse: stored count " + std::to_string(f->d_m) + " but " +
+                                      std::to_string((uint64_t)last[0] + last[1]) + " flags set");
+    f->sparse_valid = true;
+    return GG_OK;
+}
+
+static int counters_zero(Ctx *c)
+{
+    return fill_bytes(c, c->counters.p, 64, 0);
+}
+
+static int counters_read(Ctx *c, unsigned long long out[4])
+{
+    GG_HIP(hipMemcpyAsync(out, c->counters.p, 32, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+int frontier_stats_dense(Frontier *f)
+{
+    Ctx *c = f->g->ctx;
+    GG_TRY(counters_zero(c));
+    uint32_t grid = std::min<uint32_t>(blocks_for(f->n, kBlock), 4096);
+    k_dense_stats<<<grid, kBlock, 0, c->stream>>>((const uint8_t *)f->bytes.p,
+                                                   (const uint32_t *)f->g->outdeg.p, f->n,
+                                                   (unsigned long long *)c->counters.p);
+    GG_HIP(hipGetLastError());
+    unsigned long long cn[4];
+    GG_TRY(counters_read(c, cn));
+    f->d_m = cn[CNT_ACTIVE];
+    f->out_edges = cn[CNT_OUT];
+    return GG_OK;
+}
+
+static void frontier_free(Frontier *f)
+{
+    buf_free(f->bytes);
+    buf_free(f->bitmap);
+    buf_free(f->list);
+}
+
+// ============================================================================
+// edgeMap core (shared with the fused app loops)
+// ============================================================================
+
+static bool op_is_sum(int kind) { return kind == GG_OP_PAGERANK || kind == GG_OP_SPMV; }
+
+int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, int remdups,
+                 Frontier **pout, int *path, EdgeMapExtra *ex)
+{
+    Ctx *c = g->ctx;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    if (threshold == -1) threshold = (int64_t)(m / 20);
+    const int64_t dense_thr = (int64_t)(m / 2);
+    if (in->n != n)
+        return fail(GG_ERR_SIZE, "edgeMap: Sizes Don't match (frontier rows " + std::to_string(in->n) +
+                                     ", graph n " + std::to_string(n) + ")");
+    const bool f32 = op->dtype == GG_F32;
+    Frontier *out = new Frontier();
+    frontier_alloc(g, out);
+    auto bail = [&](int rc) {
+        frontier_free(out);
+        delete out;
+        return rc;
+    };
+    if (in->out_edges == 0) { // partitioned_vertices::empty() (ligra-numa.h:947)
+        out->n = 0;
+        out->d_m = 0;
+        *pout = out;
+        if (path) *path = GG_PATH_EMPTY;
+        return GG_OK;
+    }
+    int rc;
+    if ((int64_t)(in->d_m + in->out_edges) > threshold) {
+        // ---------------- dense ----------------
+        if (!in->bit && (rc = frontier_make_dense(in))) return bail(rc);
+        if ((rc = ensure_bytes(out))) return bail(rc);
+        const bool coo = (int64_t)(in->d_m + in->out_edges) > dense_thr && g->P && !g->part_vertex;
+        if (path) *path = coo ? GG_PATH_COO : GG_PATH_CSC;
+        if ((rc = counters_zero(c))) return bail(rc);
+        unsigned long long *cnt = (unsigned long long *)c->counters.p;
+        const int bit = in->bit ? 1 : 0;
+        const uint8_t *F = (const uint8_t *)in->bytes.p;
+        const uint32_t *bm = (const uint32_t *)in->bitmap.p;
+        uint8_t *next = (uint8_t *)out->bytes.p;
+        const uint32_t *outdeg = (const uint32_t *)g->outdeg.p;
+        Buf tmpv;
+        switch (op->kind) {
+        case GG_OP_PAGERANK: {
+            if ((rc = buf_alloc(c, tmpv, (f32 ? 4ull : 8ull) * n))) return bail(rc);
+            if (f32) {
+                k_pr_contrib<float><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
+                    (const float *)op->a, outdeg, F, bit, (float *)tmpv.p, n);
+                rc = launch_pull(c, g, OpPRSum<float>{(const float *)tmpv.p, (float *)op->b, next, outdeg},
+                                 cnt, nullptr, nullptr);
+            } else {
+                k_pr_contrib<double><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
+                    (const double *)op->a, outdeg, F, bit, (double *)tmpv.p, n);
+                rc = launch_pull(c, g, OpPRSum<double>{(const double *)tmpv.p, (double *)op->b, next, outdeg},
+                                 cnt, nullptr, nullptr);
+            }
+            break;
+        }
+        case GG_OP_SPMV: {
+            const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
+            if (bit) {
+                if (f32)
+                    rc = launch_pull(c, g, OpSpmvBit<float>{(const float *)op->a, tw, (float *)op->b, next, outdeg,
+                                                            (const uint32_t *)g->indeg.p},
+                                     cnt, nullptr, nullptr);
+                else
+                    rc = launch_pull(c, g, OpSpmvBit<double>{(const double *)op->a, tw, (double *)op->b, next,
+                                                             outdeg, (const uint32_t *)g->indeg.p},
+                                     cnt, nullptr, nullptr);
+            } else {
+                if (!f32)
+                    return bail(fail(GG_ERR_UNSUPPORTED,
+                                     "edgeMap SPMV with a partial frontier supports F32 values only"));
+                rc = launch_pull(c, g, OpSpmvFrontier<float>{(const float *)op->a, tw, bm, (float *)op->b, next,
+                                                             outdeg},
+                                 cnt, nullptr, nullptr);
+            }
+            break;
+        }
+        case GG_OP_CC: {
+            if (coo) {
+                if ((rc = fill_bytes(c, next, n, 0))) return bail(rc);
+                uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+                k_cc_coo<<<grid, kBlock, 0, c->stream>>>((const uint32_t *)g->coo_src.p,
+                                                          (const uint32_t *)g->coo_dst.p, m, F, bit,
+                                                          (uint32_t *)op->a, (const uint32_t *)op->b, next);
+                rc = GG_OK;
+            } else {
+                if ((rc = buf_alloc(c, tmpv, 4ull * n))) return bail(rc);
+                k_cc_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)op->b, F, bit,
+                                                                            (uint32_t *)tmpv.p, n);
+                rc = launch_pull(c, g, OpCCMin{(const uint32_t *)tmpv.p, (uint32_t *)op->a,
+                                               (const uint32_t *)op->b, next, outdeg},
+                                 cnt, nullptr, nullptr);
+            }
+            break;
+        }
+        case GG_OP_BFS: {
+            if (coo) {
+                if ((rc = fill_bytes(c, next, n, 0))) return bail(rc);
+                uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+                k_bfs_coo<<<grid, kBlock, 0, c->stream>>>((const uint32_t *)g->coo_src.p,
+                                                           (const uint32_t *)g->coo_dst.p, m, bm, bit,
+                                                           (int32_t *)op->a, ex ? ex->levels : nullptr,
+                                                           ex ? ex->level : 0, next);
+                if (ex) ex->scanned += m;
+            } else {
+                k_bfs_pull<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
+                    (const uint32_t *)g->toff.p, (const uint32_t *)g->tids.p, bm, bit, (int32_t *)op->a,
+                    ex ? ex->levels : nullptr, ex ? ex->level : 0, next, outdeg, n, cnt);
+            }
+            rc = GG_OK;
+            break;
+        }
+        default: return bail(fail(GG_ERR_INVALID, "edgeMap: unknown operator kind"));
+        }
+        if (rc) { buf_free(tmpv); return bail(rc); }
+        GG_HIP(hipGetLastError());
+        out->has_dense = true;
+        out->dense_valid = true;
+        if (coo && !op_is_sum(op->kind)) {
+            if ((rc = frontier_stats_dense(out))) return bail(rc);
+        } else {
+            unsigned long long cn[4];
+            if ((rc = counters_read(c, cn))) return bail(rc);
+            out->d_m = cn[CNT_ACTIVE];
+            out->out_edges = cn[CNT_OUT];
+            if (ex) ex->scanned += op->kind == GG_OP_BFS ? cn[CNT_SCANNED] : m;
+        }
+        buf_free(tmpv);
+        if ((rc = ensure_bitmap(out))) return bail(rc);
+        if ((rc = bitmap_from_bytes(c, (uint8_t *)out->bytes.p, (uint32_t *)out->bitmap.p, n))) return bail(rc);
+        *pout = out;
+        return GG_OK;
+    }
+    // ---------------- sparse push ----------------
+    if (path) *path = GG_PATH_SPARSE;
+    if ((rc = frontier_make_sparse(in))) return bail(rc);
+    const uint32_t k = (uint32_t)in->d_m;
+    Buf offs;
+    if ((rc = buf_alloc(c, offs, 4ull * (k + 1)))) return bail(rc);
+    {
+        struct DegOf {
+            const uint32_t *list, *outdeg;
+            uint32_t k;
+            __host__ __device__ uint32_t operator()(uint32_t i) const { return i < k ? outdeg[list[i]] : 0u; }
+        };
+        auto it = rocprim::make_transform_iterator(rocprim::make_counting_iterator<uint32_t>(0),
+                                                   DegOf{(const uint32_t *)in->list.p,
+                                                         (const uint32_t *)g->outdeg.p, k});
+        size_t tb = 0;
+        GG_HIP(rocprim::exclusive_scan(nullptr, tb, it, (uint32_t *)offs.p, 0u, (size_t)k + 1,
+                                       rocprim::plus<uint32_t>(), c->stream));
+        if ((rc = buf_ensure(c, c->tmp, tb + 16))) { buf_free(offs); return bail(rc); }
+        GG_HIP(rocprim::exclusive_scan(c->tmp.p, tb, it, (uint32_t *)offs.p, 0u, (size_t)k + 1,
+                                       rocprim::plus<uint32_t>(), c->stream));
+    }
+    uint32_t total32 = 0;
+    GG_HIP(hipMemcpyAsync(&total32, (uint32_t *)offs.p + k, 4, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = sync(c))) { buf_free(offs); return bail(rc); }
+    const uint64_t total = total32;
+    if ((rc = buf_ensure(c, out->list, 4ull * (total ? total : 1)))) { buf_free(offs); return bail(rc); }
+    if ((rc = counters_zero(c))) { buf_free(offs); return bail(rc); }
+    if (total) {
+        PushArgs A;
+        A.list = (const uint32_t *)in->list.p;
+        A.offs = (const uint32_t *)offs.p;
+        A.k = k;
+        A.total = total;
+        A.off = (const uint32_t *)g->off.p;
+        A.ids = (const uint32_t *)g->ids.p;
+        A.outdeg = (const uint32_t *)g->outdeg.p;
+        A.w = g->weighted ? (const float *)g->w.p : nullptr;
+        A.kind = op->kind == GG_OP_BFS ? PUSH_BFS
+                 : op->kind == GG_OP_CC ? PUSH_CC
+                 : op->kind == GG_OP_PAGERANK ? PUSH_PR
+                                              : PUSH_SPMV;
+        A.f32 = f32;
+        A.a = op->a;
+        A.b = op->b;
+        A.levels = ex ? ex->levels : nullptr;
+        A.level = ex ? ex->level : 0;
+        A.out = (uint32_t *)out->list.p;
+        A.counters = (unsigned long long *)c->counters.p;
+        k_push<<<blocks_for(total, kPushTile), kBlock, 0, c->stream>>>(A);
+        GG_HIP(hipGetLastError());
+    }
+    if (ex) ex->scanned += total;
+    unsigned long long cn[4];
+    if ((rc = counters_read(c, cn))) { buf_free(offs); return bail(rc); }
+    buf_free(offs);
+    out->d_m = cn[CNT_CURSOR];
+    out->out_edges = cn[CNT_OUT];
+    if (remdups && out->d_m) {
+        Buf flags, list2;
+        if ((rc = buf_alloc(c, flags, 4ull * n))) return bail(rc);
+        if ((rc = buf_alloc(c, list2, 4ull * out->d_m))) { buf_free(flags); return bail(rc); }
+        fill_u32(c, flags.p, n, 0xffffffffu);
+        counters_zero(c);
+        k_dedup_mark<<<blocks_for(out->d_m, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)out->list.p,
+                                                                              out->d_m, (uint32_t *)flags.p);
+        k_dedup_keep<<<blocks_for(out->d_m, kBlock), kBlock, 0, c->stream>>>(
+            (const uint32_t *)out->list.p, out->d_m, (uint32_t *)flags.p, (const uint32_t *)g->outdeg.p,
+            (uint32_t *)list2.p, (unsigned long long *)c->counters.p);
+        GG_HIP(hipGetLastError());
+        if ((rc = counters_read(c, cn))) { buf_free(flags); buf_free(list2); return bail(rc); }
+        std::swap(out->list, list2);
+        buf_free(flags);
+        buf_free(list2);
+        out->d_m = cn[CNT_CURSOR];
+        out->out_edges = cn[CNT_OUT];
+    }
+    if (out->d_m == 0) out->out_edges = 0; // ligra-numa.h:1013-1014
+    out->has_dense = false;
+    out->sparse_valid = true;
+    *pout = out;
+    return GG_OK;
+}
+
+int vertexmap_core(Graph *g, Frontier *f, const gg_vop_t *op)
+{
+    Ctx *c = g->ctx;
+    int mode;
+    uint64_t count;
+    const uint8_t *F = nullptr;
+    const uint32_t *list = nullptr;
+    if (!f || f->bit) {
+        mode = 0;
+        count = g->n;
+    } else if (f->n == 0) {
+        return GG_OK; // empty() frontier
+    } else if (f->has_dense) {
+        mode = 1;
+        count = g->n;
+        GG_TRY(frontier_make_dense(f));
+        F = (const uint8_t *)f->bytes.p;
+    } else {
+        mode = 2;
+        count = f->d_m;
+        list = (const uint32_t *)f->list.p;
+    }
+    if (!count) return GG_OK;
+    uint32_t grid = (uint32_t)std::min<uint64_t>((count + kBlock - 1) / kBlock, 65536);
+    if (op->dtype == GG_F32 && op->kind != GG_VOP_CC_COPY)
+        k_vmap<float><<<grid, kBlock, 0, c->stream>>>(op->kind, mode, F, list, count, op->a, op->b, op->c0,
+                                                       op->c1);
+    else
+        k_vmap<double><<<grid, kBlock, 0, c->stream>>>(op->kind, mode, F, list, count, op->a, op->b, op->c0,
+                                                        op->c1);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+int reduce_sum_core(Ctx *c, const void *dev, int dtype, uint64_t count, double *dev_out)
+{
+    const uint32_t np = 1024;
+    Buf part;
+    GG_TRY(buf_alloc(c, part, 8ull * np));
+    if (dtype == GG_F32)
+        k_sum_part<float><<<np, kBlock, 0, c->stream>>>((const float *)dev, count, (double *)part.p);
+    else
+        k_sum_part<double><<<np, kBlock, 0, c->stream>>>((const double *)dev, count, (double *)part.p);
+    k_sum_final<<<1, kBlock, 0, c->stream>>>((const double *)part.p, np, dev_out);
+    GG_HIP(hipGetLastError());
+    GG_TRY(sync(c));
+    buf_free(part);
+    return GG_OK;
+}
+
+} // namespace gg
+
+using namespace gg;
+
+// ============================================================================
+// C ABI: frontier
+// ============================================================================
+
+extern "C" int gg_frontier_create(gg_graph_t g, int kind, uint32_t v, gg_frontier_t *out)
+{
+    if (!g || !out) return fail(GG_ERR_INVALID, "gg_frontier_create: NULL argument");
+    gg_frontier_s *f = new gg_frontier_s();
+    frontier_alloc(g, f);
+    Ctx *c = g->ctx;
+    int rc = GG_OK;
+    switch (kind) {
+    case GG_FRONTIER_EMPTY:
+        f->n = 0; // numVertices = 0 as in empty() (ligra-numa.h:124)
+        break;
+    case GG_FRONTIER_SINGLE: {
+        if (v >= g->n) { delete f; return fail(GG_ERR_INVALID, "gg_frontier_create: vertex out of range"); }
+        rc = buf_alloc(c, f->list, 8);
+        if (!rc) rc = gg_memcpy_h2d(abi(c), f->list.p, &v, 4);
+        uint32_t od = 0;
+        if (!rc) rc = gg_memcpy_d2h(abi(c), &od, (uint32_t *)g->outdeg.p + v, 4);
+        f->d_m = 1;
+        f->out_edges = od;
+        f->sparse_valid = true;
+        break;
+    }
+    case GG_FRONTIER_ALL:
+        f->bit = true;
+        f->has_dense = true;
+        f->d_m = g->n;
+        f->out_edges = g->m;
+        break;
+    default: delete f; return fail(GG_ERR_INVALID, "gg_frontier_create: bad kind");
+    }
+    if (rc) { frontier_free(f); delete f; return rc; }
+    *out = f;
+    return GG_OK;
+}
+
+extern "C" int gg_frontier_from_host(gg_graph_t g, const uint8_t *dense, const uint32_t *sparse,
+                                     uint64_t count, gg_frontier_t *out)
+{
+    if (!g || !out || (!dense && !sparse && count)) return fail(GG_ERR_INVALID, "gg_frontier_from_host: NULL argument");
+    gg_frontier_s *f = new gg_frontier_s();
+    frontier_alloc(g, f);
+    Ctx *c = g->ctx;
+    int rc = GG_OK;
+    if (dense) {
+        rc = ensure_bytes(f);
+        if (!rc) rc = gg_memcpy_h2d(abi(c), f->bytes.p, dense, g->n);
+        if (!rc) rc = ensure_bitmap(f);
+        if (!rc) rc = bitmap_from_bytes(c, (uint8_t *)f->bytes.p, (uint32_t *)f->bitmap.p, g->n);
+        f->has_dense = true;
+        f->dense_valid = true;
+        if (!rc) rc = frontier_stats_dense(f);
+    } else {
+        for (uint64_t i = 0; i < count; i++)
+            if (sparse[i] >= g->n) { delete f; return fail(GG_ERR_INVALID, "gg_frontier_from_host: id out of range"); }
+        rc = buf_alloc(c, f->list, 4 * (count ? count : 1));
+        if (!rc && count) rc = gg_memcpy_h2d(abi(c), f->list.p, sparse, 4 * count);
+        f->d_m = count;
+        f->sparse_valid = true;
+        if (!rc && count) {
+            std::vector<uint32_t> od(g->n);
+            rc = gg_memcpy_d2h(abi(c), od.data(), g->outdeg.p, 4ull * g->n);
+            uint64_t s = 0;
+            for (uint64_t i = 0; i < count; i++) s += od[sparse[i]];
+            f->out_edges = s;
+        }
+    }
+    if (rc) { frontier_free(f); delete f; return rc; }
+    *out = f;
+    return GG_OK;
+}
+
+extern "C" int gg_frontier_destroy(gg_frontier_t f)
+{
+    if (!f) return fail(GG_ERR_INVALID, "gg_frontier_destroy: NULL");
+    (void)hipStreamSynchronize(f->g->ctx->stream);
+    frontier_free(f);
+    delete f;
+    return GG_OK;
+}
+
+extern "C" int gg_frontier_info(gg_frontier_t f, uint64_t *d_m, uint64_t *out_edges, int *is_dense,
+                                int *bit, uint32_t *n)
+{
+    if (!f) return fail(GG_ERR_INVALID, "gg_frontier_info: NULL");
+    if (d_m) *d_m = f->d_m;
+    if (out_edges) *out_edges = f->out_edges;
+    if (is_dense) *is_dense = f->has_dense;
+    if (bit) *bit = f->bit;
+    if (n) *n = f->n;
+    return GG_OK;
+}
+
+extern "C" int gg_frontier_to_dense(gg_frontier_t f)
+{
+    if (!f) return fail(GG_ERR_INVALID, "gg_frontier_to_dense: NULL");
+    if (f->n == 0) return GG_OK;
+    GG_TRY(frontier_make_dense(f));
+    f->has_dense = true;
+    return sync(f->g->ctx);
+}
+
+extern "C" int gg_frontier_to_sparse(gg_frontier_t f)
+{
+    if (!f) return fail(GG_ERR_INVALID, "gg_frontier_to_sparse: NULL");
+    if (f->n == 0) return GG_OK;
+    if (f->bit) return fail(GG_ERR_STATE, "toSparse on a bits() frontier reads unset flags (ligra-numa.h:250)");
+    GG_TRY(frontier_make_sparse(f));
+    f->has_dense = false;
+    return sync(f->g->ctx);
+}
+
+extern "C" int gg_frontier_copy_dense(gg_frontier_t f, uint8_t *out)
+{
+    if (!f || !out) return fail(GG_ERR_INVALID, "gg_frontier_copy_dense: NULL");
+    if (f->n == 0) return GG_OK;
+    GG_TRY(frontier_make_dense(f));
+    return gg_memcpy_d2h(abi(f->g->ctx), out, f->bytes.p, f->n);
+}
+
+extern "C" int gg_frontier_copy_sparse(gg_frontier_t f, uint32_t *out, uint64_t cap, uint64_t *count)
+{
+    if (!f || (!out && cap)) return fail(GG_ERR_INVALID, "gg_frontier_copy_sparse: NULL");
+    if (f->n == 0) {
+        if (count) *count = 0;
+        return GG_OK;
+    }
+    GG_TRY(frontier_make_sparse(f));
+    uint64_t k = f->bit ? f->n : f->d_m;
+    if (count) *count = k;
+    if (cap < k) return fail(GG_ERR_SIZE, "gg_frontier_copy_sparse: capacity too small");
+    return gg_memcpy_d2h(abi(f->g->ctx), out, f->list.p, 4 * k);
+}
+
+// ============================================================================
+// C ABI: edgeMap / vertexMap / reduce
+// ============================================================================
+
+static int check_op(gg_graph_t g, const gg_op_t *op)
+{
+    if (!op || !op->a) return fail(GG_ERR_INVALID, "edgeMap: operator or its first array is NULL");
+    switch (op->kind) {
+    case GG_OP_PAGERANK:
+    case GG_OP_SPMV:
+        if (!op->b) return fail(GG_ERR_INVALID, "edgeMap: p_next is NULL");
+        if (op->dtype != GG_F64 && op->dtype != GG_F32) return fail(GG_ERR_INVALID, "edgeMap: dtype must be F64 or F32");
+        break;
+    case GG_OP_CC:
+        if (!op->b) return fail(GG_ERR_INVALID, "edgeMap: prevIDs is NULL");
+        break;
+    case GG_OP_BFS: break;
+    default: return fail(GG_ERR_INVALID, "edgeMap: unknown operator kind");
+    }
+    (void)g;
+    return GG_OK;
+}
+
+extern "C" int gg_edgemap(gg_graph_t g, gg_frontier_t in, const gg_op_t *op, int64_t threshold,
+                          int remdups, gg_frontier_t *out, int *path)
+{
+    if (!g || !in || !out) return fail(GG_ERR_INVALID, "gg_edgemap: NULL argument");
+    GG_TRY(check_op(g, op));
+    GG_HIP(hipSetDevice(g->ctx->device));
+    Frontier *o = nullptr;
+    int rc = edgemap_core(g, in, op, threshold, remdups, &o, path, nullptr);
+    if (rc) return rc;
+    gg_frontier_s *fo = new gg_frontier_s();
+    static_cast<Frontier &>(*fo) = std::move(*o);
+    delete o;
+    *out = fo;
+    return sync(g->ctx);
+}
+
+extern "C" int gg_vertexmap(gg_graph_t g, gg_frontier_t f, const gg_vop_t *op)
+{
+    if (!g || !op || !op->a) return fail(GG_ERR_INVALID, "gg_vertexmap: NULL argument");
+    if ((op->kind == GG_VOP_PAGERANK || op->kind == GG_VOP_ABSDIFF || op->kind == GG_VOP_CC_COPY) && !op->b)
+        return fail(GG_ERR_INVALID, "gg_vertexmap: second array is NULL");
+    if (op->kind < GG_VOP_PAGERANK || op->kind > GG_VOP_ABSDIFF) return fail(GG_ERR_INVALID, "gg_vertexmap: bad kind");
+    if (f && f->n != 0 && f->n != g->n) return fail(GG_ERR_SIZE, "gg_vertexmap: frontier size mismatch");
+    GG_TRY(vertexmap_core(g, f, op));
+    return sync(g->ctx);
+}
+
+extern "C" int gg_vertexfilter(gg_graph_t g, gg_frontier_t f, const uint8_t *flags_dev, gg_frontier_t *out)
+{
+    if (!g || !f || !flags_dev || !out) return fail(GG_ERR_INVALID, "gg_vertexfilter: NULL argument");
+    if (f->n != g->n) return fail(GG_ERR_SIZE, "gg_vertexfilter: frontier size mismatch");
+    Ctx *c = g->ctx;
+    int mode = f->bit ? 0 : 1;
+    if (!f->bit) GG_TRY(frontier_make_dense(f));
+    gg_frontier_s *o = new gg_frontier_s();
+    frontier_alloc(g, o);
+    int rc = ensure_bytes(o);
+    if (!rc) {
+        k_vfilter<<<blocks_for(g->n, kBlock), kBlock, 0, c->stream>>>(mode, (const uint8_t *)f->bytes.p,
+                                                                      flags_dev, (uint8_t *)o->bytes.p, g->n);
+        o->has_dense = true;
+        o->dense_valid = true;
+        rc = ensure_bitmap(o);
+    }
+    if (!rc) rc = bitmap_from_bytes(c, (uint8_t *)o->bytes.p, (uint32_t *)o->bitmap.p, g->n);
+    if (!rc) rc = frontier_stats_dense(o);
+    if (rc) { frontier_free(o); delete o; return rc; }
+    *out = o;
+    return GG_OK;
+}
+
+extern "C" int gg_reduce_sum(gg_ctx_t c, const void *dev, int dtype, uint64_t count, double *out)
+{
+    if (!c || !out || (count && !dev)) return fail(GG_ERR_INVALID, "gg_reduce_sum: NULL argument");
+    if (dtype != GG_F64 && dtype != GG_F32) return fail(GG_ERR_INVALID, "gg_reduce_sum: F64 or F32 only");
+    Buf r;
+    GG_TRY(buf_alloc(c, r, 8));
+    int rc = reduce_sum_core(c, dev, dtype, count, (double *)r.p);
+    if (!rc) rc = gg_memcpy_d2h(abi(c), out, r.p, 8);
+    buf_free(r);
+    return rc;
+}

@@ -1,0 +1,213 @@
+/*
+ * graphgrind_amd.h -- C ABI of the MI355X (gfx950) traversal engine.
+ *
+ * Drop-in boundary for GraphGrind's hot path.  Every entry point below
+ * replaces a piece of the reference's header-only C++ template API; the
+ * comment on each names the reference interface (file:line) it stands for.
+ * The C++ front end (include/gg/ligra.h) maps the reference's app-facing
+ * API onto these calls; tests and benchmarks bind them through ctypes.
+ *
+ * Conventions
+ *  - Every function returns an int status: GG_OK (0) or a negative GG_ERR_*.
+ *    The library never aborts; gg_last_error() returns the message of the
+ *    last failing call on the calling thread.  Where the reference abort()s
+ *    or asserts, the matching call returns an error instead.
+ *  - Handles own device memory.  Host arrays passed in are copied during the
+ *    call and not retained.  "dev" pointers are device (HBM) addresses
+ *    obtained from gg_malloc (or any other hipMalloc'd memory on the
+ *    context's device).
+ *  - A context is one device and one HIP stream; it is not thread-safe.
+ *    Calls that return host-visible results synchronize the stream.
+ *  - Vertex ids and edge offsets are uint32 (n < 2^32, m < 2^32).
+ */
+#ifndef GRAPHGRIND_AMD_H
+#define GRAPHGRIND_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GG_OK 0
+#define GG_ERR_INVALID -1     /* bad argument / null handle                 */
+#define GG_ERR_HIP -2         /* HIP runtime failure                        */
+#define GG_ERR_OOM -3         /* device allocation failed                   */
+#define GG_ERR_SIZE -4        /* frontier/graph size mismatch               */
+#define GG_ERR_PARTITION -5   /* partition greedy overruns P                */
+#define GG_ERR_UNSUPPORTED -6 /* option not provided by this build          */
+#define GG_ERR_STATE -7       /* object in the wrong state for the call     */
+
+/* ---- element types ------------------------------------------------------ */
+#define GG_F64 0
+#define GG_F32 1
+#define GG_U32 2
+#define GG_I32 3
+#define GG_U8 4
+
+typedef struct gg_ctx_s *gg_ctx_t;
+typedef struct gg_graph_s *gg_graph_t;
+typedef struct gg_frontier_s *gg_frontier_t;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the Cilk runtime + NUMA setup of parallel.h:27-46 and
+ * partitioner.h:15-30: one context per GPU, one stream. */
+int gg_ctx_create(int device, gg_ctx_t *out);
+int gg_ctx_destroy(gg_ctx_t ctx);
+int gg_ctx_sync(gg_ctx_t ctx);
+/* Thread-local text of the last error (never NULL). */
+const char *gg_last_error(void);
+/* Library version string. */
+const char *gg_version(void);
+
+/* ---- device memory: mmap_ptr<T> (mm.h:246-325) -------------------------- */
+int gg_malloc(gg_ctx_t ctx, uint64_t bytes, void **dev);
+int gg_free(gg_ctx_t ctx, void *dev);
+int gg_memcpy_h2d(gg_ctx_t ctx, void *dev, const void *host, uint64_t bytes);
+int gg_memcpy_d2h(gg_ctx_t ctx, void *host, const void *dev, uint64_t bytes);
+int gg_memcpy_d2d(gg_ctx_t ctx, void *dst, const void *src, uint64_t bytes);
+/* map_vertexL init loops (app/PageRank.C:124-125, BFS.C:88,
+ * Components.C:113): dev[i] = value for i < count. */
+int gg_fill(gg_ctx_t ctx, void *dev, int dtype, uint64_t count, double value);
+/* dev[i] = i (Components.C:113: IDs[j] = j). */
+int gg_iota(gg_ctx_t ctx, void *dev, int dtype, uint64_t count);
+
+/* ---- graph: wholeGraph + partitioned_graph (graph-numa.h:500-756) ------ */
+/* From host CSR (offsets n+1, ids m, optional FP32 weights in CSR order).
+ * Builds the in-edge twin on the device (in-lists ascending by source, as
+ * IO.h:43-51,303 produce) and the out/in degree arrays.  No partitions yet. */
+int gg_graph_create_csr(gg_ctx_t ctx, uint32_t n, uint64_t m, const uint32_t *off,
+                        const uint32_t *ids, const float *w, int symmetric,
+                        gg_graph_t *out);
+/* Seeded R-MAT graph generated, deduplicated and laid out on the device
+ * (a=.57 b=.19 c=.19 d=.05, splitmix64 draws keyed by (seed, edge, level);
+ * self loops dropped; symmetric adds reverse edges).  weighted: FP32 weights
+ * (k+1)/2^24 keyed by (seed, CSR position).  The reference has no
+ * generator; this feeds its apps the inputs the benchmarks are quoted on. */
+int gg_graph_create_rmat(gg_ctx_t ctx, int scale, int edge_factor, uint64_t seed,
+                         int symmetric, int weighted, gg_graph_t *out);
+int gg_graph_destroy(gg_graph_t g);
+/* GA.n / GA.m / isSymmetric (graph-numa.h:633, 511). */
+int gg_graph_info(gg_graph_t g, uint32_t *n, uint64_t *m, int *symmetric, int *weighted);
+/* Copies of the device layout (tests, interop).  Any pointer may be NULL. */
+int gg_graph_copy_csr(gg_graph_t g, uint32_t *off, uint32_t *ids, float *w);
+int gg_graph_copy_csc(gg_graph_t g, uint32_t *toff, uint32_t *tids, float *tw);
+/* V[v].getOutDegree() / getInDegree() for all v (graph-numa.h:187-194). */
+int gg_graph_degrees(gg_graph_t g, int in_edges, uint32_t *out);
+/* Replaces the int weights of a graph (CSR order, m FP32 values). */
+int gg_graph_set_weights(gg_graph_t g, const float *w_csr);
+
+/* Device partitioner: partitioned_graph ctor (graph-numa.h:642-685) with
+ * partitionByDegree/partitionByVertex (1177-1206, 1287-1357),
+ * cscpartitionByDegree/ByVertex (1208-1285), COOPartitionByDest
+ * (1116-1143) and hilbert_sort (482-490).  by_vertex: '-v vertex';
+ * relabel: '-o'; hilbert: EDGES_HILBERT (else CSR order).  Partitions are
+ * by destination ('-P dest'); '-P source' returns GG_ERR_UNSUPPORTED.
+ * Returns GG_ERR_PARTITION where the reference greedy overruns P. */
+int gg_graph_partition(gg_graph_t g, int P, int by_vertex, int relabel, int hilbert);
+/* which 0: COO partition starts (P+1 vertex ids, partitioner.h:153-165);
+ * which 1: CSC partition starts (P+1 positions in the compressed in-vertex
+ * list, graph-numa.h:1092-1107).  *underran (may be NULL) is set when the
+ * greedy used fewer than P partitions (the reference asserts there). */
+int gg_graph_partition_starts(gg_graph_t g, int which, uint64_t *starts, int *underran);
+/* COO edge lists (get_edge_list_partition, graph-numa.h:696-699): all
+ * partitions back to back; part_off has P+1 entries.  Any may be NULL. */
+int gg_graph_coo(gg_graph_t g, uint32_t *src, uint32_t *dst, uint64_t *part_off);
+
+/* ---- frontier: partitioned_vertices (ligra-numa.h:91-272) -------------- */
+#define GG_FRONTIER_EMPTY 0  /* empty()  119-130                         */
+#define GG_FRONTIER_SINGLE 1 /* create(n, v, outdeg(v)) 103-116         */
+#define GG_FRONTIER_ALL 2    /* bits(part, n, m) 169-180: all active     */
+int gg_frontier_create(gg_graph_t g, int kind, uint32_t v, gg_frontier_t *out);
+/* From host flags (dense bool[n], 183-194) or a host id list (197-208). */
+int gg_frontier_from_host(gg_graph_t g, const uint8_t *dense, const uint32_t *sparse,
+                          uint64_t count, gg_frontier_t *out);
+int gg_frontier_destroy(gg_frontier_t f);
+/* numNonzeros (268-271), num_out_edges, has_dense, bit, numRows. */
+int gg_frontier_info(gg_frontier_t f, uint64_t *d_m, uint64_t *out_edges, int *is_dense,
+                     int *bit, uint32_t *n);
+/* toDense (228-244) / toSparse (246-260; list ascending like packIndex). */
+int gg_frontier_to_dense(gg_frontier_t f);
+int gg_frontier_to_sparse(gg_frontier_t f);
+/* Host copies: dense flags (n bytes) / sparse list (cap entries, *count). */
+int gg_frontier_copy_dense(gg_frontier_t f, uint8_t *out);
+int gg_frontier_copy_sparse(gg_frontier_t f, uint32_t *out, uint64_t cap, uint64_t *count);
+
+/* ---- edgeMap (ligra-numa.h:919-1019) ------------------------------------ */
+/* The app functors become an enumerated operator set; a and b are the
+ * functor's device arrays.  See include/gg/ligra.h for the trait mapping. */
+#define GG_OP_PAGERANK 0 /* PR_F, app/PageRank.C:28-70.  a=p_curr b=p_next
+                            (F64 or F32); p_next[d] += p_curr[s]/outdeg(s).
+                            p_curr must be >= +0 (ranks)                  */
+#define GG_OP_BFS 1      /* BFS_F, app/BFS.C:26-75.  a=Parents (I32, -1 =
+                            unvisited); cond = Parents[d] == -1           */
+#define GG_OP_CC 2       /* CC_F, app/Components.C:26-86.  a=IDs b=prevIDs
+                            (U32); min-label, frontier = changed          */
+#define GG_OP_SPMV 3     /* SPMV_F, app/SPMV.C:7-48.  a=p_curr b=p_next
+                            (F64 or F32); p_next[d] += p_curr[s]*w        */
+typedef struct {
+    int32_t kind;
+    int32_t dtype; /* GG_F64 / GG_F32 for PAGERANK and SPMV */
+    void *a;
+    void *b;
+} gg_op_t;
+#define GG_PATH_EMPTY -1
+#define GG_PATH_SPARSE 0 /* edgeMapSparseWithG 868-916 */
+#define GG_PATH_CSC 1    /* edgeMapDenseCSC    549-729 */
+#define GG_PATH_COO 2    /* edgeMapDense (COO) 803-866 */
+/* threshold -1 means m/20 (932).  remdups: remDuplicates (317-348) on the
+ * sparse output.  *out is a new frontier owned by the caller. */
+int gg_edgemap(gg_graph_t g, gg_frontier_t in, const gg_op_t *op, int64_t threshold,
+               int remdups, gg_frontier_t *out, int *path);
+
+/* ---- vertexMap (ligra-numa.h:1023-1049) / map_vertexL ------------------- */
+#define GG_VOP_PAGERANK 0 /* PR_Vertex_F (PageRank.C:73-87): b[i] = c0*b[i] + c1 */
+#define GG_VOP_RESET 1    /* PR_Vertex_Reset (90-100): a[i] = 0                */
+#define GG_VOP_CC_COPY 2  /* CC_Vertex_F (Components.C:88-99): b[i] = a[i]      */
+#define GG_VOP_ABSDIFF 3  /* map_vertexL of PageRank.C:136: a[i] = |a[i]-b[i]| */
+typedef struct {
+    int32_t kind;
+    int32_t dtype;
+    void *a;
+    void *b;
+    double c0;
+    double c1;
+} gg_vop_t;
+/* f == NULL applies to every vertex (map_vertexL over the partitioner). */
+int gg_vertexmap(gg_graph_t g, gg_frontier_t f, const gg_vop_t *op);
+/* vertexFilter (1053-1082): keep active i with a[i] (U8 flags, n) != 0. */
+int gg_vertexfilter(gg_graph_t g, gg_frontier_t f, const uint8_t *flags_dev,
+                    gg_frontier_t *out);
+/* sequence::plusReduce (utils.h:192): fixed-order device sum. */
+int gg_reduce_sum(gg_ctx_t ctx, const void *dev, int dtype, uint64_t count, double *out);
+
+/* ---- fused app loops (the Compute() of each app, on device) ------------- */
+typedef struct {
+    int32_t iters;          /* iterations / levels / rounds executed       */
+    int32_t paths[64];      /* edgeMap path of the first 64 steps          */
+    uint64_t edges_scanned; /* edges read by the traversal kernels         */
+    double ms;              /* hipEvent time of the loop (incl. init)      */
+} gg_run_info;
+/* app/PageRank.C:102-151: exactly the reference's frontier-gated loop,
+ * max_iter iterations or until L1 < eps.  ranks_dev receives the x of the
+ * last iteration (F64 or F32 per dtype). */
+int gg_pagerank(gg_graph_t g, int dtype, int max_iter, double damping, double eps,
+                void *ranks_dev, gg_run_info *info);
+/* app/BFS.C:77-101 with the direction switch of edgeMap; parents_dev (I32
+ * n, -1 unreached); levels_dev (I32 n, -1 unreached) may be NULL. */
+int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *parents_dev,
+           int32_t *levels_dev, gg_run_info *info);
+/* app/Components.C:101-126: min-label propagation to the fixed point. */
+int gg_cc(gg_graph_t g, uint32_t *labels_dev, gg_run_info *info);
+/* app/SPMV.C:64-92 generalised: y[d] = sum_{s->d} w(s->d) x[s] over the
+ * in-edge twin (unit weights when the graph has none). */
+int gg_spmv(gg_graph_t g, int dtype, const void *x_dev, void *y_dev, gg_run_info *info);
+
+/* ---- timing ------------------------------------------------------------- */
+/* Average device time (ms) of the last fused loop's dominant kernel and
+ * its launch count, measured with hipEvents on the context stream. */
+int gg_last_kernel_time(gg_ctx_t ctx, double *avg_ms, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
