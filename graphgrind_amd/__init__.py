@@ -200,9 +200,9 @@ class DeviceArray:
         return self
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:  # a closed context already released the device
             lib().gg_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -246,9 +246,9 @@ class Graph:
         return cls(ctx, h)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().gg_graph_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def partition(self, P=384, by_vertex=False, relabel=False, hilbert=True):
         """-c P, -v edge|vertex, -o, EDGES_HILBERT (ligra-numa.h:1104-1124)."""
@@ -371,9 +371,9 @@ class Frontier:
         return out[:cnt.value]
 
     def close(self):
-        if self.h:
+        if self.h and self.g.ctx.h:
             lib().gg_frontier_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

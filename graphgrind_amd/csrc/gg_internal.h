@@ -106,6 +106,7 @@ struct Graph {
 // ---- frontier ---------------------------------------------------------------
 struct Frontier {
     Graph *g = nullptr;
+    Ctx *ctx = nullptr;  // outlives the graph: destroy needs no graph
     uint32_t n = 0;      // numRows (0 for the empty() frontier)
     bool bit = false;    // all active, flags not read
     bool has_dense = false;

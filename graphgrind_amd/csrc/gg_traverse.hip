@@ -551,6 +551,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_final(const double *__restrict__
 int frontier_alloc(Graph *g, Frontier *f)
 {
     f->g = g;
+    f->ctx = g->ctx;
     f->n = g->n;
     return GG_OK;
 }
@@ -1023,7 +1024,7 @@ extern "C" int gg_frontier_from_host(gg_graph_t g, const uint8_t *dense, const u
 extern "C" int gg_frontier_destroy(gg_frontier_t f)
 {
     if (!f) return fail(GG_ERR_INVALID, "gg_frontier_destroy: NULL");
-    (void)hipStreamSynchronize(f->g->ctx->stream);
+    (void)hipStreamSynchronize(f->ctx->stream);
     frontier_free(f);
     delete f;
     return GG_OK;
