@@ -92,6 +92,7 @@ SIGNATURES = {
     "gg_graph_degrees": (C.c_int, [_VP, C.c_int, _VP]),
     "gg_graph_set_weights": (C.c_int, [_VP, _VP]),
     "gg_graph_partition": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "gg_graph_reorder": (C.c_int, [_VP, C.c_int]),
     "gg_graph_partition_starts": (C.c_int, [_VP, C.c_int, _VP, _IP]),
     "gg_graph_coo": (C.c_int, [_VP, _VP, _VP, _VP]),
     "gg_frontier_create": (C.c_int, [_VP, C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -255,6 +256,11 @@ class Graph:
         check(lib().gg_graph_partition(self.h, P, int(by_vertex), int(relabel), int(hilbert)))
         self.P = P
         self.part_vertex = by_vertex
+        return self
+
+    def reorder(self, mode=1):
+        """Internal hot-sources-first renumbering used by the pull kernels."""
+        check(lib().gg_graph_reorder(self.h, mode))
         return self
 
     def starts(self, which=0):

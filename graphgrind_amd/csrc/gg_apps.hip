@@ -37,24 +37,34 @@ struct OpPRFused {
     __device__ V gather(uint32_t, uint32_t s) const { return contrib_in[s]; }
     __device__ void finish(uint32_t d, V acc, LocalStats &st) const
     {
-        V xv = x[d];
+        // every per-vertex array is touched once per iteration: stream it
+        // past L2 so the gathered contributions stay resident
+        V xv = __builtin_nontemporal_load(&x[d]);
         const bool any = !signbit(acc);
         if (any) xv = xv + acc;
-        const bool inF = bit || Fin[d];
+        const bool inF = bit || __builtin_nontemporal_load(&Fin[d]);
         if (inF) xv = damping * xv + added;
-        const V diff = fabs(c[d] - xv);
+        const V diff = fabs(__builtin_nontemporal_load(&c[d]) - xv);
         st.l1 += (double)diff;
-        y[d] = inF ? V(0) : diff;
-        x[d] = xv;
-        Fout[d] = any;
-        const uint32_t od = outdeg[d];
-        contrib_out[d] = (any && od) ? xv / (V)od : -V(0);
+        __builtin_nontemporal_store(inF ? V(0) : diff, &y[d]);
+        __builtin_nontemporal_store(xv, &x[d]);
+        __builtin_nontemporal_store((uint8_t)any, &Fout[d]);
+        const uint32_t od = __builtin_nontemporal_load(&outdeg[d]);
+        __builtin_nontemporal_store((any && od) ? xv / (V)od : -V(0), &contrib_out[d]);
         if (any) {
             st.active++;
             st.out_edges += od;
         }
     }
 };
+
+template <class V>
+__global__ void k_unpermute(V *__restrict__ out, const V *__restrict__ in, const uint32_t *__restrict__ perm,
+                            uint32_t n)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) out[v] = in[perm[v]];
+}
 
 template <class V>
 __global__ void k_pr_init(V *c, V *x, V *contrib, const uint32_t *__restrict__ outdeg, uint32_t n, V one_over_n)
@@ -77,7 +87,8 @@ __global__ __launch_bounds__(kBlock) void k_pr_check(const double *__restrict__ 
 {
     if (*done) return;
     double s = 0;
-    for (uint32_t i = threadIdx.x; i < np; i += kBlock) s += l1_part[i];
+    (void)l1_part;
+    (void)np; // folded into l1_fix by k_pull_fixup
     for (uint32_t i = threadIdx.x; i < nf; i += kBlock) s += l1_fix[i];
     s = wave_sum_f64(s);
     __shared__ double w[kBlock / 64];
@@ -114,7 +125,10 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
     const uint64_t m = g->m;
-    PullSched &s = g->pull;
+    const bool use_plan = g->plan.mode != 0;
+    const CscView view = use_plan ? plan_view(g) : graph_view(g);
+    const uint32_t *outdeg = use_plan ? (const uint32_t *)g->plan.outdeg.p : (const uint32_t *)g->outdeg.p;
+    PullSched &s = *view.sched;
     const uint32_t nfix = blocks_for(s.tiles, kBlock);
     PRBuffers B;
     int rc = GG_OK;
@@ -140,7 +154,7 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
     fill_bytes(c, B.flags.p, 64, 0);
     fill_bytes(c, c->counters.p, 64, 0);
     k_pr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)B.R[0].p, (V *)B.R[1].p, (V *)B.K[0].p,
-                                                                   (const uint32_t *)g->outdeg.p, n, one_over_n);
+                                                                   outdeg, n, one_over_n);
     for (int it = 1; it <= max_iter; it++) {
         OpPRFused<V> op;
         op.c = (const V *)B.R[(it - 1) % 3].p;
@@ -151,16 +165,19 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
         op.bit = it == 1;
         op.Fin = it == 1 ? nullptr : (const uint8_t *)B.F[(it - 2) % 2].p;
         op.Fout = (uint8_t *)B.F[(it - 1) % 2].p;
-        op.outdeg = (const uint32_t *)g->outdeg.p;
+        op.outdeg = outdeg;
         op.damping = (V)damping;
         op.added = added;
         op.done = done;
-        rc = launch_pull(c, g, op, cnt, (double *)B.l1p.p, (double *)B.l1f.p, ev[2 * it], ev[2 * it + 1]);
+        rc = launch_pull(c, view, op, cnt, (double *)B.l1p.p, (double *)B.l1f.p, ev[2 * it], ev[2 * it + 1]);
         if (rc) break;
         k_pr_check<<<1, kBlock, 0, c->stream>>>((const double *)B.l1p.p, s.tiles, (const double *)B.l1f.p, nfix,
                                                  eps, it, done, done_iter, cnt,
                                                  (unsigned long long *)B.stats.p, (double *)B.l1h.p);
     }
+    // the rank vector in the caller's numbering: the buffer of the last
+    // executed iteration is known only on the device (break flag), so copy
+    // after reading it; with a relabelled plan this is the un-permute
     GG_HIP(hipEventRecord(ev[1], c->stream));
     GG_HIP(hipGetLastError());
     if (!rc) rc = sync(c);
@@ -197,8 +214,16 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
         }
         if (info && iters >= 1)
             info->paths[0] = g->P && !g->part_vertex ? GG_PATH_COO : GG_PATH_CSC;
-        if (!*needs_generic)
-            rc = gg_memcpy_d2d(abi(c), ranks, B.R[iters % 3].p, sizeof(V) * (uint64_t)n);
+        if (!*needs_generic) {
+            if (use_plan) {
+                k_unpermute<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
+                    (V *)ranks, (const V *)B.R[iters % 3].p, (const uint32_t *)g->plan.perm.p, n);
+                GG_HIP(hipGetLastError());
+                rc = sync(c);
+            } else {
+                rc = gg_memcpy_d2d(abi(c), ranks, B.R[iters % 3].p, sizeof(V) * (uint64_t)n);
+            }
+        }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     B.free_all();
@@ -427,8 +452,8 @@ extern "C" int gg_spmv(gg_graph_t g, int dtype, const void *x, void *y, gg_run_i
     (void)hipEventCreate(&e1);
     GG_HIP(hipEventRecord(e0, c->stream));
     int rc = dtype == GG_F64
-                 ? launch_pull(c, g, OpSpmvY<double>{(const double *)x, tw, (double *)y}, nullptr, nullptr, nullptr)
-                 : launch_pull(c, g, OpSpmvY<float>{(const float *)x, tw, (float *)y}, nullptr, nullptr, nullptr);
+                 ? launch_pull(c, graph_view(g), OpSpmvY<double>{(const double *)x, tw, (double *)y}, nullptr, nullptr, nullptr)
+                 : launch_pull(c, graph_view(g), OpSpmvY<float>{(const float *)x, tw, (float *)y}, nullptr, nullptr, nullptr);
     GG_HIP(hipEventRecord(e1, c->stream));
     if (!rc) rc = sync(c);
     float ms = 0;

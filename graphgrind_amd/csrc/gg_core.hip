@@ -522,8 +522,12 @@ static void graph_free(Graph *g)
 {
     Buf *bufs[] = {&g->off, &g->ids, &g->w, &g->toff, &g->tids, &g->tw, &g->perm, &g->outdeg,
                    &g->indeg, &g->comp, &g->coo_src, &g->coo_dst, &g->coo_w, &g->pull.start_row,
-                   &g->pull.start_edge, &g->pull.carry_row, &g->pull.carry_val, &g->pull.head_val};
+                   &g->pull.start_edge, &g->pull.carry_row, &g->pull.carry_val, &g->pull.head_val,
+                   &g->plan.perm, &g->plan.inv, &g->plan.toff, &g->plan.tids, &g->plan.tw, &g->plan.outdeg,
+                   &g->plan.sched.start_row, &g->plan.sched.start_edge, &g->plan.sched.carry_row,
+                   &g->plan.sched.carry_val, &g->plan.sched.head_val};
     for (Buf *b : bufs) buf_free(*b);
+    g->plan.mode = 0;
 }
 
 } // namespace gg
@@ -1112,6 +1116,134 @@ extern "C" int gg_graph_coo(gg_graph_t g, uint32_t *src, uint32_t *dst, uint64_t
     GG_TRY(copy_out(g->ctx, src, g->coo_src, 4 * g->m));
     GG_TRY(copy_out(g->ctx, dst, g->coo_dst, 4 * g->m));
     return sync(g->ctx);
+}
+
+// =============================================================================
+// Relabelled pull plan (hot sources first)
+// =============================================================================
+
+namespace gg {
+
+// Sort key of vertex v: out-degree descending, id ascending.
+__global__ void k_order_keys(const uint32_t *__restrict__ outdeg, uint32_t n, uint64_t *__restrict__ keys)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) keys[v] = ((uint64_t)(0xffffffffu - outdeg[v]) << 32) | v;
+}
+
+__global__ void k_order_emit(const uint64_t *__restrict__ keys, uint32_t n, const uint32_t *__restrict__ outdeg,
+                             uint32_t *__restrict__ inv, uint32_t *__restrict__ perm,
+                             uint32_t *__restrict__ outdeg2)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t v = (uint32_t)keys[k];
+    inv[k] = v;
+    perm[v] = k;
+    outdeg2[k] = outdeg[v];
+}
+
+// Relabelled in-edge key of CSR edge e: (perm[dst] << bits) | perm[src].
+__global__ void k_plan_keys(const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
+                            const uint32_t *__restrict__ perm, uint32_t n, uint64_t m, int bits,
+                            uint64_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; e < m; e += stride) {
+        uint32_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        keys[e] = ((uint64_t)perm[ids[e]] << bits) | perm[lo];
+        vals[e] = (uint32_t)e;
+    }
+}
+
+} // namespace gg
+
+extern "C" int gg_graph_reorder(gg_graph_t g, int mode)
+{
+    if (!g) return fail(GG_ERR_INVALID, "gg_graph_reorder: NULL");
+    if (mode != 0 && mode != 1) return fail(GG_ERR_INVALID, "gg_graph_reorder: mode must be 0 or 1");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    PullPlan &P = g->plan;
+    Buf *pb[] = {&P.perm, &P.inv, &P.toff, &P.tids, &P.tw, &P.outdeg, &P.sched.start_row, &P.sched.start_edge,
+                 &P.sched.carry_row, &P.sched.carry_val, &P.sched.head_val};
+    for (Buf *b : pb) buf_free(*b);
+    P.mode = 0;
+    if (mode == 0) return GG_OK;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    GG_TRY(buf_alloc(c, P.perm, 4ull * n));
+    GG_TRY(buf_alloc(c, P.inv, 4ull * n));
+    GG_TRY(buf_alloc(c, P.outdeg, 4ull * n));
+    GG_TRY(buf_alloc(c, P.toff, 4ull * (n + 1)));
+    GG_TRY(buf_alloc(c, P.tids, 4ull * (m ? m : 1)));
+    {
+        Buf k0, k1;
+        GG_TRY(buf_alloc(c, k0, 8ull * n));
+        GG_TRY(buf_alloc(c, k1, 8ull * n));
+        k_order_keys<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)g->outdeg.p, n,
+                                                                       (uint64_t *)k0.p);
+        size_t tb = 0;
+        GG_HIP(rocprim::radix_sort_keys(nullptr, tb, (uint64_t *)k0.p, (uint64_t *)k1.p, (size_t)n, 0, 64,
+                                        c->stream));
+        GG_TRY(buf_ensure(c, c->tmp, tb + 16));
+        GG_HIP(rocprim::radix_sort_keys(c->tmp.p, tb, (uint64_t *)k0.p, (uint64_t *)k1.p, (size_t)n, 0, 64,
+                                        c->stream));
+        k_order_emit<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint64_t *)k1.p, n,
+                                                                       (const uint32_t *)g->outdeg.p,
+                                                                       (uint32_t *)P.inv.p, (uint32_t *)P.perm.p,
+                                                                       (uint32_t *)P.outdeg.p);
+        GG_HIP(hipGetLastError());
+        GG_TRY(sync(c));
+        buf_free(k0);
+        buf_free(k1);
+    }
+    Buf deg;
+    GG_TRY(buf_alloc(c, deg, 4ull * n));
+    GG_TRY(fill_u32(c, deg.p, n, 0));
+    if (m) {
+        int bits = bits_for(n);
+        Buf k0, k1, v0, v1;
+        GG_TRY(buf_alloc(c, k0, 8 * m));
+        GG_TRY(buf_alloc(c, k1, 8 * m));
+        GG_TRY(buf_alloc(c, v0, 4 * m));
+        GG_TRY(buf_alloc(c, v1, 4 * m));
+        uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+        k_plan_keys<<<grid, kBlock, 0, c->stream>>>((const uint32_t *)g->off.p, (const uint32_t *)g->ids.p,
+                                                     (const uint32_t *)P.perm.p, n, m, bits, (uint64_t *)k0.p,
+                                                     (uint32_t *)v0.p);
+        size_t tb = 0;
+        GG_HIP(rocprim::radix_sort_pairs(nullptr, tb, (uint64_t *)k0.p, (uint64_t *)k1.p, (uint32_t *)v0.p,
+                                         (uint32_t *)v1.p, (size_t)m, 0, (unsigned)(2 * bits), c->stream));
+        GG_TRY(buf_ensure(c, c->tmp, tb + 16));
+        GG_HIP(rocprim::radix_sort_pairs(c->tmp.p, tb, (uint64_t *)k0.p, (uint64_t *)k1.p, (uint32_t *)v0.p,
+                                         (uint32_t *)v1.p, (size_t)m, 0, (unsigned)(2 * bits), c->stream));
+        k_csc_decode<<<grid, kBlock, 0, c->stream>>>((const uint64_t *)k1.p, m, bits, (uint32_t *)P.tids.p,
+                                                      (uint32_t *)deg.p);
+        if (g->weighted) {
+            GG_TRY(buf_alloc(c, P.tw, 4 * m));
+            k_gather_f32<<<grid, kBlock, 0, c->stream>>>((const float *)g->w.p, (const uint32_t *)v1.p,
+                                                          (float *)P.tw.p, m);
+        }
+        GG_HIP(hipGetLastError());
+        GG_TRY(sync(c));
+        buf_free(k0);
+        buf_free(k1);
+        buf_free(v0);
+        buf_free(v1);
+    }
+    GG_TRY(scan_offsets(c, (const uint32_t *)deg.p, (uint32_t *)P.toff.p, n));
+    GG_TRY(sched_build(c, P.sched, (const uint32_t *)P.toff.p, n, m));
+    GG_TRY(sync(c));
+    buf_free(deg);
+    P.mode = mode;
+    return GG_OK;
 }
 
 extern "C" int gg_last_kernel_time(gg_ctx_t c, double *avg_ms, int *launches)

@@ -65,6 +65,27 @@ struct PullSched {
     Buf head_val;   // 8 bytes [tiles] partial of the first row if it began earlier
 };
 
+// ---- relabelled pull plan ----------------------------------------------------
+// An internal renumbering of the vertices (hot sources first) with its own
+// in-edge twin and schedule; kernels run in the new numbering and results
+// are mapped back.  Built by gg_graph_reorder, like the reference's offline
+// VEBO reordering but kept inside the library.
+struct PullPlan {
+    int mode = 0;        // 0 none, 1 out-degree descending
+    Buf perm;            // uint32[n]: new id of old vertex
+    Buf inv;             // uint32[n]: old id of new vertex
+    Buf toff, tids, tw;  // relabelled in-edge twin (tw when weighted)
+    Buf outdeg;          // uint32[n] in the new numbering
+    PullSched sched;
+};
+
+// In-edge arrays + schedule a pull runs over.
+struct CscView {
+    const uint32_t *toff = nullptr;
+    const uint32_t *tids = nullptr;
+    PullSched *sched = nullptr;
+};
+
 // ---- context ----------------------------------------------------------------
 struct Ctx {
     int device = 0;
@@ -101,6 +122,7 @@ struct Graph {
     Buf coo_w;            // float[m] when weighted
     // merge-path schedule over the in-edge twin
     PullSched pull;
+    PullPlan plan;
 };
 
 // ---- frontier ---------------------------------------------------------------

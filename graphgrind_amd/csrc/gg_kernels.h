@@ -100,10 +100,12 @@ __global__ __launch_bounds__(kBlock) void k_pull(Op op, const uint32_t *__restri
     const uint32_t r0 = srow[t], r1 = srow[t + 1];
     const uint32_t e0 = sedge[t], e1 = sedge[t + 1];
     const uint32_t nr = r1 - r0, ne = e1 - e0;
-    for (uint32_t i = tid; i < nr; i += kBlock) s_end[i] = toff[r0 + i + 1] - e0;
+    // the in-edge arrays are streamed once: non-temporal loads keep them
+    // from evicting the gathered source values from L2
+    for (uint32_t i = tid; i < nr; i += kBlock) s_end[i] = __builtin_nontemporal_load(&toff[r0 + i + 1]) - e0;
     for (uint32_t j = tid; j < ne; j += kBlock) {
         const uint32_t e = e0 + j;
-        s_val[j] = op.gather(e, tids[e]);
+        s_val[j] = op.gather(e, __builtin_nontemporal_load(&tids[e]));
     }
     const bool head_open = toff[r0] < e0; // first row has edges in earlier tiles
     __syncthreads();
@@ -171,7 +173,9 @@ __global__ __launch_bounds__(kBlock) void k_pull(Op op, const uint32_t *__restri
 }
 
 // Finishes rows that began in an earlier tile: row r0 of tile t when its
-// end lies in t.  Partials are combined in tile order.
+// end lies in t.  Partials are combined in tile order.  Block b also folds
+// the per-tile residuals l1_part[256b .. 256b+255] into l1_fix[b] in a fixed
+// tree, so the per-iteration check reads one value per fixup block.
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_pull_fixup(Op op, const uint32_t *__restrict__ toff,
                                                        const uint32_t *__restrict__ srow,
@@ -181,6 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_fixup(Op op, const uint32_t *__
                                                        const typename Op::acc_t *__restrict__ head_val,
                                                        uint32_t tiles,
                                                        unsigned long long *__restrict__ counters,
+                                                       const double *__restrict__ l1_part,
                                                        double *__restrict__ l1_fix)
 {
     using A = typename Op::acc_t;
@@ -197,30 +202,48 @@ __global__ __launch_bounds__(kBlock) void k_pull_fixup(Op op, const uint32_t *__
             acc = op.combine(acc, head_val[t]);
             op.finish(r0, acc, st);
         }
+        if (l1_part) st.l1 += l1_part[t];
     }
     double l1 = block_stats(st, counters);
     if (l1_fix && threadIdx.x == 0) l1_fix[blockIdx.x] = l1;
 }
 
 template <class Op>
-int launch_pull(Ctx *c, Graph *g, const Op &op, unsigned long long *counters, double *l1_part,
+int launch_pull(Ctx *c, const CscView &v, const Op &op, unsigned long long *counters, double *l1_part,
                 double *l1_fix, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr)
 {
     using A = typename Op::acc_t;
     static_assert(sizeof(A) <= 8, "carry buffers hold 8-byte partials");
-    PullSched &s = g->pull;
+    PullSched &s = *v.sched;
     if (ev_a) GG_HIP(hipEventRecord(ev_a, c->stream));
     k_pull<Op><<<s.tiles, kBlock, 0, c->stream>>>(
-        op, (const uint32_t *)g->toff.p, (const uint32_t *)g->tids.p, (const uint32_t *)s.start_row.p,
-        (const uint32_t *)s.start_edge.p, (uint32_t *)s.carry_row.p, (A *)s.carry_val.p,
-        (A *)s.head_val.p, counters, l1_part);
+        op, v.toff, v.tids, (const uint32_t *)s.start_row.p, (const uint32_t *)s.start_edge.p,
+        (uint32_t *)s.carry_row.p, (A *)s.carry_val.p, (A *)s.head_val.p, counters, l1_part);
     if (ev_b) GG_HIP(hipEventRecord(ev_b, c->stream));
     k_pull_fixup<Op><<<blocks_for(s.tiles, kBlock), kBlock, 0, c->stream>>>(
-        op, (const uint32_t *)g->toff.p, (const uint32_t *)s.start_row.p,
-        (const uint32_t *)s.start_edge.p, (const uint32_t *)s.carry_row.p, (const A *)s.carry_val.p,
-        (const A *)s.head_val.p, s.tiles, counters, l1_fix);
+        op, v.toff, (const uint32_t *)s.start_row.p, (const uint32_t *)s.start_edge.p,
+        (const uint32_t *)s.carry_row.p, (const A *)s.carry_val.p, (const A *)s.head_val.p, s.tiles, counters,
+        l1_part, l1_fix);
     GG_HIP(hipGetLastError());
     return GG_OK;
+}
+
+inline CscView graph_view(Graph *g)
+{
+    CscView v;
+    v.toff = (const uint32_t *)g->toff.p;
+    v.tids = (const uint32_t *)g->tids.p;
+    v.sched = &g->pull;
+    return v;
+}
+
+inline CscView plan_view(Graph *g)
+{
+    CscView v;
+    v.toff = (const uint32_t *)g->plan.toff.p;
+    v.tids = (const uint32_t *)g->plan.tids.p;
+    v.sched = &g->plan.sched;
+    return v;
 }
 
 // BFS level bookkeeping carried through edgemap_core by the fused loops.

@@ -721,12 +721,12 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             if (f32) {
                 k_pr_contrib<float><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
                     (const float *)op->a, outdeg, F, bit, (float *)tmpv.p, n);
-                rc = launch_pull(c, g, OpPRSum<float>{(const float *)tmpv.p, (float *)op->b, next, outdeg},
+                rc = launch_pull(c, graph_view(g), OpPRSum<float>{(const float *)tmpv.p, (float *)op->b, next, outdeg},
                                  cnt, nullptr, nullptr);
             } else {
                 k_pr_contrib<double><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
                     (const double *)op->a, outdeg, F, bit, (double *)tmpv.p, n);
-                rc = launch_pull(c, g, OpPRSum<double>{(const double *)tmpv.p, (double *)op->b, next, outdeg},
+                rc = launch_pull(c, graph_view(g), OpPRSum<double>{(const double *)tmpv.p, (double *)op->b, next, outdeg},
                                  cnt, nullptr, nullptr);
             }
             break;
@@ -735,18 +735,18 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
             if (bit) {
                 if (f32)
-                    rc = launch_pull(c, g, OpSpmvBit<float>{(const float *)op->a, tw, (float *)op->b, next, outdeg,
+                    rc = launch_pull(c, graph_view(g), OpSpmvBit<float>{(const float *)op->a, tw, (float *)op->b, next, outdeg,
                                                             (const uint32_t *)g->indeg.p},
                                      cnt, nullptr, nullptr);
                 else
-                    rc = launch_pull(c, g, OpSpmvBit<double>{(const double *)op->a, tw, (double *)op->b, next,
+                    rc = launch_pull(c, graph_view(g), OpSpmvBit<double>{(const double *)op->a, tw, (double *)op->b, next,
                                                              outdeg, (const uint32_t *)g->indeg.p},
                                      cnt, nullptr, nullptr);
             } else {
                 if (!f32)
                     return bail(fail(GG_ERR_UNSUPPORTED,
                                      "edgeMap SPMV with a partial frontier supports F32 values only"));
-                rc = launch_pull(c, g, OpSpmvFrontier<float>{(const float *)op->a, tw, bm, (float *)op->b, next,
+                rc = launch_pull(c, graph_view(g), OpSpmvFrontier<float>{(const float *)op->a, tw, bm, (float *)op->b, next,
                                                              outdeg},
                                  cnt, nullptr, nullptr);
             }
@@ -764,7 +764,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
                 if ((rc = buf_alloc(c, tmpv, 4ull * n))) return bail(rc);
                 k_cc_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)op->b, F, bit,
                                                                             (uint32_t *)tmpv.p, n);
-                rc = launch_pull(c, g, OpCCMin{(const uint32_t *)tmpv.p, (uint32_t *)op->a,
+                rc = launch_pull(c, graph_view(g), OpCCMin{(const uint32_t *)tmpv.p, (uint32_t *)op->a,
                                                (const uint32_t *)op->b, next, outdeg},
                                  cnt, nullptr, nullptr);
             }

@@ -96,6 +96,13 @@ int gg_graph_degrees(gg_graph_t g, int in_edges, uint32_t *out);
 /* Replaces the int weights of a graph (CSR order, m FP32 values). */
 int gg_graph_set_weights(gg_graph_t g, const float *w_csr);
 
+/* Internal vertex renumbering for the pull kernels (no reference
+ * counterpart in the engine; the reference reorders offline with VEBO,
+ * graphtools/VEBO.C, and '-o').  mode 1: vertices by out-degree descending,
+ * so the sources most edges read sit together in L2; mode 0 drops it.
+ * Kernels that use it map results back to the caller's numbering. */
+int gg_graph_reorder(gg_graph_t g, int mode);
+
 /* Device partitioner: partitioned_graph ctor (graph-numa.h:642-685) with
  * partitionByDegree/partitionByVertex (1177-1206, 1287-1357),
  * cscpartitionByDegree/ByVertex (1208-1285), COOPartitionByDest

@@ -144,6 +144,27 @@ def test_pagerank_fp64_fp32_vs_oracle(ctx, scale, P):
     g.close()
 
 
+@pytest.mark.parametrize("scale,P", [(12, 64), (20, 384)])
+def test_pagerank_reordered_plan(ctx, scale, P):
+    """The hot-sources-first renumbering changes only summation order."""
+    print(f"pagerank reordered scale={scale}", flush=True)
+    off, ids = oracle.rmat_csr(scale, 16, 1)
+    og = oracle.Graph(off, ids, P=P)
+    want, it, _ = og.pagerank(10)
+    g = _dev_graph(ctx, off, ids).partition(P).reorder(1)
+    r64, info = gg.pagerank(g, np.float64)
+    assert info.iters == it
+    assert R.rel_l1(r64.to_host(), want) < TOL64
+    r32, _ = gg.pagerank(g, np.float32)
+    assert R.rel_l1(r32.to_host(), want) < TOL32
+    again, _ = gg.pagerank(g, np.float64)
+    assert np.array_equal(again.to_host().view(np.uint64), r64.to_host().view(np.uint64))
+    g.reorder(0)
+    plain, _ = gg.pagerank(g, np.float64)
+    assert R.rel_l1(plain.to_host(), want) < TOL64
+    g.close()
+
+
 def oracle_path(p):
     return {oracle.PATH_SPARSE: "sparse", oracle.PATH_CSC: "csc", oracle.PATH_COO: "coo"}[p]
 
