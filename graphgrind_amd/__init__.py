@@ -53,6 +53,12 @@ class VOp(C.Structure):
                 ("c0", C.c_double), ("c1", C.c_double)]
 
 
+class GraphView(C.Structure):
+    _fields_ = [("toff", C.c_void_p), ("tids", C.c_void_p), ("outdeg", C.c_void_p), ("perm", C.c_void_p),
+                ("tile_row", C.c_void_p), ("tile_edge", C.c_void_p), ("tiles", C.c_uint32),
+                ("n", C.c_uint32), ("m", C.c_uint64)]
+
+
 class RunInfo(C.Structure):
     _fields_ = [("iters", C.c_int32), ("paths", C.c_int32 * 64), ("edges_scanned", C.c_uint64),
                 ("ms", C.c_double)]
@@ -93,6 +99,7 @@ SIGNATURES = {
     "gg_graph_set_weights": (C.c_int, [_VP, _VP]),
     "gg_graph_partition": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int]),
     "gg_graph_reorder": (C.c_int, [_VP, C.c_int]),
+    "gg_graph_device_view": (C.c_int, [_VP, C.c_int, C.POINTER(GraphView)]),
     "gg_graph_partition_starts": (C.c_int, [_VP, C.c_int, _VP, _IP]),
     "gg_graph_coo": (C.c_int, [_VP, _VP, _VP, _VP]),
     "gg_frontier_create": (C.c_int, [_VP, C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -262,6 +269,11 @@ class Graph:
         """Internal hot-sources-first renumbering used by the pull kernels."""
         check(lib().gg_graph_reorder(self.h, mode))
         return self
+
+    def device_view(self, which=0):
+        v = GraphView()
+        check(lib().gg_graph_device_view(self.h, which, C.byref(v)))
+        return v
 
     def starts(self, which=0):
         s = np.zeros(self.P + 1, np.uint64)

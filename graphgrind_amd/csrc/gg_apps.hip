@@ -35,25 +35,41 @@ struct OpPRFused {
     __device__ V identity() const { return -V(0); }
     __device__ V combine(V a, V b) const { return a + b; }
     __device__ V gather(uint32_t, uint32_t s) const { return contrib_in[s]; }
-    __device__ void finish(uint32_t d, V acc, LocalStats &st) const
+    // per-destination inputs, prefetched with the gathers; every per-vertex
+    // array is touched once per iteration, so it is streamed past L2 to
+    // keep the gathered contributions resident
+    struct row_t {
+        V x, c;
+        uint32_t od;
+        uint8_t f;
+    };
+    __device__ row_t load_row(uint32_t d) const
     {
-        // every per-vertex array is touched once per iteration: stream it
-        // past L2 so the gathered contributions stay resident
-        V xv = __builtin_nontemporal_load(&x[d]);
+        row_t r;
+        r.x = __builtin_nontemporal_load(&x[d]);
+        r.c = __builtin_nontemporal_load(&c[d]);
+        r.od = __builtin_nontemporal_load(&outdeg[d]);
+        // unconditional load (a branch here would drain the load queue);
+        // Fin always points at a valid array, 'bit' forces membership
+        r.f = __builtin_nontemporal_load(&Fin[d]) | (uint8_t)bit;
+        return r;
+    }
+    __device__ void store_row(uint32_t d, const row_t &r, V acc, LocalStats &st) const
+    {
+        V xv = r.x;
         const bool any = !signbit(acc);
         if (any) xv = xv + acc;
-        const bool inF = bit || __builtin_nontemporal_load(&Fin[d]);
+        const bool inF = r.f != 0;
         if (inF) xv = damping * xv + added;
-        const V diff = fabs(__builtin_nontemporal_load(&c[d]) - xv);
+        const V diff = fabs(r.c - xv);
         st.l1 += (double)diff;
         __builtin_nontemporal_store(inF ? V(0) : diff, &y[d]);
         __builtin_nontemporal_store(xv, &x[d]);
         __builtin_nontemporal_store((uint8_t)any, &Fout[d]);
-        const uint32_t od = __builtin_nontemporal_load(&outdeg[d]);
-        __builtin_nontemporal_store((any && od) ? xv / (V)od : -V(0), &contrib_out[d]);
+        __builtin_nontemporal_store((any && r.od) ? xv / (V)r.od : -V(0), &contrib_out[d]);
         if (any) {
             st.active++;
-            st.out_edges += od;
+            st.out_edges += r.od;
         }
     }
 };
@@ -155,6 +171,7 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
     fill_bytes(c, c->counters.p, 64, 0);
     k_pr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)B.R[0].p, (V *)B.R[1].p, (V *)B.K[0].p,
                                                                    outdeg, n, one_over_n);
+    fill_bytes(c, B.F[1].p, n, 1);
     for (int it = 1; it <= max_iter; it++) {
         OpPRFused<V> op;
         op.c = (const V *)B.R[(it - 1) % 3].p;
@@ -163,7 +180,8 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
         op.contrib_in = (const V *)B.K[(it - 1) % 2].p;
         op.contrib_out = (V *)B.K[it % 2].p;
         op.bit = it == 1;
-        op.Fin = it == 1 ? nullptr : (const uint8_t *)B.F[(it - 2) % 2].p;
+        // iteration 1 runs on bits(): its flags are not read (bit forces them)
+        op.Fin = (const uint8_t *)B.F[it == 1 ? 1 : (it - 2) % 2].p;
         op.Fout = (uint8_t *)B.F[(it - 1) % 2].p;
         op.outdeg = outdeg;
         op.damping = (V)damping;

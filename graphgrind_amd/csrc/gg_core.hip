@@ -1246,6 +1246,25 @@ extern "C" int gg_graph_reorder(gg_graph_t g, int mode)
     return GG_OK;
 }
 
+extern "C" int gg_graph_device_view(gg_graph_t g, int which, gg_graph_view *out)
+{
+    if (!g || !out) return fail(GG_ERR_INVALID, "gg_graph_device_view: NULL");
+    if (which != 0 && which != 1) return fail(GG_ERR_INVALID, "gg_graph_device_view: which must be 0 or 1");
+    if (which == 1 && !g->plan.mode) return fail(GG_ERR_STATE, "gg_graph_device_view: no renumbered plan");
+    const bool p = which == 1;
+    out->toff = (const uint32_t *)(p ? g->plan.toff.p : g->toff.p);
+    out->tids = (const uint32_t *)(p ? g->plan.tids.p : g->tids.p);
+    out->outdeg = (const uint32_t *)(p ? g->plan.outdeg.p : g->outdeg.p);
+    out->perm = p ? (const uint32_t *)g->plan.perm.p : nullptr;
+    const PullSched &s = p ? g->plan.sched : g->pull;
+    out->tile_row = (const uint32_t *)s.start_row.p;
+    out->tile_edge = (const uint32_t *)s.start_edge.p;
+    out->tiles = s.tiles;
+    out->n = g->n;
+    out->m = g->m;
+    return GG_OK;
+}
+
 extern "C" int gg_last_kernel_time(gg_ctx_t c, double *avg_ms, int *launches)
 {
     if (!c) return fail(GG_ERR_INVALID, "gg_last_kernel_time: NULL");

@@ -1,6 +1,8 @@
 // gg_kernels.h -- device kernel templates shared by the traversal and app
 // translation units.  Included by .hip files only.
 #pragma once
+#include <type_traits>
+
 #include "gg_internal.h"
 
 namespace gg {
@@ -67,17 +69,56 @@ __device__ __forceinline__ bool bit_test(const uint32_t *bm, uint32_t v)
     return (bm[v >> 5] >> (v & 31)) & 1u;
 }
 
+// Optional per-row prefetch.  An operator that defines row_t, load_row(d)
+// and store_row(d, row, acc, stats) has its per-destination inputs loaded in
+// the staging phase together with the gathers (so the epilogue waits on no
+// memory); other operators are finished through finish(d, acc, stats).
+struct NoRow {};
+template <class Op, class = void>
+struct has_row : std::false_type {};
+template <class Op>
+struct has_row<Op, std::void_t<typename Op::row_t>> : std::true_type {};
+template <class Op, bool = has_row<Op>::value>
+struct row_of {
+    using type = NoRow;
+};
+template <class Op>
+struct row_of<Op, true> {
+    using type = typename Op::row_t;
+};
+
+template <class Op>
+__device__ __forceinline__ typename row_of<Op>::type row_load(const Op &op, uint32_t d)
+{
+    if constexpr (has_row<Op>::value) return op.load_row(d);
+    else return NoRow{};
+}
+
+template <class Op, class R, class A>
+__device__ __forceinline__ void row_store(const Op &op, uint32_t d, const R &r, A acc, LocalStats &st)
+{
+    if constexpr (has_row<Op>::value) op.store_row(d, r, acc, st);
+    else op.finish(d, acc, st);
+}
+
 // ---------------------------------------------------------------------------
 // Merge-path pull over an in-edge (CSC-like) array: every destination row is
 // owned by exactly one work-group, so per-row results have a fixed
 // combination order (no floating-point atomics).  Tile t covers kPullIPB
-// consecutive items of the merge of row ends and edges.  Gathered values
-// are staged in LDS, each thread walks kPullIPT items, and a block-wide
-// segmented scan joins rows that cross thread boundaries.  Rows crossing
-// tile boundaries are finished by k_pull_fixup from per-tile carries.
+// consecutive items of the merge of row ends and edges.
+//   1. staging: each thread issues its index loads, its per-row prefetches
+//      and then its gathers (kPullIPT independent requests per lane), and
+//      parks gathered values and row ends in LDS;
+//   2. each thread walks kPullIPT consecutive merge items; a block-wide
+//      segmented scan joins rows that cross thread boundaries; row sums go
+//      to registers, then to LDS;
+//   3. epilogue: rows are finished in a coalesced pass, row i by thread
+//      i mod kBlock, with its prefetched inputs.
+// Rows crossing tile boundaries are finished by k_pull_fixup from per-tile
+// carries.
 //
 // Op: acc_t, identity(), combine(earlier, later), gather(edge, src),
-//     finish(row, acc, LocalStats&).
+//     finish(row, acc, LocalStats&) [or row_t/load_row/store_row].
 // ---------------------------------------------------------------------------
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_pull(Op op, const uint32_t *__restrict__ toff,
@@ -91,6 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(Op op, const uint32_t *__restri
                                                  double *__restrict__ l1_part)
 {
     using A = typename Op::acc_t;
+    using R = typename row_of<Op>::type;
     __shared__ A s_val[kPullIPB];
     __shared__ uint32_t s_end[kPullIPB];
     __shared__ uint32_t s_key[kBlock];
@@ -100,18 +142,48 @@ __global__ __launch_bounds__(kBlock) void k_pull(Op op, const uint32_t *__restri
     const uint32_t r0 = srow[t], r1 = srow[t + 1];
     const uint32_t e0 = sedge[t], e1 = sedge[t + 1];
     const uint32_t nr = r1 - r0, ne = e1 - e0;
-    // the in-edge arrays are streamed once: non-temporal loads keep them
-    // from evicting the gathered source values from L2
-    for (uint32_t i = tid; i < nr; i += kBlock) s_end[i] = __builtin_nontemporal_load(&toff[r0 + i + 1]) - e0;
-    for (uint32_t j = tid; j < ne; j += kBlock) {
-        const uint32_t e = e0 + j;
-        s_val[j] = op.gather(e, __builtin_nontemporal_load(&tids[e]));
-    }
     const bool head_open = toff[r0] < e0; // first row has edges in earlier tiles
+    // 1. staging (the in-edge arrays are streamed once: non-temporal loads
+    //    keep them from evicting the gathered source values from L2)
+    // Every load below is unconditional, with its index clamped into the
+    // tile: a load under a divergent branch makes the compiler wait for it at
+    // the branch join, which would serialise the round trips.  Only the
+    // block-uniform emptiness tests branch.
+    R rin[kPullIPT];
+    {
+        const uint32_t nel = ne ? ne - 1 : 0u, nrl = nr ? nr - 1 : 0u;
+        uint32_t src[kPullIPT];
+        uint32_t rend[kPullIPT];
+        A val[kPullIPT];
+        if (ne) {
+#pragma unroll
+            for (int k = 0; k < kPullIPT; k++)
+                src[k] = __builtin_nontemporal_load(&tids[e0 + min(tid + k * kBlock, nel)]);
+        }
+        if (nr) {
+#pragma unroll
+            for (int k = 0; k < kPullIPT; k++) {
+                const uint32_t jr = min(tid + k * kBlock, nrl);
+                rend[k] = __builtin_nontemporal_load(&toff[r0 + jr + 1]);
+                rin[k] = row_load(op, r0 + jr);
+            }
+        }
+        if (ne) {
+#pragma unroll
+            for (int k = 0; k < kPullIPT; k++) val[k] = op.gather(e0 + min(tid + k * kBlock, nel), src[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kPullIPT; k++) {
+            const uint32_t j = tid + k * kBlock;
+            if (j < ne) s_val[j] = val[k];
+            if (j < nr) s_end[j] = rend[k] - e0;
+        }
+    }
     __syncthreads();
+    // 2. merge-path walk
     const uint32_t tot = nr + ne;
     const uint32_t diag = min(tid * (uint32_t)kPullIPT, tot);
-    const uint32_t dend = min(diag + (uint32_t)kPullIPT, tot);
+    const uint32_t nitems = min(diag + (uint32_t)kPullIPT, tot) - diag;
     uint32_t lo = diag > ne ? diag - ne : 0u, hi = min(diag, nr);
     while (lo < hi) {
         const uint32_t p = (lo + hi) >> 1;
@@ -119,16 +191,18 @@ __global__ __launch_bounds__(kBlock) void k_pull(Op op, const uint32_t *__restri
         else hi = p;
     }
     const uint32_t ri0 = lo, ej0 = diag - lo;
-    // pass A: carry-out of this thread
     uint32_t ri = ri0, ej = ej0;
     A acc = op.identity();
-    for (uint32_t d = diag; d < dend; d++) {
-        if (ri < nr && ej >= s_end[ri]) {
-            acc = op.identity();
-            ri++;
-        } else {
-            acc = op.combine(acc, s_val[ej]);
-            ej++;
+#pragma unroll
+    for (int k = 0; k < kPullIPT; k++) {
+        if ((uint32_t)k < nitems) {
+            if (ri < nr && ej >= s_end[ri]) {
+                acc = op.identity();
+                ri++;
+            } else {
+                acc = op.combine(acc, s_val[ej]);
+                ej++;
+            }
         }
     }
     s_key[tid] = ri;
@@ -149,24 +223,45 @@ __global__ __launch_bounds__(kBlock) void k_pull(Op op, const uint32_t *__restri
     }
     A run = op.identity();
     if (tid > 0 && s_key[tid - 1] == ri0) run = s_sv[tid - 1];
-    // pass B: finish rows that end in this thread's items
-    LocalStats st;
+    // pass B: row sums of the rows ending in this thread's items
+    uint32_t rloc[kPullIPT];
+    A rsum[kPullIPT];
     ri = ri0;
     ej = ej0;
-    for (uint32_t d = diag; d < dend; d++) {
-        if (ri < nr && ej >= s_end[ri]) {
-            if (ri == 0 && head_open) head_val[t] = run;
-            else op.finish(r0 + ri, run, st);
-            run = op.identity();
-            ri++;
-        } else {
-            run = op.combine(run, s_val[ej]);
-            ej++;
+#pragma unroll
+    for (int k = 0; k < kPullIPT; k++) {
+        rloc[k] = 0xffffffffu;
+        rsum[k] = op.identity();
+        if ((uint32_t)k < nitems) {
+            if (ri < nr && ej >= s_end[ri]) {
+                rloc[k] = ri;
+                rsum[k] = run;
+                run = op.identity();
+                ri++;
+            } else {
+                run = op.combine(run, s_val[ej]);
+                ej++;
+            }
         }
     }
-    if (diag < dend && dend == tot) {
+    if (nitems && diag + nitems == tot) {
         carry_row[t] = r0 + ri;
         carry_val[t] = run;
+    }
+    __syncthreads(); // s_val is free: reuse it for the row sums
+#pragma unroll
+    for (int k = 0; k < kPullIPT; k++)
+        if (rloc[k] != 0xffffffffu) s_val[rloc[k]] = rsum[k];
+    __syncthreads();
+    // 3. coalesced epilogue
+    LocalStats st;
+#pragma unroll
+    for (int k = 0; k < kPullIPT; k++) {
+        const uint32_t i = tid + k * kBlock;
+        if (i < nr) {
+            if (i == 0 && head_open) head_val[t] = s_val[0];
+            else row_store(op, r0 + i, rin[k], s_val[i], st);
+        }
     }
     double l1 = block_stats(st, counters);
     if (l1_part && tid == 0) l1_part[t] = l1;
@@ -200,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_fixup(Op op, const uint32_t *__
             A acc = op.identity();
             for (uint32_t k = tb; k < t; k++) acc = op.combine(acc, carry_val[k]);
             acc = op.combine(acc, head_val[t]);
-            op.finish(r0, acc, st);
+            row_store(op, r0, row_load(op, r0), acc, st);
         }
         if (l1_part) st.l1 += l1_part[t];
     }

@@ -103,6 +103,24 @@ int gg_graph_set_weights(gg_graph_t g, const float *w_csr);
  * Kernels that use it map results back to the caller's numbering. */
 int gg_graph_reorder(gg_graph_t g, int mode);
 
+/* Raw device addresses of a graph's layout, for interop (no reference
+ * counterpart: the reference's arrays are host memory its apps read
+ * directly).  which 0: the graph as given; which 1: the renumbered plan of
+ * gg_graph_reorder (perm = new id of each old vertex).  Pointers stay valid
+ * until the graph is destroyed or re-planned. */
+typedef struct {
+    const uint32_t *toff;       /* n+1 in-edge offsets          */
+    const uint32_t *tids;       /* m in-edge sources            */
+    const uint32_t *outdeg;     /* n out-degrees                */
+    const uint32_t *perm;       /* n (which 1) or NULL          */
+    const uint32_t *tile_row;   /* tiles+1 merge-path row starts  */
+    const uint32_t *tile_edge;  /* tiles+1 merge-path edge starts */
+    uint32_t tiles;
+    uint32_t n;
+    uint64_t m;
+} gg_graph_view;
+int gg_graph_device_view(gg_graph_t g, int which, gg_graph_view *out);
+
 /* Device partitioner: partitioned_graph ctor (graph-numa.h:642-685) with
  * partitionByDegree/partitionByVertex (1177-1206, 1287-1357),
  * cscpartitionByDegree/ByVertex (1208-1285), COOPartitionByDest
