@@ -1,0 +1,812 @@
+// gg/ligra-numa.h -- drop-in C++ front end of the MI355X traversal engine.
+//
+// Application sources written against GraphGrind's app-facing API
+// (graphgrind/ligra-numa.h, graph-numa.h, partitioner.h, mm.h, utils.h)
+// compile against this header unchanged: build with -I include/gg and
+// link -lgraphgrind_amd.  The traversal runs on the GPU through the C ABI
+// of include/graphgrind_amd.h; nothing here launches kernels itself.
+//
+// How the reference API maps onto the device:
+//  - edgeMap / vertexMap take the app's functor structs.  A functor is
+//    recognised by its data members (the reference functors are plain
+//    structs of arrays) and becomes an enumerated device operator:
+//      Parents                       -> GG_OP_BFS       (app/BFS.C:26-75)
+//      IDs + prevIDs                 -> GG_OP_CC        (app/Components.C:26-86)
+//      p_curr + p_next + V           -> GG_OP_PAGERANK  (app/PageRank.C:28-70)
+//      p_curr + p_next (no V)        -> GG_OP_SPMV      (app/SPMV.C:7-48)
+//    and for vertexMap
+//      damping + addedConstant       -> GG_VOP_PAGERANK (app/PageRank.C:73-87)
+//      IDs + prevIDs                 -> GG_VOP_CC_COPY  (app/Components.C:88-99)
+//      p_curr only                   -> GG_VOP_RESET    (app/PageRank.C:90-100)
+//    Other vertexMap functors run on the host over the frontier's members.
+//  - mmap_ptr<T> (mm.h) is an HBM buffer with a host shadow.  Host code
+//    (map_vertexL lambdas, operator[], get()) works on the shadow; device
+//    operators work on HBM; each side is refreshed when the other wrote
+//    last.  Those copies are the cost of running unmodified app source;
+//    the fused device loops (gg_pagerank, gg_bfs, gg_cc, gg_spmv) avoid them.
+//  - partitioned_graph / partitioned_vertices hold C-ABI handles.
+//  - intT is 32-bit here: the device engine uses uint32 ids and offsets.
+#ifndef GG_LIGRA_NUMA_H
+#define GG_LIGRA_NUMA_H
+
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <map>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../graphgrind_amd.h"
+
+using namespace std;
+
+// ---- widths and parallel loops (parallel.h:27-106) ---------------------------
+typedef int intT;
+typedef unsigned int uintT;
+typedef int intE;
+typedef unsigned int uintE;
+#define INT_T_MAX INT32_MAX
+#define UINT_T_MAX UINT32_MAX
+#ifdef _OPENMP
+#include <omp.h>
+#define parallel_for _Pragma("omp parallel for") for
+#else
+#define parallel_for for
+#endif
+#define parallel_main main
+#define cilk_spawn
+#define cilk_sync
+
+namespace gg_front {
+
+[[noreturn]] inline void die(const char *what)
+{
+    std::cerr << what << ": " << gg_last_error() << std::endl;
+    std::abort();
+}
+
+inline void check(int rc, const char *what)
+{
+    if (rc != GG_OK) die(what);
+}
+
+inline gg_ctx_t &context()
+{
+    static gg_ctx_t ctx = nullptr;
+    if (!ctx) {
+        const char *dev = getenv("GG_DEVICE");
+        check(gg_ctx_create(dev ? atoi(dev) : 0, &ctx), "gg_ctx_create");
+    }
+    return ctx;
+}
+
+// ---- host/device coherence of mmap_ptr buffers ---------------------------------
+struct Buffer {
+    void *host = nullptr;
+    void *dev = nullptr;
+    size_t bytes = 0;
+    enum State { HOST, DEVICE, BOTH } state = HOST;
+};
+
+inline std::map<const void *, Buffer *> &registry()
+{
+    static std::map<const void *, Buffer *> r;
+    return r;
+}
+
+inline void to_host(Buffer *b)
+{
+    if (b->state == Buffer::DEVICE) {
+        check(gg_memcpy_d2h(context(), b->host, b->dev, b->bytes), "download");
+        b->state = Buffer::BOTH;
+    }
+}
+
+inline void to_device(Buffer *b)
+{
+    if (b->state == Buffer::HOST) {
+        check(gg_memcpy_h2d(context(), b->dev, b->host, b->bytes), "upload");
+        b->state = Buffer::BOTH;
+    }
+}
+
+// Every host phase may read and write any buffer.
+inline void host_phase_begin()
+{
+    for (auto &kv : registry()) to_host(kv.second);
+}
+inline void host_phase_end()
+{
+    for (auto &kv : registry()) kv.second->state = Buffer::HOST;
+}
+
+// Device address of an app array (a host shadow pointer handed to a functor).
+inline void *device_of(const void *host, bool written)
+{
+    auto it = registry().find(host);
+    if (it == registry().end()) die("device operator on an array not allocated with mmap_ptr");
+    Buffer *b = it->second;
+    to_device(b);
+    if (written) b->state = Buffer::DEVICE;
+    return b->dev;
+}
+
+} // namespace gg_front
+
+// ---- partitioner (partitioner.h:38-200) -----------------------------------------
+class partitioner
+{
+    std::vector<intT> starts_;
+    intT elements_ = 0;
+
+public:
+    partitioner() {}
+    partitioner(intT P, intT elements) : starts_(P + 1, 0), elements_(elements) {}
+    void set_starts(const std::vector<uint64_t> &s)
+    {
+        starts_.assign(s.begin(), s.end());
+    }
+    int get_num_per_node_partitions() const { return get_num_partitions(); }
+    int get_num_partitions() const { return (int)starts_.size() - 1; }
+    intT get_num_elements() const { return elements_; }
+    intT start_of(intT p) const { return starts_[p]; }
+    intT get_size(intT p) const { return starts_[p + 1] - starts_[p]; }
+    intT partition_of(intT v) const
+    {
+        return (intT)(std::upper_bound(starts_.begin(), starts_.end(), v) - starts_.begin()) - 1;
+    }
+    intT offset_of(intT v) const { return v - starts_[partition_of(v)]; }
+};
+
+// ---- mmap_ptr<T> (mm.h:246-325) ---------------------------------------------------
+template <typename T>
+class mmap_ptr
+{
+    gg_front::Buffer *buf_ = nullptr;
+
+public:
+    mmap_ptr() {}
+    explicit mmap_ptr(const partitioner &part) { part_allocate(part); }
+    explicit mmap_ptr(size_t elements) { Interleave_allocate(elements); }
+    mmap_ptr(size_t elements, size_t) { Interleave_allocate(elements); }
+
+    void part_allocate(const partitioner &part) { Interleave_allocate(part.get_num_elements()); }
+    void local_allocate(size_t elements, int) { Interleave_allocate(elements); }
+    void Interleave_allocate(size_t elements)
+    {
+        buf_ = new gg_front::Buffer();
+        buf_->bytes = std::max<size_t>(elements, 1) * sizeof(T);
+        buf_->host = ::operator new(buf_->bytes);
+        gg_front::check(gg_malloc(gg_front::context(), buf_->bytes, &buf_->dev), "gg_malloc");
+        buf_->state = gg_front::Buffer::HOST;
+        gg_front::registry()[buf_->host] = buf_;
+    }
+    void del()
+    {
+        if (!buf_) return;
+        gg_front::registry().erase(buf_->host);
+        gg_free(gg_front::context(), buf_->dev);
+        ::operator delete(buf_->host);
+        delete buf_;
+        buf_ = nullptr;
+    }
+    // Host view: refreshed from HBM when a device operator wrote last, and
+    // considered written afterwards.
+    operator T *()
+    {
+        if (!buf_) return nullptr;
+        gg_front::to_host(buf_);
+        buf_->state = gg_front::Buffer::HOST;
+        return reinterpret_cast<T *>(buf_->host);
+    }
+    operator const T *() const
+    {
+        if (!buf_) return nullptr;
+        gg_front::to_host(buf_);
+        return reinterpret_cast<const T *>(buf_->host);
+    }
+    T *get() const
+    {
+        if (!buf_) return nullptr;
+        gg_front::to_host(buf_);
+        buf_->state = gg_front::Buffer::HOST;
+        return reinterpret_cast<T *>(buf_->host);
+    }
+    explicit operator bool() const { return buf_ != nullptr; }
+    size_t get_bytes() const { return buf_ ? buf_->bytes : 0; }
+    T *device() const { return reinterpret_cast<T *>(gg_front::device_of(buf_->host, false)); }
+};
+
+// ---- vertex types (graph-numa.h:39-217): degrees only on the host ------------------
+class symmetricVertex
+{
+    intT degree = 0;
+
+public:
+    symmetricVertex() {}
+    intT getInDegree() const { return degree; }
+    intT getOutDegree() const { return degree; }
+    void setInDegree(intT d) { degree = d; }
+    void setOutDegree(intT d) { degree = d; }
+    void flipEdges() {}
+};
+
+class asymmetricVertex
+{
+    intT inDegree = 0, outDegree = 0;
+
+public:
+    asymmetricVertex() {}
+    intT getInDegree() const { return inDegree; }
+    intT getOutDegree() const { return outDegree; }
+    void setInDegree(intT d) { inDegree = d; }
+    void setOutDegree(intT d) { outDegree = d; }
+    void flipEdges() { std::swap(inDegree, outDegree); }
+};
+
+// graph<vertex> (graph-numa.h:560-613): the vertex table apps index (V[v]).
+template <class vertex>
+class graph
+{
+public:
+    intT n = 0, m = 0;
+    vertex *V = nullptr;
+    partitioner csc;
+    const partitioner &get_csc_partitioner() const { return csc; }
+};
+
+// partitioned_graph (graph-numa.h:626-756) holding the device graph.
+template <class vertex>
+class partitioned_graph
+{
+public:
+    typedef vertex vertex_type;
+    intT n = 0, m = 0;
+    bool source = false, part_ver = false, part_relabel = false;
+    gg_graph_t h = nullptr;
+    partitioner coo_partition;
+    graph<vertex> *WG = nullptr;
+
+    partitioned_graph() {}
+    // Takes ownership of a device graph and partitions it ('-c', '-v', '-o').
+    partitioned_graph(gg_graph_t g, int P, bool by_source, bool by_vertex, bool relabel)
+        : source(by_source), part_ver(by_vertex), part_relabel(relabel), h(g)
+    {
+        if (by_source) {
+            std::cerr << "-P source: only destination partitioning is provided" << std::endl;
+            std::abort();
+        }
+        uint32_t nn;
+        uint64_t mm;
+        gg_front::check(gg_graph_info(g, &nn, &mm, nullptr, nullptr), "gg_graph_info");
+        n = (intT)nn;
+        m = (intT)mm;
+        gg_front::check(gg_graph_partition(g, P, by_vertex, relabel, 1), "gg_graph_partition");
+        std::vector<uint64_t> s(P + 1);
+        coo_partition = partitioner(P, n);
+        gg_front::check(gg_graph_partition_starts(g, 0, s.data(), nullptr), "partition starts");
+        coo_partition.set_starts(s);
+        WG = new graph<vertex>();
+        WG->n = n;
+        WG->m = m;
+        WG->V = new vertex[n];
+        std::vector<uint32_t> od(n), id(n);
+        gg_front::check(gg_graph_degrees(g, 0, od.data()), "out-degrees");
+        gg_front::check(gg_graph_degrees(g, 1, id.data()), "in-degrees");
+        for (intT v = 0; v < n; v++) {
+            WG->V[v].setInDegree((intT)id[v]);
+            WG->V[v].setOutDegree((intT)od[v]);
+        }
+        uint64_t nc = 0;
+        gg_front::check(gg_graph_partition_starts(g, 1, s.data(), nullptr), "csc starts");
+        nc = s[P];
+        WG->csc = partitioner(P, (intT)nc);
+        WG->csc.set_starts(s);
+    }
+    void del()
+    {
+        if (h) gg_graph_destroy(h);
+        h = nullptr;
+        if (WG) {
+            delete[] WG->V;
+            delete WG;
+            WG = nullptr;
+        }
+    }
+    graph<vertex> &get_partition() { return *WG; }
+    const partitioner &get_partitioner() const { return coo_partition; }
+    const partitioner &get_coo_partitioner() const { return coo_partition; }
+    int get_num_coo_partitions() const { return coo_partition.get_num_partitions(); }
+    bool transposed() const { return false; }
+    void transpose()
+    {
+        std::cerr << "transpose(): the device graph keeps one orientation" << std::endl;
+        std::abort();
+    }
+};
+
+// ---- partitioned_vertices (ligra-numa.h:91-272) -------------------------------------
+class partitioned_vertices
+{
+public:
+    gg_frontier_t h = nullptr;
+    intT numVertices = 0;
+    intT d_m = 0;
+    intT num_out_edges = 0;
+    bool bit = false;
+    bool has_dense = false;
+
+    void refresh()
+    {
+        uint64_t dm = 0, oe = 0;
+        int dense = 0, b = 0;
+        uint32_t nn = 0;
+        gg_front::check(gg_frontier_info(h, &dm, &oe, &dense, &b, &nn), "gg_frontier_info");
+        d_m = (intT)dm;
+        num_out_edges = (intT)oe;
+        has_dense = dense;
+        bit = b;
+        numVertices = (intT)nn;
+    }
+    static partitioned_vertices wrap(gg_frontier_t f)
+    {
+        partitioned_vertices pv;
+        pv.h = f;
+        pv.refresh();
+        return pv;
+    }
+    template <class vertex>
+    static partitioned_vertices create(partitioned_graph<vertex> &GA, intT v)
+    {
+        gg_frontier_t f;
+        gg_front::check(gg_frontier_create(GA.h, GG_FRONTIER_SINGLE, (uint32_t)v, &f), "create");
+        return wrap(f);
+    }
+    // create(n, v, outdeg) / bits(part, n, m) / empty(): the graph is the
+    // one the driver loaded (one graph per process, as in the reference)
+    static partitioned_vertices create(intT n, intT v, intT);
+    static partitioned_vertices bits(const partitioner &, intT n, intT m);
+    static partitioned_vertices empty();
+    void del()
+    {
+        if (h) gg_frontier_destroy(h);
+        h = nullptr;
+    }
+    bool isEmpty() const { return d_m == 0; }
+    intT numRows() const { return numVertices; }
+    intT numNonzeros() const { return d_m; }
+    void toDense(const partitioner &)
+    {
+        gg_front::check(gg_frontier_to_dense(h), "toDense");
+        refresh();
+    }
+    void toSparse()
+    {
+        gg_front::check(gg_frontier_to_sparse(h), "toSparse");
+        refresh();
+    }
+    std::vector<uint8_t> members() const
+    {
+        std::vector<uint8_t> d((size_t)std::max<intT>(numVertices, 0));
+        if (numVertices) gg_front::check(gg_frontier_copy_dense(h, d.data()), "copy frontier");
+        return d;
+    }
+};
+
+namespace gg_front {
+inline gg_graph_t &current_graph()
+{
+    static gg_graph_t g = nullptr;
+    return g;
+}
+} // namespace gg_front
+
+inline partitioned_vertices partitioned_vertices::create(intT, intT v, intT)
+{
+    gg_frontier_t f;
+    gg_front::check(gg_frontier_create(gg_front::current_graph(), GG_FRONTIER_SINGLE, (uint32_t)v, &f), "create");
+    return wrap(f);
+}
+inline partitioned_vertices partitioned_vertices::bits(const partitioner &, intT, intT)
+{
+    gg_frontier_t f;
+    gg_front::check(gg_frontier_create(gg_front::current_graph(), GG_FRONTIER_ALL, 0, &f), "bits");
+    return wrap(f);
+}
+inline partitioned_vertices partitioned_vertices::empty()
+{
+    gg_frontier_t f;
+    gg_front::check(gg_frontier_create(gg_front::current_graph(), GG_FRONTIER_EMPTY, 0, &f), "empty");
+    return wrap(f);
+}
+
+// ---- functor traits ------------------------------------------------------------------
+namespace gg_front {
+#define GG_HAS_MEMBER(name)                                                                  \
+    template <class F, class = void>                                                         \
+    struct has_##name : std::false_type {};                                                  \
+    template <class F>                                                                       \
+    struct has_##name<F, std::void_t<decltype(std::declval<F &>().name)>> : std::true_type {};
+GG_HAS_MEMBER(Parents)
+GG_HAS_MEMBER(IDs)
+GG_HAS_MEMBER(prevIDs)
+GG_HAS_MEMBER(p_curr)
+GG_HAS_MEMBER(p_next)
+GG_HAS_MEMBER(V)
+GG_HAS_MEMBER(damping)
+GG_HAS_MEMBER(addedConstant)
+#undef GG_HAS_MEMBER
+
+template <class P>
+constexpr int dtype_of()
+{
+    using T = typename std::remove_cv<typename std::remove_pointer<P>::type>::type;
+    return std::is_same<T, float>::value ? GG_F32 : GG_F64;
+}
+
+template <class F>
+gg_op_t edge_op(F &f)
+{
+    gg_op_t op{};
+    if constexpr (has_Parents<F>::value) {
+        op.kind = GG_OP_BFS;
+        op.dtype = GG_I32;
+        op.a = device_of(f.Parents, true);
+    } else if constexpr (has_IDs<F>::value && has_prevIDs<F>::value) {
+        op.kind = GG_OP_CC;
+        op.dtype = GG_U32;
+        op.a = device_of(f.IDs, true);
+        op.b = device_of(f.prevIDs, false);
+    } else if constexpr (has_p_curr<F>::value && has_p_next<F>::value) {
+        op.kind = has_V<F>::value ? GG_OP_PAGERANK : GG_OP_SPMV;
+        op.dtype = dtype_of<decltype(f.p_curr)>();
+        op.a = device_of(f.p_curr, false);
+        op.b = device_of(f.p_next, true);
+    } else {
+        static_assert(sizeof(F) == 0, "edgeMap: functor not recognised by the device engine");
+    }
+    return op;
+}
+} // namespace gg_front
+
+enum options { DENSE, DENSE_FORWARD };
+
+inline bool cond_true(intT) { return 1; }
+
+// edgeMap (ligra-numa.h:919-1019)
+template <class F, class vertex>
+partitioned_vertices edgeMap(partitioned_graph<vertex> GA, partitioned_vertices Localfrontier, F f,
+                             intT threshold = -1, char option = DENSE, bool remDups = false)
+{
+    (void)option;
+    gg_op_t op = gg_front::edge_op(f);
+    gg_frontier_t out = nullptr;
+    int path = 0;
+    gg_front::check(gg_edgemap(GA.h, Localfrontier.h, &op, threshold, remDups, &out, &path), "edgeMap");
+    return partitioned_vertices::wrap(out);
+}
+
+// vertexMap (ligra-numa.h:1023-1049)
+template <class F>
+void vertexMap(const partitioner &part, partitioned_vertices V, F add)
+{
+    using namespace gg_front;
+    gg_vop_t op{};
+    bool device = true;
+    if constexpr (has_damping<F>::value && has_addedConstant<F>::value && has_p_next<F>::value) {
+        op.kind = GG_VOP_PAGERANK;
+        op.dtype = dtype_of<decltype(add.p_next)>();
+        op.a = device_of(add.p_next, true);
+        op.b = op.a;
+        op.c0 = add.damping;
+        op.c1 = add.addedConstant;
+    } else if constexpr (has_IDs<F>::value && has_prevIDs<F>::value) {
+        op.kind = GG_VOP_CC_COPY;
+        op.dtype = GG_U32;
+        op.a = device_of(add.IDs, false);
+        op.b = device_of(add.prevIDs, true);
+    } else if constexpr (has_p_curr<F>::value && !has_p_next<F>::value) {
+        op.kind = GG_VOP_RESET;
+        op.dtype = dtype_of<decltype(add.p_curr)>();
+        op.a = device_of(add.p_curr, true);
+    } else {
+        device = false;
+    }
+    if (device) {
+        check(gg_vertexmap(current_graph(), V.h, &op), "vertexMap");
+        return;
+    }
+    // any other functor: run it on the host over the members
+    std::vector<uint8_t> d = V.bit ? std::vector<uint8_t>() : V.members();
+    host_phase_begin();
+    for (intT j = 0; j < part.get_num_elements(); j++)
+        if (V.bit || d[j]) add(j);
+    host_phase_end();
+}
+
+// map_vertexL (partitioner.h:376-386): a host loop over all vertices.
+template <typename Fn>
+void map_vertexL(const partitioner &part, Fn fn)
+{
+    gg_front::host_phase_begin();
+    const intT n = part.get_num_elements();
+    parallel_for(intT v = 0; v < n; ++v) fn(v);
+    gg_front::host_phase_end();
+}
+
+// ---- utils.h pieces the apps reference ---------------------------------------------
+template <class ET>
+inline bool CAS(ET *ptr, ET oldv, ET newv)
+{
+    return __sync_bool_compare_and_swap(ptr, oldv, newv);
+}
+template <class ET>
+inline bool writeMin(ET *a, ET b)
+{
+    ET c;
+    bool r = 0;
+    do c = *a;
+    while (c > b && !(r = CAS(a, c, b)));
+    return r;
+}
+template <class ET>
+inline void writeAdd(ET *a, ET b)
+{
+    ET o, n;
+    do {
+        o = *a;
+        n = o + b;
+    } while (!__sync_bool_compare_and_swap(reinterpret_cast<uint64_t *>(a),
+                                          *reinterpret_cast<uint64_t *>(&o),
+                                          *reinterpret_cast<uint64_t *>(&n)));
+}
+
+namespace sequence {
+// plusReduce (utils.h:192): 1024-element serial blocks, then over the sums.
+template <class OT, class IT>
+OT plusReduce(OT *A, IT n)
+{
+    if (n <= 0) return OT(0);
+    std::vector<OT> sums;
+    for (IT s = 0; s < n; s += 1024) {
+        IT e = std::min<IT>(s + 1024, n);
+        OT r = A[s];
+        for (IT j = s + 1; j < e; j++) r = r + A[j];
+        sums.push_back(r);
+    }
+    if (sums.size() == 1) return sums[0];
+    return plusReduce(sums.data(), (IT)sums.size());
+}
+} // namespace sequence
+
+// ---- timer (gettime.h:31-143) ----------------------------------------------------------
+struct timer {
+    double total = 0;
+    std::chrono::steady_clock::time_point t0;
+    void start() { t0 = std::chrono::steady_clock::now(); }
+    double next()
+    {
+        auto t1 = std::chrono::steady_clock::now();
+        double d = std::chrono::duration<double>(t1 - t0).count();
+        total += d;
+        t0 = t1;
+        return d;
+    }
+};
+static timer _tm;
+#define startTime() _tm.start();
+#define nextTime(_string) std::cout << _string << " : " << _tm.next() << std::endl;
+#define reportAvg(r) std::cout << "Average" << " : " << _tm.total / (r) << std::endl;
+
+// ---- command line (parseCommandLine.h:32-113) -------------------------------------------
+class commandLine
+{
+    int argc;
+    char **argv;
+    std::string usage;
+
+public:
+    commandLine(int ac, char **av, std::string u) : argc(ac), argv(av), usage(u) {}
+    [[noreturn]] void badArgument()
+    {
+        std::cout << "usage: " << argv[0] << " " << usage << std::endl;
+        std::exit(0);
+    }
+    char *getArgument(int i)
+    {
+        if (argc < 2 + i) badArgument();
+        return argv[argc - 1 - i];
+    }
+    bool getOption(const std::string &o)
+    {
+        for (int i = 1; i < argc; i++)
+            if ((std::string)argv[i] == o) return true;
+        return false;
+    }
+    char *getOptionValue(const std::string &o)
+    {
+        for (int i = 1; i < argc - 1; i++)
+            if ((std::string)argv[i] == o) return argv[i + 1];
+        return nullptr;
+    }
+    long getOptionLongValue(const std::string &o, long d)
+    {
+        char *v = getOptionValue(o);
+        return v ? std::atol(v) : d;
+    }
+};
+
+// ---- input (IO.h:136-361 text PBBS format; 365-656 Galois .gr) ----------------------------
+namespace gg_front {
+struct HostCSR {
+    uint32_t n = 0;
+    uint64_t m = 0;
+    std::vector<uint32_t> off, ids;
+    std::vector<float> w;
+};
+
+inline bool read_text(const char *path, bool weighted, HostCSR &G)
+{
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return false;
+    std::vector<char> buf;
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    buf.resize((size_t)sz + 1);
+    size_t got = std::fread(buf.data(), 1, (size_t)sz, f);
+    std::fclose(f);
+    buf[got] = 0;
+    char *p = buf.data();
+    auto word = [&](std::string &out) {
+        while (*p && std::isspace((unsigned char)*p)) p++;
+        char *b = p;
+        while (*p && !std::isspace((unsigned char)*p)) p++;
+        out.assign(b, p);
+        return !out.empty();
+    };
+    std::string tok;
+    if (!word(tok) || tok != (weighted ? "WeightedAdjacencyGraph" : "AdjacencyGraph")) {
+        std::cout << "Bad input file" << std::endl;
+        return false;
+    }
+    auto num = [&]() { return std::strtoull(p, &p, 10); };
+    G.n = (uint32_t)num();
+    G.m = num();
+    G.off.resize(G.n + 1);
+    for (uint32_t i = 0; i < G.n; i++) G.off[i] = (uint32_t)num();
+    G.off[G.n] = (uint32_t)G.m;
+    G.ids.resize(G.m);
+    for (uint64_t e = 0; e < G.m; e++) G.ids[e] = (uint32_t)num();
+    if (weighted) {
+        G.w.resize(G.m);
+        for (uint64_t e = 0; e < G.m; e++) G.w[e] = (float)std::strtoll(p, &p, 10);
+    }
+    return true;
+}
+
+inline bool read_galois(const char *path, bool weighted, HostCSR &G)
+{
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return false;
+    uint64_t hdr[4];
+    if (std::fread(hdr, 8, 4, f) != 4) return false;
+    G.n = (uint32_t)hdr[2];
+    G.m = hdr[3];
+    std::vector<uint64_t> ends(G.n);
+    bool ok = std::fread(ends.data(), 8, G.n, f) == G.n;
+    G.ids.resize(G.m);
+    ok = ok && std::fread(G.ids.data(), 4, G.m, f) == G.m;
+    if (ok && (G.m & 1)) {
+        uint32_t pad;
+        ok = std::fread(&pad, 4, 1, f) == 1;
+    }
+    if (ok && weighted && hdr[1] == 4) {
+        G.w.resize(G.m);
+        std::vector<uint32_t> wi(G.m);
+        ok = std::fread(wi.data(), 4, G.m, f) == G.m;
+        for (uint64_t e = 0; e < G.m; e++) G.w[e] = (float)wi[e];
+    }
+    std::fclose(f);
+    G.off.assign(G.n + 1, 0);
+    for (uint32_t i = 0; i < G.n; i++) G.off[i + 1] = (uint32_t)ends[i];
+    return ok;
+}
+} // namespace gg_front
+
+template <class GraphType>
+void Compute(GraphType &, long);
+
+#ifndef WEIGHTED
+#define GG_FRONT_WEIGHTED false
+#else
+#define GG_FRONT_WEIGHTED true
+#endif
+
+// driver (ligra-numa.h:1095-1208) with one extension: "-rmat S EF SEED"
+// generates the seeded R-MAT graph on the device instead of reading a file
+int parallel_main(int argc, char *argv[])
+{
+    commandLine P(argc, argv, " [-s] [-b] [-r root] [-rounds R] [-c P] [-v edge|vertex] [-o] <inFile | -rmat S EF SEED>");
+    bool symmetric = P.getOption("-s");
+    bool binary = P.getOption("-b");
+    long start = P.getOptionLongValue("-r", 100);
+    long rounds = P.getOptionLongValue("-rounds", 3);
+    int numOfCoo = (int)P.getOptionLongValue("-c", 384);
+    char *part_how = P.getOptionValue("-P");
+    char *vertex_edge = P.getOptionValue("-v");
+    bool relabel = P.getOption("-o");
+    bool part_src = false, part_vertex = false;
+    if (part_how && strcmp(part_how, "dest") && strcmp(part_how, "source")) {
+        std::cerr << "Illegal value for -P: \"" << part_how << "\". Allowed values: dest source. Default: dest\n";
+        return 1;
+    }
+    part_src = part_how && !strcmp(part_how, "source");
+    if (vertex_edge && strcmp(vertex_edge, "edge") && strcmp(vertex_edge, "vertex")) {
+        std::cerr << "Illegal value for -v: \"" << vertex_edge << "\". Allowed values: edge vertex. Default: edge\n";
+        return 1;
+    }
+    part_vertex = vertex_edge && !strcmp(vertex_edge, "vertex");
+    gg_ctx_t ctx = gg_front::context();
+    gg_graph_t g = nullptr;
+    char *rm = P.getOptionValue("-rmat");
+    if (rm) {
+        int i = 1;
+        while (i < argc && strcmp(argv[i], "-rmat")) i++;
+        if (i + 3 >= argc) P.badArgument();
+        gg_front::check(gg_graph_create_rmat(ctx, atoi(argv[i + 1]), atoi(argv[i + 2]),
+                                             strtoull(argv[i + 3], nullptr, 10), symmetric, GG_FRONT_WEIGHTED, &g),
+                        "gg_graph_create_rmat");
+    } else {
+        gg_front::HostCSR G;
+        char *iFile = P.getArgument(0);
+        bool ok = binary ? gg_front::read_galois(iFile, GG_FRONT_WEIGHTED, G)
+                         : gg_front::read_text(iFile, GG_FRONT_WEIGHTED, G);
+        if (!ok) {
+            std::cerr << "cannot read graph " << iFile << std::endl;
+            return 1;
+        }
+        gg_front::check(gg_graph_create_csr(ctx, G.n, G.m, G.off.data(), G.ids.data(),
+                                            G.w.empty() ? nullptr : G.w.data(), symmetric, &g),
+                        "gg_graph_create_csr");
+    }
+    gg_front::current_graph() = g;
+    int rc = 0;
+    auto run = [&](auto &PG) {
+        if (start < 0 || start >= PG.n) {
+            std::cerr << "start vertex " << start << " out of range" << std::endl;
+            rc = 1;
+            return;
+        }
+        for (long r = 0; r < rounds; r++) {
+            startTime();
+            Compute(PG, start);
+            nextTime("Running");
+        }
+        reportAvg(rounds);
+    };
+    if (symmetric) {
+        partitioned_graph<symmetricVertex> PG(g, numOfCoo, part_src, part_vertex, relabel);
+        run(PG);
+        PG.del();
+    } else {
+        partitioned_graph<asymmetricVertex> PG(g, numOfCoo, part_src, part_vertex, relabel);
+        run(PG);
+        PG.del();
+    }
+    return rc;
+}
+
+#endif
