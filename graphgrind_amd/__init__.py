@@ -8,6 +8,7 @@ reference's app-facing API (graphgrind/ligra-numa.h, graph-numa.h):
     edge_map         edgeMap(GA, frontier, functor, threshold, remDups)
     vertex_map       vertexMap(part, frontier, functor) / map_vertexL
     pagerank, bfs, components, spmv   the apps' Compute() loops
+    spmv_bwd, wpagerank, wpagerank_adjoint   differentiated forms (adjoints)
 
 Errors come back as GGError carrying the C status and message (the
 reference abort()s in the same places).  The compute path is the HIP
@@ -121,6 +122,11 @@ SIGNATURES = {
     "gg_cc": (C.c_int, [_VP, _VP, C.POINTER(RunInfo)]),
     "gg_spmv": (C.c_int, [_VP, C.c_int, _VP, _VP, C.POINTER(RunInfo)]),
     "gg_last_kernel_time": (C.c_int, [_VP, C.POINTER(C.c_double), _IP]),
+    "gg_spmv_bwd": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP, C.POINTER(RunInfo)]),
+    "gg_wpagerank": (C.c_int, [_VP, C.c_int, C.c_double, C.c_double, C.c_int, _VP, _VP,
+                               C.POINTER(RunInfo)]),
+    "gg_wpagerank_adjoint": (C.c_int, [_VP, C.c_int, C.c_double, C.c_double, C.c_int, _VP, _VP, _VP,
+                                       _VP, _VP, C.POINTER(RunInfo)]),
 }
 
 
@@ -148,6 +154,23 @@ def check(rc):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def _dev(a):
+    """(device pointer, numpy dtype) of a DeviceArray, or of a contiguous
+    device tensor exposing data_ptr() (a torch tensor on a HIP device; the
+    caller orders its own stream before the call)."""
+    if a is None:
+        return None, None
+    if isinstance(a, DeviceArray):
+        return a.ptr, a.dtype
+    if hasattr(a, "data_ptr"):
+        if getattr(a.device, "type", None) != "cuda":
+            raise ValueError("device tensor required")
+        if not a.is_contiguous():
+            raise ValueError("contiguous tensor required")
+        return a.data_ptr(), np.dtype(str(a.dtype).replace("torch.", ""))
+    raise TypeError(f"not a device array: {type(a)!r}")
 
 
 # ----------------------------------------------------------------- memory --
@@ -309,6 +332,8 @@ class Graph:
 
     def set_weights(self, w_csr):
         w = np.ascontiguousarray(w_csr, np.float32)
+        if w.size != self.m:
+            raise ValueError(f"set_weights: {w.size} weights for {self.m} edges")
         check(lib().gg_graph_set_weights(self.h, w.ctypes.data))
         self.weighted = True
 
@@ -452,9 +477,59 @@ def components(g: Graph, out=None):
     return lab, info
 
 
-def spmv(g: Graph, x: DeviceArray, y: DeviceArray = None):
-    """y[d] = sum_{s->d} w * x[s] over the in-edge twin (app/SPMV.C)."""
-    yy = y if y is not None else DeviceArray(g.ctx, g.n, x.dtype)
+def spmv(g: Graph, x, y=None):
+    """y[d] = sum_{s->d} w * x[s] over the in-edge twin (app/SPMV.C).  x and y
+    are DeviceArrays or device tensors."""
+    xp, dt = _dev(x)
+    yy = y if y is not None else DeviceArray(g.ctx, g.n, dt)
     info = RunInfo()
-    check(lib().gg_spmv(g.h, DT_OF[x.dtype], x.ptr, yy.ptr, C.byref(info)))
+    check(lib().gg_spmv(g.h, DT_OF[dt], xp, _dev(yy)[0], C.byref(info)))
     return yy, info
+
+
+# ---------------------------------------------------- differentiated forms --
+
+def spmv_bwd(g: Graph, x, gy, gx=None, gw=None, want_gx=True, want_gw=True):
+    """Backward of spmv for dL/dy = gy: gx[s] = sum_e w_e gy[d] (pull over
+    the out-edge CSR), gw[e] = x[s] gy[d] in CSR order.  Arrays are
+    DeviceArrays or device tensors of one float dtype; returns (gx, gw, info)."""
+    gp, dt = _dev(gy)
+    xp, xdt = _dev(x)
+    if xdt is not None and xdt != dt:
+        raise ValueError("x and gy differ in dtype")
+    if want_gx and gx is None:
+        gx = DeviceArray(g.ctx, g.n, dt)
+    if want_gw and gw is None:
+        gw = DeviceArray(g.ctx, max(g.m, 1), dt)
+    info = RunInfo()
+    check(lib().gg_spmv_bwd(g.h, DT_OF[dt], xp, gp, _dev(gx)[0] if want_gx else None,
+                            _dev(gw)[0] if want_gw else None, C.byref(info)))
+    return (gx if want_gx else None), (gw if want_gw else None), info
+
+
+def wpagerank(g: Graph, dtype=np.float64, alpha=0.85, eps=1e-7, max_iter=100, v=None, out=None):
+    """Row-normalised weighted PageRank with all vertices active (teleport v,
+    uniform when None) until the L1 step is below eps; returns (r, info)."""
+    dt = np.dtype(dtype)
+    r = out if out is not None else DeviceArray(g.ctx, g.n, dt)
+    info = RunInfo()
+    check(lib().gg_wpagerank(g.h, DT_OF[dt], alpha, eps, max_iter, _dev(v)[0], _dev(r)[0], C.byref(info)))
+    return r, info
+
+
+def wpagerank_adjoint(g: Graph, r, u, alpha=0.85, eps=1e-7, max_iter=100, gw=None, gv=None, lam=None,
+                      want_gw=True, want_gv=True, want_lam=False):
+    """Implicit adjoint of wpagerank at r for dL/dr = u.  Returns
+    (gw per edge in CSR order, gv = dL/dv, lambda, info)."""
+    rp, dt = _dev(r)
+    if want_gw and gw is None:
+        gw = DeviceArray(g.ctx, max(g.m, 1), dt)
+    if want_gv and gv is None:
+        gv = DeviceArray(g.ctx, g.n, dt)
+    if want_lam and lam is None:
+        lam = DeviceArray(g.ctx, g.n, dt)
+    info = RunInfo()
+    check(lib().gg_wpagerank_adjoint(g.h, DT_OF[dt], alpha, eps, max_iter, rp, _dev(u)[0],
+                                     _dev(gw)[0] if want_gw else None, _dev(gv)[0] if want_gv else None,
+                                     _dev(lam)[0] if want_lam else None, C.byref(info)))
+    return (gw if want_gw else None), (gv if want_gv else None), (lam if want_lam else None), info

@@ -525,7 +525,9 @@ static void graph_free(Graph *g)
                    &g->pull.start_edge, &g->pull.carry_row, &g->pull.carry_val, &g->pull.head_val,
                    &g->plan.perm, &g->plan.inv, &g->plan.toff, &g->plan.tids, &g->plan.tw, &g->plan.outdeg,
                    &g->plan.sched.start_row, &g->plan.sched.start_edge, &g->plan.sched.carry_row,
-                   &g->plan.sched.carry_val, &g->plan.sched.head_val};
+                   &g->plan.sched.carry_val, &g->plan.sched.head_val, &g->csr_pull.start_row,
+                   &g->csr_pull.start_edge, &g->csr_pull.carry_row, &g->csr_pull.carry_val,
+                   &g->csr_pull.head_val, &g->wsum};
     for (Buf *b : bufs) buf_free(*b);
     g->plan.mode = 0;
 }
@@ -725,6 +727,7 @@ extern "C" int gg_graph_set_weights(gg_graph_t g, const float *w_csr)
         GG_TRY(buf_alloc(c, g->tw, 4 * (m ? m : 1)));
         g->weighted = true;
     }
+    g->wsum_ok = false;
     if (!m) return GG_OK;
     GG_HIP(hipMemcpyAsync(g->w.p, w_csr, 4 * m, hipMemcpyHostToDevice, c->stream));
     uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);

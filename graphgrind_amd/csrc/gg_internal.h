@@ -123,6 +123,12 @@ struct Graph {
     // merge-path schedule over the in-edge twin
     PullSched pull;
     PullPlan plan;
+    // transpose traversals (adjoints): schedule over the out-edge CSR and the
+    // FP64 out-weight sums, built on first use
+    PullSched csr_pull;
+    bool csr_pull_ok = false;
+    Buf wsum;            // double[n]
+    bool wsum_ok = false;
 };
 
 // ---- frontier ---------------------------------------------------------------

@@ -14,7 +14,15 @@ struct LocalStats {
     unsigned long long active = 0;
     unsigned long long out_edges = 0;
     double l1 = 0.0;
+    double aux = 0.0; // second residual term, reduced only for ops with kAux
 };
+
+// Operators that set 'static constexpr bool kAux = true' have LocalStats::aux
+// reduced beside l1 (written after the l1 partials in the same arrays).
+template <class Op, class = void>
+struct has_aux : std::false_type {};
+template <class Op>
+struct has_aux<Op, std::void_t<decltype(Op::kAux)>> : std::bool_constant<Op::kAux> {};
 
 // counters layout (unsigned long long): [0] active, [1] out_edges,
 // [2] edges scanned, [3] list cursor
@@ -33,29 +41,38 @@ __device__ __forceinline__ double wave_sum_f64(double v)
 }
 
 // Block reduction of LocalStats in a fixed tree; thread 0 adds the counts to
-// the global counters and returns the block's l1 (meaningful in thread 0).
-__device__ __forceinline__ double block_stats(const LocalStats &st, unsigned long long *counters)
+// the global counters and returns the block's l1 (meaningful in thread 0);
+// with kAux it also writes the block's aux sum to *aux_tot.
+template <bool kAux = false>
+__device__ __forceinline__ double block_stats(const LocalStats &st, unsigned long long *counters,
+                                              double *aux_tot = nullptr)
 {
     __shared__ unsigned long long s_a[kBlock / 64], s_o[kBlock / 64];
-    __shared__ double s_l[kBlock / 64];
+    __shared__ double s_l[kBlock / 64], s_x[kBlock / 64];
     unsigned long long a = wave_sum_u64(st.active);
     unsigned long long o = wave_sum_u64(st.out_edges);
     double l = wave_sum_f64(st.l1);
+    double x = 0;
+    if constexpr (kAux) x = wave_sum_f64(st.aux);
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) {
         s_a[w] = a;
         s_o[w] = o;
         s_l[w] = l;
+        if constexpr (kAux) s_x[w] = x;
     }
     __syncthreads();
     double tot = 0;
     if (threadIdx.x == 0) {
         unsigned long long ta = 0, to = 0;
+        double tx = 0;
         for (int k = 0; k < kBlock / 64; k++) {
             ta += s_a[k];
             to += s_o[k];
             tot += s_l[k];
+            if constexpr (kAux) tx += s_x[k];
         }
+        if constexpr (kAux) *aux_tot = tx;
         if (counters) {
             if (ta) atomicAdd(&counters[CNT_ACTIVE], ta);
             if (to) atomicAdd(&counters[CNT_OUT], to);
@@ -115,7 +132,8 @@ __device__ __forceinline__ void row_store(const Op &op, uint32_t d, const R &r, 
 //   3. epilogue: rows are finished in a coalesced pass, row i by thread
 //      i mod kBlock, with its prefetched inputs.
 // Rows crossing tile boundaries are finished by k_pull_fixup from per-tile
-// carries.
+// carries.  l1_part receives each tile's residual (and, for kAux ops, the
+// aux sums in l1_part[tiles ..]).
 //
 // Op: acc_t, identity(), combine(earlier, later), gather(edge, src),
 //     finish(row, acc, LocalStats&) [or row_t/load_row/store_row].
@@ -263,8 +281,13 @@ __global__ __launch_bounds__(kBlock) void k_pull(Op op, const uint32_t *__restri
             else row_store(op, r0 + i, rin[k], s_val[i], st);
         }
     }
-    double l1 = block_stats(st, counters);
-    if (l1_part && tid == 0) l1_part[t] = l1;
+    constexpr bool kAux = has_aux<Op>::value;
+    double aux = 0;
+    double l1 = block_stats<kAux>(st, counters, &aux);
+    if (l1_part && tid == 0) {
+        l1_part[t] = l1;
+        if constexpr (kAux) l1_part[gridDim.x + t] = aux;
+    }
 }
 
 // Finishes rows that began in an earlier tile: row r0 of tile t when its
@@ -297,10 +320,18 @@ __global__ __launch_bounds__(kBlock) void k_pull_fixup(Op op, const uint32_t *__
             acc = op.combine(acc, head_val[t]);
             row_store(op, r0, row_load(op, r0), acc, st);
         }
-        if (l1_part) st.l1 += l1_part[t];
+        if (l1_part) {
+            st.l1 += l1_part[t];
+            if constexpr (has_aux<Op>::value) st.aux += l1_part[tiles + t];
+        }
     }
-    double l1 = block_stats(st, counters);
-    if (l1_fix && threadIdx.x == 0) l1_fix[blockIdx.x] = l1;
+    constexpr bool kAux = has_aux<Op>::value;
+    double aux = 0;
+    double l1 = block_stats<kAux>(st, counters, &aux);
+    if (l1_fix && threadIdx.x == 0) {
+        l1_fix[blockIdx.x] = l1;
+        if constexpr (kAux) l1_fix[gridDim.x + blockIdx.x] = aux;
+    }
 }
 
 template <class Op>

@@ -227,7 +227,34 @@ int gg_cc(gg_graph_t g, uint32_t *labels_dev, gg_run_info *info);
  * in-edge twin (unit weights when the graph has none). */
 int gg_spmv(gg_graph_t g, int dtype, const void *x_dev, void *y_dev, gg_run_info *info);
 
-/* ---- timing ------------------------------------------------------------- */
+/* ---- differentiated forms ------------------------------------------------
+ * No counterpart in the reference: gradients of SPMV.C's product and of the
+ * all-active PageRank (app/PageRankBit.C:145) with row-normalised weights,
+ * on the same weighted graph.  Arrays are device pointers in 'dtype' (F64
+ * or F32); per-edge arrays are in the CSR edge order the graph was created
+ * with; weights stay FP32 on the graph. */
+/* Backward of gg_spmv for a loss with dL/dy = gy: gx[s] = sum_{e=(s->d)}
+ * w_e gy[d] (a pull over the out-edge CSR, the transpose traversal) and
+ * gw[e] = x[s] gy[d].  gx or gw may be NULL (x is needed only for gw). */
+int gg_spmv_bwd(gg_graph_t g, int dtype, const void *x_dev, const void *gy_dev, void *gx_dev,
+                void *gw_dev, gg_run_info *info);
+/* Weighted PageRank: r = v, then r <- (1-alpha) v + alpha P r with
+ * P[d][s] = w_{s->d} / S_s (S_s = FP64 sum of s's out-weights; dangling
+ * sources contribute nothing) until sum|r_new - r| < eps or max_iter.
+ * v_dev NULL means the uniform teleport 1/n.  Needs edge weights. */
+int gg_wpagerank(gg_graph_t g, int dtype, double alpha, double eps, int max_iter,
+                 const void *v_dev, void *r_dev, gg_run_info *info);
+/* Implicit adjoint of gg_wpagerank at its output r for dL/dr = u:
+ *   lambda = u, lambda <- u + alpha P^T lambda (pull over the out-edge CSR)
+ *   until sum|lambda_new - lambda| <= eps * sum|lambda_new| or max_iter;
+ *   gw[e] = (alpha r_s / S_s) (lambda_d - (lambda_s - u_s) / alpha)
+ *   gv    = (1 - alpha) lambda            (dL/d teleport)
+ * gw, gv and lambda (the converged adjoint) may each be NULL. */
+int gg_wpagerank_adjoint(gg_graph_t g, int dtype, double alpha, double eps, int max_iter,
+                         const void *r_dev, const void *u_dev, void *gw_dev, void *gv_dev,
+                         void *lambda_dev, gg_run_info *info);
+
+/* ---- timing------------------------------------------------------------- */
 /* Average device time (ms) of the last fused loop's dominant kernel and
  * its launch count, measured with hipEvents on the context stream. */
 int gg_last_kernel_time(gg_ctx_t ctx, double *avg_ms, int *launches);

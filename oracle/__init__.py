@@ -85,6 +85,14 @@ def lib():
                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                        C.POINTER(C.c_int)]
         L.ggo_graph_error.restype = C.c_int
+        f32 = P(np.float32, flags="C_CONTIGUOUS")
+        L.ggo_out_wsum.argtypes = [C.c_uint32, u32, f32, f64]
+        L.ggo_spmv_csr.argtypes = [C.c_uint32, u32, u32, C.c_void_p, f64, f64]
+        L.ggo_spmv_bwd.argtypes = [C.c_uint32, u32, u32, C.c_void_p, f64, f64, C.c_void_p, C.c_void_p]
+        L.ggo_wpagerank.argtypes = [C.c_uint32, u32, u32, f32, C.c_double, C.c_double, C.c_int,
+                                    C.c_void_p, f64, C.c_void_p]
+        L.ggo_wpagerank_adjoint.argtypes = [C.c_uint32, u32, u32, f32, C.c_double, C.c_double, C.c_int,
+                                            f64, f64, f64, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -244,6 +252,64 @@ class Graph:
 
 
 # ------------------------------------------------------ reference harness --
+
+# ------------------------------------------------- differentiated forms --
+
+def _vp(a):
+    return None if a is None else a.ctypes.data
+
+
+def out_wsum(off, w):
+    n = len(off) - 1
+    S = np.empty(n, np.float64)
+    lib().ggo_out_wsum(n, off, np.ascontiguousarray(w, np.float32), S)
+    return S
+
+
+def spmv_csr(off, ids, w, x):
+    """y[d] = sum_{e=(s->d)} w_e x[s] in FP64 (w None: unweighted)."""
+    n = len(off) - 1
+    w = None if w is None else np.ascontiguousarray(w, np.float32)
+    y = np.empty(n, np.float64)
+    lib().ggo_spmv_csr(n, off, ids, _vp(w), np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def spmv_bwd(off, ids, w, x, gy):
+    """(gx, gw): gx[s] = sum_e w_e gy[d]; gw[e] = x[s] gy[d] in CSR order."""
+    n, m = len(off) - 1, len(ids)
+    w = None if w is None else np.ascontiguousarray(w, np.float32)
+    gx = np.empty(n, np.float64)
+    gw = np.empty(max(m, 1), np.float64)
+    lib().ggo_spmv_bwd(n, off, ids, _vp(w), np.ascontiguousarray(x, np.float64),
+                       np.ascontiguousarray(gy, np.float64), _vp(gx), _vp(gw))
+    return gx, gw[:m]
+
+
+def wpagerank(off, ids, w, alpha=0.85, eps=1e-7, max_iter=100, v=None):
+    """Row-normalised weighted PageRank (all active); returns (r, iters, l1 history)."""
+    n = len(off) - 1
+    r = np.empty(n, np.float64)
+    hist = np.zeros(max_iter, np.float64)
+    v = None if v is None else np.ascontiguousarray(v, np.float64)
+    it = lib().ggo_wpagerank(n, off, ids, np.ascontiguousarray(w, np.float32), alpha, eps, max_iter,
+                             _vp(v), r, _vp(hist))
+    assert it >= 0
+    return r, it, hist[:it]
+
+
+def wpagerank_adjoint(off, ids, w, r, u, alpha=0.85, eps=1e-7, max_iter=100):
+    """Implicit adjoint at r for dL/dr = u; returns (lambda, gw (CSR order), gv, iters)."""
+    n, m = len(off) - 1, len(ids)
+    lam = np.empty(n, np.float64)
+    gw = np.empty(max(m, 1), np.float64)
+    gv = np.empty(n, np.float64)
+    it = lib().ggo_wpagerank_adjoint(n, off, ids, np.ascontiguousarray(w, np.float32), alpha, eps, max_iter,
+                                     np.ascontiguousarray(r, np.float64), np.ascontiguousarray(u, np.float64),
+                                     lam, _vp(gw), _vp(gv))
+    assert it >= 0
+    return lam, gw[:m], gv, it
+
 
 def ref_available(app: str = "pagerank") -> bool:
     return os.path.exists(os.path.join(REF_DIR, f"ref_{app}"))

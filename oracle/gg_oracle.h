@@ -130,6 +130,30 @@ int ggo_edgemap_once(const ggo_graph *g, int op, const uint8_t *in_dense,
                      int64_t *a_i64, int64_t *b_i64, uint8_t *out_dense,
                      int64_t *out_dm, int64_t *out_edges, int *path);
 
+/* ---- differentiated forms (gg_oracle_grad.c; no reference counterpart) --- */
+/* S[s] = sum of s's out-edge weights, CSR order, FP64. */
+void ggo_out_wsum(uint32_t n, const uint32_t *off, const float *w, double *S);
+/* y[d] = sum_{e=(s->d)} w_e x[s] (w NULL: 1). */
+void ggo_spmv_csr(uint32_t n, const uint32_t *off, const uint32_t *ids, const float *w,
+                  const double *x, double *y);
+/* Backward of ggo_spmv_csr: gx[s] = sum_e w_e gy[d], gw[e] = x[s] gy[d]
+ * (CSR order); gx or gw may be NULL. */
+void ggo_spmv_bwd(uint32_t n, const uint32_t *off, const uint32_t *ids, const float *w,
+                  const double *x, const double *gy, double *gx, double *gw);
+/* Row-normalised weighted PageRank, all vertices active: r = v (1/n when v
+ * is NULL), then r <- (1-alpha) v + alpha P r until sum|r_new - r| < eps or
+ * max_iter.  Returns the iteration count. */
+int ggo_wpagerank(uint32_t n, const uint32_t *off, const uint32_t *ids, const float *w,
+                  double alpha, double eps, int max_iter, const double *v, double *r,
+                  double *l1_hist);
+/* Implicit adjoint at r for dL/dr = u: lambda = u, then
+ * lambda <- u + alpha P^T lambda until sum|dlambda| <= eps * sum|lambda| or
+ * max_iter; gw[e] (CSR order) and gv as in gg_oracle_grad.c.  Returns the
+ * iteration count. */
+int ggo_wpagerank_adjoint(uint32_t n, const uint32_t *off, const uint32_t *ids, const float *w,
+                          double alpha, double eps, int max_iter, const double *r, const double *u,
+                          double *lambda, double *gw, double *gv);
+
 #ifdef __cplusplus
 }
 #endif
