@@ -10,10 +10,15 @@ untimed, as in ligra-numa.h:1155-1157).
 
 Also measured on the same line: BFS from root 100 and connected components
 on the symmetric R-MAT scale-24 graph (seed 2), the FP32 PageRank variant,
-the roofline of the dominant kernel (merge-path pull, HIP events on the
-library stream) and the reference engine's CPU time on a bounded sample.
+weighted SpMV and its adjoint (FP32, R-MAT scale 24, seed 3), the
+differentiable PageRank forward and implicit adjoint with the adjoint /
+forward time ratio (FP64, R-MAT scale 22, seed 4, 4096 targets), the
+batched personalised PageRank + adjoint over 512 packed R-MAT graphs and
+32-source BFS on symmetric R-MAT scale 22, the roofline of the dominant
+kernel (HIP events on the library stream) and the reference engine's CPU
+time on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 24] [--no-extras]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 24] [--no-extras] [--no-batch]
 """
 from __future__ import annotations
 
@@ -114,6 +119,182 @@ def bfs_cc_bench(gg, ctx, scale, steps, root=100, P=384):
     return res
 
 
+def spmv_adjoint_bench(gg, ctx, scale, steps, seed=3):
+    """Weighted SpMV and its adjoint (FP32): y = A x over the in-edge twin;
+    for L = 0.5|y - t|^2, dL/dx over the out-edge CSR and dL/dw per edge."""
+    g = gg.Graph.rmat(ctx, scale, 16, seed, weighted=True)
+    n, m = g.n, g.m
+    rng = np.random.default_rng(seed)
+    x = gg.DeviceArray.from_host(ctx, rng.uniform(-1, 1, n).astype(np.float32))
+    t = rng.standard_normal(n).astype(np.float32)
+    y = gg.DeviceArray(ctx, n, np.float32)
+    gx = gg.DeviceArray(ctx, n, np.float32)
+    gw = gg.DeviceArray(ctx, m, np.float32)
+    gg.spmv(g, x, y)
+    gy = gg.DeviceArray.from_host(ctx, y.to_host() - t)
+    gg.spmv_bwd(g, x, gy, gx=gx, gw=gw)  # warm-up (builds the CSR schedule)
+    fwd, bwd = [], []
+    for _ in range(max(steps, 3)):
+        _, i1 = gg.spmv(g, x, y)
+        fwd.append(i1.ms)
+        _, _, i2 = gg.spmv_bwd(g, x, gy, gx=gx, gw=gw)
+        bwd.append(i2.ms)
+    f, b = float(np.median(fwd)), float(np.median(bwd))
+    # FP32: per edge 4 B id + 4 B weight + 4 B gathered x; per row offset + y
+    fbytes = 12 * m + 8 * n
+    # dx pass as the forward; dw pass: id + gathered gy + written gw per edge,
+    # x per row
+    bbytes = 12 * m + 8 * n + 12 * m + 8 * n
+    res = {"n": n, "m": m, "fwd_ms": f, "bwd_ms": b, "bwd_fwd_ratio": b / f,
+           "fwd_gbs": fbytes / f / 1e6, "bwd_gbs": bbytes / b / 1e6}
+    g.close()
+    return res
+
+
+def wpagerank_adjoint_bench(gg, ctx, scale, steps, seed=4, targets=4096, alpha=0.85, eps=1e-7):
+    """Differentiable PageRank (FP64): forward to L1 step < 1e-7 (at most 100
+    iterations), loss = sum over 'targets' vertices of r^2, implicit adjoint
+    giving dL/dw (per edge) and dL/dv (teleport)."""
+    g = gg.Graph.rmat(ctx, scale, 16, seed, weighted=True)
+    n, m = g.n, g.m
+    r = gg.DeviceArray(ctx, n, np.float64)
+    gw = gg.DeviceArray(ctx, m, np.float64)
+    gv = gg.DeviceArray(ctx, n, np.float64)
+    gg.wpagerank(g, np.float64, alpha, eps, 100, out=r)  # warm-up (weight sums, schedules)
+    rh = r.to_host()
+    T = np.random.default_rng(seed).choice(n, targets, replace=False)
+    u = np.zeros(n)
+    u[T] = 2 * rh[T]
+    ud = gg.DeviceArray.from_host(ctx, u)
+    gg.wpagerank_adjoint(g, r, ud, alpha, eps, 100, gw=gw, gv=gv)
+    fwd, adj, fi, ai, fk, ak = [], [], [], [], [], []
+    for _ in range(max(steps, 3)):
+        _, i1 = gg.wpagerank(g, np.float64, alpha, eps, 100, out=r)
+        fwd.append(i1.ms)
+        fi.append(i1.iters)
+        fk.append(ctx.kernel_time()[0])  # mean pull time per iteration
+        _, _, _, i2 = gg.wpagerank_adjoint(g, r, ud, alpha, eps, 100, gw=gw, gv=gv)
+        adj.append(i2.ms)
+        ai.append(i2.iters)
+        ak.append(ctx.kernel_time()[0])
+    f, a = float(np.median(fwd)), float(np.median(adj))
+    # per iteration, FP64: 4 B id + 4 B weight + 8 B gathered value per edge;
+    # per row offset, old value, teleport/seed, weight sum, new value and
+    # (forward) the next contribution
+    it_bytes = 16 * m + 36 * n
+    res = {"n": n, "m": m, "fwd_ms": f, "adj_ms": a, "adjoint_forward_ratio": a / f,
+           "fwd_iters": fi[-1], "adj_iters": ai[-1],
+           "fwd_iter_ms": float(np.median(fk)), "adj_iter_ms": float(np.median(ak)),
+           "fwd_iter_gbs": it_bytes / float(np.median(fk)) / 1e6,
+           "adj_iter_gbs": it_bytes / float(np.median(ak)) / 1e6}
+    g.close()
+    return res
+
+
+def batched_bench(gg, ctx, count=512, seed=5, iters=30, msbfs_scale=22):
+    """Batched uneven workloads: 'count' directed R-MAT graphs (scale uniform
+    in 10..18, edge factor in {8, 16, 32}, seeds derived from 'seed') packed
+    into one arena; personalised PageRank with 1-8 seeds per graph (FP32,
+    exactly 'iters' iterations) and its adjoint (loss: sum of r^2 over 8
+    targets per graph), each one captured hipGraph replay, after the
+    autotuner picked the pull schedule; beside it the same solves graph by
+    graph.  Then 32-source BFS on symmetric R-MAT scale 22 against 32
+    single-source BFS runs."""
+    rng = np.random.default_rng(seed)
+    scales = rng.integers(10, 19, count)
+    efs = rng.choice([8, 16, 32], count)
+    t0 = time.perf_counter()
+    parts = [gg.Graph.rmat(ctx, int(s), int(e), seed * 100003 + i) for i, (s, e) in enumerate(zip(scales, efs))]
+    arena, vb, eb = gg.Graph.arena(ctx, parts)
+    ones = gg.DeviceArray(ctx, max(arena.m, 1), np.float32).fill(1.0)
+    arena.set_weights_dev(ones)
+    log(f"[bench] batch: {count} graphs, n={arena.n} m={arena.m} built in {time.perf_counter() - t0:.1f}s")
+    n = arena.n
+    v = np.zeros(n, np.float32)
+    u = np.zeros(n, np.float32)
+    for i in range(count):
+        ni = int(vb[i + 1] - vb[i])
+        k = int(rng.integers(1, 9))
+        v[vb[i] + rng.choice(ni, k, replace=False)] += np.float32(1.0 / k)
+    vd = gg.DeviceArray.from_host(ctx, v)
+    tune = gg.autotune_pull(arena, np.float32, trials=3)
+    r = gg.DeviceArray(ctx, n, np.float32)
+    gg.wpagerank(arena, np.float32, 0.85, -1.0, iters, v=vd, out=r)  # capture
+    rh = r.to_host()
+    for i in range(count):
+        t = vb[i] + rng.choice(int(vb[i + 1] - vb[i]), 8, replace=False)
+        u[t] = 2 * rh[t]
+    ud = gg.DeviceArray.from_host(ctx, u)
+    gw = gg.DeviceArray(ctx, max(arena.m, 1), np.float32)
+    gv = gg.DeviceArray(ctx, n, np.float32)
+    gg.wpagerank_adjoint(arena, r, ud, 0.85, -1.0, iters, gw=gw, gv=gv)  # capture
+    fw, ad = [], []
+    for _ in range(3):
+        _, i1 = gg.wpagerank(arena, np.float32, 0.85, -1.0, iters, v=vd, out=r)
+        fw.append(i1.ms)
+        _, _, _, i2 = gg.wpagerank_adjoint(arena, r, ud, 0.85, -1.0, iters, gw=gw, gv=gv)
+        ad.append(i2.ms)
+    tb_f, tb_a = float(np.median(fw)), float(np.median(ad))
+    # the same solves graph by graph (each its own captured loop)
+    per_f, per_a = [], []
+    for i, p in enumerate(parts):
+        ni, mi = p.n, p.m
+        if not mi:
+            continue
+        w1 = gg.DeviceArray(ctx, mi, np.float32).fill(1.0)
+        p.set_weights_dev(w1)
+        w1.free()
+        vi = gg.DeviceArray.from_host(ctx, v[vb[i]:vb[i + 1]])
+        ui = gg.DeviceArray.from_host(ctx, u[vb[i]:vb[i + 1]])
+        ri = gg.DeviceArray(ctx, ni, np.float32)
+        gwi = gg.DeviceArray(ctx, mi, np.float32)
+        gvi = gg.DeviceArray(ctx, ni, np.float32)
+        gg.wpagerank(p, np.float32, 0.85, -1.0, iters, v=vi, out=ri)
+        gg.wpagerank_adjoint(p, ri, ui, 0.85, -1.0, iters, gw=gwi, gv=gvi)
+        t1 = time.perf_counter()
+        gg.wpagerank(p, np.float32, 0.85, -1.0, iters, v=vi, out=ri)
+        t2 = time.perf_counter()
+        gg.wpagerank_adjoint(p, ri, ui, 0.85, -1.0, iters, gw=gwi, gv=gvi)
+        t3 = time.perf_counter()
+        per_f.append((t2 - t1) * 1e3)
+        per_a.append((t3 - t2) * 1e3)
+        for a in (vi, ui, ri, gwi, gvi):
+            a.free()
+    M = arena.m
+    res = {"graphs": count, "n": n, "m": M, "iterations": iters,
+           "tuned": {"schedule": tune["schedule"], "edges_per_lane": tune["edges_per_lane"]},
+           "batched_fwd_ms": tb_f, "batched_adj_ms": tb_a,
+           "batched_fwd_gteps": iters * M / tb_f / 1e6, "batched_adj_gteps": iters * M / tb_a / 1e6,
+           "batched_latency_ms_p50_p99": [tb_f + tb_a, tb_f + tb_a],
+           "unbatched_total_ms": float(np.sum(per_f) + np.sum(per_a)),
+           "unbatched_fwd_gteps": iters * M / float(np.sum(per_f)) / 1e6,
+           "unbatched_latency_ms_p50": float(np.percentile(np.add(per_f, per_a), 50)),
+           "unbatched_latency_ms_p99": float(np.percentile(np.add(per_f, per_a), 99))}
+    arena.close()
+    for p in parts:
+        p.close()
+    # 32-source BFS on the BC shape
+    g = gg.Graph.rmat(ctx, msbfs_scale, 16, seed, symmetric=True)
+    g.partition(384)
+    deg = g.degrees(False)
+    src = np.random.default_rng(seed).choice(np.flatnonzero(deg > 0), 32, replace=False)
+    lv, _ = gg.msbfs(g, src)  # warm-up (row blocks)
+    t1 = time.perf_counter()
+    lv, info = gg.msbfs(g, src, out=lv)
+    tm = time.perf_counter() - t1
+    L = lv.to_host().reshape(32, g.n)
+    und = sum(int(deg[L[k] >= 0].astype(np.int64).sum()) // 2 for k in range(32))
+    par = gg.DeviceArray(ctx, g.n, np.int32)
+    t1 = time.perf_counter()
+    for s in src:
+        gg.bfs(g, int(s), parents=par, levels=False)
+    ts = time.perf_counter() - t1
+    res["msbfs"] = {"scale": msbfs_scale, "sources": 32, "levels": info.iters, "ms": tm * 1e3,
+                    "gteps": und / tm / 1e9, "single_source_loop_ms": ts * 1e3, "single_source_gteps": und / ts / 1e9}
+    g.close()
+    return res
+
+
 def cpu_baseline(scale=20, rounds=3):
     """The reference engine (oracle/_ref, compiled from the reference's own
     sources with its OpenMP branch) on a bounded sample: PageRank.C on the
@@ -147,6 +328,36 @@ def cpu_baseline(scale=20, rounds=3):
             "sample": f"C restatement, R-MAT scale 18, 10 iterations ({t * 1e3:.0f} ms)"}
 
 
+def copy_calibration(gg, ctx, nbytes=1 << 30, reps=3):
+    """Device-to-device copies of a known size, for scaling the HBM counters
+    (FETCH_SIZE / WRITE_SIZE per dispatch) of a profiled run to bytes."""
+    a = gg.DeviceArray(ctx, nbytes // 4, np.uint32).fill(1)
+    b = gg.DeviceArray(ctx, nbytes // 4, np.uint32)
+    for _ in range(reps):
+        gg.lib().gg_memcpy_d2d(ctx.h, b.ptr, a.ptr, nbytes)
+    ctx.sync()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
+def traffic_from_profile(kernel, n, m):
+    """HBM bytes per launch of 'kernel' on this workload, as recorded from
+    the rocprofv3 counter runs in profiles/ (None when absent)."""
+    path = os.path.join(ROOT, "profiles", "k_pull_pagerank_rmat24_fp64_counters.md")
+    if not os.path.exists(path):
+        return None
+    rec = {}
+    for line in open(path):
+        if line.startswith("    ") and "=" in line:
+            k, v = line.strip().split("=", 1)
+            rec[k.strip()] = v.strip()
+    try:
+        if rec.get("kernel") != kernel or int(rec["n"]) != n or int(rec["m"]) != m:
+            return None
+        return float(rec["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +366,9 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 1 GiB device copies (counter calibration under rocprofv3)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every run of this build uses a single MI355X; "
@@ -163,6 +377,10 @@ def main():
     import graphgrind_amd as gg
 
     ctx = gg.Context(0)
+    if a.calibrate_copy:
+        copy_calibration(gg, ctx)
+        ctx.close()
+        return
     pr = pagerank_bench(gg, ctx, a.scale, a.steps, a.warmup, np.float64)
     log(f"[bench] pagerank fp64: {pr}")
     extras = {}
@@ -173,6 +391,16 @@ def main():
         bc = bfs_cc_bench(gg, ctx, a.scale, a.steps)
         log(f"[bench] bfs/cc: {bc}")
         extras["bfs_cc_sym_rmat"] = bc
+        sa = spmv_adjoint_bench(gg, ctx, a.scale, a.steps)
+        log(f"[bench] spmv adjoint: {sa}")
+        extras["spmv_adjoint_fp32"] = sa
+        wa = wpagerank_adjoint_bench(gg, ctx, a.scale - 2, a.steps)
+        log(f"[bench] pagerank adjoint: {wa}")
+        extras["pagerank_adjoint_fp64"] = wa
+        if not a.no_batch:
+            bb = batched_bench(gg, ctx, msbfs_scale=a.scale - 2)
+            log(f"[bench] batched: {bb}")
+            extras["batched"] = bb
     ctx.close()
     cpu = None if a.no_cpu else cpu_baseline()
     line = {
@@ -192,7 +420,8 @@ def main():
                                f"edge factor 16, seed 1, 384 edge-balanced Hilbert COO partitions, FP64",
                    "n": pr["n"], "m": pr["m"], "iterations": 10, "partitions": 384},
         "roofline": {"bound": "hbm", "achieved": pr["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": pr["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": pr["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile("k_pull<OpPRFused<double>>", pr["n"], pr["m"]),
                      "kernel": "k_pull<OpPRFused<double>>", "kernel_ms": pr["kernel_ms"],
                      "algo_bytes_per_launch": pr["algo_bytes_per_launch"]},
         "cpu_baseline": cpu,

@@ -37,6 +37,7 @@ VOP_PAGERANK, VOP_RESET, VOP_CC_COPY, VOP_ABSDIFF = 0, 1, 2, 3
 FRONTIER_EMPTY, FRONTIER_SINGLE, FRONTIER_ALL = 0, 1, 2
 PATH_EMPTY, PATH_SPARSE, PATH_CSC, PATH_COO = -1, 0, 1, 2
 PATH_NAMES = {-1: "empty", 0: "sparse", 1: "csc", 2: "coo"}
+OPT_PULL_SCHEDULE, OPT_ROW_EDGES_PER_LANE = 0, 1
 
 
 class GGError(RuntimeError):
@@ -57,7 +58,14 @@ class VOp(C.Structure):
 class GraphView(C.Structure):
     _fields_ = [("toff", C.c_void_p), ("tids", C.c_void_p), ("outdeg", C.c_void_p), ("perm", C.c_void_p),
                 ("tile_row", C.c_void_p), ("tile_edge", C.c_void_p), ("tiles", C.c_uint32),
-                ("n", C.c_uint32), ("m", C.c_uint64)]
+                ("n", C.c_uint32), ("m", C.c_uint64), ("row_work", C.c_void_p),
+                ("row_blocks", C.c_uint32), ("row_long", C.c_uint32)]
+
+
+class TuneInfo(C.Structure):
+    _fields_ = [("schedule", C.c_int32), ("edges_per_lane", C.c_int32), ("ms", C.c_double),
+                ("candidates", C.c_int32), ("cand_schedule", C.c_int32 * 8),
+                ("cand_edges_per_lane", C.c_int32 * 8), ("cand_ms", C.c_double * 8)]
 
 
 class RunInfo(C.Structure):
@@ -122,6 +130,12 @@ SIGNATURES = {
     "gg_cc": (C.c_int, [_VP, _VP, C.POINTER(RunInfo)]),
     "gg_spmv": (C.c_int, [_VP, C.c_int, _VP, _VP, C.POINTER(RunInfo)]),
     "gg_last_kernel_time": (C.c_int, [_VP, C.POINTER(C.c_double), _IP]),
+    "gg_graph_set_option": (C.c_int, [_VP, C.c_int, C.c_int64]),
+    "gg_graph_get_option": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int64)]),
+    "gg_graph_set_weights_dev": (C.c_int, [_VP, _VP]),
+    "gg_graph_create_arena": (C.c_int, [_VP, C.c_int, _VP, C.POINTER(C.c_void_p), _VP, _VP]),
+    "gg_msbfs": (C.c_int, [_VP, C.c_int, _VP, _VP, C.POINTER(RunInfo)]),
+    "gg_autotune_pull": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(TuneInfo)]),
     "gg_spmv_bwd": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP, C.POINTER(RunInfo)]),
     "gg_wpagerank": (C.c_int, [_VP, C.c_int, C.c_double, C.c_double, C.c_int, _VP, _VP,
                                C.POINTER(RunInfo)]),
@@ -276,6 +290,18 @@ class Graph:
                                          int(weighted), C.byref(h)))
         return cls(ctx, h)
 
+    @classmethod
+    def arena(cls, ctx, parts):
+        """Packs graphs into one block-diagonal arena graph.  Returns (arena,
+        vbase, ebase): graph i owns vertices [vbase[i], vbase[i+1]) and edges
+        [ebase[i], ebase[i+1]) of the arena."""
+        hs = (C.c_void_p * len(parts))(*[p.h for p in parts])
+        vb = np.zeros(len(parts) + 1, np.uint32)
+        eb = np.zeros(len(parts) + 1, np.uint64)
+        h = C.c_void_p()
+        check(lib().gg_graph_create_arena(ctx.h, len(parts), hs, C.byref(h), vb.ctypes.data, eb.ctypes.data))
+        return cls(ctx, h), vb.astype(np.int64), eb.astype(np.int64)
+
     def close(self):
         if self.h and self.ctx.h:
             lib().gg_graph_destroy(self.h)
@@ -329,6 +355,24 @@ class Graph:
         d = np.empty(self.n, np.uint32)
         check(lib().gg_graph_degrees(self.h, int(in_edges), d.ctypes.data))
         return d
+
+    def set_option(self, option, value):
+        """Schedule choices of the device pull (OPT_PULL_SCHEDULE, OPT_ROW_EDGES_PER_LANE)."""
+        check(lib().gg_graph_set_option(self.h, option, int(value)))
+        return self
+
+    def get_option(self, option):
+        v = C.c_int64()
+        check(lib().gg_graph_get_option(self.h, option, C.byref(v)))
+        return v.value
+
+    def set_weights_dev(self, w):
+        """Edge weights from a device array (m FP32 values, CSR order)."""
+        p, dt = _dev(w)
+        if dt != np.float32:
+            raise ValueError("weights are FP32")
+        check(lib().gg_graph_set_weights_dev(self.h, p))
+        self.weighted = True
 
     def set_weights(self, w_csr):
         w = np.ascontiguousarray(w_csr, np.float32)
@@ -533,3 +577,27 @@ def wpagerank_adjoint(g: Graph, r, u, alpha=0.85, eps=1e-7, max_iter=100, gw=Non
                                      _dev(gw)[0] if want_gw else None, _dev(gv)[0] if want_gv else None,
                                      _dev(lam)[0] if want_lam else None, C.byref(info)))
     return (gw if want_gw else None), (gv if want_gv else None), (lam if want_lam else None), info
+
+
+# -------------------------------------------------- batching and tuning --
+
+def msbfs(g: Graph, sources, out=None):
+    """BFS from up to 32 sources at once (the forward phase of app/BC.C).
+    Returns (levels: int32 DeviceArray [len(sources) * n], -1 unreached,
+    RunInfo)."""
+    src = np.ascontiguousarray(sources, np.uint32)
+    lv = out if out is not None else DeviceArray(g.ctx, len(src) * g.n, np.int32)
+    info = RunInfo()
+    check(lib().gg_msbfs(g.h, len(src), src.ctypes.data, _dev(lv)[0], C.byref(info)))
+    return lv, info
+
+
+def autotune_pull(g: Graph, dtype=np.float64, trials=5):
+    """Times the candidate pull schedules on g, keeps the fastest on the
+    graph and returns every candidate's time."""
+    t = TuneInfo()
+    check(lib().gg_autotune_pull(g.h, DT_OF[np.dtype(dtype)], trials, C.byref(t)))
+    k = t.candidates
+    return {"schedule": t.schedule, "edges_per_lane": t.edges_per_lane, "ms": t.ms, "candidates": k,
+            "cand_schedule": list(t.cand_schedule[:k]), "cand_edges_per_lane": list(t.cand_edges_per_lane[:k]),
+            "cand_ms": list(t.cand_ms[:k])}

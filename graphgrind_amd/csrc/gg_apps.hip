@@ -144,14 +144,14 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
     const bool use_plan = g->plan.mode != 0;
     const CscView view = use_plan ? plan_view(g) : graph_view(g);
     const uint32_t *outdeg = use_plan ? (const uint32_t *)g->plan.outdeg.p : (const uint32_t *)g->outdeg.p;
-    PullSched &s = *view.sched;
-    const uint32_t nfix = blocks_for(s.tiles, kBlock);
+    const uint32_t nblk = view.blocks();
+    const uint32_t nfix = view.fix_blocks();
     PRBuffers B;
     int rc = GG_OK;
     for (int i = 0; i < 3 && !rc; i++) rc = buf_alloc(c, B.R[i], sizeof(V) * (uint64_t)n);
     for (int i = 0; i < 2 && !rc; i++) rc = buf_alloc(c, B.K[i], sizeof(V) * (uint64_t)n);
     for (int i = 0; i < 2 && !rc; i++) rc = buf_alloc(c, B.F[i], (uint64_t)n + 64);
-    if (!rc) rc = buf_alloc(c, B.l1p, 8ull * s.tiles);
+    if (!rc) rc = buf_alloc(c, B.l1p, 8ull * nblk);
     if (!rc) rc = buf_alloc(c, B.l1f, 8ull * nfix);
     if (!rc) rc = buf_alloc(c, B.flags, 64);
     if (!rc) rc = buf_alloc(c, B.stats, 16ull * (max_iter + 1));
@@ -189,7 +189,7 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
         op.done = done;
         rc = launch_pull(c, view, op, cnt, (double *)B.l1p.p, (double *)B.l1f.p, ev[2 * it], ev[2 * it + 1]);
         if (rc) break;
-        k_pr_check<<<1, kBlock, 0, c->stream>>>((const double *)B.l1p.p, s.tiles, (const double *)B.l1f.p, nfix,
+        k_pr_check<<<1, kBlock, 0, c->stream>>>((const double *)B.l1p.p, nblk, (const double *)B.l1f.p, nfix,
                                                  eps, it, done, done_iter, cnt,
                                                  (unsigned long long *)B.stats.p, (double *)B.l1h.p);
     }

@@ -1,0 +1,223 @@
+// gg_batch.hip -- many sources / many graphs in one pass, and the pull
+// autotuner.
+//
+// No counterpart in the reference.  Multi-source BFS is the forward phase
+// of app/BC.C (BC_F, 28-77) run for up to 32 sources at once: every vertex
+// carries a 32-bit mask, one bit per source, and one pull per level ORs the
+// frontier masks of its in-neighbours:
+//   next[v]  = (OR_{u in N_in(v)} frontier[u]) & ~seen[v]
+//   seen[v] |= next[v];  level_k[v] = level for every bit k of next[v]
+// OR is order-free, so the result is deterministic.  Many independent graphs
+// are batched by packing them into one arena graph (gg_graph_create_arena);
+// the fixed-iteration weighted PageRank and its adjoint then run over the
+// arena as one captured loop.
+#include <cstring>
+
+#include "gg_kernels.h"
+
+namespace gg {
+
+struct OpMSBFS {
+    using acc_t = uint32_t;
+    const uint32_t *fin;
+    uint32_t *fout;
+    uint32_t *seen;
+    int32_t *levels; // [nsrc][n]
+    uint32_t n;
+    int32_t level;
+    __device__ bool skip() const { return false; }
+    __device__ uint32_t identity() const { return 0u; }
+    __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a | b; }
+    __device__ uint32_t gather(uint32_t, uint32_t s) const { return fin[s]; }
+    struct row_t {
+        uint32_t seen;
+    };
+    __device__ row_t load_row(uint32_t v) const { return row_t{seen[v]}; }
+    __device__ void store_row(uint32_t v, const row_t &q, uint32_t acc, LocalStats &st) const
+    {
+        const uint32_t nw = acc & ~q.seen;
+        fout[v] = nw;
+        if (nw) {
+            seen[v] = q.seen | nw;
+            st.active++;
+            uint32_t b = nw;
+            while (b) {
+                const int k = __builtin_ctz(b);
+                b &= b - 1;
+                levels[(uint64_t)k * n + v] = level;
+            }
+        }
+    }
+};
+
+__global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_t n, uint32_t *__restrict__ fin,
+                             uint32_t *__restrict__ seen, int32_t *__restrict__ levels)
+{
+    // one thread per source; sources may repeat, so bits are ORed atomically
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsrc) return;
+    const uint32_t v = src[k];
+    atomicOr(&fin[v], 1u << k);
+    atomicOr(&seen[v], 1u << k);
+    levels[(uint64_t)k * n + v] = 0;
+}
+
+} // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t *levels, gg_run_info *info)
+{
+    if (!g || !sources || !levels) return fail(GG_ERR_INVALID, "gg_msbfs: NULL argument");
+    if (nsrc < 1 || nsrc > 32) return fail(GG_ERR_INVALID, "gg_msbfs: 1 to 32 sources");
+    for (int k = 0; k < nsrc; k++)
+        if (sources[k] >= g->n) return fail(GG_ERR_INVALID, "gg_msbfs: source out of range");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    if (info) std::memset(info, 0, sizeof(*info));
+    const uint32_t n = g->n;
+    CscView view;
+    GG_TRY(pull_view(g, &view));
+    Buf F[2], seen, src;
+    int rc = buf_alloc(c, F[0], 4ull * n);
+    if (!rc) rc = buf_alloc(c, F[1], 4ull * n);
+    if (!rc) rc = buf_alloc(c, seen, 4ull * n);
+    if (!rc) rc = buf_alloc(c, src, 4ull * nsrc);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int level = 0;
+    if (!rc) {
+        (void)hipEventRecord(e0, c->stream);
+        rc = fill_u32(c, F[0].p, n, 0);
+        if (!rc) rc = fill_u32(c, seen.p, n, 0);
+        if (!rc) rc = gg_fill(abi(c), levels, GG_I32, (uint64_t)nsrc * n, -1.0);
+        if (!rc && hipMemcpyAsync(src.p, sources, 4ull * nsrc, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            rc = fail(GG_ERR_HIP, "gg_msbfs: upload sources");
+        if (!rc) {
+            k_msbfs_init<<<1, 64, 0, c->stream>>>((const uint32_t *)src.p, nsrc, n, (uint32_t *)F[0].p,
+                                                  (uint32_t *)seen.p, levels);
+            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_msbfs: init");
+        }
+        unsigned long long *cnt = (unsigned long long *)c->counters.p;
+        // one pull per level until no vertex gains a bit (at most n levels)
+        while (!rc && level < (int)n) {
+            level++;
+            OpMSBFS op{(const uint32_t *)F[(level - 1) % 2].p, (uint32_t *)F[level % 2].p, (uint32_t *)seen.p,
+                       levels, n, level};
+            rc = fill_bytes(c, c->counters.p, 64, 0);
+            if (!rc) rc = launch_pull(c, view, op, cnt, nullptr, nullptr);
+            unsigned long long active = 0;
+            if (!rc && hipMemcpyAsync(&active, cnt + CNT_ACTIVE, 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+                rc = fail(GG_ERR_HIP, "gg_msbfs: readback");
+            if (!rc) rc = sync(c);
+            if (!rc && info && level <= 64) info->paths[level - 1] = GG_PATH_CSC;
+            if (!rc && active == 0) break;
+        }
+        (void)hipEventRecord(e1, c->stream);
+        if (!rc) rc = sync(c);
+    }
+    float ms = 0;
+    if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
+    if (info && !rc) {
+        info->iters = level;
+        info->ms = ms;
+        info->edges_scanned = (uint64_t)level * g->m;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    buf_free(F[0]);
+    buf_free(F[1]);
+    buf_free(seen);
+    buf_free(src);
+    return rc;
+}
+
+namespace gg {
+// The autotuner's probe: one pull of a gathered value per edge and one
+// write per row, the access pattern of every PageRank-like iteration.
+template <class V>
+struct OpProbe {
+    using acc_t = V;
+    const V *x;
+    V *y;
+    __device__ bool skip() const { return false; }
+    __device__ V identity() const { return V(0); }
+    __device__ V combine(V a, V b) const { return a + b; }
+    __device__ V gather(uint32_t, uint32_t s) const { return x[s]; }
+    __device__ void finish(uint32_t d, V acc, LocalStats &) const { y[d] = acc; }
+};
+
+template <class V>
+static int tune_time(Graph *g, const V *x, V *y, int trials, double *ms)
+{
+    Ctx *c = g->ctx;
+    CscView view;
+    GG_TRY(pull_view(g, &view));
+    hipEvent_t e0, e1;
+    GG_HIP(hipEventCreate(&e0));
+    GG_HIP(hipEventCreate(&e1));
+    int rc = launch_pull(c, view, OpProbe<V>{x, y}, nullptr, nullptr, nullptr); // warm-up
+    if (!rc) rc = (hipEventRecord(e0, c->stream) == hipSuccess) ? GG_OK : fail(GG_ERR_HIP, "event");
+    for (int t = 0; t < trials && !rc; t++) rc = launch_pull(c, view, OpProbe<V>{x, y}, nullptr, nullptr, nullptr);
+    if (!rc) rc = (hipEventRecord(e1, c->stream) == hipSuccess) ? GG_OK : fail(GG_ERR_HIP, "event");
+    if (!rc) rc = sync(c);
+    float f = 0;
+    if (!rc) (void)hipEventElapsedTime(&f, e0, e1);
+    *ms = f / trials;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+} // namespace gg
+
+extern "C" int gg_autotune_pull(gg_graph_t g, int dtype, int trials, gg_tune_info *out)
+{
+    if (!g || !out) return fail(GG_ERR_INVALID, "gg_autotune_pull: NULL argument");
+    if (dtype != GG_F64 && dtype != GG_F32) return fail(GG_ERR_INVALID, "gg_autotune_pull: dtype must be F64 or F32");
+    if (trials < 1 || trials > 1000) return fail(GG_ERR_INVALID, "gg_autotune_pull: trials in [1, 1000]");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    std::memset(out, 0, sizeof(*out));
+    const size_t vs = dtype == GG_F64 ? 8 : 4;
+    Buf x, y;
+    GG_TRY(buf_alloc(c, x, vs * g->n));
+    int rc = buf_alloc(c, y, vs * g->n);
+    if (!rc) rc = gg_fill(abi(c), x.p, dtype, g->n, 1.0);
+    // candidates: merge-path tiles, then row blocks at 1..32 edges per lane
+    static const int kEpl[] = {1, 2, 4, 8, 16, 32};
+    const int ncand = 1 + (int)(sizeof(kEpl) / sizeof(kEpl[0]));
+    int best = -1;
+    double best_ms = 0;
+    const int keep_sched = g->pull_schedule, keep_epl = g->edges_per_lane;
+    for (int k = 0; k < ncand && !rc; k++) {
+        const int sched = k == 0 ? 0 : 1, epl = k == 0 ? keep_epl : kEpl[k - 1];
+        rc = gg_graph_set_option(g, GG_OPT_PULL_SCHEDULE, sched);
+        if (!rc) rc = gg_graph_set_option(g, GG_OPT_ROW_EDGES_PER_LANE, epl);
+        double ms = 0;
+        if (!rc)
+            rc = dtype == GG_F64 ? tune_time<double>(g, (const double *)x.p, (double *)y.p, trials, &ms)
+                                 : tune_time<float>(g, (const float *)x.p, (float *)y.p, trials, &ms);
+        if (rc) break;
+        out->cand_schedule[k] = sched;
+        out->cand_edges_per_lane[k] = epl;
+        out->cand_ms[k] = ms;
+        if (best < 0 || ms < best_ms) {
+            best = k;
+            best_ms = ms;
+        }
+    }
+    buf_free(x);
+    buf_free(y);
+    if (rc) {
+        (void)gg_graph_set_option(g, GG_OPT_PULL_SCHEDULE, keep_sched);
+        (void)gg_graph_set_option(g, GG_OPT_ROW_EDGES_PER_LANE, keep_epl);
+        return rc;
+    }
+    out->candidates = ncand;
+    out->schedule = out->cand_schedule[best];
+    out->edges_per_lane = out->cand_edges_per_lane[best];
+    out->ms = best_ms;
+    GG_TRY(gg_graph_set_option(g, GG_OPT_PULL_SCHEDULE, out->schedule));
+    return gg_graph_set_option(g, GG_OPT_ROW_EDGES_PER_LANE, out->edges_per_lane);
+}

@@ -192,6 +192,118 @@ int sched_build(Ctx *c, PullSched &s, const uint32_t *d_off, uint32_t rows, uint
     return GG_OK;
 }
 
+void graph_exec_free(GraphExec &x)
+{
+    if (x.exec) (void)hipGraphExecDestroy(x.exec);
+    if (x.graph) (void)hipGraphDestroy(x.graph);
+    x.exec = nullptr;
+    x.graph = nullptr;
+    for (auto &k : x.key) k = 0;
+    for (auto &b : x.own) buf_free(b);
+}
+
+void graph_exec_drop(Graph *g)
+{
+    graph_exec_free(g->fwd_exec);
+    graph_exec_free(g->adj_exec);
+}
+
+int ensure_rows(Graph *g)
+{
+    if (g->rows_ok) return GG_OK;
+    GG_TRY(rows_build(g->ctx, g->rows, (const uint32_t *)g->toff.p, g->n, g->m, g->edges_per_lane));
+    g->rows_ok = true;
+    return GG_OK;
+}
+
+void rows_free(RowSched &s)
+{
+    buf_free(s.work);
+    buf_free(s.long_rows);
+    buf_free(s.slot_val);
+    s.blocks = s.nlong = s.slots = 0;
+}
+
+// Greedy packing of rows in index order (host pass over the offsets; part of
+// the untimed graph preparation).  A block closes when the next row would
+// exceed kRowEdges edges or kRowCap rows, or when the next row is on the
+// other side of kRowBig edges than the block's rows (so one long row does
+// not hold up a block of short ones); rows above kRowEdges edges become
+// chunked long rows.  Lanes per row: the power of two G <= 64 that gives
+// each lane about 'edges_per_lane' of the block's average row.
+int rows_build(Ctx *c, RowSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges, int edges_per_lane)
+{
+    rows_free(s);
+    std::vector<uint32_t> off((size_t)rows + 1);
+    GG_HIP(hipMemcpyAsync(off.data(), d_off, 4ull * (rows + 1), hipMemcpyDeviceToHost, c->stream));
+    GG_TRY(sync(c));
+    if ((uint64_t)off[rows] != edges) return fail(GG_ERR_SIZE, "rows_build: offsets do not end at the edge count");
+    const uint32_t kRowBig = 256;
+    std::vector<RowWork> work;
+    std::vector<uint32_t> longs;
+    work.reserve(edges / kRowEdges + rows / kRowCap + 16);
+    uint32_t slots = 0;
+    uint32_t r0 = 0, nr = 0;
+    uint64_t ne = 0;
+    bool big = false;
+    auto close = [&]() {
+        if (!nr) return;
+        RowWork w{};
+        w.r0 = r0;
+        w.r1 = r0 + nr;
+        w.e0 = off[r0];
+        w.e1 = off[r0 + nr];
+        const uint64_t per = (ne + (uint64_t)nr * edges_per_lane - 1) / ((uint64_t)nr * edges_per_lane);
+        uint32_t lg = 0;
+        while (lg < 6 && (1ull << lg) < per) lg++;
+        w.kind = lg;
+        work.push_back(w);
+        nr = 0;
+        ne = 0;
+    };
+    for (uint32_t r = 0; r < rows; r++) {
+        const uint32_t d = off[r + 1] - off[r];
+        if (d > (uint32_t)kRowEdges) {
+            close();
+            longs.push_back(r);
+            longs.push_back(slots);
+            for (uint32_t e = off[r]; e < off[r + 1]; e += kRowEdges) {
+                RowWork w{};
+                w.r0 = r;
+                w.r1 = r + 1;
+                w.e0 = e;
+                w.e1 = std::min<uint32_t>(e + kRowEdges, off[r + 1]);
+                w.kind = kRowLong;
+                w.slot = slots++;
+                work.push_back(w);
+            }
+            longs.push_back(slots);
+            continue;
+        }
+        const bool b = d > kRowBig;
+        if (nr && (ne + d > (uint64_t)kRowEdges || nr + 1 > (uint32_t)kRowCap || b != big)) close();
+        if (!nr) {
+            r0 = r;
+            big = b;
+        }
+        nr++;
+        ne += d;
+    }
+    close();
+    if (work.size() >= (1ull << 31)) return fail(GG_ERR_SIZE, "rows_build: too many blocks");
+    s.blocks = (uint32_t)work.size();
+    s.nlong = (uint32_t)(longs.size() / 3);
+    s.slots = slots;
+    GG_TRY(buf_alloc(c, s.work, sizeof(RowWork) * (uint64_t)std::max<size_t>(work.size(), 1)));
+    GG_TRY(buf_alloc(c, s.long_rows, 4ull * std::max<size_t>(longs.size(), 1)));
+    GG_TRY(buf_alloc(c, s.slot_val, 8ull * std::max<uint32_t>(slots, 1)));
+    if (!work.empty())
+        GG_HIP(hipMemcpyAsync(s.work.p, work.data(), sizeof(RowWork) * work.size(), hipMemcpyHostToDevice, c->stream));
+    if (!longs.empty())
+        GG_HIP(hipMemcpyAsync(s.long_rows.p, longs.data(), 4 * longs.size(), hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
 } // namespace gg
 
 using namespace gg;
@@ -529,6 +641,10 @@ static void graph_free(Graph *g)
                    &g->csr_pull.start_edge, &g->csr_pull.carry_row, &g->csr_pull.carry_val,
                    &g->csr_pull.head_val, &g->wsum};
     for (Buf *b : bufs) buf_free(*b);
+    rows_free(g->plan.rows);
+    rows_free(g->rows);
+    g->rows_ok = false;
+    graph_exec_drop(g);
     g->plan.mode = 0;
 }
 
@@ -560,6 +676,76 @@ extern "C" int gg_graph_create_csr(gg_ctx_t c, uint32_t n, uint64_t m, const uin
     if (!rc && hipMemcpyAsync(g->off.p, off, 4ull * (n + 1), hipMemcpyHostToDevice, c->stream)) rc = fail(GG_ERR_HIP, "upload offsets");
     if (!rc && m && hipMemcpyAsync(g->ids.p, ids, 4ull * m, hipMemcpyHostToDevice, c->stream)) rc = fail(GG_ERR_HIP, "upload ids");
     if (!rc && w && m && hipMemcpyAsync(g->w.p, w, 4ull * m, hipMemcpyHostToDevice, c->stream)) rc = fail(GG_ERR_HIP, "upload weights");
+    if (!rc) rc = build_twin(g);
+    if (rc) {
+        graph_free(g);
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return GG_OK;
+}
+
+namespace gg {
+// out[i] = in[i] + add
+__global__ void k_shift_u32(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t count,
+                            uint32_t add)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) out[i] = in[i] + add;
+}
+} // namespace gg
+
+extern "C" int gg_graph_create_arena(gg_ctx_t c, int count, const gg_graph_t *parts, gg_graph_t *out,
+                                     uint32_t *vbase, uint64_t *ebase)
+{
+    if (!c || !parts || !out || count < 1) return fail(GG_ERR_INVALID, "gg_graph_create_arena: bad argument");
+    uint64_t n = 0, m = 0;
+    bool weighted = true, symmetric = true;
+    for (int i = 0; i < count; i++) {
+        if (!parts[i] || parts[i]->ctx != c)
+            return fail(GG_ERR_INVALID, "gg_graph_create_arena: graph " + std::to_string(i) + " missing or on another context");
+        n += parts[i]->n;
+        m += parts[i]->m;
+        weighted = weighted && parts[i]->weighted;
+        symmetric = symmetric && parts[i]->symmetric;
+    }
+    if (n >= 0xffffffffULL || m >= 0xffffffffULL)
+        return fail(GG_ERR_SIZE, "gg_graph_create_arena: the arena needs n, m < 2^32");
+    GG_HIP(hipSetDevice(c->device));
+    gg_graph_s *g = new gg_graph_s();
+    g->ctx = c;
+    g->n = (uint32_t)n;
+    g->m = m;
+    g->symmetric = symmetric;
+    g->weighted = weighted;
+    int rc = buf_alloc(c, g->off, 4ull * (n + 1));
+    if (!rc) rc = buf_alloc(c, g->ids, 4ull * (m ? m : 1));
+    if (!rc && weighted) rc = buf_alloc(c, g->w, 4ull * (m ? m : 1));
+    uint64_t vb = 0, eb = 0;
+    for (int i = 0; i < count && !rc; i++) {
+        const Graph *p = parts[i];
+        if (vbase) vbase[i] = (uint32_t)vb;
+        if (ebase) ebase[i] = eb;
+        const uint32_t gv = blocks_for(std::max<uint64_t>(p->n, 1), kBlock);
+        k_shift_u32<<<std::min<uint32_t>(gv, 1u << 16), kBlock, 0, c->stream>>>(
+            (const uint32_t *)p->off.p, (uint32_t *)g->off.p + vb, p->n, (uint32_t)eb);
+        if (p->m) {
+            const uint32_t ge = blocks_for(p->m, kBlock);
+            k_shift_u32<<<std::min<uint32_t>(ge, 1u << 16), kBlock, 0, c->stream>>>(
+                (const uint32_t *)p->ids.p, (uint32_t *)g->ids.p + eb, p->m, (uint32_t)vb);
+            if (weighted && hipMemcpyAsync((float *)g->w.p + eb, p->w.p, 4ull * p->m, hipMemcpyDeviceToDevice,
+                                           c->stream) != hipSuccess)
+                rc = fail(GG_ERR_HIP, "gg_graph_create_arena: weight copy");
+        }
+        vb += p->n;
+        eb += p->m;
+    }
+    if (vbase) vbase[count] = (uint32_t)vb;
+    if (ebase) ebase[count] = eb;
+    if (!rc) rc = fill_u32(c, (uint32_t *)g->off.p + n, 1, (uint32_t)m);
+    if (!rc && hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_graph_create_arena: launch");
     if (!rc) rc = build_twin(g);
     if (rc) {
         graph_free(g);
@@ -717,6 +903,41 @@ extern "C" int gg_graph_degrees(gg_graph_t g, int in_edges, uint32_t *out)
     return sync(g->ctx);
 }
 
+namespace gg {
+static int weights_changed(gg_graph_t g)
+{
+    Ctx *c = g->ctx;
+    const uint64_t m = g->m;
+    g->wsum_ok = false;
+    graph_exec_drop(g);
+    if (!m) return GG_OK;
+    uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+    k_gather_f32<<<grid, kBlock, 0, c->stream>>>((float *)g->w.p, (uint32_t *)g->perm.p, (float *)g->tw.p, m);
+    GG_HIP(hipGetLastError());
+    if (g->coo_w.p) {
+        // COO weights follow the COO order: rebuild them by re-partitioning
+        int rc = gg_graph_partition(g, g->P, g->part_vertex, g->relabel, g->hilbert);
+        if (rc) return rc;
+    }
+    return sync(c);
+}
+} // namespace gg
+
+extern "C" int gg_graph_set_weights_dev(gg_graph_t g, const float *w_dev)
+{
+    if (!g || !w_dev) return fail(GG_ERR_INVALID, "gg_graph_set_weights_dev: NULL");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    const uint64_t m = g->m;
+    if (!g->weighted) {
+        GG_TRY(buf_alloc(c, g->w, 4 * (m ? m : 1)));
+        GG_TRY(buf_alloc(c, g->tw, 4 * (m ? m : 1)));
+        g->weighted = true;
+    }
+    if (m) GG_HIP(hipMemcpyAsync(g->w.p, w_dev, 4 * m, hipMemcpyDeviceToDevice, c->stream));
+    return weights_changed(g);
+}
+
 extern "C" int gg_graph_set_weights(gg_graph_t g, const float *w_csr)
 {
     if (!g || !w_csr) return fail(GG_ERR_INVALID, "gg_graph_set_weights: NULL");
@@ -727,20 +948,8 @@ extern "C" int gg_graph_set_weights(gg_graph_t g, const float *w_csr)
         GG_TRY(buf_alloc(c, g->tw, 4 * (m ? m : 1)));
         g->weighted = true;
     }
-    g->wsum_ok = false;
-    if (!m) return GG_OK;
-    GG_HIP(hipMemcpyAsync(g->w.p, w_csr, 4 * m, hipMemcpyHostToDevice, c->stream));
-    uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
-    k_gather_f32<<<grid, kBlock, 0, c->stream>>>((float *)g->w.p, (uint32_t *)g->perm.p,
-                                                  (float *)g->tw.p, m);
-    GG_HIP(hipGetLastError());
-    if (g->coo_w.p) {
-        // COO weights follow the COO order; rebuild them from the partition
-        // by re-partitioning (rare path)
-        int rc = gg_graph_partition(g, g->P, g->part_vertex, g->relabel, g->hilbert);
-        if (rc) return rc;
-    }
-    return sync(c);
+    if (m) GG_HIP(hipMemcpyAsync(g->w.p, w_csr, 4 * m, hipMemcpyHostToDevice, c->stream));
+    return weights_changed(g);
 }
 
 // =============================================================================
@@ -1177,6 +1386,7 @@ extern "C" int gg_graph_reorder(gg_graph_t g, int mode)
     Buf *pb[] = {&P.perm, &P.inv, &P.toff, &P.tids, &P.tw, &P.outdeg, &P.sched.start_row, &P.sched.start_edge,
                  &P.sched.carry_row, &P.sched.carry_val, &P.sched.head_val};
     for (Buf *b : pb) buf_free(*b);
+    rows_free(P.rows);
     P.mode = 0;
     if (mode == 0) return GG_OK;
     const uint32_t n = g->n;
@@ -1245,8 +1455,47 @@ extern "C" int gg_graph_reorder(gg_graph_t g, int mode)
     GG_TRY(sched_build(c, P.sched, (const uint32_t *)P.toff.p, n, m));
     GG_TRY(sync(c));
     buf_free(deg);
+    GG_TRY(rows_build(c, P.rows, (const uint32_t *)P.toff.p, n, m, g->edges_per_lane));
     P.mode = mode;
     return GG_OK;
+}
+
+extern "C" int gg_graph_set_option(gg_graph_t g, int option, int64_t value)
+{
+    if (!g) return fail(GG_ERR_INVALID, "gg_graph_set_option: NULL");
+    GG_HIP(hipSetDevice(g->ctx->device));
+    switch (option) {
+    case GG_OPT_PULL_SCHEDULE:
+        if (value != 0 && value != 1) return fail(GG_ERR_INVALID, "gg_graph_set_option: schedule must be 0 or 1");
+        if (g->pull_schedule != (int)value) graph_exec_drop(g);
+        g->pull_schedule = (int)value;
+        return GG_OK;
+    case GG_OPT_ROW_EDGES_PER_LANE:
+        if (value < 1 || value > kRowEdges)
+            return fail(GG_ERR_INVALID, "gg_graph_set_option: edges per lane out of range");
+        if (g->edges_per_lane != (int)value) {
+            graph_exec_drop(g);
+            rows_free(g->rows);
+            g->rows_ok = false;
+        }
+        g->edges_per_lane = (int)value;
+        if (g->plan.mode)
+            return rows_build(g->ctx, g->plan.rows, (const uint32_t *)g->plan.toff.p, g->n, g->m,
+                              g->edges_per_lane);
+        return GG_OK;
+    default:
+        return fail(GG_ERR_INVALID, "gg_graph_set_option: unknown option " + std::to_string(option));
+    }
+}
+
+extern "C" int gg_graph_get_option(gg_graph_t g, int option, int64_t *value)
+{
+    if (!g || !value) return fail(GG_ERR_INVALID, "gg_graph_get_option: NULL");
+    switch (option) {
+    case GG_OPT_PULL_SCHEDULE: *value = g->pull_schedule; return GG_OK;
+    case GG_OPT_ROW_EDGES_PER_LANE: *value = g->edges_per_lane; return GG_OK;
+    default: return fail(GG_ERR_INVALID, "gg_graph_get_option: unknown option " + std::to_string(option));
+    }
 }
 
 extern "C" int gg_graph_device_view(gg_graph_t g, int which, gg_graph_view *out)
@@ -1265,6 +1514,10 @@ extern "C" int gg_graph_device_view(gg_graph_t g, int which, gg_graph_view *out)
     out->tiles = s.tiles;
     out->n = g->n;
     out->m = g->m;
+    const RowSched *rs = p ? &g->plan.rows : nullptr;
+    out->row_work = rs && rs->blocks ? (const uint32_t *)rs->work.p : nullptr;
+    out->row_blocks = rs ? rs->blocks : 0;
+    out->row_long = rs ? rs->nlong : 0;
     return GG_OK;
 }
 

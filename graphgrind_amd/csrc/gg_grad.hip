@@ -348,10 +348,9 @@ static int run_fixed_point(Graph *g, const CscView &view, int max_iter, int rela
                            SolveEvents &E, int *iters, std::vector<double> *hist_out)
 {
     Ctx *c = g->ctx;
-    PullSched &s = *view.sched;
-    const uint32_t nfix = blocks_for(s.tiles, kBlock);
+    const uint32_t nfix = view.fix_blocks();
     Buf l1p, l1f, flags, hist;
-    int rc = buf_alloc(c, l1p, 16ull * s.tiles);
+    int rc = buf_alloc(c, l1p, 16ull * view.blocks());
     if (!rc) rc = buf_alloc(c, l1f, 16ull * nfix);
     if (!rc) rc = buf_alloc(c, flags, 64);
     if (!rc) rc = buf_alloc(c, hist, 16ull * max_iter);
@@ -413,13 +412,158 @@ static void record_times(Ctx *c, SolveEvents &E, int iters, gg_run_info *info, u
 }
 
 template <class V>
+__global__ void k_copy(V *__restrict__ out, const V *__restrict__ in, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+template <class V, bool kV>
+static OpWPR<V, kV> wpr_op(Graph *g, const Buf *K, int it, V *r, const V *v, V vconst, double alpha,
+                           const int *done)
+{
+    OpWPR<V, kV> op;
+    op.tw = (const float *)g->tw.p;
+    op.cin = (const V *)K[(it - 1) % 2].p;
+    op.cout = (V *)K[it % 2].p;
+    op.r = r;
+    op.v = v;
+    op.vconst = vconst;
+    op.S = (const double *)g->wsum.p;
+    op.alpha = (V)alpha;
+    op.beta = (V)(1 - alpha);
+    op.done = done;
+    return op;
+}
+
+template <class V>
+static OpWPRAdj<V> adj_op(Graph *g, const Buf *L, int it, const V *u, double alpha, const int *done)
+{
+    OpWPRAdj<V> op;
+    op.w = (const float *)g->w.p;
+    op.lin = (const V *)L[(it - 1) % 2].p;
+    op.lout = (V *)L[it % 2].p;
+    op.u = u;
+    op.S = (const double *)g->wsum.p;
+    op.alpha = (V)alpha;
+    op.done = done;
+    return op;
+}
+
+static uint64_t dbits(double x)
+{
+    uint64_t b;
+    std::memcpy(&b, &x, 8);
+    return b;
+}
+
+// Captures what 'body' enqueues on the context stream into X (a hipGraph and
+// its executable); the stream leaves capture mode on every path.
+template <class Body>
+static int capture(Ctx *c, Body body, GraphExec &X)
+{
+    GG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    if (rc != GG_OK || e != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc != GG_OK ? rc : fail(GG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    const hipError_t e2 = hipGraphInstantiate(&X.exec, gr, nullptr, nullptr, 0);
+    if (e2 != hipSuccess) {
+        (void)hipGraphDestroy(gr);
+        X.exec = nullptr;
+        return fail(GG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    }
+    X.graph = gr;
+    return GG_OK;
+}
+
+// Replays a captured loop between two events; ms = the replay's device time.
+static int replay(Ctx *c, GraphExec &X, float *ms)
+{
+    hipEvent_t e0, e1;
+    GG_HIP(hipEventCreate(&e0));
+    GG_HIP(hipEventCreate(&e1));
+    int rc = GG_OK;
+    if (hipEventRecord(e0, c->stream) != hipSuccess || hipGraphLaunch(X.exec, c->stream) != hipSuccess ||
+        hipEventRecord(e1, c->stream) != hipSuccess)
+        rc = fail(GG_ERR_HIP, "hipGraphLaunch");
+    if (!rc) rc = sync(c);
+    if (!rc) (void)hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+static void fixed_info(Ctx *c, float ms, int iters, uint64_t m, gg_run_info *info)
+{
+    c->last_kernel_ms = ms;
+    c->last_kernel_launches = iters;
+    if (info) {
+        info->iters = iters;
+        info->ms = ms;
+        info->edges_scanned = (uint64_t)iters * m;
+    }
+}
+
+// Exactly 'iters' forward iterations (no stop rule), captured once per
+// (buffers, parameters) and replayed as one hipGraph launch.
+template <class V>
+static int wpagerank_fixed(Graph *g, const CscView &view, double alpha, int iters, const V *v, V *r,
+                           gg_run_info *info)
+{
+    Ctx *c = g->ctx;
+    const uint32_t n = g->n;
+    const V vconst = (V)(1 / (double)n);
+    GraphExec &X = g->fwd_exec;
+    const uint64_t key[10] = {1, sizeof(V), (uint64_t)(uintptr_t)r, (uint64_t)(uintptr_t)v, dbits(alpha),
+                              (uint64_t)iters, 0, 0, 0, 0};
+    if (!X.exec || std::memcmp(X.key, key, sizeof key) != 0) {
+        graph_exec_free(X);
+        GG_TRY(buf_alloc(c, X.own[0], sizeof(V) * (uint64_t)n));
+        GG_TRY(buf_alloc(c, X.own[1], sizeof(V) * (uint64_t)n));
+        GG_TRY(buf_alloc(c, X.own[2], 64));
+        GG_TRY(fill_bytes(c, X.own[2].p, 64, 0));
+        GG_TRY(sync(c));
+        const int *done = (const int *)X.own[2].p;
+        auto body = [&]() -> int {
+            k_wpr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(r, (V *)X.own[0].p, v, vconst,
+                                                                           (const double *)g->wsum.p, n);
+            for (int it = 1; it <= iters; it++) {
+                const int rc = v ? launch_pull(c, view, wpr_op<V, true>(g, X.own, it, r, v, vconst, alpha, done),
+                                               nullptr, nullptr, nullptr)
+                                 : launch_pull(c, view, wpr_op<V, false>(g, X.own, it, r, v, vconst, alpha, done),
+                                               nullptr, nullptr, nullptr);
+                if (rc) return rc;
+            }
+            return GG_OK;
+        };
+        const int rc = capture(c, body, X);
+        if (rc) {
+            graph_exec_free(X);
+            return rc;
+        }
+        std::memcpy(X.key, key, sizeof key);
+    }
+    float ms = 0;
+    GG_TRY(replay(c, X, &ms));
+    fixed_info(c, ms, iters, g->m, info);
+    return GG_OK;
+}
+
+template <class V>
 static int wpagerank_impl(Graph *g, double alpha, double eps, int max_iter, const V *v, V *r, gg_run_info *info)
 {
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
+    GG_TRY(ensure_wsum(g));
+    CscView view;
+    GG_TRY(pull_view(g, &view));
+    if (eps < 0) return wpagerank_fixed<V>(g, view, alpha, max_iter, v, r, info);
     SolveEvents E(max_iter);
     GG_HIP(hipEventRecord(E.ev[0], c->stream));
-    GG_TRY(ensure_wsum(g));
     Buf K[2];
     int rc = buf_alloc(c, K[0], sizeof(V) * (uint64_t)n);
     if (!rc) rc = buf_alloc(c, K[1], sizeof(V) * (uint64_t)n);
@@ -428,36 +572,10 @@ static int wpagerank_impl(Graph *g, double alpha, double eps, int max_iter, cons
         k_wpr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(r, (V *)K[0].p, v, vconst,
                                                                        (const double *)g->wsum.p, n);
         int iters = 0;
-        auto make_v = [&](int it, const int *done) {
-            OpWPR<V, true> op;
-            op.tw = (const float *)g->tw.p;
-            op.cin = (const V *)K[(it - 1) % 2].p;
-            op.cout = (V *)K[it % 2].p;
-            op.r = r;
-            op.v = v;
-            op.vconst = vconst;
-            op.S = (const double *)g->wsum.p;
-            op.alpha = (V)alpha;
-            op.beta = (V)(1 - alpha);
-            op.done = done;
-            return op;
-        };
-        auto make_u = [&](int it, const int *done) {
-            OpWPR<V, false> op;
-            op.tw = (const float *)g->tw.p;
-            op.cin = (const V *)K[(it - 1) % 2].p;
-            op.cout = (V *)K[it % 2].p;
-            op.r = r;
-            op.v = nullptr;
-            op.vconst = vconst;
-            op.S = (const double *)g->wsum.p;
-            op.alpha = (V)alpha;
-            op.beta = (V)(1 - alpha);
-            op.done = done;
-            return op;
-        };
-        rc = v ? run_fixed_point(g, graph_view(g), max_iter, 0, eps, make_v, E, &iters, nullptr)
-               : run_fixed_point(g, graph_view(g), max_iter, 0, eps, make_u, E, &iters, nullptr);
+        auto make_v = [&](int it, const int *done) { return wpr_op<V, true>(g, K, it, r, v, vconst, alpha, done); };
+        auto make_u = [&](int it, const int *done) { return wpr_op<V, false>(g, K, it, r, v, vconst, alpha, done); };
+        rc = v ? run_fixed_point(g, view, max_iter, 0, eps, make_v, E, &iters, nullptr)
+               : run_fixed_point(g, view, max_iter, 0, eps, make_u, E, &iters, nullptr);
         if (!rc) {
             GG_HIP(hipEventRecord(E.ev[1], c->stream));
             rc = sync(c);
@@ -469,50 +587,86 @@ static int wpagerank_impl(Graph *g, double alpha, double eps, int max_iter, cons
     return rc;
 }
 
+// Enqueues the gradient outputs from the converged adjoint 'lam'.
+template <class V>
+static int adjoint_outputs(Graph *g, double alpha, const V *r, const V *u, const V *lam, V *gw, V *gv, V *lam_out)
+{
+    Ctx *c = g->ctx;
+    const uint32_t n = g->n;
+    if (gw && g->m) {
+        EdgeWPRGrad<V> op;
+        op.r = r;
+        op.lam = lam;
+        op.u = u;
+        op.S = (const double *)g->wsum.p;
+        op.alpha = (V)alpha;
+        op.gw = gw;
+        GG_TRY(launch_edge(c, g, op));
+    }
+    if (gv) k_scale<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(gv, lam, (V)(1 - alpha), n);
+    if (lam_out) k_copy<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(lam_out, lam, n);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+template <class V>
+static int wpagerank_adjoint_fixed(Graph *g, double alpha, int iters, const V *r, const V *u, V *gw, V *gv,
+                                   V *lam_out, gg_run_info *info)
+{
+    Ctx *c = g->ctx;
+    const uint32_t n = g->n;
+    GraphExec &X = g->adj_exec;
+    const uint64_t key[10] = {2, sizeof(V), (uint64_t)(uintptr_t)r, (uint64_t)(uintptr_t)u,
+                              (uint64_t)(uintptr_t)gw, (uint64_t)(uintptr_t)gv, (uint64_t)(uintptr_t)lam_out,
+                              dbits(alpha), (uint64_t)iters, 0};
+    if (!X.exec || std::memcmp(X.key, key, sizeof key) != 0) {
+        graph_exec_free(X);
+        GG_TRY(buf_alloc(c, X.own[0], sizeof(V) * (uint64_t)n));
+        GG_TRY(buf_alloc(c, X.own[1], sizeof(V) * (uint64_t)n));
+        GG_TRY(buf_alloc(c, X.own[2], 64));
+        GG_TRY(fill_bytes(c, X.own[2].p, 64, 0));
+        GG_TRY(sync(c));
+        const int *done = (const int *)X.own[2].p;
+        auto body = [&]() -> int {
+            k_copy<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)X.own[0].p, u, n);
+            for (int it = 1; it <= iters; it++)
+                GG_TRY(launch_pull(c, csr_view(g), adj_op<V>(g, X.own, it, u, alpha, done), nullptr, nullptr,
+                                   nullptr));
+            return adjoint_outputs<V>(g, alpha, r, u, (const V *)X.own[iters % 2].p, gw, gv, lam_out);
+        };
+        const int rc = capture(c, body, X);
+        if (rc) {
+            graph_exec_free(X);
+            return rc;
+        }
+        std::memcpy(X.key, key, sizeof key);
+    }
+    float ms = 0;
+    GG_TRY(replay(c, X, &ms));
+    fixed_info(c, ms, iters, g->m, info);
+    return GG_OK;
+}
+
 template <class V>
 static int wpagerank_adjoint_impl(Graph *g, double alpha, double eps, int max_iter, const V *r, const V *u, V *gw,
                                   V *gv, V *lam_out, gg_run_info *info)
 {
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
+    GG_TRY(ensure_wsum(g));
+    GG_TRY(ensure_csr_sched(g));
+    if (eps < 0) return wpagerank_adjoint_fixed<V>(g, alpha, max_iter, r, u, gw, gv, lam_out, info);
     SolveEvents E(max_iter);
     GG_HIP(hipEventRecord(E.ev[0], c->stream));
-    GG_TRY(ensure_wsum(g));
     Buf L[2];
     int rc = buf_alloc(c, L[0], sizeof(V) * (uint64_t)n);
     if (!rc) rc = buf_alloc(c, L[1], sizeof(V) * (uint64_t)n);
     if (!rc) {
-        GG_HIP(hipMemcpyAsync(L[0].p, u, sizeof(V) * (uint64_t)n, hipMemcpyDeviceToDevice, c->stream));
+        k_copy<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)L[0].p, u, n);
         int iters = 0;
-        auto make = [&](int it, const int *done) {
-            OpWPRAdj<V> op;
-            op.w = (const float *)g->w.p;
-            op.lin = (const V *)L[(it - 1) % 2].p;
-            op.lout = (V *)L[it % 2].p;
-            op.u = u;
-            op.S = (const double *)g->wsum.p;
-            op.alpha = (V)alpha;
-            op.done = done;
-            return op;
-        };
+        auto make = [&](int it, const int *done) { return adj_op<V>(g, L, it, u, alpha, done); };
         rc = run_fixed_point(g, csr_view(g), max_iter, 1, eps, make, E, &iters, nullptr);
-        const V *lam = (const V *)L[iters % 2].p;
-        if (!rc && gw && g->m) {
-            EdgeWPRGrad<V> op;
-            op.r = r;
-            op.lam = lam;
-            op.u = u;
-            op.S = (const double *)g->wsum.p;
-            op.alpha = (V)alpha;
-            op.gw = gw;
-            rc = launch_edge(c, g, op);
-        }
-        if (!rc && gv) {
-            k_scale<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(gv, lam, (V)(1 - alpha), n);
-            GG_HIP(hipGetLastError());
-        }
-        if (!rc && lam_out)
-            GG_HIP(hipMemcpyAsync(lam_out, lam, sizeof(V) * (uint64_t)n, hipMemcpyDeviceToDevice, c->stream));
+        if (!rc) rc = adjoint_outputs<V>(g, alpha, r, u, (const V *)L[iters % 2].p, gw, gv, lam_out);
         if (!rc) {
             GG_HIP(hipEventRecord(E.ev[1], c->stream));
             rc = sync(c);

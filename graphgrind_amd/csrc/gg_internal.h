@@ -65,6 +65,34 @@ struct PullSched {
     Buf head_val;   // 8 bytes [tiles] partial of the first row if it began earlier
 };
 
+// ---- row-block pull schedule (CSR-Adaptive) -----------------------------------
+// Consecutive complete rows are packed into blocks of at most kRowEdges
+// edges and kRowCap rows (Greathouse & Daga, "Efficient Sparse Matrix-Vector
+// Multiplication on GPUs using the CSR Storage Format", SC 2014): a block
+// streams its edges' gathered values into LDS with coalesced loads and then
+// reduces each row without a scan, with 2^kind lanes per row (1 lane = one
+// thread per row; up to a whole wave), chosen from the block's average
+// degree.  A row with more than kRowEdges edges is split into chunks, one
+// block each; its partials are combined in chunk order by the fix-up kernel.
+constexpr int kRowEdges = kBlock * kPullIPT; // edges staged per block
+constexpr int kRowCap = 4 * kBlock;          // rows per block
+constexpr uint32_t kRowLong = 0xffu;         // kind of a long-row chunk
+struct RowWork {
+    uint32_t r0, r1;   // rows [r0, r1) (long: the row in r0)
+    uint32_t e0, e1;   // edges [e0, e1)
+    uint32_t kind;     // log2(lanes per row), or kRowLong
+    uint32_t slot;     // long: partial slot of this chunk
+    uint32_t pad[2];
+};
+struct RowSched {
+    uint32_t blocks = 0; // work items, one per block
+    uint32_t nlong = 0;  // rows split into chunks
+    uint32_t slots = 0;  // chunks of those rows
+    Buf work;            // RowWork[blocks]
+    Buf long_rows;       // uint32[3*nlong]: row, first slot, end slot
+    Buf slot_val;        // 8 bytes [slots]
+};
+
 // ---- relabelled pull plan ----------------------------------------------------
 // An internal renumbering of the vertices (hot sources first) with its own
 // in-edge twin and schedule; kernels run in the new numbering and results
@@ -77,14 +105,40 @@ struct PullPlan {
     Buf toff, tids, tw;  // relabelled in-edge twin (tw when weighted)
     Buf outdeg;          // uint32[n] in the new numbering
     PullSched sched;
+    RowSched rows;
 };
 
-// In-edge arrays + schedule a pull runs over.
+// In-edge arrays + schedule a pull runs over: the row-block schedule when
+// 'rows' is set, else the merge-path tiles.
 struct CscView {
     const uint32_t *toff = nullptr;
     const uint32_t *tids = nullptr;
     PullSched *sched = nullptr;
+    RowSched *rows = nullptr;
+    // work-groups of the main pull kernel (per-block residual partials)
+    uint32_t blocks() const { return rows ? rows->blocks : sched->tiles; }
+    // work-groups of the fix-up kernel (per-block folded residuals)
+    uint32_t fix_blocks() const
+    {
+        return rows ? blocks_for(rows->nlong > rows->blocks ? rows->nlong : rows->blocks, kBlock)
+                    : blocks_for(sched->tiles, kBlock);
+    }
 };
+int rows_build(Ctx *c, RowSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges, int edges_per_lane);
+void rows_free(RowSched &s);
+
+// ---- captured fixed-iteration loops ---------------------------------------------
+// A loop with a fixed iteration count has no data-dependent exit, so it is
+// captured once into a hipGraph and replayed as one launch.  The capture
+// bakes in buffer addresses and parameters; 'key' records them and the
+// buffers the loop owns live here, so a call with the same key replays.
+struct GraphExec {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t key[10] = {};
+    Buf own[4];
+};
+void graph_exec_free(GraphExec &x);
 
 // ---- context ----------------------------------------------------------------
 struct Ctx {
@@ -129,7 +183,19 @@ struct Graph {
     bool csr_pull_ok = false;
     Buf wsum;            // double[n]
     bool wsum_ok = false;
+    // row blocks over the in-edge twin, built on first use
+    RowSched rows;
+    bool rows_ok = false;
+    // captured fixed-iteration loops (weighted PageRank forward / adjoint)
+    GraphExec fwd_exec, adj_exec;
+    // tuning (gg_graph_set_option)
+    int pull_schedule = 1;  // 0 merge-path tiles, 1 row blocks where built
+    int edges_per_lane = 8; // row blocks: target edges per lane of a row group
 };
+// Drops captured loops (their addresses or schedules changed).
+void graph_exec_drop(Graph *g);
+// Row blocks of the in-edge twin (built on first use).
+int ensure_rows(Graph *g);
 
 // ---- frontier ---------------------------------------------------------------
 struct Frontier {

@@ -334,12 +334,207 @@ __global__ __launch_bounds__(kBlock) void k_pull_fixup(Op op, const uint32_t *__
     }
 }
 
+// Lane shuffle of any 4- or 8-byte value within segments of 'width' lanes.
+template <class T>
+__device__ __forceinline__ T shfl_down_any(T v, int off, int width = 64)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte values");
+    if constexpr (sizeof(T) == 4) {
+        int x;
+        __builtin_memcpy(&x, &v, 4);
+        x = __shfl_down(x, off, width);
+        __builtin_memcpy(&v, &x, 4);
+    } else {
+        long long x;
+        __builtin_memcpy(&x, &v, 8);
+        x = __shfl_down(x, off, width);
+        __builtin_memcpy(&v, &x, 8);
+    }
+    return v;
+}
+
+// Wave reduction in a fixed tree; the result is in lane 0.
+template <class Op, class A>
+__device__ __forceinline__ A wave_combine(const Op &op, A v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = op.combine(v, shfl_down_any(v, off));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Row-block pull (CSR-Adaptive).  Work item b (one per work-group):
+//   rows [r0,r1) with all their edges [e0,e1) (<= kRowEdges): the gathered
+//     values are staged in LDS with coalesced loads; then groups of
+//     G = 2^kind lanes take one row at a time, each lane combining every
+//     G-th edge in edge order and the group combining its lanes in a fixed
+//     tree (G = 1: one thread per row, edges in order);
+//   long (kind kRowLong): edges [e0,e1) of one row with more than kRowEdges
+//     edges; the block's partial (lane-strided, then a fixed tree) goes to
+//     slot_val[slot]; k_rows_fix combines a row's slots in order.
+// Each row is finished by exactly one thread, so results are deterministic.
+// ---------------------------------------------------------------------------
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_rows(Op op, const uint32_t *__restrict__ toff,
+                                                 const uint32_t *__restrict__ tids,
+                                                 const RowWork *__restrict__ work,
+                                                 typename Op::acc_t *__restrict__ slot_val,
+                                                 unsigned long long *__restrict__ counters,
+                                                 double *__restrict__ l1_part)
+{
+    using A = typename Op::acc_t;
+    using R = typename row_of<Op>::type;
+    constexpr int kRPT = kRowCap / kBlock;
+    __shared__ A s_val[kRowEdges];
+    __shared__ A s_sum[kRowCap];
+    __shared__ uint32_t s_rs[kRowCap + 1];
+    __shared__ A s_w[kBlock / 64];
+    if (op.skip()) return;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const RowWork w = work[blockIdx.x];
+    const uint32_t ne = w.e1 - w.e0;
+    const uint32_t nel = ne ? ne - 1 : 0u;
+    // gathered values of this block's edges (clamped, unconditional loads:
+    // a load under a divergent branch is waited for at the branch join; the
+    // emptiness test is block-uniform)
+    A val[kPullIPT];
+    if (ne) {
+        uint32_t src[kPullIPT];
+#pragma unroll
+        for (int k = 0; k < kPullIPT; k++)
+            src[k] = __builtin_nontemporal_load(&tids[w.e0 + min(tid + k * kBlock, nel)]);
+#pragma unroll
+        for (int k = 0; k < kPullIPT; k++) val[k] = op.gather(w.e0 + min(tid + k * kBlock, nel), src[k]);
+    }
+    if (w.kind == kRowLong) {
+        A acc = op.identity();
+#pragma unroll
+        for (int k = 0; k < kPullIPT; k++)
+            if (tid + k * kBlock < ne) acc = op.combine(acc, val[k]);
+        acc = wave_combine(op, acc);
+        if (lane == 0) s_w[wv] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            A t = s_w[0];
+            for (int k = 1; k < kBlock / 64; k++) t = op.combine(t, s_w[k]);
+            slot_val[w.slot] = t;
+            if (l1_part) {
+                l1_part[blockIdx.x] = 0.0;
+                if constexpr (has_aux<Op>::value) l1_part[gridDim.x + blockIdx.x] = 0.0;
+            }
+        }
+        return;
+    }
+    const uint32_t nr = w.r1 - w.r0;
+    const uint32_t nrl = nr ? nr - 1 : 0u;
+    uint32_t rs[kRPT], re[kRPT];
+    R rin[kRPT];
+#pragma unroll
+    for (int k = 0; k < kRPT; k++) {
+        const uint32_t i = w.r0 + min(tid + k * kBlock, nrl);
+        rs[k] = __builtin_nontemporal_load(&toff[i]) - w.e0;
+        re[k] = __builtin_nontemporal_load(&toff[i + 1]) - w.e0;
+        rin[k] = row_load(op, i);
+    }
+#pragma unroll
+    for (int k = 0; k < kPullIPT; k++) {
+        const uint32_t j = tid + k * kBlock;
+        if (j < ne) s_val[j] = val[k];
+    }
+    const uint32_t lg = w.kind;
+    if (lg) {
+#pragma unroll
+        for (int k = 0; k < kRPT; k++) {
+            const uint32_t i = tid + k * kBlock;
+            if (i < nr) s_rs[i] = rs[k];
+        }
+        if (tid == 0) s_rs[nr] = ne;
+    }
+    __syncthreads();
+    LocalStats st;
+    if (lg == 0) {
+#pragma unroll
+        for (int k = 0; k < kRPT; k++) {
+            if (tid + k * kBlock < nr) {
+                A acc = op.identity();
+                for (uint32_t j = rs[k]; j < re[k]; j++) acc = op.combine(acc, s_val[j]);
+                row_store(op, w.r0 + tid + k * kBlock, rin[k], acc, st);
+            }
+        }
+    } else {
+        const uint32_t G = 1u << lg, q = tid & (G - 1);
+        for (uint32_t i = tid >> lg; i < nr; i += kBlock >> lg) {
+            const uint32_t a = s_rs[i], b = s_rs[i + 1];
+            A acc = op.identity();
+            for (uint32_t j = a + q; j < b; j += G) acc = op.combine(acc, s_val[j]);
+            for (uint32_t off = G >> 1; off > 0; off >>= 1) acc = op.combine(acc, shfl_down_any(acc, off, G));
+            if (q == 0) s_sum[i] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kRPT; k++) {
+            const uint32_t i = tid + k * kBlock;
+            if (i < nr) row_store(op, w.r0 + i, rin[k], s_sum[i], st);
+        }
+    }
+    constexpr bool kAux = has_aux<Op>::value;
+    double aux = 0;
+    double l1 = block_stats<kAux>(st, counters, &aux);
+    if (l1_part && tid == 0) {
+        l1_part[blockIdx.x] = l1;
+        if constexpr (kAux) l1_part[gridDim.x + blockIdx.x] = aux;
+    }
+}
+
+// Finishes the long rows (their chunk partials combined in chunk order) and
+// folds the per-block residuals l1_part[256b .. 256b+255] into l1_fix[b].
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_rows_fix(Op op, const uint32_t *__restrict__ long_rows, uint32_t nlong,
+                                                     const typename Op::acc_t *__restrict__ slot_val,
+                                                     uint32_t nblocks, unsigned long long *__restrict__ counters,
+                                                     const double *__restrict__ l1_part,
+                                                     double *__restrict__ l1_fix)
+{
+    using A = typename Op::acc_t;
+    if (op.skip()) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    LocalStats st;
+    if (t < nlong) {
+        const uint32_t row = long_rows[3 * t], s0 = long_rows[3 * t + 1], s1 = long_rows[3 * t + 2];
+        A acc = op.identity();
+        for (uint32_t k = s0; k < s1; k++) acc = op.combine(acc, slot_val[k]);
+        row_store(op, row, row_load(op, row), acc, st);
+    }
+    if (l1_part && t < nblocks) {
+        st.l1 += l1_part[t];
+        if constexpr (has_aux<Op>::value) st.aux += l1_part[nblocks + t];
+    }
+    constexpr bool kAux = has_aux<Op>::value;
+    double aux = 0;
+    double l1 = block_stats<kAux>(st, counters, &aux);
+    if (l1_fix && threadIdx.x == 0) {
+        l1_fix[blockIdx.x] = l1;
+        if constexpr (kAux) l1_fix[gridDim.x + blockIdx.x] = aux;
+    }
+}
+
 template <class Op>
 int launch_pull(Ctx *c, const CscView &v, const Op &op, unsigned long long *counters, double *l1_part,
                 double *l1_fix, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr)
 {
     using A = typename Op::acc_t;
     static_assert(sizeof(A) <= 8, "carry buffers hold 8-byte partials");
+    if (v.rows) {
+        RowSched &r = *v.rows;
+        if (ev_a) GG_HIP(hipEventRecord(ev_a, c->stream));
+        k_rows<Op><<<r.blocks, kBlock, 0, c->stream>>>(op, v.toff, v.tids, (const RowWork *)r.work.p,
+                                                        (A *)r.slot_val.p, counters, l1_part);
+        if (ev_b) GG_HIP(hipEventRecord(ev_b, c->stream));
+        k_rows_fix<Op><<<v.fix_blocks(), kBlock, 0, c->stream>>>(op, (const uint32_t *)r.long_rows.p, r.nlong,
+                                                                 (const A *)r.slot_val.p, r.blocks, counters,
+                                                                 l1_part, l1_fix);
+        GG_HIP(hipGetLastError());
+        return GG_OK;
+    }
     PullSched &s = *v.sched;
     if (ev_a) GG_HIP(hipEventRecord(ev_a, c->stream));
     k_pull<Op><<<s.tiles, kBlock, 0, c->stream>>>(
@@ -363,12 +558,25 @@ inline CscView graph_view(Graph *g)
     return v;
 }
 
+// The in-edge twin with the graph's chosen schedule: row blocks (built on
+// first use) or merge-path tiles.
+inline int pull_view(Graph *g, CscView *v)
+{
+    *v = graph_view(g);
+    if (g->pull_schedule == 1) {
+        GG_TRY(ensure_rows(g));
+        v->rows = &g->rows;
+    }
+    return GG_OK;
+}
+
 inline CscView plan_view(Graph *g)
 {
     CscView v;
     v.toff = (const uint32_t *)g->plan.toff.p;
     v.tids = (const uint32_t *)g->plan.tids.p;
     v.sched = &g->plan.sched;
+    if (g->pull_schedule == 1 && g->plan.rows.blocks) v.rows = &g->plan.rows;
     return v;
 }
 

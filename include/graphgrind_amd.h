@@ -4,7 +4,7 @@
  * Drop-in boundary for GraphGrind's hot path.  Every entry point below
  * replaces a piece of the reference's header-only C++ template API; the
  * comment on each names the reference interface (file:line) it stands for.
- * The C++ front end (include/gg/ligra.h) maps the reference's app-facing
+ * The C++ front end (include/gg/ligra-numa.h) maps the reference's app-facing
  * API onto these calls; tests and benchmarks bind them through ctypes.
  *
  * Conventions
@@ -95,6 +95,16 @@ int gg_graph_copy_csc(gg_graph_t g, uint32_t *toff, uint32_t *tids, float *tw);
 int gg_graph_degrees(gg_graph_t g, int in_edges, uint32_t *out);
 /* Replaces the int weights of a graph (CSR order, m FP32 values). */
 int gg_graph_set_weights(gg_graph_t g, const float *w_csr);
+/* The same from a device array (m FP32 values in CSR order). */
+int gg_graph_set_weights_dev(gg_graph_t g, const float *w_dev);
+/* Batching (no reference counterpart): packs 'count' graphs of one context
+ * into one arena graph, a block-diagonal union whose CSR is the parts' CSRs
+ * back to back.  vbase[i] / ebase[i] (count+1 entries each, may be NULL)
+ * receive graph i's first vertex id and first edge position in the arena.
+ * The arena is weighted when every part is.  Per-graph results are slices
+ * of the arena's arrays. */
+int gg_graph_create_arena(gg_ctx_t ctx, int count, const gg_graph_t *parts, gg_graph_t *out,
+                          uint32_t *vbase, uint64_t *ebase);
 
 /* Internal vertex renumbering for the pull kernels (no reference
  * counterpart in the engine; the reference reorders offline with VEBO,
@@ -118,6 +128,10 @@ typedef struct {
     uint32_t tiles;
     uint32_t n;
     uint64_t m;
+    const uint32_t *row_work;   /* row blocks: 8 uint32 each (r0, r1, e0, e1,
+                                   kind, slot, 0, 0); NULL when not built  */
+    uint32_t row_blocks;
+    uint32_t row_long;          /* rows split into chunks                */
 } gg_graph_view;
 int gg_graph_device_view(gg_graph_t g, int which, gg_graph_view *out);
 
@@ -244,6 +258,12 @@ int gg_spmv_bwd(gg_graph_t g, int dtype, const void *x_dev, const void *gy_dev, 
  * v_dev NULL means the uniform teleport 1/n.  Needs edge weights. */
 int gg_wpagerank(gg_graph_t g, int dtype, double alpha, double eps, int max_iter,
                  const void *v_dev, void *r_dev, gg_run_info *info);
+/* eps < 0 (here and in gg_wpagerank_adjoint): exactly max_iter iterations
+ * with no stop rule.  Such a loop is captured into a hipGraph on first use
+ * and replayed as one launch while the buffers and parameters stay the same
+ * (one captured forward and one adjoint per graph).  Over an arena graph
+ * this runs every packed graph's personalised PageRank (teleport v = each
+ * graph's seed distribution) in one replay. */
 /* Implicit adjoint of gg_wpagerank at its output r for dL/dr = u:
  *   lambda = u, lambda <- u + alpha P^T lambda (pull over the out-edge CSR)
  *   until sum|lambda_new - lambda| <= eps * sum|lambda_new| or max_iter;
@@ -253,6 +273,35 @@ int gg_wpagerank(gg_graph_t g, int dtype, double alpha, double eps, int max_iter
 int gg_wpagerank_adjoint(gg_graph_t g, int dtype, double alpha, double eps, int max_iter,
                          const void *r_dev, const void *u_dev, void *gw_dev, void *gv_dev,
                          void *lambda_dev, gg_run_info *info);
+
+/* ---- multi-source BFS -----------------------------------------------------
+ * No counterpart in the reference: the forward (BFS) phase of app/BC.C for
+ * up to 32 sources at once, one 32-bit mask per vertex.  levels_dev is I32
+ * [nsrc][n]: levels_dev[k*n + v] = hop distance of v from sources[k]
+ * (host array), -1 when unreached.  info->iters = levels run. */
+int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t *levels_dev, gg_run_info *info);
+
+/* ---- tuning ---------------------------------------------------------------
+ * No counterpart in the reference (its only knobs are -c/-v/-o): schedule
+ * choices of the device pull, set per graph; the autotuner sweeps them. */
+#define GG_OPT_PULL_SCHEDULE 0      /* 0 merge-path tiles, 1 row blocks (default) */
+#define GG_OPT_ROW_EDGES_PER_LANE 1 /* row blocks: target edges per lane of a row group (8) */
+int gg_graph_set_option(gg_graph_t g, int option, int64_t value);
+int gg_graph_get_option(gg_graph_t g, int option, int64_t *value);
+/* Autotuner: times 'trials' pulls of a probe operator (one gathered value
+ * per edge, one write per row, in 'dtype') under each candidate schedule --
+ * merge-path tiles, and row blocks at 1, 2, 4, 8, 16, 32 edges per lane --
+ * then sets the fastest on the graph and reports all of them. */
+typedef struct {
+    int32_t schedule;            /* chosen GG_OPT_PULL_SCHEDULE             */
+    int32_t edges_per_lane;      /* chosen GG_OPT_ROW_EDGES_PER_LANE        */
+    double ms;                   /* its mean pull time                      */
+    int32_t candidates;
+    int32_t cand_schedule[8];
+    int32_t cand_edges_per_lane[8];
+    double cand_ms[8];
+} gg_tune_info;
+int gg_autotune_pull(gg_graph_t g, int dtype, int trials, gg_tune_info *out);
 
 /* ---- timing------------------------------------------------------------- */
 /* Average device time (ms) of the last fused loop's dominant kernel and

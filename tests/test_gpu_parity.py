@@ -144,14 +144,46 @@ def test_pagerank_fp64_fp32_vs_oracle(ctx, scale, P):
     g.close()
 
 
+def _check_row_blocks(g):
+    """The row-block table of the renumbered plan: every in-edge in exactly
+    one block, blocks of complete rows within the staging limits, long rows
+    chunked, every row finished once."""
+    v = g.device_view(1)
+    assert v.row_work and v.row_blocks > 0
+    tab = np.empty(8 * v.row_blocks, np.uint32)
+    gg.check(gg.lib().gg_memcpy_d2h(g.ctx.h, tab.ctypes.data, v.row_work, tab.nbytes))
+    tab = tab.reshape(-1, 8).astype(np.int64)
+    toff = np.empty(g.n + 1, np.uint32)
+    gg.check(gg.lib().gg_memcpy_d2h(g.ctx.h, toff.ctypes.data, v.toff, toff.nbytes))
+    toff = toff.astype(np.int64)
+    r0, r1, e0, e1, kind = tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3], tab[:, 4]
+    assert np.all((e1 - e0) <= 2048) and np.all(e1 >= e0)
+    # blocks tile the edge range in order
+    assert e0[0] == 0 and e1[-1] == g.m and np.array_equal(e0[1:], e1[:-1])
+    short = kind != 0xFF
+    assert np.all(kind[short] <= 6) and np.all((r1 - r0)[short] >= 1) and np.all((r1 - r0)[short] <= 1024)
+    assert np.array_equal(toff[r0[short]], e0[short]) and np.array_equal(toff[r1[short]], e1[short])
+    rows = np.zeros(g.n, np.int64)
+    np.add.at(rows, np.concatenate([np.arange(a, b) for a, b in zip(r0[short], r1[short])]), 1)
+    lrows = np.unique(r0[~short])
+    assert len(lrows) == v.row_long and np.all(np.diff(toff)[lrows] > 2048)
+    rows[lrows] += 1
+    assert np.all(rows == 1)
+
+
 @pytest.mark.parametrize("scale,P", [(12, 64), (20, 384)])
-def test_pagerank_reordered_plan(ctx, scale, P):
-    """The hot-sources-first renumbering changes only summation order."""
-    print(f"pagerank reordered scale={scale}", flush=True)
+@pytest.mark.parametrize("schedule", [1, 0])
+def test_pagerank_reordered_plan(ctx, scale, P, schedule):
+    """The hot-sources-first renumbering changes only summation order; both
+    pull schedules (row blocks, merge-path tiles) run it."""
+    print(f"pagerank reordered scale={scale} schedule={schedule}", flush=True)
     off, ids = oracle.rmat_csr(scale, 16, 1)
     og = oracle.Graph(off, ids, P=P)
     want, it, _ = og.pagerank(10)
     g = _dev_graph(ctx, off, ids).partition(P).reorder(1)
+    g.set_option(gg.OPT_PULL_SCHEDULE, schedule)
+    if schedule == 1:
+        _check_row_blocks(g)
     r64, info = gg.pagerank(g, np.float64)
     assert info.iters == it
     assert R.rel_l1(r64.to_host(), want) < TOL64
