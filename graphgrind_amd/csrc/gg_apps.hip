@@ -107,6 +107,18 @@ __global__ __launch_bounds__(kBlock) void k_pr_check(const double *__restrict__ 
     (void)np; // folded into l1_fix by k_pull_fixup
     for (uint32_t i = threadIdx.x; i < nf; i += kBlock) s += l1_fix[i];
     s = wave_sum_f64(s);
+    // this iteration's frontier statistics: the counter stripes (the first
+    // wave reads one stripe per lane and clears it)
+    unsigned long long a = 0, o = 0;
+    if (threadIdx.x < kStripes) {
+        unsigned long long *st = counters + kCntStripeBase + threadIdx.x * kStripeStride;
+        a = st[CNT_ACTIVE];
+        o = st[CNT_OUT];
+        st[CNT_ACTIVE] = 0;
+        st[CNT_OUT] = 0;
+    }
+    a = wave_sum_u64(a);
+    o = wave_sum_u64(o);
     __shared__ double w[kBlock / 64];
     if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
     __syncthreads();
@@ -114,10 +126,8 @@ __global__ __launch_bounds__(kBlock) void k_pr_check(const double *__restrict__ 
         double t = 0;
         for (int k = 0; k < kBlock / 64; k++) t += w[k];
         l1_hist[iter - 1] = t;
-        stats[2 * (iter - 1)] = counters[CNT_ACTIVE];
-        stats[2 * (iter - 1) + 1] = counters[CNT_OUT];
-        counters[CNT_ACTIVE] = 0;
-        counters[CNT_OUT] = 0;
+        stats[2 * (iter - 1)] = a;
+        stats[2 * (iter - 1) + 1] = o;
         if (t < eps) {
             *done = 1;
             *done_iter = iter;
@@ -168,7 +178,7 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
     // timed region: init + iterations (the reference times Compute() whole)
     GG_HIP(hipEventRecord(ev[0], c->stream));
     fill_bytes(c, B.flags.p, 64, 0);
-    fill_bytes(c, c->counters.p, 64, 0);
+    counters_zero(c);
     k_pr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)B.R[0].p, (V *)B.R[1].p, (V *)B.K[0].p,
                                                                    outdeg, n, one_over_n);
     fill_bytes(c, B.F[1].p, n, 1);

@@ -105,12 +105,11 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
             level++;
             OpMSBFS op{(const uint32_t *)F[(level - 1) % 2].p, (uint32_t *)F[level % 2].p, (uint32_t *)seen.p,
                        levels, n, level};
-            rc = fill_bytes(c, c->counters.p, 64, 0);
+            rc = counters_zero(c);
             if (!rc) rc = launch_pull(c, view, op, cnt, nullptr, nullptr);
-            unsigned long long active = 0;
-            if (!rc && hipMemcpyAsync(&active, cnt + CNT_ACTIVE, 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-                rc = fail(GG_ERR_HIP, "gg_msbfs: readback");
-            if (!rc) rc = sync(c);
+            unsigned long long cn[4] = {0, 0, 0, 0};
+            if (!rc) rc = counters_read(c, cn);
+            const unsigned long long active = cn[CNT_ACTIVE];
             if (!rc && info && level <= 64) info->paths[level - 1] = GG_PATH_CSC;
             if (!rc && active == 0) break;
         }

@@ -334,7 +334,8 @@ extern "C" int gg_ctx_create(int device, gg_ctx_t *out)
     }
     (void)hipEventCreate(&c->ev[0]);
     (void)hipEventCreate(&c->ev[1]);
-    int rc = buf_alloc(c, c->counters, 4096);
+    int rc = buf_alloc(c, c->counters, kCounterBytes);
+    if (!rc) rc = fill_bytes(c, c->counters.p, kCounterBytes, 0);
     if (rc) {
         (void)hipStreamDestroy(c->stream);
         delete c;

@@ -75,7 +75,7 @@ struct PullSched {
 // degree.  A row with more than kRowEdges edges is split into chunks, one
 // block each; its partials are combined in chunk order by the fix-up kernel.
 constexpr int kRowEdges = kBlock * kPullIPT; // edges staged per block
-constexpr int kRowCap = 4 * kBlock;          // rows per block
+constexpr int kRowCap = 2 * kBlock;          // rows per block
 constexpr uint32_t kRowLong = 0xffu;         // kind of a long-row chunk
 struct RowWork {
     uint32_t r0, r1;   // rows [r0, r1) (long: the row in r0)
@@ -140,12 +140,24 @@ struct GraphExec {
 };
 void graph_exec_free(GraphExec &x);
 
+// ---- per-context counters ---------------------------------------------------
+// u64 [0..3]: totals (active, out_edges, scanned, cursor); from u64
+// kCntStripeBase on: kStripes stripes of kStripeStride u64 (128 bytes)
+// holding partial active / out_edges / scanned counts.
+constexpr int kStripes = 64;
+constexpr int kStripeStride = 16;
+constexpr int kCntStripeBase = 64;
+constexpr uint64_t kCounterBytes = 8ull * (kCntStripeBase + kStripes * kStripeStride);
+struct Ctx;
+int counters_zero(Ctx *c);                           // totals and stripes
+int counters_read(Ctx *c, unsigned long long out[4]); // folds the stripes first
+
 // ---- context ----------------------------------------------------------------
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    Buf counters;   // small scratch: uint64 counters/flags (256 B)
+    Buf counters;   // uint64 counters: totals + stripes (kCounterBytes)
     Buf host_pin;   // unused placeholder for pinned staging
     Buf tmp;        // grow-only temporary (sort space etc.)
     double last_kernel_ms = 0;

@@ -25,8 +25,22 @@ template <class Op>
 struct has_aux<Op, std::void_t<decltype(Op::kAux)>> : std::bool_constant<Op::kAux> {};
 
 // counters layout (unsigned long long): [0] active, [1] out_edges,
-// [2] edges scanned, [3] list cursor
+// [2] edges scanned, [3] list cursor.  Kernels with many work-groups add
+// their active / out_edges / scanned counts into one of kStripes stripes
+// (each on its own 128-byte line, picked by the work-group index) instead of
+// the totals: same-address atomics from ~10^5 work-groups serialise at one
+// L2 channel.  counters_fold() (host) and the PageRank check kernel sum the
+// stripes into the totals.
 enum { CNT_ACTIVE = 0, CNT_OUT = 1, CNT_SCANNED = 2, CNT_CURSOR = 3 };
+
+__device__ __forceinline__ void cnt_add(unsigned long long *counters, int which, unsigned long long v)
+{
+    if (v) atomicAdd(&counters[kCntStripeBase + (blockIdx.x & (kStripes - 1)) * kStripeStride + which], v);
+}
+
+// Sums the stripes into counters[0..2] and clears them (one work-group of
+// kStripes threads).
+__global__ void k_counters_fold(unsigned long long *counters);
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 {
@@ -74,8 +88,8 @@ __device__ __forceinline__ double block_stats(const LocalStats &st, unsigned lon
         }
         if constexpr (kAux) *aux_tot = tx;
         if (counters) {
-            if (ta) atomicAdd(&counters[CNT_ACTIVE], ta);
-            if (to) atomicAdd(&counters[CNT_OUT], to);
+            cnt_add(counters, CNT_ACTIVE, ta);
+            cnt_add(counters, CNT_OUT, to);
         }
     }
     return tot;
@@ -425,57 +439,58 @@ __global__ __launch_bounds__(kBlock) void k_rows(Op op, const uint32_t *__restri
         return;
     }
     const uint32_t nr = w.r1 - w.r0;
-    const uint32_t nrl = nr ? nr - 1 : 0u;
-    uint32_t rs[kRPT], re[kRPT];
-    R rin[kRPT];
+    // row starts (relative to e0) in LDS; no per-row state is held in
+    // registers across the staging, which keeps the kernel's register
+    // footprint, and so its occupancy, that of a plain gather
 #pragma unroll
     for (int k = 0; k < kRPT; k++) {
-        const uint32_t i = w.r0 + min(tid + k * kBlock, nrl);
-        rs[k] = __builtin_nontemporal_load(&toff[i]) - w.e0;
-        re[k] = __builtin_nontemporal_load(&toff[i + 1]) - w.e0;
-        rin[k] = row_load(op, i);
+        const uint32_t i = tid + k * kBlock;
+        if (i < nr) s_rs[i] = __builtin_nontemporal_load(&toff[w.r0 + i]) - w.e0;
     }
+    if (tid == 0) s_rs[nr] = ne;
 #pragma unroll
     for (int k = 0; k < kPullIPT; k++) {
         const uint32_t j = tid + k * kBlock;
         if (j < ne) s_val[j] = val[k];
     }
-    const uint32_t lg = w.kind;
-    if (lg) {
-#pragma unroll
-        for (int k = 0; k < kRPT; k++) {
-            const uint32_t i = tid + k * kBlock;
-            if (i < nr) s_rs[i] = rs[k];
-        }
-        if (tid == 0) s_rs[nr] = ne;
-    }
     __syncthreads();
-    LocalStats st;
+    const uint32_t lg = w.kind;
+    A acc[kRPT];
     if (lg == 0) {
 #pragma unroll
         for (int k = 0; k < kRPT; k++) {
-            if (tid + k * kBlock < nr) {
-                A acc = op.identity();
-                for (uint32_t j = rs[k]; j < re[k]; j++) acc = op.combine(acc, s_val[j]);
-                row_store(op, w.r0 + tid + k * kBlock, rin[k], acc, st);
+            acc[k] = op.identity();
+            const uint32_t i = tid + k * kBlock;
+            if (i < nr) {
+                const uint32_t a = s_rs[i], b = s_rs[i + 1];
+                for (uint32_t j = a; j < b; j++) acc[k] = op.combine(acc[k], s_val[j]);
             }
         }
     } else {
         const uint32_t G = 1u << lg, q = tid & (G - 1);
         for (uint32_t i = tid >> lg; i < nr; i += kBlock >> lg) {
             const uint32_t a = s_rs[i], b = s_rs[i + 1];
-            A acc = op.identity();
-            for (uint32_t j = a + q; j < b; j += G) acc = op.combine(acc, s_val[j]);
-            for (uint32_t off = G >> 1; off > 0; off >>= 1) acc = op.combine(acc, shfl_down_any(acc, off, G));
-            if (q == 0) s_sum[i] = acc;
+            A t = op.identity();
+            for (uint32_t j = a + q; j < b; j += G) t = op.combine(t, s_val[j]);
+            for (uint32_t off = G >> 1; off > 0; off >>= 1) t = op.combine(t, shfl_down_any(t, off, G));
+            if (q == 0) s_sum[i] = t;
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kRPT; k++) {
             const uint32_t i = tid + k * kBlock;
-            if (i < nr) row_store(op, w.r0 + i, rin[k], s_sum[i], st);
+            acc[k] = i < nr ? s_sum[i] : op.identity();
         }
     }
+    // epilogue: the rows' own inputs are loaded now, all before any store
+    LocalStats st;
+    R rin[kRPT];
+    const uint32_t nrl = nr - 1;
+#pragma unroll
+    for (int k = 0; k < kRPT; k++) rin[k] = row_load(op, w.r0 + min(tid + k * kBlock, nrl));
+#pragma unroll
+    for (int k = 0; k < kRPT; k++)
+        if (tid + k * kBlock < nr) row_store(op, w.r0 + tid + k * kBlock, rin[k], acc[k], st);
     constexpr bool kAux = has_aux<Op>::value;
     double aux = 0;
     double l1 = block_stats<kAux>(st, counters, &aux);

@@ -197,47 +197,8 @@ __global__ __launch_bounds__(kBlock) void k_bfs_pull(const uint32_t *__restrict_
         next[d] = nx;
     }
     unsigned long long sc = wave_sum_u64(scanned);
-    if ((threadIdx.x & 63) == 0 && sc) atomicAdd(&counters[CNT_SCANNED], sc);
+    if ((threadIdx.x & 63) == 0) cnt_add(counters, CNT_SCANNED, sc);
     block_stats(st, counters);
-}
-
-// Dense COO for BFS_F (edgeMapDense with PART96, ligra-numa.h:838-863):
-// one thread per COO edge; CAS elects one parent per destination.
-__global__ void k_bfs_coo(const uint32_t *__restrict__ csrc, const uint32_t *__restrict__ cdst,
-                          uint64_t m, const uint32_t *__restrict__ bm, int bit,
-                          int32_t *__restrict__ parents, int32_t *__restrict__ levels, int32_t level,
-                          uint8_t *__restrict__ next)
-{
-    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (; k < m; k += stride) {
-        const uint32_t s = csrc[k], d = cdst[k];
-        if (parents[d] != -1) continue; // cond(d)
-        if (!bit && !bit_test(bm, s)) continue;
-        if (atomicCAS(&parents[d], -1, (int32_t)s) == -1) {
-            next[d] = 1;
-            if (levels) levels[d] = level;
-        }
-    }
-}
-
-// Dense COO for CC_F: atomicMin of the source's prevID (Jacobi; the
-// reference reads IDs[s] here, which only changes the round count).
-__global__ void k_cc_coo(const uint32_t *__restrict__ csrc, const uint32_t *__restrict__ cdst,
-                         uint64_t m, const uint8_t *__restrict__ F, int bit,
-                         uint32_t *__restrict__ ids, const uint32_t *__restrict__ prev,
-                         uint8_t *__restrict__ next)
-{
-    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (; k < m; k += stride) {
-        const uint32_t s = csrc[k], d = cdst[k];
-        if (!bit && !F[s]) continue;
-        const uint32_t v = prev[s];
-        if (v >= ids[d]) continue;
-        const uint32_t old = atomicMin(&ids[d], v);
-        if (v < old && old == prev[d]) next[d] = 1;
-    }
 }
 
 // Count and out-degree sum of a dense flag array (GoutDegree + reduce,
@@ -364,8 +325,8 @@ __global__ __launch_bounds__(kBlock) void k_push(PushArgs A)
         unsigned long long base = 0;
         if (lane == 0 && cnt) {
             base = atomicAdd(&A.counters[CNT_CURSOR], (unsigned long long)cnt);
-            atomicAdd(&A.counters[CNT_ACTIVE], (unsigned long long)cnt);
-            atomicAdd(&A.counters[CNT_OUT], od);
+            cnt_add(A.counters, CNT_ACTIVE, (unsigned long long)cnt);
+            cnt_add(A.counters, CNT_OUT, od);
         }
         base = __shfl(base, 0, 64);
         if (emit) {
@@ -402,8 +363,8 @@ __global__ void k_dedup_keep(const uint32_t *__restrict__ list, uint64_t k, uint
     unsigned long long base = 0;
     if (lane == 0 && cnt) {
         base = atomicAdd(&counters[CNT_CURSOR], (unsigned long long)cnt);
-        atomicAdd(&counters[CNT_ACTIVE], (unsigned long long)cnt);
-        atomicAdd(&counters[CNT_OUT], od);
+        cnt_add(counters, CNT_ACTIVE, (unsigned long long)cnt);
+        cnt_add(counters, CNT_OUT, od);
     }
     base = __shfl(base, 0, 64);
     if (keep) out[base + __popcll(mask & ((1ULL << lane) - 1ULL))] = v;
@@ -634,16 +595,37 @@ int frontier_make_sparse(Frontier *f)
     return GG_OK;
 }
 
-static int counters_zero(Ctx *c)
+__global__ void k_counters_fold(unsigned long long *counters)
 {
-    return fill_bytes(c, c->counters.p, 64, 0);
+    const int t = threadIdx.x;
+    unsigned long long *st = counters + kCntStripeBase + t * kStripeStride;
+    unsigned long long a = st[CNT_ACTIVE], o = st[CNT_OUT], sc = st[CNT_SCANNED];
+    st[CNT_ACTIVE] = 0;
+    st[CNT_OUT] = 0;
+    st[CNT_SCANNED] = 0;
+    a = wave_sum_u64(a);
+    o = wave_sum_u64(o);
+    sc = wave_sum_u64(sc);
+    if (t == 0) {
+        counters[CNT_ACTIVE] += a;
+        counters[CNT_OUT] += o;
+        counters[CNT_SCANNED] += sc;
+    }
 }
 
-static int counters_read(Ctx *c, unsigned long long out[4])
+int counters_zero(Ctx *c)
 {
+    return fill_bytes(c, c->counters.p, kCounterBytes, 0);
+}
+
+int counters_read(Ctx *c, unsigned long long out[4])
+{
+    k_counters_fold<<<1, kStripes, 0, c->stream>>>((unsigned long long *)c->counters.p);
+    GG_HIP(hipGetLastError());
     GG_HIP(hipMemcpyAsync(out, c->counters.p, 32, hipMemcpyDeviceToHost, c->stream));
     return sync(c);
 }
+
 
 int frontier_stats_dense(Frontier *f)
 {
@@ -753,14 +735,11 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             break;
         }
         case GG_OP_CC: {
-            if (coo) {
-                if ((rc = fill_bytes(c, next, n, 0))) return bail(rc);
-                uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
-                k_cc_coo<<<grid, kBlock, 0, c->stream>>>((const uint32_t *)g->coo_src.p,
-                                                          (const uint32_t *)g->coo_dst.p, m, F, bit,
-                                                          (uint32_t *)op->a, (const uint32_t *)op->b, next);
-                rc = GG_OK;
-            } else {
+            // The COO and CSC branches compute the same Jacobi round (the
+            // minimum prevID over active in-neighbours); the destination-
+            // owned pull reads 4 bytes per edge where the COO lists read 8
+            // and needs no atomics, so both branches run it.
+            {
                 if ((rc = buf_alloc(c, tmpv, 4ull * n))) return bail(rc);
                 k_cc_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)op->b, F, bit,
                                                                             (uint32_t *)tmpv.p, n);
@@ -771,15 +750,13 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             break;
         }
         case GG_OP_BFS: {
-            if (coo) {
-                if ((rc = fill_bytes(c, next, n, 0))) return bail(rc);
-                uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
-                k_bfs_coo<<<grid, kBlock, 0, c->stream>>>((const uint32_t *)g->coo_src.p,
-                                                           (const uint32_t *)g->coo_dst.p, m, bm, bit,
-                                                           (int32_t *)op->a, ex ? ex->levels : nullptr,
-                                                           ex ? ex->level : 0, next);
-                if (ex) ex->scanned += m;
-            } else {
+            // Both dense branches visit the same vertices in this level; the
+            // bottom-up pull stops at an unvisited vertex's first frontier
+            // in-neighbour (ligra-numa.h:711-712) where the COO lists scan
+            // every edge, so both branches run the pull.  Levels are
+            // identical; parents are a valid choice either way (the
+            // reference's own choice depends on its thread schedule).
+            {
                 k_bfs_pull<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
                     (const uint32_t *)g->toff.p, (const uint32_t *)g->tids.p, bm, bit, (int32_t *)op->a,
                     ex ? ex->levels : nullptr, ex ? ex->level : 0, next, outdeg, n, cnt);
@@ -793,9 +770,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         GG_HIP(hipGetLastError());
         out->has_dense = true;
         out->dense_valid = true;
-        if (coo && !op_is_sum(op->kind)) {
-            if ((rc = frontier_stats_dense(out))) return bail(rc);
-        } else {
+        {
             unsigned long long cn[4];
             if ((rc = counters_read(c, cn))) return bail(rc);
             out->d_m = cn[CNT_ACTIVE];
