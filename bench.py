@@ -68,7 +68,10 @@ def pagerank_bench(gg, ctx, scale, steps, warmup, dtype, P=384):
     # rank read/write, out-degree and frontier flags
     algo_bytes = (4 + item) * m + (32 if item == 8 else 18) * n
     avg_kernel = float(np.mean(kern_ms)) / 1e3  # seconds per launch
+    vname = "double" if dtype == np.float64 else "float"
+    kern = "k_rows" if g.get_option(gg.OPT_PULL_SCHEDULE) == 1 else "k_pull"
     res = {
+        "kernel": f"{kern}<OpPRFused<{vname}>>",
         "n": n, "m": m, "step_ms": wall * 1e3, "event_ms": float(np.mean(ev_ms)),
         "gteps": 10 * m / wall / 1e9,
         "kernel_ms": avg_kernel * 1e3,
@@ -342,7 +345,7 @@ def copy_calibration(gg, ctx, nbytes=1 << 30, reps=3):
 def traffic_from_profile(kernel, n, m):
     """HBM bytes per launch of 'kernel' on this workload, as recorded from
     the rocprofv3 counter runs in profiles/ (None when absent)."""
-    path = os.path.join(ROOT, "profiles", "k_pull_pagerank_rmat24_fp64_counters.md")
+    path = os.path.join(ROOT, "profiles", "k_rows_pagerank_rmat24_fp64.md")
     if not os.path.exists(path):
         return None
     rec = {}
@@ -387,7 +390,7 @@ def main():
     if not a.no_extras:
         pr32 = pagerank_bench(gg, ctx, a.scale, a.steps, a.warmup, np.float32)
         log(f"[bench] pagerank fp32: {pr32}")
-        extras["pagerank_fp32"] = {k: pr32[k] for k in ("gteps", "step_ms", "kernel_ms", "achieved_gbs")}
+        extras["pagerank_fp32"] = {k: pr32[k] for k in ("gteps", "step_ms", "kernel_ms", "achieved_gbs", "kernel")}
         bc = bfs_cc_bench(gg, ctx, a.scale, a.steps)
         log(f"[bench] bfs/cc: {bc}")
         extras["bfs_cc_sym_rmat"] = bc
@@ -421,8 +424,8 @@ def main():
                    "n": pr["n"], "m": pr["m"], "iterations": 10, "partitions": 384},
         "roofline": {"bound": "hbm", "achieved": pr["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": pr["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile("k_pull<OpPRFused<double>>", pr["n"], pr["m"]),
-                     "kernel": "k_pull<OpPRFused<double>>", "kernel_ms": pr["kernel_ms"],
+                     "traffic": traffic_from_profile(pr["kernel"], pr["n"], pr["m"]),
+                     "kernel": pr["kernel"], "kernel_ms": pr["kernel_ms"],
                      "algo_bytes_per_launch": pr["algo_bytes_per_launch"]},
         "cpu_baseline": cpu,
         "extras": extras,

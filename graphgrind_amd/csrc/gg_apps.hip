@@ -135,12 +135,15 @@ __global__ __launch_bounds__(kBlock) void k_pr_check(const double *__restrict__ 
     }
 }
 
+// The fused loop's buffers, views of the graph's workspace (allocated on the
+// first call and kept: per-call allocation of seven n-sized arrays would
+// cost more than a PageRank iteration).
 struct PRBuffers {
-    Buf R[3], K[2], F[2], l1p, l1f, flags, stats, l1h;
-    void free_all()
+    Buf *R, *K, *F, &l1p, &l1f, &flags, &stats, &l1h;
+    explicit PRBuffers(Graph *g)
+        : R(g->pr_ws), K(g->pr_ws + 3), F(g->pr_ws + 5), l1p(g->pr_ws[7]), l1f(g->pr_ws[8]),
+          flags(g->pr_ws[9]), stats(g->pr_ws[10]), l1h(g->pr_ws[11])
     {
-        Buf *all[] = {&R[0], &R[1], &R[2], &K[0], &K[1], &F[0], &F[1], &l1p, &l1f, &flags, &stats, &l1h};
-        for (Buf *b : all) buf_free(*b);
     }
 };
 
@@ -156,17 +159,17 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
     const uint32_t *outdeg = use_plan ? (const uint32_t *)g->plan.outdeg.p : (const uint32_t *)g->outdeg.p;
     const uint32_t nblk = view.blocks();
     const uint32_t nfix = view.fix_blocks();
-    PRBuffers B;
+    PRBuffers B(g);
     int rc = GG_OK;
-    for (int i = 0; i < 3 && !rc; i++) rc = buf_alloc(c, B.R[i], sizeof(V) * (uint64_t)n);
-    for (int i = 0; i < 2 && !rc; i++) rc = buf_alloc(c, B.K[i], sizeof(V) * (uint64_t)n);
-    for (int i = 0; i < 2 && !rc; i++) rc = buf_alloc(c, B.F[i], (uint64_t)n + 64);
-    if (!rc) rc = buf_alloc(c, B.l1p, 8ull * nblk);
-    if (!rc) rc = buf_alloc(c, B.l1f, 8ull * nfix);
-    if (!rc) rc = buf_alloc(c, B.flags, 64);
-    if (!rc) rc = buf_alloc(c, B.stats, 16ull * (max_iter + 1));
-    if (!rc) rc = buf_alloc(c, B.l1h, 8ull * (max_iter + 1));
-    if (rc) { B.free_all(); return rc; }
+    for (int i = 0; i < 3 && !rc; i++) rc = buf_ensure(c, B.R[i], sizeof(V) * (uint64_t)n);
+    for (int i = 0; i < 2 && !rc; i++) rc = buf_ensure(c, B.K[i], sizeof(V) * (uint64_t)n);
+    for (int i = 0; i < 2 && !rc; i++) rc = buf_ensure(c, B.F[i], (uint64_t)n + 64);
+    if (!rc) rc = buf_ensure(c, B.l1p, 8ull * nblk);
+    if (!rc) rc = buf_ensure(c, B.l1f, 8ull * nfix);
+    if (!rc) rc = buf_ensure(c, B.flags, 64);
+    if (!rc) rc = buf_ensure(c, B.stats, 16ull * (max_iter + 1));
+    if (!rc) rc = buf_ensure(c, B.l1h, 8ull * (max_iter + 1));
+    if (rc) return rc;
     int *done = (int *)B.flags.p;
     int *done_iter = done + 1;
     unsigned long long *cnt = (unsigned long long *)c->counters.p;
@@ -254,7 +257,6 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
         }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
-    B.free_all();
     return rc;
 }
 

@@ -646,6 +646,7 @@ static void graph_free(Graph *g)
     rows_free(g->rows);
     g->rows_ok = false;
     graph_exec_drop(g);
+    for (Buf &b : g->pr_ws) buf_free(b);
     g->plan.mode = 0;
 }
 

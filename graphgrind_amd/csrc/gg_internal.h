@@ -200,6 +200,8 @@ struct Graph {
     bool rows_ok = false;
     // captured fixed-iteration loops (weighted PageRank forward / adjoint)
     GraphExec fwd_exec, adj_exec;
+    // workspace of the fused PageRank loop, kept between calls (grow-only)
+    Buf pr_ws[12];
     // tuning (gg_graph_set_option)
     int pull_schedule = 1;  // 0 merge-path tiles, 1 row blocks where built
     int edges_per_lane = 8; // row blocks: target edges per lane of a row group
