@@ -9,6 +9,8 @@ reference's app-facing API (graphgrind/ligra-numa.h, graph-numa.h):
     vertex_map       vertexMap(part, frontier, functor) / map_vertexL
     pagerank, bfs, components, spmv   the apps' Compute() loops
     spmv_bwd, wpagerank, wpagerank_adjoint   differentiated forms (adjoints)
+    bc, sssp         app/BC.C and app/BellmanFord.C on the same engine
+    msbfs, Graph.arena, autotune_pull        batching and tuning
 
 Errors come back as GGError carrying the C status and message (the
 reference abort()s in the same places).  The compute path is the HIP
@@ -136,6 +138,8 @@ SIGNATURES = {
     "gg_graph_create_arena": (C.c_int, [_VP, C.c_int, _VP, C.POINTER(C.c_void_p), _VP, _VP]),
     "gg_msbfs": (C.c_int, [_VP, C.c_int, _VP, _VP, C.POINTER(RunInfo)]),
     "gg_autotune_pull": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(TuneInfo)]),
+    "gg_bc": (C.c_int, [_VP, C.c_uint32, C.c_int, _VP, _VP, C.POINTER(RunInfo)]),
+    "gg_sssp": (C.c_int, [_VP, C.c_uint32, _VP, C.POINTER(RunInfo)]),
     "gg_spmv_bwd": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP, C.POINTER(RunInfo)]),
     "gg_wpagerank": (C.c_int, [_VP, C.c_int, C.c_double, C.c_double, C.c_int, _VP, _VP,
                                C.POINTER(RunInfo)]),
@@ -577,6 +581,26 @@ def wpagerank_adjoint(g: Graph, r, u, alpha=0.85, eps=1e-7, max_iter=100, gw=Non
                                      _dev(gw)[0] if want_gw else None, _dev(gv)[0] if want_gv else None,
                                      _dev(lam)[0] if want_lam else None, C.byref(info)))
     return (gw if want_gw else None), (gv if want_gv else None), (lam if want_lam else None), info
+
+
+# ------------------------------------------------------- further apps --
+
+def bc(g: Graph, root, dtype=np.float32, out=None, levels=True):
+    """app/BC.C from root: (scores, levels|None, RunInfo); NaN where unreached."""
+    sc = out if out is not None else DeviceArray(g.ctx, g.n, dtype)
+    lv = DeviceArray(g.ctx, g.n, np.int32) if levels else None
+    info = RunInfo()
+    p, dt = _dev(sc)
+    check(lib().gg_bc(g.h, root, DT_OF[dt], p, None if lv is None else lv.ptr, C.byref(info)))
+    return sc, lv, info
+
+
+def sssp(g: Graph, root, out=None):
+    """app/BellmanFord.C from root with integer lengths: (dist int32, RunInfo)."""
+    d = out if out is not None else DeviceArray(g.ctx, g.n, np.int32)
+    info = RunInfo()
+    check(lib().gg_sssp(g.h, root, _dev(d)[0], C.byref(info)))
+    return d, info
 
 
 # -------------------------------------------------- batching and tuning --

@@ -286,22 +286,6 @@ __global__ __launch_bounds__(kBlock) void k_fp_check(const double *__restrict__ 
 
 // ---- lazily built transpose-side state ----------------------------------------
 
-static int ensure_csr_sched(Graph *g)
-{
-    if (g->csr_pull_ok) return GG_OK;
-    GG_TRY(sched_build(g->ctx, g->csr_pull, (const uint32_t *)g->off.p, g->n, g->m));
-    g->csr_pull_ok = true;
-    return GG_OK;
-}
-
-static CscView csr_view(Graph *g)
-{
-    CscView v;
-    v.toff = (const uint32_t *)g->off.p;
-    v.tids = (const uint32_t *)g->ids.p;
-    v.sched = &g->csr_pull;
-    return v;
-}
 
 static int ensure_wsum(Graph *g)
 {

@@ -573,6 +573,25 @@ inline CscView graph_view(Graph *g)
     return v;
 }
 
+// Transpose traversal: a pull over the out-edge CSR (rows = sources), with
+// merge-path tiles built on first use.
+inline int ensure_csr_sched(Graph *g)
+{
+    if (g->csr_pull_ok) return GG_OK;
+    GG_TRY(sched_build(g->ctx, g->csr_pull, (const uint32_t *)g->off.p, g->n, g->m));
+    g->csr_pull_ok = true;
+    return GG_OK;
+}
+
+inline CscView csr_view(Graph *g)
+{
+    CscView v;
+    v.toff = (const uint32_t *)g->off.p;
+    v.tids = (const uint32_t *)g->ids.p;
+    v.sched = &g->csr_pull;
+    return v;
+}
+
 // The in-edge twin with the graph's chosen schedule: row blocks (built on
 // first use) or merge-path tiles.
 inline int pull_view(Graph *g, CscView *v)

@@ -736,7 +736,10 @@ void Compute(GraphType &, long);
 #endif
 
 // driver (ligra-numa.h:1095-1208) with one extension: "-rmat S EF SEED"
-// generates the seeded R-MAT graph on the device instead of reading a file
+// generates the seeded R-MAT graph on the device instead of reading a file.
+// Tools that include this header for its readers or types and bring their
+// own main() define GG_FRONT_NO_DRIVER.
+#ifndef GG_FRONT_NO_DRIVER
 int parallel_main(int argc, char *argv[])
 {
     commandLine P(argc, argv, " [-s] [-b] [-r root] [-rounds R] [-c P] [-v edge|vertex] [-o] <inFile | -rmat S EF SEED>");
@@ -808,5 +811,6 @@ int parallel_main(int argc, char *argv[])
     }
     return rc;
 }
+#endif // GG_FRONT_NO_DRIVER
 
 #endif

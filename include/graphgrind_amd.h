@@ -241,6 +241,23 @@ int gg_cc(gg_graph_t g, uint32_t *labels_dev, gg_run_info *info);
  * in-edge twin (unit weights when the graph has none). */
 int gg_spmv(gg_graph_t g, int dtype, const void *x_dev, void *y_dev, gg_run_info *info);
 
+/* ---- further apps on the engine --------------------------------------------- */
+/* app/BC.C:154-226 from 'root': forward pulls level by level summing
+ * NumPaths, backward pulls over the out-edge CSR (the true transpose)
+ * summing Dependencies; scores_dev (F64 or F32 per dtype, n) receives
+ * (Dependencies - inv) / inv, NaN where unreached; levels_dev (I32 n, may be
+ * NULL) the forward level, -1 unreached.  info->iters = forward rounds.
+ * The reference's transpose() leaves the COO lists unflipped
+ * (graph-numa.h:721-725), so its backward phase on a directed graph can
+ * run along the wrong direction; this follows the algorithm. */
+int gg_bc(gg_graph_t g, uint32_t root, int dtype, void *scores_dev, int32_t *levels_dev, gg_run_info *info);
+/* app/BellmanFord.C:100-140 from 'root' with integer edge lengths (the
+ * graph's weights truncated to integers, as intE; unit lengths when the
+ * graph has none): dist_dev (I32 n) = shortest distance, INT_MAX/2 when
+ * unreached, all -(INT_MAX/2) after a negative cycle (n rounds).  Each
+ * round is a Jacobi relaxation from the last round's frontier. */
+int gg_sssp(gg_graph_t g, uint32_t root, int32_t *dist_dev, gg_run_info *info);
+
 /* ---- differentiated forms ------------------------------------------------
  * No counterpart in the reference: gradients of SPMV.C's product and of the
  * all-active PageRank (app/PageRankBit.C:145) with row-normalised weights,

@@ -93,6 +93,9 @@ def lib():
                                     C.c_void_p, f64, C.c_void_p]
         L.ggo_wpagerank_adjoint.argtypes = [C.c_uint32, u32, u32, f32, C.c_double, C.c_double, C.c_int,
                                             f64, f64, f64, C.c_void_p, C.c_void_p]
+        i64 = P(np.int64, flags="C_CONTIGUOUS")
+        L.ggo_bc.argtypes = [C.c_uint32, u32, u32, C.c_uint32, f64, i32, C.c_void_p]
+        L.ggo_sssp.argtypes = [C.c_uint32, u32, u32, f32, C.c_uint32, i64]
         _lib = L
     return _lib
 
@@ -309,6 +312,30 @@ def wpagerank_adjoint(off, ids, w, r, u, alpha=0.85, eps=1e-7, max_iter=100):
                                      lam, _vp(gw), _vp(gv))
     assert it >= 0
     return lam, gw[:m], gv, it
+
+
+# ------------------------------------------------------- further apps --
+
+def bc(off, ids, root):
+    """app/BC.C from root: (score, level, sigma, rounds); score NaN where unreached."""
+    n = len(off) - 1
+    score = np.empty(n, np.float64)
+    level = np.empty(n, np.int32)
+    sigma = np.empty(n, np.float64)
+    r = lib().ggo_bc(n, np.ascontiguousarray(off, np.uint32), np.ascontiguousarray(ids, np.uint32), root,
+                     score, level, sigma.ctypes.data)
+    assert r >= 0
+    return score, level, sigma, r
+
+
+def sssp(off, ids, w, root):
+    """app/BellmanFord.C from root with integer lengths: (dist int64, rounds)."""
+    n = len(off) - 1
+    dist = np.empty(n, np.int64)
+    r = lib().ggo_sssp(n, np.ascontiguousarray(off, np.uint32), np.ascontiguousarray(ids, np.uint32),
+                       np.ascontiguousarray(w, np.float32), root, dist)
+    assert r >= 0
+    return dist, r
 
 
 def ref_available(app: str = "pagerank") -> bool:

@@ -154,6 +154,18 @@ int ggo_wpagerank_adjoint(uint32_t n, const uint32_t *off, const uint32_t *ids, 
                           double alpha, double eps, int max_iter, const double *r, const double *u,
                           double *lambda, double *gw, double *gv);
 
+/* ---- apps beyond the four (gg_oracle_apps.c) ------------------------------ */
+/* app/BC.C:154-226 from 'root' on the out-edge CSR (the backward phase runs
+ * on the true transpose).  score = (Dependencies - inv) / inv, NaN where
+ * unreached; level = BFS round of first visit (-1 unreached); sigma_out (may
+ * be NULL) = NumPaths.  FP64.  Returns the number of forward rounds. */
+int ggo_bc(uint32_t n, const uint32_t *off, const uint32_t *ids, uint32_t root, double *score,
+           int32_t *level, double *sigma_out);
+/* app/BellmanFord.C:100-140 with integer edge lengths (w truncated to
+ * integers, as intE): dist = shortest path length, INT_MAX/2 unreached,
+ * all -(INT_MAX/2) after a negative cycle.  Returns the round count. */
+int ggo_sssp(uint32_t n, const uint32_t *off, const uint32_t *ids, const float *w, uint32_t root, int64_t *dist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,3 +81,112 @@ def test_reference_apps_run_on_gpu(app, args):
     r = subprocess.run([exe] + args + ["-rounds", "2"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     assert r.stdout.count("Running") == 2 and "Average" in r.stdout
+
+
+# ---- input formats (IO.h:136-361 PBBS text, 365-656 Galois .gr) -------------
+
+def write_pbbs(path, off, ids, w=None):
+    """PBBS AdjacencyGraph / WeightedAdjacencyGraph text: header, n, m, the n
+    row offsets, the m ids (then the m integer weights)."""
+    n = len(off) - 1
+    with open(path, "w") as f:
+        f.write("WeightedAdjacencyGraph\n" if w is not None else "AdjacencyGraph\n")
+        f.write(f"{n}\n{len(ids)}\n")
+        f.write("\n".join(str(int(x)) for x in off[:-1]) + "\n")
+        f.write("\n".join(str(int(x)) for x in ids) + "\n")
+        if w is not None:
+            f.write("\n".join(str(int(x)) for x in w) + "\n")
+
+
+def write_galois(path, off, ids, w=None):
+    """Galois .gr v1: u64 {1, edge-data size, n, m}, u64 row ends, u32 ids
+    (padded to an even count), u32 edge data."""
+    n, m = len(off) - 1, len(ids)
+    with open(path, "wb") as f:
+        np.array([1, 4 if w is not None else 0, n, m], np.uint64).tofile(f)
+        np.asarray(off[1:], np.uint64).tofile(f)
+        np.asarray(ids, np.uint32).tofile(f)
+        if m % 2:
+            np.zeros(1, np.uint32).tofile(f)
+        if w is not None:
+            np.asarray(w, np.uint32).tofile(f)
+
+
+def _read(path, binary=False, weighted=False):
+    exe = os.path.join(BUILD, "read_graph")
+    if not os.path.exists(exe):
+        pytest.skip("apps not built")
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "csr.bin")
+        args = [exe] + (["-b"] if binary else []) + (["-w"] if weighted else []) + [path, out]
+        r = subprocess.run(args, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, (r.stdout, r.stderr)
+        raw = open(out, "rb").read()
+    n, m = np.frombuffer(raw[:16], np.uint64)
+    n, m = int(n), int(m)
+    off = np.frombuffer(raw[16:16 + 4 * (n + 1)], np.uint32)
+    ids = np.frombuffer(raw[16 + 4 * (n + 1):16 + 4 * (n + 1 + m)], np.uint32)
+    w = np.frombuffer(raw[16 + 4 * (n + 1 + m):], np.float32) if weighted else None
+    return off, ids, w
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("binary", [False, True])
+def test_readers_roundtrip(tmp_path, binary, weighted):
+    off, ids = oracle.rmat_csr(9, 8, 11)
+    w = (1 + np.arange(len(ids)) % 19).astype(np.int64) if weighted else None
+    path = str(tmp_path / ("g.gr" if binary else "g.adj"))
+    (write_galois if binary else write_pbbs)(path, off, ids, w)
+    got_off, got_ids, got_w = _read(path, binary, weighted)
+    assert np.array_equal(got_off, off) and np.array_equal(got_ids, ids)
+    if weighted:
+        assert np.array_equal(got_w, w.astype(np.float32))
+
+
+def test_reader_odd_edge_count_padding(tmp_path):
+    off, ids = oracle.csr_from_edges(4, [0, 1, 2], [1, 2, 3])  # m = 3: ids padded
+    w = np.array([5, 6, 7])
+    path = str(tmp_path / "odd.gr")
+    write_galois(path, off, ids, w)
+    o, i, ww = _read(path, True, True)
+    assert np.array_equal(o, off) and np.array_equal(i, ids) and np.array_equal(ww, w.astype(np.float32))
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/sixnode"), reason="reference tree not present")
+def test_reader_reference_sixnode():
+    """The reference's own sample graph file parses to the CSR the golden
+    fixtures were built from."""
+    off, ids, _ = _read("/root/reference/sixnode")
+    g = R.load_golden(os.path.join(ROOT, "tests", "golden", "sixnode_p2.npz"))
+    assert np.array_equal(off, g["off"]) and np.array_equal(ids, g["ids"])
+
+
+def test_reader_rejects_wrong_header(tmp_path):
+    path = str(tmp_path / "bad.adj")
+    with open(path, "w") as f:
+        f.write("EdgeArray\n1\n0\n0\n")
+    exe = os.path.join(BUILD, "read_graph")
+    if not os.path.exists(exe):
+        pytest.skip("apps not built")
+    r = subprocess.run([exe, path, str(tmp_path / "o.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Bad input file" in r.stdout
+
+
+@pytest.mark.gpu
+def test_frontend_pagerank_from_pbbs_file(tmp_path):
+    """The driver reading a PBBS file (no -rmat) gives the oracle's ranks."""
+    off, ids = oracle.rmat_csr(12, 16, 4)
+    path = str(tmp_path / "g.adj")
+    write_pbbs(path, off, ids)
+    want, it, _ = oracle.Graph(off, ids, P=64).pagerank(10)
+    got, iters, _ = _run("frontend_parity", "pagerank", ["-c", "64", path])
+    assert iters == it and R.rel_l1(got, want) < 1e-12
+
+
+@pytest.mark.gpu
+def test_frontend_bfs_from_galois_file(tmp_path):
+    off, ids = oracle.rmat_csr(12, 16, 2, symmetric=True)
+    path = str(tmp_path / "g.gr")
+    write_galois(path, off, ids)
+    got, _, _ = _run("frontend_parity", "bfs", ["-s", "-b", "-c", "64", "-r", "100", path])
+    R.check_parents(off, ids, 100, got.astype(np.int64))
