@@ -298,6 +298,37 @@ def batched_bench(gg, ctx, count=512, seed=5, iters=30, msbfs_scale=22):
     return res
 
 
+def apps_bench(gg, ctx, scale, steps):
+    """The further apps on the same engine: BC.C from root 100 (FP32, the
+    reference's fType) on the symmetric R-MAT graph of the BFS leg, and
+    BellmanFord.C from vertex 0 on a directed R-MAT graph (seed 4) with
+    integer lengths 1..19."""
+    g = gg.Graph.rmat(ctx, scale, 16, 2, symmetric=True)
+    sc = gg.DeviceArray(ctx, g.n, np.float32)
+    gg.bc(g, 100, np.float32, out=sc)
+    ts = []
+    for _ in range(max(steps, 3)):
+        _, _, info = gg.bc(g, 100, np.float32, out=sc)
+        ts.append(info.ms)
+    res = {"bc": {"scale": scale, "root": 100, "rounds": info.iters, "ms": float(np.median(ts)),
+                  "edges_read": int(info.edges_scanned),
+                  "gbs": 8 * info.edges_scanned / float(np.median(ts)) / 1e6}}
+    g.close()
+    g = gg.Graph.rmat(ctx, scale - 2, 16, 4)
+    w = (1 + (np.arange(g.m, dtype=np.uint64) * np.uint64(2654435761)) % np.uint64(19)).astype(np.float32)
+    g.set_weights(w)
+    d = gg.DeviceArray(ctx, g.n, np.int32)
+    gg.sssp(g, 0, out=d)
+    ts = []
+    for _ in range(max(steps, 3)):
+        _, info = gg.sssp(g, 0, out=d)
+        ts.append(info.ms)
+    res["sssp"] = {"scale": scale - 2, "rounds": info.iters, "ms": float(np.median(ts)),
+                   "edges_read": int(info.edges_scanned)}
+    g.close()
+    return res
+
+
 def cpu_baseline(scale=20, rounds=3):
     """The reference engine (oracle/_ref, compiled from the reference's own
     sources with its OpenMP branch) on a bounded sample: PageRank.C on the
@@ -400,6 +431,9 @@ def main():
         wa = wpagerank_adjoint_bench(gg, ctx, a.scale - 2, a.steps)
         log(f"[bench] pagerank adjoint: {wa}")
         extras["pagerank_adjoint_fp64"] = wa
+        ap_ = apps_bench(gg, ctx, a.scale, a.steps)
+        log(f"[bench] apps: {ap_}")
+        extras["apps"] = ap_
         if not a.no_batch:
             bb = batched_bench(gg, ctx, msbfs_scale=a.scale - 2)
             log(f"[bench] batched: {bb}")

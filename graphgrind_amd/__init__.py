@@ -34,7 +34,7 @@ F64, F32, U32, I32, U8 = 0, 1, 2, 3, 4
 NP_OF = {F64: np.float64, F32: np.float32, U32: np.uint32, I32: np.int32, U8: np.uint8}
 DT_OF = {np.dtype(np.float64): F64, np.dtype(np.float32): F32, np.dtype(np.uint32): U32,
          np.dtype(np.int32): I32, np.dtype(np.uint8): U8}
-OP_PAGERANK, OP_BFS, OP_CC, OP_SPMV = 0, 1, 2, 3
+OP_PAGERANK, OP_BFS, OP_CC, OP_SPMV, OP_SSSP, OP_PATHSUM, OP_SUM = 0, 1, 2, 3, 4, 5, 6
 VOP_PAGERANK, VOP_RESET, VOP_CC_COPY, VOP_ABSDIFF = 0, 1, 2, 3
 FRONTIER_EMPTY, FRONTIER_SINGLE, FRONTIER_ALL = 0, 1, 2
 PATH_EMPTY, PATH_SPARSE, PATH_CSC, PATH_COO = -1, 0, 1, 2
@@ -110,6 +110,7 @@ SIGNATURES = {
     "gg_graph_set_weights": (C.c_int, [_VP, _VP]),
     "gg_graph_partition": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int]),
     "gg_graph_reorder": (C.c_int, [_VP, C.c_int]),
+    "gg_graph_transpose": (C.c_int, [_VP]),
     "gg_graph_device_view": (C.c_int, [_VP, C.c_int, C.POINTER(GraphView)]),
     "gg_graph_partition_starts": (C.c_int, [_VP, C.c_int, _VP, _IP]),
     "gg_graph_coo": (C.c_int, [_VP, _VP, _VP, _VP]),
@@ -318,6 +319,12 @@ class Graph:
         self.part_vertex = by_vertex
         return self
 
+    def transpose(self):
+        """GA.transpose() (graph-numa.h:547-555, 721-725): later calls run on
+        the reversed edges; a second call restores the graph."""
+        check(lib().gg_graph_transpose(self.h))
+        return self
+
     def reorder(self, mode=1):
         """Internal hot-sources-first renumbering used by the pull kernels."""
         check(lib().gg_graph_reorder(self.h, mode))
@@ -476,7 +483,7 @@ class Frontier:
 def edge_map(g: Graph, frontier: Frontier, kind, a: DeviceArray, b: DeviceArray = None,
              threshold=-1, remdups=False):
     """edgeMap(GA, frontier, F, threshold, DENSE, remDups) -> (output, path)."""
-    dtype = DT_OF[a.dtype] if kind in (OP_PAGERANK, OP_SPMV) else DT_OF[a.dtype]
+    dtype = DT_OF[a.dtype]
     op = Op(kind, dtype, a.ptr, None if b is None else b.ptr)
     h = C.c_void_p()
     path = C.c_int()

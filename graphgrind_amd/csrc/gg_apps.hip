@@ -155,7 +155,9 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
     const uint32_t n = g->n;
     const uint64_t m = g->m;
     const bool use_plan = g->plan.mode != 0;
-    const CscView view = use_plan ? plan_view(g) : graph_view(g);
+    CscView view;
+    if (use_plan) view = plan_view(g);
+    else GG_TRY(pull_view(g, &view));
     const uint32_t *outdeg = use_plan ? (const uint32_t *)g->plan.outdeg.p : (const uint32_t *)g->outdeg.p;
     const uint32_t nblk = view.blocks();
     const uint32_t nfix = view.fix_blocks();
@@ -477,13 +479,15 @@ extern "C" int gg_spmv(gg_graph_t g, int dtype, const void *x, void *y, gg_run_i
     GG_HIP(hipSetDevice(c->device));
     if (info) std::memset(info, 0, sizeof(*info));
     const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
+    CscView pv;
+    GG_TRY(pull_view(g, &pv));
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     GG_HIP(hipEventRecord(e0, c->stream));
     int rc = dtype == GG_F64
-                 ? launch_pull(c, graph_view(g), OpSpmvY<double>{(const double *)x, tw, (double *)y}, nullptr, nullptr, nullptr)
-                 : launch_pull(c, graph_view(g), OpSpmvY<float>{(const float *)x, tw, (float *)y}, nullptr, nullptr, nullptr);
+                 ? launch_pull(c, pv, OpSpmvY<double>{(const double *)x, tw, (double *)y}, nullptr, nullptr, nullptr)
+                 : launch_pull(c, pv, OpSpmvY<float>{(const float *)x, tw, (float *)y}, nullptr, nullptr, nullptr);
     GG_HIP(hipEventRecord(e1, c->stream));
     if (!rc) rc = sync(c);
     float ms = 0;

@@ -208,6 +208,14 @@ void graph_exec_drop(Graph *g)
     graph_exec_free(g->adj_exec);
 }
 
+int ensure_csr_rows(Graph *g)
+{
+    if (g->csr_rows_ok) return GG_OK;
+    GG_TRY(rows_build(g->ctx, g->csr_rows, (const uint32_t *)g->off.p, g->n, g->m, g->edges_per_lane));
+    g->csr_rows_ok = true;
+    return GG_OK;
+}
+
 int ensure_rows(Graph *g)
 {
     if (g->rows_ok) return GG_OK;
@@ -644,7 +652,9 @@ static void graph_free(Graph *g)
     for (Buf *b : bufs) buf_free(*b);
     rows_free(g->plan.rows);
     rows_free(g->rows);
+    rows_free(g->csr_rows);
     g->rows_ok = false;
+    g->csr_rows_ok = false;
     graph_exec_drop(g);
     for (Buf &b : g->pr_ws) buf_free(b);
     g->plan.mode = 0;
@@ -1462,6 +1472,56 @@ extern "C" int gg_graph_reorder(gg_graph_t g, int mode)
     return GG_OK;
 }
 
+namespace gg {
+// inv[perm[k]] = k
+__global__ void k_invert_perm(const uint32_t *__restrict__ perm, uint32_t *__restrict__ inv, uint64_t m)
+{
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; k < m; k += stride) inv[perm[k]] = (uint32_t)k;
+}
+} // namespace gg
+
+extern "C" int gg_graph_transpose(gg_graph_t g)
+{
+    if (!g) return fail(GG_ERR_INVALID, "gg_graph_transpose: NULL");
+    // a symmetric graph is its own transpose (graph-numa.h:547-555 skips it)
+    if (g->symmetric) return GG_OK;
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    // both directions' schedules exist before the swap
+    if (!g->csr_pull_ok) {
+        GG_TRY(sched_build(c, g->csr_pull, (const uint32_t *)g->off.p, g->n, g->m));
+        g->csr_pull_ok = true;
+    }
+    Buf inv;
+    if (g->m) {
+        GG_TRY(buf_alloc(c, inv, 4ull * g->m));
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((g->m + kBlock - 1) / kBlock, 1u << 20);
+        k_invert_perm<<<grid, kBlock, 0, c->stream>>>((const uint32_t *)g->perm.p, (uint32_t *)inv.p, g->m);
+        GG_HIP(hipGetLastError());
+        GG_TRY(sync(c));
+        std::swap(g->perm, inv);
+        buf_free(inv);
+    }
+    std::swap(g->off, g->toff);
+    std::swap(g->ids, g->tids);
+    std::swap(g->w, g->tw);
+    std::swap(g->outdeg, g->indeg);
+    std::swap(g->pull, g->csr_pull);
+    std::swap(g->rows, g->csr_rows);
+    std::swap(g->rows_ok, g->csr_rows_ok);
+    g->csr_pull_ok = true;
+    // state tied to one direction
+    g->wsum_ok = false;
+    graph_exec_drop(g);
+    // destination partitions and COO lists follow the new in-edges
+    if (g->P) GG_TRY(gg_graph_partition(g, g->P, g->part_vertex, g->relabel, g->hilbert));
+    if (g->plan.mode) GG_TRY(gg_graph_reorder(g, 0));
+    g->transposed = !g->transposed;
+    return GG_OK;
+}
+
 extern "C" int gg_graph_set_option(gg_graph_t g, int option, int64_t value)
 {
     if (!g) return fail(GG_ERR_INVALID, "gg_graph_set_option: NULL");
@@ -1478,7 +1538,9 @@ extern "C" int gg_graph_set_option(gg_graph_t g, int option, int64_t value)
         if (g->edges_per_lane != (int)value) {
             graph_exec_drop(g);
             rows_free(g->rows);
+            rows_free(g->csr_rows);
             g->rows_ok = false;
+            g->csr_rows_ok = false;
         }
         g->edges_per_lane = (int)value;
         if (g->plan.mode)

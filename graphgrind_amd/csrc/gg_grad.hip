@@ -291,9 +291,10 @@ static int ensure_wsum(Graph *g)
 {
     if (g->wsum_ok) return GG_OK;
     if (!g->weighted) return fail(GG_ERR_STATE, "graph has no edge weights");
-    GG_TRY(ensure_csr_sched(g));
+    CscView cv;
+    GG_TRY(csr_pull_view(g, &cv));
     GG_TRY(buf_ensure(g->ctx, g->wsum, 8ull * g->n));
-    GG_TRY(launch_pull(g->ctx, csr_view(g), OpWSum{(const float *)g->w.p, (double *)g->wsum.p}, nullptr, nullptr,
+    GG_TRY(launch_pull(g->ctx, cv, OpWSum{(const float *)g->w.p, (double *)g->wsum.p}, nullptr, nullptr,
                        nullptr));
     g->wsum_ok = true;
     return GG_OK;
@@ -594,8 +595,8 @@ static int adjoint_outputs(Graph *g, double alpha, const V *r, const V *u, const
 }
 
 template <class V>
-static int wpagerank_adjoint_fixed(Graph *g, double alpha, int iters, const V *r, const V *u, V *gw, V *gv,
-                                   V *lam_out, gg_run_info *info)
+static int wpagerank_adjoint_fixed(Graph *g, const CscView &cv, double alpha, int iters, const V *r, const V *u,
+                                   V *gw, V *gv, V *lam_out, gg_run_info *info)
 {
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
@@ -614,8 +615,7 @@ static int wpagerank_adjoint_fixed(Graph *g, double alpha, int iters, const V *r
         auto body = [&]() -> int {
             k_copy<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)X.own[0].p, u, n);
             for (int it = 1; it <= iters; it++)
-                GG_TRY(launch_pull(c, csr_view(g), adj_op<V>(g, X.own, it, u, alpha, done), nullptr, nullptr,
-                                   nullptr));
+                GG_TRY(launch_pull(c, cv, adj_op<V>(g, X.own, it, u, alpha, done), nullptr, nullptr, nullptr));
             return adjoint_outputs<V>(g, alpha, r, u, (const V *)X.own[iters % 2].p, gw, gv, lam_out);
         };
         const int rc = capture(c, body, X);
@@ -639,7 +639,9 @@ static int wpagerank_adjoint_impl(Graph *g, double alpha, double eps, int max_it
     const uint32_t n = g->n;
     GG_TRY(ensure_wsum(g));
     GG_TRY(ensure_csr_sched(g));
-    if (eps < 0) return wpagerank_adjoint_fixed<V>(g, alpha, max_iter, r, u, gw, gv, lam_out, info);
+    CscView cv;
+    GG_TRY(csr_pull_view(g, &cv));
+    if (eps < 0) return wpagerank_adjoint_fixed<V>(g, cv, alpha, max_iter, r, u, gw, gv, lam_out, info);
     SolveEvents E(max_iter);
     GG_HIP(hipEventRecord(E.ev[0], c->stream));
     Buf L[2];
@@ -649,7 +651,7 @@ static int wpagerank_adjoint_impl(Graph *g, double alpha, double eps, int max_it
         k_copy<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)L[0].p, u, n);
         int iters = 0;
         auto make = [&](int it, const int *done) { return adj_op<V>(g, L, it, u, alpha, done); };
-        rc = run_fixed_point(g, csr_view(g), max_iter, 1, eps, make, E, &iters, nullptr);
+        rc = run_fixed_point(g, cv, max_iter, 1, eps, make, E, &iters, nullptr);
         if (!rc) rc = adjoint_outputs<V>(g, alpha, r, u, (const V *)L[iters % 2].p, gw, gv, lam_out);
         if (!rc) {
             GG_HIP(hipEventRecord(E.ev[1], c->stream));
@@ -682,6 +684,8 @@ extern "C" int gg_spmv_bwd(gg_graph_t g, int dtype, const void *x, const void *g
     GG_HIP(hipSetDevice(c->device));
     if (info) std::memset(info, 0, sizeof(*info));
     GG_TRY(ensure_csr_sched(g));
+    CscView cv;
+    GG_TRY(csr_pull_view(g, &cv));
     const float *w = g->weighted ? (const float *)g->w.p : nullptr;
     hipEvent_t e0, e1, k0, k1;
     (void)hipEventCreate(&e0);
@@ -692,9 +696,9 @@ extern "C" int gg_spmv_bwd(gg_graph_t g, int dtype, const void *x, const void *g
     GG_HIP(hipEventRecord(e0, c->stream));
     if (gx) {
         rc = dtype == GG_F64
-                 ? launch_pull(c, csr_view(g), OpSpmvBwdX<double>{w, (const double *)gy, (double *)gx}, nullptr,
+                 ? launch_pull(c, cv, OpSpmvBwdX<double>{w, (const double *)gy, (double *)gx}, nullptr,
                                nullptr, nullptr, k0, k1)
-                 : launch_pull(c, csr_view(g), OpSpmvBwdX<float>{w, (const float *)gy, (float *)gx}, nullptr,
+                 : launch_pull(c, cv, OpSpmvBwdX<float>{w, (const float *)gy, (float *)gx}, nullptr,
                                nullptr, nullptr, k0, k1);
     }
     if (!rc && gw && g->m) {

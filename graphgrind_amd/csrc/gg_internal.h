@@ -195,21 +195,24 @@ struct Graph {
     bool csr_pull_ok = false;
     Buf wsum;            // double[n]
     bool wsum_ok = false;
-    // row blocks over the in-edge twin, built on first use
-    RowSched rows;
-    bool rows_ok = false;
+    // row blocks over the in-edge twin and over the out-edge CSR, built on
+    // first use
+    RowSched rows, csr_rows;
+    bool rows_ok = false, csr_rows_ok = false;
     // captured fixed-iteration loops (weighted PageRank forward / adjoint)
     GraphExec fwd_exec, adj_exec;
     // workspace of the fused PageRank loop, kept between calls (grow-only)
     Buf pr_ws[12];
+    bool transposed = false; // gg_graph_transpose flips the two directions
     // tuning (gg_graph_set_option)
     int pull_schedule = 1;  // 0 merge-path tiles, 1 row blocks where built
     int edges_per_lane = 8; // row blocks: target edges per lane of a row group
 };
 // Drops captured loops (their addresses or schedules changed).
 void graph_exec_drop(Graph *g);
-// Row blocks of the in-edge twin (built on first use).
+// Row blocks of the in-edge twin / of the out-edge CSR (built on first use).
 int ensure_rows(Graph *g);
+int ensure_csr_rows(Graph *g);
 
 // ---- frontier ---------------------------------------------------------------
 struct Frontier {

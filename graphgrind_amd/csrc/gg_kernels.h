@@ -592,6 +592,19 @@ inline CscView csr_view(Graph *g)
     return v;
 }
 
+// The out-edge CSR with the graph's chosen schedule (the transpose
+// traversal): row blocks or merge-path tiles, built on first use.
+inline int csr_pull_view(Graph *g, CscView *v)
+{
+    GG_TRY(ensure_csr_sched(g));
+    *v = csr_view(g);
+    if (g->pull_schedule == 1) {
+        GG_TRY(ensure_csr_rows(g));
+        v->rows = &g->csr_rows;
+    }
+    return GG_OK;
+}
+
 // The in-edge twin with the graph's chosen schedule: row blocks (built on
 // first use) or merge-path tiles.
 inline int pull_view(Graph *g, CscView *v)

@@ -119,9 +119,9 @@ static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_ru
 {
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
-    CscView fwd;
+    CscView fwd, bwd;
     GG_TRY(pull_view(g, &fwd));
-    GG_TRY(ensure_csr_sched(g));
+    GG_TRY(csr_pull_view(g, &bwd));
     Buf sigma, level, dep, inv, gate;
     int rc = buf_alloc(c, sigma, sizeof(V) * (uint64_t)n);
     if (!rc) rc = buf_alloc(c, level, 4ull * n);
@@ -159,7 +159,7 @@ static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_ru
             for (int r = round - 2; r >= 0 && !rc; r--) {
                 k_bc_gate<V><<<nb, kBlock, 0, c->stream>>>((const int32_t *)level.p, (const V *)dep.p, r + 1, V(0),
                                                            (V *)gate.p, n);
-                rc = launch_pull(c, csr_view(g),
+                rc = launch_pull(c, bwd,
                                  OpBCBack<V>{(const V *)gate.p, (V *)dep.p, (const V *)inv.p,
                                              (const int32_t *)level.p, r},
                                  nullptr, nullptr, nullptr);

@@ -28,10 +28,42 @@ namespace gg {
 // Operators of the generic edgeMap
 // ============================================================================
 
-// PR_F (app/PageRank.C:28-70): contributions p_curr[s]/outdeg(s) come from
-// a prepass; an inactive source holds -0.0 (ranks are >= +0), the sum
-// starts at -0.0, so the sign of a row's sum says whether any source of
-// the frontier reached it.
+// "No contribution": a NaN of a fixed payload that the sum operators'
+// combine() treats as their identity.  A source outside the frontier
+// gathers it, so "no frontier in-neighbour" stays apart from every sum of
+// real values, negative and zero sums included (PageRankDelta.C pushes
+// negative changes).  Arithmetic never produces this payload.
+template <class V>
+struct PathNone;
+template <>
+struct PathNone<float> {
+    using bits_t = uint32_t;
+    static constexpr bits_t kBits = 0x7fa5a5a5u;
+};
+template <>
+struct PathNone<double> {
+    using bits_t = uint64_t;
+    static constexpr bits_t kBits = 0x7ff5a5a5a5a5a5a5ull;
+};
+template <class V>
+__device__ __forceinline__ bool path_none(V x)
+{
+    typename PathNone<V>::bits_t b;
+    __builtin_memcpy(&b, &x, sizeof(V));
+    return b == PathNone<V>::kBits;
+}
+template <class V>
+__device__ __forceinline__ V path_none_value()
+{
+    const typename PathNone<V>::bits_t b = PathNone<V>::kBits;
+    V x;
+    __builtin_memcpy(&x, &b, sizeof(V));
+    return x;
+}
+
+// PR_F (app/PageRank.C:28-70, app/PageRankDelta.C:27-72): contributions
+// a[s]/outdeg(s) come from a prepass, "no contribution" for sources outside
+// the frontier; a row whose sum is not that value had a frontier source.
 template <class V>
 struct OpPRSum {
     using acc_t = V;
@@ -40,12 +72,12 @@ struct OpPRSum {
     uint8_t *next;
     const uint32_t *outdeg;
     __device__ bool skip() const { return false; }
-    __device__ V identity() const { return -V(0); }
-    __device__ V combine(V a, V b) const { return a + b; }
+    __device__ V identity() const { return path_none_value<V>(); }
+    __device__ V combine(V a, V b) const { return path_none(a) ? b : path_none(b) ? a : a + b; }
     __device__ V gather(uint32_t, uint32_t s) const { return contrib[s]; }
     __device__ void finish(uint32_t d, V acc, LocalStats &st) const
     {
-        const bool any = !signbit(acc);
+        const bool any = !path_none(acc);
         if (any) {
             p_next[d] = p_next[d] + acc;
             st.active++;
@@ -80,14 +112,12 @@ struct OpSpmvBit {
     }
 };
 
-// SPMV_F with a frontier: the source membership travels with the value.
-struct SpmvAcc32 {
-    float v;
-    uint32_t c;
-};
+// SPMV_F (and PageRankConverage's PR_F, tw NULL) with a frontier: sources
+// outside it gather "no contribution", so a row's sum says whether any
+// frontier source reached it.
 template <class V>
 struct OpSpmvFrontier {
-    using acc_t = SpmvAcc32; // 8-byte partials; values are float here
+    using acc_t = V;
     const V *x;
     const float *tw;
     const uint32_t *bm;
@@ -95,18 +125,18 @@ struct OpSpmvFrontier {
     uint8_t *next;
     const uint32_t *outdeg;
     __device__ bool skip() const { return false; }
-    __device__ acc_t identity() const { return {0.0f, 0u}; }
-    __device__ acc_t combine(acc_t a, acc_t b) const { return {a.v + b.v, a.c + b.c}; }
-    __device__ acc_t gather(uint32_t e, uint32_t s) const
+    __device__ V identity() const { return path_none_value<V>(); }
+    __device__ V combine(V a, V b) const { return path_none(a) ? b : path_none(b) ? a : a + b; }
+    __device__ V gather(uint32_t e, uint32_t s) const
     {
-        if (!bit_test(bm, s)) return {0.0f, 0u};
-        return {(float)(tw ? x[s] * (V)tw[e] : x[s]), 1u};
+        if (!bit_test(bm, s)) return path_none_value<V>();
+        return tw ? x[s] * (V)tw[e] : x[s];
     }
-    __device__ void finish(uint32_t d, acc_t acc, LocalStats &st) const
+    __device__ void finish(uint32_t d, V acc, LocalStats &st) const
     {
-        const bool any = acc.c != 0;
+        const bool any = !path_none(acc);
         if (any) {
-            p_next[d] = p_next[d] + (V)acc.v;
+            p_next[d] = p_next[d] + acc;
             st.active++;
             st.out_edges += outdeg[d];
         }
@@ -152,7 +182,7 @@ __global__ void k_pr_contrib(const V *__restrict__ p_curr, const uint32_t *__res
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const uint32_t od = outdeg[v];
-    contrib[v] = ((bit || F[v]) && od) ? p_curr[v] / (V)od : -V(0);
+    contrib[v] = ((bit || F[v]) && od) ? p_curr[v] / (V)od : path_none_value<V>();
 }
 
 __global__ void k_cc_gate(const uint32_t *__restrict__ prev, const uint8_t *__restrict__ F, int bit,
@@ -201,6 +231,91 @@ __global__ __launch_bounds__(kBlock) void k_bfs_pull(const uint32_t *__restrict_
     block_stats(st, counters);
 }
 
+// BC_F / BC_Back_F (app/BC.C:28-124) on the dense branch: a[d] += sum of
+// the frontier in-neighbours' values for d not yet visited; d joins the
+// output when a[d] was 0 (its first update in the reference's loop).
+// Sources outside the frontier gather the "no contribution" NaN.
+template <class V>
+__global__ void k_pathsum_gate(const V *__restrict__ a, const uint8_t *__restrict__ F, int bit, V *__restrict__ gate,
+                               uint32_t n)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) gate[v] = (bit || F[v]) ? a[v] : path_none_value<V>();
+}
+
+template <class V>
+struct OpPathSum {
+    using acc_t = V;
+    const V *gate;
+    V *a;
+    const uint8_t *vis;
+    uint8_t *next;
+    const uint32_t *outdeg;
+    __device__ bool skip() const { return false; }
+    __device__ V identity() const { return path_none_value<V>(); }
+    __device__ V combine(V x, V y) const { return path_none(x) ? y : path_none(y) ? x : x + y; }
+    __device__ V gather(uint32_t, uint32_t s) const { return gate[s]; }
+    __device__ void finish(uint32_t d, V acc, LocalStats &st) const
+    {
+        bool emit = false;
+        if (!vis[d] && !path_none(acc)) {
+            const V old = a[d];
+            a[d] = old + acc;
+            emit = old == V(0);
+        }
+        next[d] = emit;
+        if (emit) {
+            st.active++;
+            st.out_edges += outdeg[d];
+        }
+    }
+};
+
+// BF_F (app/BellmanFord.C:28-92) on the dense branch: the frontier's
+// distances are copied first (gate, INT_MIN elsewhere), so every vertex
+// relaxes against last round's values and the round is deterministic.
+__global__ void k_sp_gate(const int32_t *__restrict__ sp, const uint8_t *__restrict__ F, int bit,
+                          int32_t *__restrict__ gate, uint32_t n)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) gate[v] = (bit || F[v]) ? sp[v] : INT32_MIN;
+}
+
+struct OpSPRelax {
+    using acc_t = int32_t;
+    const int32_t *gate;
+    const float *tw; // in-edge order, NULL: unit lengths
+    int32_t *sp;
+    int32_t *vis;
+    uint8_t *next;
+    const uint32_t *outdeg;
+    __device__ bool skip() const { return false; }
+    __device__ int32_t identity() const { return INT32_MAX; }
+    __device__ int32_t combine(int32_t a, int32_t b) const { return a < b ? a : b; }
+    __device__ int32_t gather(uint32_t e, uint32_t s) const
+    {
+        const int32_t g = gate[s];
+        const int32_t w = tw ? (int32_t)__builtin_nontemporal_load(&tw[e]) : 1;
+        return g == INT32_MIN ? INT32_MAX : g + w;
+    }
+    __device__ void finish(uint32_t d, int32_t acc, LocalStats &st) const
+    {
+        bool emit = false;
+        if (acc < sp[d]) {
+            sp[d] = acc;
+            if (vis[d] == 0) {
+                vis[d] = 1;
+                emit = true;
+            }
+        }
+        next[d] = emit;
+        if (emit) {
+            st.active++;
+            st.out_edges += outdeg[d];
+        }
+    }
+};
+
 // Count and out-degree sum of a dense flag array (GoutDegree + reduce,
 // ligra-numa.h:985-988).
 __global__ __launch_bounds__(kBlock) void k_dense_stats(const uint8_t *__restrict__ F,
@@ -220,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_stats(const uint8_t *__restric
 
 // ---- sparse push (edgeMapSparseWithG, ligra-numa.h:868-916) -----------------
 
-enum { PUSH_PR = 0, PUSH_BFS = 1, PUSH_CC = 2, PUSH_SPMV = 3 };
+enum { PUSH_PR = 0, PUSH_BFS = 1, PUSH_CC = 2, PUSH_SPMV = 3, PUSH_SSSP = 4, PUSH_PATHSUM = 5 };
 
 struct PushArgs {
     const uint32_t *list;   // frontier ids
@@ -295,6 +410,29 @@ __global__ __launch_bounds__(kBlock) void k_push(PushArgs A)
                 if (val < ids[ngh]) {
                     const uint32_t old = atomicMin(&ids[ngh], val);
                     emit = val < old && old == prev[ngh];
+                }
+                break;
+            }
+            case PUSH_PATHSUM: { // BC_F::updateAtomic: an add; first update emits
+                const uint8_t *vis = (const uint8_t *)A.b;
+                if (!vis[ngh]) {
+                    if (A.f32) {
+                        float *x = (float *)A.a;
+                        emit = atomicAdd(&x[ngh], x[v]) == 0.0f;
+                    } else {
+                        double *x = (double *)A.a;
+                        emit = atomicAdd(&x[ngh], x[v]) == 0.0;
+                    }
+                }
+                break;
+            }
+            case PUSH_SSSP: { // BF_F::updateAtomic: writeMin && CAS(Visited, 0, 1)
+                int32_t *sp = (int32_t *)A.a;
+                int32_t *vis = (int32_t *)A.b;
+                const int32_t nd = sp[v] + (A.w ? (int32_t)A.w[pos] : 1);
+                if (nd < sp[ngh]) {
+                    const int32_t old = atomicMin(&sp[ngh], nd);
+                    if (nd < old) emit = atomicCAS(&vis[ngh], 0, 1) == 0;
                 }
                 break;
             }
@@ -654,8 +792,6 @@ static void frontier_free(Frontier *f)
 // edgeMap core (shared with the fused app loops)
 // ============================================================================
 
-static bool op_is_sum(int kind) { return kind == GG_OP_PAGERANK || kind == GG_OP_SPMV; }
-
 int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, int remdups,
                  Frontier **pout, int *path, EdgeMapExtra *ex)
 {
@@ -697,39 +833,43 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         uint8_t *next = (uint8_t *)out->bytes.p;
         const uint32_t *outdeg = (const uint32_t *)g->outdeg.p;
         Buf tmpv;
+        CscView pv;
+        if ((rc = pull_view(g, &pv))) return bail(rc);
         switch (op->kind) {
         case GG_OP_PAGERANK: {
             if ((rc = buf_alloc(c, tmpv, (f32 ? 4ull : 8ull) * n))) return bail(rc);
             if (f32) {
                 k_pr_contrib<float><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
                     (const float *)op->a, outdeg, F, bit, (float *)tmpv.p, n);
-                rc = launch_pull(c, graph_view(g), OpPRSum<float>{(const float *)tmpv.p, (float *)op->b, next, outdeg},
+                rc = launch_pull(c, pv, OpPRSum<float>{(const float *)tmpv.p, (float *)op->b, next, outdeg},
                                  cnt, nullptr, nullptr);
             } else {
                 k_pr_contrib<double><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
                     (const double *)op->a, outdeg, F, bit, (double *)tmpv.p, n);
-                rc = launch_pull(c, graph_view(g), OpPRSum<double>{(const double *)tmpv.p, (double *)op->b, next, outdeg},
+                rc = launch_pull(c, pv, OpPRSum<double>{(const double *)tmpv.p, (double *)op->b, next, outdeg},
                                  cnt, nullptr, nullptr);
             }
             break;
         }
-        case GG_OP_SPMV: {
-            const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
+        case GG_OP_SPMV:
+        case GG_OP_SUM: {
+            const float *tw = (g->weighted && op->kind == GG_OP_SPMV) ? (const float *)g->tw.p : nullptr;
             if (bit) {
                 if (f32)
-                    rc = launch_pull(c, graph_view(g), OpSpmvBit<float>{(const float *)op->a, tw, (float *)op->b, next, outdeg,
+                    rc = launch_pull(c, pv, OpSpmvBit<float>{(const float *)op->a, tw, (float *)op->b, next, outdeg,
                                                             (const uint32_t *)g->indeg.p},
                                      cnt, nullptr, nullptr);
                 else
-                    rc = launch_pull(c, graph_view(g), OpSpmvBit<double>{(const double *)op->a, tw, (double *)op->b, next,
+                    rc = launch_pull(c, pv, OpSpmvBit<double>{(const double *)op->a, tw, (double *)op->b, next,
                                                              outdeg, (const uint32_t *)g->indeg.p},
                                      cnt, nullptr, nullptr);
-            } else {
-                if (!f32)
-                    return bail(fail(GG_ERR_UNSUPPORTED,
-                                     "edgeMap SPMV with a partial frontier supports F32 values only"));
-                rc = launch_pull(c, graph_view(g), OpSpmvFrontier<float>{(const float *)op->a, tw, bm, (float *)op->b, next,
+            } else if (f32) {
+                rc = launch_pull(c, pv, OpSpmvFrontier<float>{(const float *)op->a, tw, bm, (float *)op->b, next,
                                                              outdeg},
+                                 cnt, nullptr, nullptr);
+            } else {
+                rc = launch_pull(c, pv, OpSpmvFrontier<double>{(const double *)op->a, tw, bm, (double *)op->b, next,
+                                                              outdeg},
                                  cnt, nullptr, nullptr);
             }
             break;
@@ -743,7 +883,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
                 if ((rc = buf_alloc(c, tmpv, 4ull * n))) return bail(rc);
                 k_cc_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)op->b, F, bit,
                                                                             (uint32_t *)tmpv.p, n);
-                rc = launch_pull(c, graph_view(g), OpCCMin{(const uint32_t *)tmpv.p, (uint32_t *)op->a,
+                rc = launch_pull(c, pv, OpCCMin{(const uint32_t *)tmpv.p, (uint32_t *)op->a,
                                                (const uint32_t *)op->b, next, outdeg},
                                  cnt, nullptr, nullptr);
             }
@@ -762,6 +902,37 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
                     ex ? ex->levels : nullptr, ex ? ex->level : 0, next, outdeg, n, cnt);
             }
             rc = GG_OK;
+            break;
+        }
+        case GG_OP_PATHSUM: {
+            // the frontier's values are copied first (the identity NaN
+            // elsewhere): the sources are visited and the updated vertices
+            // are not, so the copy changes no value but keeps the pull free
+            // of aliasing between gathered and written entries
+            if ((rc = buf_alloc(c, tmpv, (f32 ? 4ull : 8ull) * n))) return bail(rc);
+            if (f32) {
+                k_pathsum_gate<float><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const float *)op->a, F, bit,
+                                                                                       (float *)tmpv.p, n);
+                rc = launch_pull(c, pv, OpPathSum<float>{(const float *)tmpv.p, (float *)op->a,
+                                                         (const uint8_t *)op->b, next, outdeg},
+                                 cnt, nullptr, nullptr);
+            } else {
+                k_pathsum_gate<double><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
+                    (const double *)op->a, F, bit, (double *)tmpv.p, n);
+                rc = launch_pull(c, pv, OpPathSum<double>{(const double *)tmpv.p, (double *)op->a,
+                                                          (const uint8_t *)op->b, next, outdeg},
+                                 cnt, nullptr, nullptr);
+            }
+            break;
+        }
+        case GG_OP_SSSP: {
+            if ((rc = buf_alloc(c, tmpv, 4ull * n))) return bail(rc);
+            k_sp_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const int32_t *)op->a, F, bit,
+                                                                        (int32_t *)tmpv.p, n);
+            const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
+            rc = launch_pull(c, pv, OpSPRelax{(const int32_t *)tmpv.p, tw, (int32_t *)op->a, (int32_t *)op->b, next,
+                                              outdeg},
+                             cnt, nullptr, nullptr);
             break;
         }
         default: return bail(fail(GG_ERR_INVALID, "edgeMap: unknown operator kind"));
@@ -820,10 +991,12 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         A.off = (const uint32_t *)g->off.p;
         A.ids = (const uint32_t *)g->ids.p;
         A.outdeg = (const uint32_t *)g->outdeg.p;
-        A.w = g->weighted ? (const float *)g->w.p : nullptr;
+        A.w = (g->weighted && op->kind != GG_OP_SUM) ? (const float *)g->w.p : nullptr;
         A.kind = op->kind == GG_OP_BFS ? PUSH_BFS
                  : op->kind == GG_OP_CC ? PUSH_CC
                  : op->kind == GG_OP_PAGERANK ? PUSH_PR
+                 : op->kind == GG_OP_SSSP ? PUSH_SSSP
+                 : op->kind == GG_OP_PATHSUM ? PUSH_PATHSUM
                                               : PUSH_SPMV;
         A.f32 = f32;
         A.a = op->a;
@@ -1068,6 +1241,7 @@ static int check_op(gg_graph_t g, const gg_op_t *op)
     switch (op->kind) {
     case GG_OP_PAGERANK:
     case GG_OP_SPMV:
+    case GG_OP_SUM:
         if (!op->b) return fail(GG_ERR_INVALID, "edgeMap: p_next is NULL");
         if (op->dtype != GG_F64 && op->dtype != GG_F32) return fail(GG_ERR_INVALID, "edgeMap: dtype must be F64 or F32");
         break;
@@ -1075,6 +1249,13 @@ static int check_op(gg_graph_t g, const gg_op_t *op)
         if (!op->b) return fail(GG_ERR_INVALID, "edgeMap: prevIDs is NULL");
         break;
     case GG_OP_BFS: break;
+    case GG_OP_SSSP:
+        if (!op->b) return fail(GG_ERR_INVALID, "edgeMap: Visited is NULL");
+        break;
+    case GG_OP_PATHSUM:
+        if (!op->b) return fail(GG_ERR_INVALID, "edgeMap: Visited is NULL");
+        if (op->dtype != GG_F64 && op->dtype != GG_F32) return fail(GG_ERR_INVALID, "edgeMap: dtype must be F64 or F32");
+        break;
     default: return fail(GG_ERR_INVALID, "edgeMap: unknown operator kind");
     }
     (void)g;

@@ -14,6 +14,10 @@
 //      IDs + prevIDs                 -> GG_OP_CC        (app/Components.C:26-86)
 //      p_curr + p_next + V           -> GG_OP_PAGERANK  (app/PageRank.C:28-70)
 //      p_curr + p_next (no V)        -> GG_OP_SPMV      (app/SPMV.C:7-48)
+//      ... + add_factor              -> GG_OP_SUM       (app/PageRankConverage.C:34-88)
+//      V + Delta + nghSum            -> GG_OP_PAGERANK  (app/PageRankDelta.C:27-72)
+//      ShortestPathLen + Visited     -> GG_OP_SSSP      (app/BellmanFord.C:28-92)
+//      NumPaths|Dependencies+Visited -> GG_OP_PATHSUM   (app/BC.C:28-124)
 //    and for vertexMap
 //      damping + addedConstant       -> GG_VOP_PAGERANK (app/PageRank.C:73-87)
 //      IDs + prevIDs                 -> GG_VOP_CC_COPY  (app/Components.C:88-99)
@@ -36,6 +40,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -264,6 +269,8 @@ public:
     intT n = 0, m = 0;
     vertex *V = nullptr;
     partitioner csc;
+    bool isSymmetric = false;
+    bool transposed = false;
     const partitioner &get_csc_partitioner() const { return csc; }
 };
 
@@ -290,7 +297,8 @@ public:
         }
         uint32_t nn;
         uint64_t mm;
-        gg_front::check(gg_graph_info(g, &nn, &mm, nullptr, nullptr), "gg_graph_info");
+        int sym = 0;
+        gg_front::check(gg_graph_info(g, &nn, &mm, &sym, nullptr), "gg_graph_info");
         n = (intT)nn;
         m = (intT)mm;
         gg_front::check(gg_graph_partition(g, P, by_vertex, relabel, 1), "gg_graph_partition");
@@ -301,6 +309,7 @@ public:
         WG = new graph<vertex>();
         WG->n = n;
         WG->m = m;
+        WG->isSymmetric = sym != 0;
         WG->V = new vertex[n];
         std::vector<uint32_t> od(n), id(n);
         gg_front::check(gg_graph_degrees(g, 0, od.data()), "out-degrees");
@@ -329,11 +338,17 @@ public:
     const partitioner &get_partitioner() const { return coo_partition; }
     const partitioner &get_coo_partitioner() const { return coo_partition; }
     int get_num_coo_partitions() const { return coo_partition.get_num_partitions(); }
-    bool transposed() const { return false; }
+    // transpose() (graph-numa.h:547-555, 721-725): the device graph swaps
+    // its two edge directions; the host vertex table flips its degrees.
+    // Copies of this object share the device graph and the vertex table.
+    bool transposed() const { return WG && WG->transposed; }
     void transpose()
     {
-        std::cerr << "transpose(): the device graph keeps one orientation" << std::endl;
-        std::abort();
+        gg_front::check(gg_graph_transpose(h), "transpose");
+        if (WG && !WG->isSymmetric) {
+            for (intT v = 0; v < n; v++) WG->V[v].flipEdges();
+            WG->transposed = !WG->transposed;
+        }
     }
 };
 
@@ -447,6 +462,13 @@ GG_HAS_MEMBER(p_next)
 GG_HAS_MEMBER(V)
 GG_HAS_MEMBER(damping)
 GG_HAS_MEMBER(addedConstant)
+GG_HAS_MEMBER(Delta)
+GG_HAS_MEMBER(nghSum)
+GG_HAS_MEMBER(ShortestPathLen)
+GG_HAS_MEMBER(Visited)
+GG_HAS_MEMBER(NumPaths)
+GG_HAS_MEMBER(Dependencies)
+GG_HAS_MEMBER(add_factor)
 #undef GG_HAS_MEMBER
 
 template <class P>
@@ -469,11 +491,42 @@ gg_op_t edge_op(F &f)
         op.dtype = GG_U32;
         op.a = device_of(f.IDs, true);
         op.b = device_of(f.prevIDs, false);
+    } else if constexpr (has_p_next<F>::value && has_add_factor<F>::value) {
+        // PR_F of app/PageRankConverage.C:34-88: p_next[d] += add_factor[s]
+        op.kind = GG_OP_SUM;
+        op.dtype = dtype_of<decltype(f.add_factor)>();
+        op.a = device_of(f.add_factor, false);
+        op.b = device_of(f.p_next, true);
     } else if constexpr (has_p_curr<F>::value && has_p_next<F>::value) {
         op.kind = has_V<F>::value ? GG_OP_PAGERANK : GG_OP_SPMV;
         op.dtype = dtype_of<decltype(f.p_curr)>();
         op.a = device_of(f.p_curr, false);
         op.b = device_of(f.p_next, true);
+    } else if constexpr (has_V<F>::value && has_Delta<F>::value && has_nghSum<F>::value) {
+        // PR_F of app/PageRankDelta.C:27-72: nghSum[d] += Delta[s]/outdeg(s)
+        op.kind = GG_OP_PAGERANK;
+        op.dtype = dtype_of<decltype(f.Delta)>();
+        op.a = device_of(f.Delta, false);
+        op.b = device_of(f.nghSum, true);
+    } else if constexpr (has_ShortestPathLen<F>::value && has_Visited<F>::value) {
+        // BF_F of app/BellmanFord.C:28-92
+        static_assert(sizeof(*f.ShortestPathLen) == 4 && sizeof(*f.Visited) == 4, "BF_F: 32-bit intT arrays");
+        op.kind = GG_OP_SSSP;
+        op.dtype = GG_I32;
+        op.a = device_of(f.ShortestPathLen, true);
+        op.b = device_of(f.Visited, true);
+    } else if constexpr ((has_NumPaths<F>::value || has_Dependencies<F>::value) && has_Visited<F>::value) {
+        // BC_F / BC_Back_F of app/BC.C:28-124: a[d] += a[s] for unvisited d
+        static_assert(sizeof(*f.Visited) == 1, "BC_F: bool Visited");
+        op.kind = GG_OP_PATHSUM;
+        if constexpr (has_NumPaths<F>::value) {
+            op.dtype = dtype_of<decltype(f.NumPaths)>();
+            op.a = device_of(f.NumPaths, true);
+        } else {
+            op.dtype = dtype_of<decltype(f.Dependencies)>();
+            op.a = device_of(f.Dependencies, true);
+        }
+        op.b = device_of(f.Visited, false);
     } else {
         static_assert(sizeof(F) == 0, "edgeMap: functor not recognised by the device engine");
     }
@@ -536,6 +589,43 @@ void vertexMap(const partitioner &part, partitioned_vertices V, F add)
     host_phase_end();
 }
 
+// vertexFilter (ligra-numa.h:1053-1082): the vertices of V for which F
+// returns true, as a new dense frontier.  F runs on the host over V's
+// members (the filters of the apps are arbitrary vertex code).
+template <class vertex, class F>
+partitioned_vertices vertexFilter(partitioned_graph<vertex> GA, partitioned_vertices V, F filter)
+{
+    using namespace gg_front;
+    const intT n = GA.n;
+    std::vector<uint8_t> d = V.bit ? std::vector<uint8_t>() : V.members();
+    std::vector<uint8_t> keep((size_t)n, 0);
+    host_phase_begin();
+    for (intT j = 0; j < n; j++)
+        if (V.bit || d[j]) keep[j] = filter(j) ? 1 : 0;
+    host_phase_end();
+    gg_frontier_t f;
+    check(gg_frontier_from_host(GA.h, keep.data(), nullptr, 0, &f), "vertexFilter");
+    return partitioned_vertices::wrap(f);
+}
+
+// map_partitionL / map_partition / map_vertex (partitioner.h:338-399):
+// host loops over the partitions of 'part' (and over their vertices).
+template <typename Fn>
+void map_partitionL(const partitioner &part, Fn fn)
+{
+    gg_front::host_phase_begin();
+    const intT np = part.get_num_partitions();
+    parallel_for(intT p = 0; p < np; ++p) fn(p);
+    gg_front::host_phase_end();
+}
+#define map_partition(vname, part, ...) map_partitionL((part), [&](intT vname) { __VA_ARGS__; })
+#define map_vertex(vname, part, ...)                                                    \
+    map_partitionL((part), [&](intT gg_part_) {                                        \
+        for (intT vname = (part).start_of(gg_part_); vname < (part).start_of(gg_part_ + 1); ++vname) { \
+            __VA_ARGS__;                                                                \
+        }                                                                               \
+    })
+
 // map_vertexL (partitioner.h:376-386): a host loop over all vertices.
 template <typename Fn>
 void map_vertexL(const partitioner &part, Fn fn)
@@ -547,10 +637,19 @@ void map_vertexL(const partitioner &part, Fn fn)
 }
 
 // ---- utils.h pieces the apps reference ---------------------------------------------
+// CAS (utils.h:423-442): 4- and 8-byte values, floats compared by bits
 template <class ET>
 inline bool CAS(ET *ptr, ET oldv, ET newv)
 {
-    return __sync_bool_compare_and_swap(ptr, oldv, newv);
+    if constexpr (std::is_floating_point<ET>::value) {
+        using U = typename std::conditional<sizeof(ET) == 4, uint32_t, uint64_t>::type;
+        U o, n;
+        std::memcpy(&o, &oldv, sizeof(ET));
+        std::memcpy(&n, &newv, sizeof(ET));
+        return __sync_bool_compare_and_swap(reinterpret_cast<U *>(ptr), o, n);
+    } else {
+        return __sync_bool_compare_and_swap(ptr, oldv, newv);
+    }
 }
 template <class ET>
 inline bool writeMin(ET *a, ET b)
@@ -797,6 +896,8 @@ int parallel_main(int argc, char *argv[])
             startTime();
             Compute(PG, start);
             nextTime("Running");
+            // an app that transposed the graph (BC.C) gets it back (1195)
+            if (PG.transposed()) PG.transpose();
         }
         reportAvg(rounds);
     };

@@ -106,6 +106,16 @@ int gg_graph_set_weights_dev(gg_graph_t g, const float *w_dev);
 int gg_graph_create_arena(gg_ctx_t ctx, int count, const gg_graph_t *parts, gg_graph_t *out,
                           uint32_t *vbase, uint64_t *ebase);
 
+/* GA.transpose() (graph-numa.h:547-555, 721-725): swaps the out-edge CSR
+ * and the in-edge twin (with weights, degrees and schedules), so later
+ * edgeMaps run on the transposed graph; calling it again restores the
+ * graph.  A symmetric graph is left as it is, as in the reference.  The
+ * reference flips only the vertex views of its CSC graph and leaves the
+ * COO partitions as they were; here every structure follows the transpose:
+ * a partitioned graph is re-partitioned by the new destinations (same P and
+ * options) and a relabelled plan (gg_graph_reorder) is dropped. */
+int gg_graph_transpose(gg_graph_t g);
+
 /* Internal vertex renumbering for the pull kernels (no reference
  * counterpart in the engine; the reference reorders offline with VEBO,
  * graphtools/VEBO.C, and '-o').  mode 1: vertices by out-degree descending,
@@ -174,18 +184,33 @@ int gg_frontier_copy_sparse(gg_frontier_t f, uint32_t *out, uint64_t cap, uint64
 /* ---- edgeMap (ligra-numa.h:919-1019) ------------------------------------ */
 /* The app functors become an enumerated operator set; a and b are the
  * functor's device arrays.  See include/gg/ligra.h for the trait mapping. */
-#define GG_OP_PAGERANK 0 /* PR_F, app/PageRank.C:28-70.  a=p_curr b=p_next
-                            (F64 or F32); p_next[d] += p_curr[s]/outdeg(s).
-                            p_curr must be >= +0 (ranks)                  */
+#define GG_OP_PAGERANK 0 /* PR_F, app/PageRank.C:28-70 and
+                            app/PageRankDelta.C:27-72.  a=p_curr (Delta)
+                            b=p_next (nghSum), F64 or F32;
+                            p_next[d] += p_curr[s]/outdeg(s); any sign   */
 #define GG_OP_BFS 1      /* BFS_F, app/BFS.C:26-75.  a=Parents (I32, -1 =
                             unvisited); cond = Parents[d] == -1           */
 #define GG_OP_CC 2       /* CC_F, app/Components.C:26-86.  a=IDs b=prevIDs
                             (U32); min-label, frontier = changed          */
 #define GG_OP_SPMV 3     /* SPMV_F, app/SPMV.C:7-48.  a=p_curr b=p_next
                             (F64 or F32); p_next[d] += p_curr[s]*w        */
+#define GG_OP_SSSP 4     /* BF_F, app/BellmanFord.C:28-92.  a=ShortestPathLen
+                            b=Visited (I32 both); d joins the output when
+                            SP[s] + w < SP[d] lowers SP[d] and Visited[d]
+                            was 0 (then set to 1).  Integer lengths: the
+                            graph's weights truncated (1 when unweighted).
+                            The dense branch relaxes from a copy of the
+                            frontier's distances (a Jacobi step).       */
+#define GG_OP_PATHSUM 5  /* BC_F / BC_Back_F, app/BC.C:28-124.  a=NumPaths
+                            or Dependencies (F64 or F32), b=Visited (U8);
+                            cond = !Visited[d]; a[d] += a[s]; d joins the
+                            output when a[d] was 0 before this edgeMap   */
+#define GG_OP_SUM 6      /* PR_F of app/PageRankConverage.C:34-88.
+                            a=add_factor b=p_next (F64 or F32);
+                            p_next[d] += a[s], edge weights not used     */
 typedef struct {
     int32_t kind;
-    int32_t dtype; /* GG_F64 / GG_F32 for PAGERANK and SPMV */
+    int32_t dtype; /* GG_F64 / GG_F32 for PAGERANK, SPMV, PATHSUM, SUM */
     void *a;
     void *b;
 } gg_op_t;

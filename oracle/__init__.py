@@ -96,6 +96,8 @@ def lib():
         i64 = P(np.int64, flags="C_CONTIGUOUS")
         L.ggo_bc.argtypes = [C.c_uint32, u32, u32, C.c_uint32, f64, i32, C.c_void_p]
         L.ggo_sssp.argtypes = [C.c_uint32, u32, u32, f32, C.c_uint32, i64]
+        L.ggo_prdelta.argtypes = [C.c_uint32, u32, u32, C.c_double, C.c_double, C.c_double, C.c_int, f64]
+        L.ggo_prconv.argtypes = [C.c_uint32, u32, u32, C.c_double, C.c_double, C.c_int, f64]
         _lib = L
     return _lib
 
@@ -336,6 +338,26 @@ def sssp(off, ids, w, root):
                        np.ascontiguousarray(w, np.float32), root, dist)
     assert r >= 0
     return dist, r
+
+
+def prdelta(off, ids, damping=0.85, eps=1e-7, eps2=0.01, max_rounds=1 << 30):
+    """app/PageRankDelta.C: (p, rounds)."""
+    n = len(off) - 1
+    p = np.empty(n, np.float64)
+    r = lib().ggo_prdelta(n, np.ascontiguousarray(off, np.uint32), np.ascontiguousarray(ids, np.uint32),
+                          damping, eps, eps2, max_rounds, p)
+    assert r >= 0
+    return p, r
+
+
+def prconv(off, ids, damping=0.85, eps=1e-7, max_iter=100):
+    """app/PageRankConverage.C: (p, iterations)."""
+    n = len(off) - 1
+    p = np.empty(n, np.float64)
+    r = lib().ggo_prconv(n, np.ascontiguousarray(off, np.uint32), np.ascontiguousarray(ids, np.uint32),
+                         damping, eps, max_iter, p)
+    assert r >= 0
+    return p, r
 
 
 def ref_available(app: str = "pagerank") -> bool:

@@ -165,6 +165,18 @@ int ggo_bc(uint32_t n, const uint32_t *off, const uint32_t *ids, uint32_t root, 
  * integers, as intE): dist = shortest path length, INT_MAX/2 unreached,
  * all -(INT_MAX/2) after a negative cycle.  Returns the round count. */
 int ggo_sssp(uint32_t n, const uint32_t *off, const uint32_t *ids, const float *w, uint32_t root, int64_t *dist);
+/* app/PageRankDelta.C:124-176: delta-propagating PageRank; only vertices
+ * whose last change exceeded eps2 * p push their change.  Stops when the
+ * L1 norm of the changes falls below eps, or after max_rounds.  p = ranks.
+ * Returns the number of rounds. */
+int ggo_prdelta(uint32_t n, const uint32_t *off, const uint32_t *ids, double damping, double eps, double eps2,
+                int max_rounds, double *p);
+/* app/PageRankConverage.C:227-310: PageRank with the lost (dangling) mass
+ * spread uniformly, p normalised each iteration; at most max_iter (the
+ * app's MaxIter = 100) iterations while the L1 change exceeds eps.
+ * p = ranks.  Returns the iteration count. */
+int ggo_prconv(uint32_t n, const uint32_t *off, const uint32_t *ids, double damping, double eps, int max_iter,
+               double *p);
 
 #ifdef __cplusplus
 }

@@ -15,9 +15,23 @@
  *            reference; this restatement computes in FP64.
  *   ggo_sssp app/BellmanFord.C: distances by relaxation rounds over the
  *            frontier of changed vertices (integer weights).
+ *   ggo_prdelta app/PageRankDelta.C:124-176: each round the active
+ *            vertices push Delta[s]/outdeg(s) along their out-edges
+ *            (PR_F, 27-72) into nghSum; then every vertex updates p and
+ *            Delta (PR_Vertex_F_FirstRound in round 1, PR_Vertex_F after,
+ *            74-110) and stays active while |Delta| > eps2 * p; stop when
+ *            sum |Delta| < eps.
+ *   ggo_prconv app/PageRankConverage.C:227-310: add_factor = 0.85 p/outdeg
+ *            pushed along out-edges (PR_F, 34-88), then p_next += (1 -
+ *            sum p_next)/n (PR_Vertex_Csum), L1 = sum |p_curr - p_next|,
+ *            p_next /= sum p_next (PR_Vertex_norm), swap.
  *
  * Pins (tests/test_oracle_apps.py): textbook Brandes dependencies and path
- * counts on small graphs; textbook Bellman-Ford / Dijkstra distances.
+ * counts on small graphs; textbook Bellman-Ford / Dijkstra distances;
+ * PageRankDelta with eps2 = 0 against PageRank power iteration (the two are
+ * the same recurrence then) and with a huge eps2 against one power step;
+ * PageRankConverage's first iteration in closed form and its converged
+ * ranks against a direct solve of the dangling-corrected PageRank system.
  */
 #include <math.h>
 #include <stdint.h>
@@ -145,4 +159,86 @@ int ggo_sssp(uint32_t n, const uint32_t *off, const uint32_t *ids, const float *
     }
     free(vis), free(cur), free(nxt);
     return (int)(round > 2147483647 ? 2147483647 : round);
+}
+
+int ggo_prconv(uint32_t n, const uint32_t *off, const uint32_t *ids, double damping, double eps, int max_iter,
+               double *p)
+{
+    if (!n) return 0;
+    double *cur = malloc(sizeof(double) * n), *nxt = calloc(n, sizeof(double));
+    double *add = malloc(sizeof(double) * n);
+    if (!cur || !nxt || !add) {
+        free(cur), free(nxt), free(add);
+        return -1;
+    }
+    for (uint32_t v = 0; v < n; v++) cur[v] = 1.0 / (double)n;
+    int count = 0;
+    double l1 = 2.0;
+    while (count < max_iter && l1 > eps) {
+        for (uint32_t v = 0; v < n; v++) add[v] = damping / (double)(off[v + 1] - off[v]) * cur[v];
+        for (uint32_t s = 0; s < n; s++)
+            for (uint32_t e = off[s]; e < off[s + 1]; e++) nxt[ids[e]] += add[s];
+        double sum = 0.0;
+        for (uint32_t v = 0; v < n; v++) sum += nxt[v];
+        const double csum = (1.0 - sum) / (double)n;
+        for (uint32_t v = 0; v < n; v++) nxt[v] += csum;
+        l1 = 0.0;
+        for (uint32_t v = 0; v < n; v++) l1 += fabs(cur[v] - nxt[v]);
+        sum = 0.0;
+        for (uint32_t v = 0; v < n; v++) sum += nxt[v];
+        for (uint32_t v = 0; v < n; v++) nxt[v] /= sum;
+        for (uint32_t v = 0; v < n; v++) cur[v] = 0.0;
+        count++;
+        double *t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    memcpy(p, cur, sizeof(double) * n);
+    free(cur), free(nxt), free(add);
+    return count;
+}
+
+int ggo_prdelta(uint32_t n, const uint32_t *off, const uint32_t *ids, double damping, double eps, double eps2,
+                int max_rounds, double *p)
+{
+    if (!n) return 0;
+    double *delta = malloc(sizeof(double) * n), *ngh = calloc(n, sizeof(double));
+    uint8_t *act = malloc(n);
+    if (!delta || !ngh || !act) {
+        free(delta), free(ngh), free(act);
+        return -1;
+    }
+    const double one_over_n = 1.0 / (double)n;
+    const double added = (1.0 - damping) * one_over_n;
+    for (uint32_t v = 0; v < n; v++) {
+        p[v] = 0.0;
+        delta[v] = one_over_n;
+        act[v] = 1;
+    }
+    int round = 0;
+    while (round < max_rounds) {
+        round++;
+        for (uint32_t s = 0; s < n; s++) {
+            if (!act[s]) continue;
+            const double c = delta[s] / (double)(off[s + 1] - off[s]);
+            for (uint32_t e = off[s]; e < off[s + 1]; e++) ngh[ids[e]] += c;
+        }
+        double l1 = 0.0;
+        for (uint32_t v = 0; v < n; v++) {
+            if (round == 1) {
+                delta[v] = damping * (p[v] + ngh[v]) + added - p[v];
+                p[v] += delta[v];
+                delta[v] -= one_over_n;
+            } else {
+                delta[v] = ngh[v] * damping;
+                p[v] += delta[v];
+            }
+            act[v] = fabs(delta[v]) > eps2 * p[v];
+            l1 += fabs(delta[v]);
+        }
+        if (l1 < eps) break;
+        memset(ngh, 0, sizeof(double) * n);
+    }
+    free(delta), free(ngh), free(act);
+    return round;
 }

@@ -21,7 +21,8 @@ REF_APP = "/root/reference/app"
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_APP), reason="reference sources not present")
-@pytest.mark.parametrize("app", ["PageRank", "BFS", "Components", "SPMV"])
+@pytest.mark.parametrize("app", ["PageRank", "BFS", "Components", "SPMV", "BC", "BellmanFord", "PageRankDelta",
+                                 "PageRankBit", "PageRankConverage"])
 def test_reference_app_compiles_unchanged(app):
     with open(os.path.join(REF_APP, app + ".C")) as src:
         r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++", "-", "-I.", "-I.."],
@@ -34,7 +35,8 @@ def _run(exe, app, args):
     with tempfile.TemporaryDirectory() as td:
         out = os.path.join(td, "out.bin")
         env = dict(os.environ, GG_APP=app, GG_DUMP=out)
-        r = subprocess.run([os.path.join(BUILD, exe)] + args + ["-rounds", "1"], env=env,
+        # the input file is the last argument (commandLine::getArgument(0))
+        r = subprocess.run([os.path.join(BUILD, exe), "-rounds", "1"] + args, env=env,
                            capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr[-2000:]
         raw = np.fromfile(out, np.float64)
@@ -73,7 +75,12 @@ def test_frontend_spmv_weighted():
 @pytest.mark.parametrize("app,args", [("PageRank", ["-c", "384", "-rmat", "18", "16", "1"]),
                                       ("BFS", ["-s", "-c", "384", "-r", "100", "-rmat", "18", "16", "2"]),
                                       ("Components", ["-s", "-c", "384", "-rmat", "18", "16", "2"]),
-                                      ("SPMV", ["-c", "384", "-rmat", "18", "16", "3"])])
+                                      ("SPMV", ["-c", "384", "-rmat", "18", "16", "3"]),
+                                      ("BC", ["-c", "384", "-r", "100", "-rmat", "16", "16", "2"]),
+                                      ("BellmanFord", ["-s", "-c", "384", "-r", "100", "-rmat", "16", "16", "2"]),
+                                      ("PageRankDelta", ["-c", "384", "-rmat", "16", "16", "1"]),
+                                      ("PageRankBit", ["-c", "384", "-rmat", "16", "16", "1"]),
+                                      ("PageRankConverage", ["-c", "384", "-rmat", "16", "16", "1"])])
 def test_reference_apps_run_on_gpu(app, args):
     exe = os.path.join(BUILD, app)
     if not os.path.exists(exe):
@@ -81,6 +88,65 @@ def test_reference_apps_run_on_gpu(app, args):
     r = subprocess.run([exe] + args + ["-rounds", "2"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     assert r.stdout.count("Running") == 2 and "Average" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sym", [False, True])
+def test_frontend_bc(sym):
+    """BC.C's loop (both phases, host vertexMap functors, GA.transpose())
+    through the front end against the oracle (pinned by Brandes):
+    FP64 <= 1e-12 rel L1, the reference's FP32 fType <= 1e-4, NaN exactly
+    where unreached."""
+    off, ids = oracle.rmat_csr(12, 16, 2, symmetric=sym)
+    want, lv, _, rounds = oracle.bc(off, ids, 100)
+    reached = lv >= 0
+    args = (["-s"] if sym else []) + ["-c", "64", "-r", "100", "-rmat", "12", "16", "2"]
+    for app, tol in (("bc", 1e-12), ("bc32", 1e-4)):
+        got, r, _ = _run("frontend_parity", app, args)
+        assert r == rounds
+        assert np.all(np.isnan(got[~reached])) and not np.any(np.isnan(got[reached]))
+        assert R.rel_l1(got[reached], want[reached]) < tol, app
+
+
+@pytest.mark.gpu
+def test_frontend_bellman_ford(tmp_path):
+    """BellmanFord.C's loop through the front end: unit lengths (unweighted
+    build, as the reference without WEIGHTED) and integer lengths read from a
+    PBBS WeightedAdjacencyGraph file; distances bit-exact."""
+    off, ids = oracle.rmat_csr(12, 8, 3)
+    want1, _ = oracle.sssp(off, ids, np.ones(len(ids), np.float32), 100)
+    got, _, _ = _run("frontend_parity", "sssp", ["-c", "64", "-r", "100", "-rmat", "12", "8", "3"])
+    assert np.array_equal(got.astype(np.int64), want1)
+    w = (1 + (np.arange(len(ids)) * 2654435761 % 23)).astype(np.int64)
+    path = str(tmp_path / "w.adj")
+    write_pbbs(path, off, ids, w)
+    want, _ = oracle.sssp(off, ids, w.astype(np.float32), 100)
+    got, _, _ = _run("frontend_parity_w", "sssp", ["-c", "64", "-r", "100", path])
+    assert np.array_equal(got.astype(np.int64), want)
+
+
+@pytest.mark.gpu
+def test_frontend_pagerank_delta():
+    """PageRankDelta.C's loop (signed pushes from a filtered frontier, host
+    vertexFilter functors) through the front end against the oracle (pinned
+    by power iteration): same round count, ranks <= 1e-10 rel L1."""
+    off, ids = oracle.rmat_csr(13, 16, 1)
+    want, rounds = oracle.prdelta(off, ids)
+    got, r, _ = _run("frontend_parity", "prdelta", ["-c", "64", "-rmat", "13", "16", "1"])
+    assert r == rounds
+    assert R.rel_l1(got, want) < 1e-10
+
+
+@pytest.mark.gpu
+def test_frontend_pagerank_converage():
+    """PageRankConverage.C's loop (edgeMap of add_factor, host Csum / norm
+    functors over map_partition sums) through the front end against the
+    oracle (pinned by a closed-form first step and a direct solve)."""
+    off, ids = oracle.rmat_csr(13, 16, 1)
+    want, it = oracle.prconv(off, ids)
+    got, r, _ = _run("frontend_parity", "prconv", ["-c", "64", "-rmat", "13", "16", "1"])
+    assert r == it
+    assert R.rel_l1(got, want) < 1e-10
 
 
 # ---- input formats (IO.h:136-361 PBBS text, 365-656 Galois .gr) -------------

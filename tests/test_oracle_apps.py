@@ -1,5 +1,7 @@
 """Pins for the oracle's further apps (oracle/gg_oracle_apps.c): BC.C
-against textbook Brandes accumulation, BellmanFord.C against Dijkstra."""
+against textbook Brandes accumulation, BellmanFord.C against Dijkstra,
+PageRankDelta.C against PageRank power iteration and a one-step closed form,
+PageRankConverage.C against its first step in closed form and a direct solve."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -86,3 +88,92 @@ def test_sssp_negative_cycle():
     w = np.array([1, 1, -3], np.float32)
     dist, rounds = oracle.sssp(off, ids, w, 0)
     assert rounds == 3 and np.all(dist == -INF)
+
+
+def power_pagerank(off, ids, damping=0.85, eps=1e-7, max_iter=10000):
+    """Textbook PageRank power iteration with scipy (no dangling-mass
+    correction, as the reference apps): p_k = (1-d)/n + d A^T (p_{k-1}/deg)
+    from the uniform vector, until ||p_k - p_{k-1}||_1 < eps.  (p, k)."""
+    n = len(off) - 1
+    deg = np.diff(off).astype(np.float64)
+    src = np.repeat(np.arange(n), np.diff(off))
+    AT = sp.csr_matrix((np.ones(len(ids)), (ids, src)), shape=(n, n))
+    p = np.full(n, 1.0 / n)
+    for k in range(1, max_iter + 1):
+        q = (1 - damping) / n + damping * (AT @ np.where(deg > 0, p / np.maximum(deg, 1), 0.0))
+        done = np.abs(q - p).sum() < eps
+        p = q
+        if done:
+            return p, k
+    return p, max_iter
+
+
+@pytest.mark.parametrize("scale,seed", [(8, 1), (11, 4)])
+def test_prdelta_without_filter_is_power_iteration(scale, seed):
+    """With eps2 = 0 every vertex with a non-zero change stays active, and
+    the delta recurrence p_k = p_{k-1} + Delta_k, Delta_k = d A^T
+    (Delta_{k-1}/deg), is the power iteration exactly: same ranks (to
+    rounding) and the same round at which the L1 change drops below eps."""
+    off, ids = oracle.rmat_csr(scale, 8, seed)
+    want, it = power_pagerank(off, ids)
+    got, rounds = oracle.prdelta(off, ids, eps2=0.0)
+    assert rounds == it
+    assert np.abs(got - want).sum() / np.abs(want).sum() < 1e-12
+
+
+def test_prdelta_huge_filter_is_one_power_step():
+    """With eps2 so large that no vertex stays active, round 2 pushes nothing,
+    so p is one damped step from the uniform vector:
+    p = (1-d)/n + d * A^T (1/n / outdeg) (scipy), after exactly 2 rounds."""
+    off, ids = oracle.rmat_csr(9, 8, 7)
+    want, _ = power_pagerank(off, ids, max_iter=1)
+    got, rounds = oracle.prdelta(off, ids, eps2=1e300)
+    assert rounds == 2
+    assert np.allclose(got, want, rtol=1e-13, atol=0)
+
+
+def test_prdelta_filter_converges_near_pagerank():
+    """With the app's eps2 = 0.01 the filtered run stays close to the fixed
+    point: its L1 distance to converged power iteration is small."""
+    off, ids = oracle.rmat_csr(11, 16, 2)
+    want, _ = power_pagerank(off, ids, eps=1e-13)
+    got, rounds = oracle.prdelta(off, ids)
+    assert 2 < rounds < 200
+    assert np.abs(got - want).sum() / np.abs(want).sum() < 5e-2
+
+
+def _dangling_system(off, ids, damping=0.85):
+    n = len(off) - 1
+    deg = np.diff(off).astype(np.float64)
+    src = np.repeat(np.arange(n), np.diff(off))
+    AT = sp.csr_matrix((damping / deg[src], (ids, src)), shape=(n, n))  # d A^T D^-1
+    return n, deg, AT
+
+
+def test_prconv_first_iteration_closed_form():
+    """One iteration: x = d A^T (p0/deg); x += (1 - sum x)/n; x /= sum x,
+    evaluated with scipy from the uniform vector (max_iter = 1)."""
+    off, ids = oracle.rmat_csr(10, 8, 3)
+    n, deg, AT = _dangling_system(off, ids)
+    x = AT @ np.full(n, 1.0 / n)
+    x = x + (1 - x.sum()) / n
+    x = x / x.sum()
+    got, it = oracle.prconv(off, ids, max_iter=1)
+    assert it == 1 and np.allclose(got, x, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("scale,seed", [(8, 1), (10, 5)])
+def test_prconv_converges_to_dangling_pagerank(scale, seed):
+    """The fixed point spreads the non-forwarded mass uniformly: p = d A^T
+    D^-1 p + (1 - d sum_{deg>0} p)/n with sum p = 1.  Solved directly
+    (dense LU); the run ends within the app's 100 iterations once the L1
+    change is below 1e-7, so p agrees to about 1e-6."""
+    off, ids = oracle.rmat_csr(scale, 8, seed)
+    n, deg, AT = _dangling_system(off, ids)
+    fwd = (deg > 0).astype(np.float64)
+    M = np.eye(n) - AT.toarray() + (0.85 / n) * np.outer(np.ones(n), fwd)
+    want = np.linalg.solve(M, np.full(n, 1.0 / n))
+    got, it = oracle.prconv(off, ids)
+    assert it < 100
+    assert abs(got.sum() - 1) < 1e-12
+    assert np.abs(got - want).sum() < 1e-5
