@@ -272,9 +272,9 @@ static int pagerank_generic(Graph *g, int max_iter, double damping, double eps, 
     const uint32_t n = g->n;
     const int dt = sizeof(V) == 8 ? GG_F64 : GG_F32;
     Buf pc, pn, red;
-    GG_TRY(buf_alloc(c, pc, sizeof(V) * (uint64_t)n));
-    GG_TRY(buf_alloc(c, pn, sizeof(V) * (uint64_t)n));
-    GG_TRY(buf_alloc(c, red, 8));
+    GG_TRY(pool_take(c, pc, sizeof(V) * (uint64_t)n));
+    GG_TRY(pool_take(c, pn, sizeof(V) * (uint64_t)n));
+    GG_TRY(pool_take(c, red, 8));
     GG_TRY(gg_fill(abi(c), pc.p, dt, n, 1 / (double)n));
     GG_TRY(gg_fill(abi(c), pn.p, dt, n, 0));
     Frontier *F = new Frontier();
@@ -321,9 +321,9 @@ static int pagerank_generic(Graph *g, int max_iter, double damping, double eps, 
         info->edges_scanned = (uint64_t)count * g->m;
     }
     frontier_delete(F);
-    buf_free(pc);
-    buf_free(pn);
-    buf_free(red);
+    pool_give(c, pc);
+    pool_give(c, pn);
+    pool_give(c, red);
     return rc;
 }
 
@@ -369,7 +369,7 @@ extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *p
     Frontier *F = new Frontier();
     frontier_alloc(g, F);
     uint32_t od = 0;
-    int rc0 = buf_alloc(c, F->list, 8);
+    int rc0 = pool_take(c, F->list, 8);
     if (!rc0 && hipMemcpyAsync(F->list.p, &root, 4, hipMemcpyHostToDevice, c->stream)) rc0 = fail(GG_ERR_HIP, "root upload");
     if (!rc0 && hipMemcpyAsync(&od, (uint32_t *)g->outdeg.p + root, 4, hipMemcpyDeviceToHost, c->stream))
         rc0 = fail(GG_ERR_HIP, "outdeg readback");
@@ -414,7 +414,7 @@ extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
     GG_HIP(hipSetDevice(c->device));
     if (info) std::memset(info, 0, sizeof(*info));
     Buf prev;
-    GG_TRY(buf_alloc(c, prev, 4ull * g->n));
+    GG_TRY(pool_take(c, prev, 4ull * g->n));
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -447,7 +447,7 @@ extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     frontier_delete(F);
-    buf_free(prev);
+    pool_give(c, prev);
     if (info) {
         info->iters = rounds;
         info->ms = ms;

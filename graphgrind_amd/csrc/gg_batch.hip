@@ -79,10 +79,10 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
     CscView view;
     GG_TRY(pull_view(g, &view));
     Buf F[2], seen, src;
-    int rc = buf_alloc(c, F[0], 4ull * n);
-    if (!rc) rc = buf_alloc(c, F[1], 4ull * n);
-    if (!rc) rc = buf_alloc(c, seen, 4ull * n);
-    if (!rc) rc = buf_alloc(c, src, 4ull * nsrc);
+    int rc = pool_take(c, F[0], 4ull * n);
+    if (!rc) rc = pool_take(c, F[1], 4ull * n);
+    if (!rc) rc = pool_take(c, seen, 4ull * n);
+    if (!rc) rc = pool_take(c, src, 4ull * nsrc);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -125,10 +125,10 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    buf_free(F[0]);
-    buf_free(F[1]);
-    buf_free(seen);
-    buf_free(src);
+    pool_give(c, F[0]);
+    pool_give(c, F[1]);
+    pool_give(c, seen);
+    pool_give(c, src);
     return rc;
 }
 
@@ -180,8 +180,8 @@ extern "C" int gg_autotune_pull(gg_graph_t g, int dtype, int trials, gg_tune_inf
     std::memset(out, 0, sizeof(*out));
     const size_t vs = dtype == GG_F64 ? 8 : 4;
     Buf x, y;
-    GG_TRY(buf_alloc(c, x, vs * g->n));
-    int rc = buf_alloc(c, y, vs * g->n);
+    GG_TRY(pool_take(c, x, vs * g->n));
+    int rc = pool_take(c, y, vs * g->n);
     if (!rc) rc = gg_fill(abi(c), x.p, dtype, g->n, 1.0);
     // candidates: merge-path tiles, then row blocks at 1..32 edges per lane
     static const int kEpl[] = {1, 2, 4, 8, 16, 32};
@@ -206,8 +206,8 @@ extern "C" int gg_autotune_pull(gg_graph_t g, int dtype, int trials, gg_tune_inf
             best_ms = ms;
         }
     }
-    buf_free(x);
-    buf_free(y);
+    pool_give(c, x);
+    pool_give(c, y);
     if (rc) {
         (void)gg_graph_set_option(g, GG_OPT_PULL_SCHEDULE, keep_sched);
         (void)gg_graph_set_option(g, GG_OPT_ROW_EDGES_PER_LANE, keep_epl);

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <set>
 
 #include "gg_internal.h"
 
@@ -57,6 +58,71 @@ void buf_free(Buf &b)
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
+}
+
+// Contexts alive now: a frontier may be destroyed after its context, and
+// then its buffers are freed rather than pooled.
+static std::mutex g_ctx_mu;
+static std::set<Ctx *> g_live_ctx;
+constexpr uint64_t kPoolCap = 8ull << 30; // bytes kept per context
+
+void pool_drain(Ctx *c)
+{
+    for (auto &kv : c->pool) (void)hipFree(kv.second);
+    c->pool.clear();
+    c->pool_bytes = 0;
+}
+
+int pool_take(Ctx *c, Buf &b, uint64_t bytes)
+{
+    if (b.p) pool_give(c, b);
+    if (bytes == 0) bytes = 8;
+    auto it = c->pool.lower_bound(bytes);
+    if (it != c->pool.end() && it->first <= std::max<uint64_t>(2 * bytes, bytes + (1ull << 20))) {
+        b.p = it->second;
+        b.bytes = it->first;
+        c->pool_bytes -= it->first;
+        c->pool.erase(it);
+        return GG_OK;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess && !c->pool.empty()) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(c->stream);
+        pool_drain(c);
+        e = hipMalloc(&b.p, bytes);
+    }
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        b.bytes = 0;
+        return fail(GG_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return GG_OK;
+}
+
+int pool_ensure(Ctx *c, Buf &b, uint64_t bytes)
+{
+    if (b.p && b.bytes >= bytes) return GG_OK;
+    return pool_take(c, b, bytes);
+}
+
+void pool_give(Ctx *c, Buf &b)
+{
+    if (!b.p) return;
+    bool live;
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        live = c && g_live_ctx.count(c);
+    }
+    if (live && c->pool_bytes + b.bytes <= kPoolCap) {
+        c->pool.emplace(b.bytes, b.p);
+        c->pool_bytes += b.bytes;
+        b.p = nullptr;
+        b.bytes = 0;
+        return;
+    }
+    buf_free(b);
 }
 
 int sync(Ctx *c)
@@ -349,6 +415,10 @@ extern "C" int gg_ctx_create(int device, gg_ctx_t *out)
         delete c;
         return rc;
     }
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        g_live_ctx.insert(c);
+    }
     *out = c;
     return GG_OK;
 }
@@ -358,6 +428,11 @@ extern "C" int gg_ctx_destroy(gg_ctx_t c)
     if (!c) return fail(GG_ERR_INVALID, "gg_ctx_destroy: NULL");
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        g_live_ctx.erase(c);
+    }
+    pool_drain(c);
     buf_free(c->counters);
     buf_free(c->tmp);
     buf_free(c->host_pin);

@@ -335,10 +335,10 @@ static int run_fixed_point(Graph *g, const CscView &view, int max_iter, int rela
     Ctx *c = g->ctx;
     const uint32_t nfix = view.fix_blocks();
     Buf l1p, l1f, flags, hist;
-    int rc = buf_alloc(c, l1p, 16ull * view.blocks());
-    if (!rc) rc = buf_alloc(c, l1f, 16ull * nfix);
-    if (!rc) rc = buf_alloc(c, flags, 64);
-    if (!rc) rc = buf_alloc(c, hist, 16ull * max_iter);
+    int rc = pool_take(c, l1p, 16ull * view.blocks());
+    if (!rc) rc = pool_take(c, l1f, 16ull * nfix);
+    if (!rc) rc = pool_take(c, flags, 64);
+    if (!rc) rc = pool_take(c, hist, 16ull * max_iter);
     int done_iter = max_iter;
     if (!rc) {
         fill_bytes(c, flags.p, 64, 0);
@@ -372,10 +372,10 @@ static int run_fixed_point(Graph *g, const CscView &view, int max_iter, int rela
             }
         }
     }
-    buf_free(l1p);
-    buf_free(l1f);
-    buf_free(flags);
-    buf_free(hist);
+    pool_give(c, l1p);
+    pool_give(c, l1f);
+    pool_give(c, flags);
+    pool_give(c, hist);
     return rc;
 }
 
@@ -550,8 +550,8 @@ static int wpagerank_impl(Graph *g, double alpha, double eps, int max_iter, cons
     SolveEvents E(max_iter);
     GG_HIP(hipEventRecord(E.ev[0], c->stream));
     Buf K[2];
-    int rc = buf_alloc(c, K[0], sizeof(V) * (uint64_t)n);
-    if (!rc) rc = buf_alloc(c, K[1], sizeof(V) * (uint64_t)n);
+    int rc = pool_take(c, K[0], sizeof(V) * (uint64_t)n);
+    if (!rc) rc = pool_take(c, K[1], sizeof(V) * (uint64_t)n);
     const V vconst = (V)(1 / (double)n);
     if (!rc) {
         k_wpr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(r, (V *)K[0].p, v, vconst,
@@ -567,8 +567,8 @@ static int wpagerank_impl(Graph *g, double alpha, double eps, int max_iter, cons
         }
         if (!rc) record_times(c, E, iters, info, g->m);
     }
-    buf_free(K[0]);
-    buf_free(K[1]);
+    pool_give(c, K[0]);
+    pool_give(c, K[1]);
     return rc;
 }
 
@@ -645,8 +645,8 @@ static int wpagerank_adjoint_impl(Graph *g, double alpha, double eps, int max_it
     SolveEvents E(max_iter);
     GG_HIP(hipEventRecord(E.ev[0], c->stream));
     Buf L[2];
-    int rc = buf_alloc(c, L[0], sizeof(V) * (uint64_t)n);
-    if (!rc) rc = buf_alloc(c, L[1], sizeof(V) * (uint64_t)n);
+    int rc = pool_take(c, L[0], sizeof(V) * (uint64_t)n);
+    if (!rc) rc = pool_take(c, L[1], sizeof(V) * (uint64_t)n);
     if (!rc) {
         k_copy<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)L[0].p, u, n);
         int iters = 0;
@@ -659,8 +659,8 @@ static int wpagerank_adjoint_impl(Graph *g, double alpha, double eps, int max_it
         }
         if (!rc) record_times(c, E, iters, info, g->m);
     }
-    buf_free(L[0]);
-    buf_free(L[1]);
+    pool_give(c, L[0]);
+    pool_give(c, L[1]);
     return rc;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -149,6 +150,14 @@ constexpr int kStripeStride = 16;
 constexpr int kCntStripeBase = 64;
 constexpr uint64_t kCounterBytes = 8ull * (kCntStripeBase + kStripes * kStripeStride);
 struct Ctx;
+// Pooled buffers of a context: take reuses a pooled buffer of at least
+// 'bytes' (and at most about twice that) or allocates one; give returns the
+// buffer to the pool (or frees it when the pool is full or the context is
+// gone).  ensure grows a buffer through the pool.
+int pool_take(Ctx *c, Buf &b, uint64_t bytes);
+int pool_ensure(Ctx *c, Buf &b, uint64_t bytes);
+void pool_give(Ctx *c, Buf &b);
+void pool_drain(Ctx *c);
 int counters_zero(Ctx *c);                           // totals and stripes
 int counters_read(Ctx *c, unsigned long long out[4]); // folds the stripes first
 
@@ -160,6 +169,14 @@ struct Ctx {
     Buf counters;   // uint64 counters: totals + stripes (kCounterBytes)
     Buf host_pin;   // unused placeholder for pinned staging
     Buf tmp;        // grow-only temporary (sort space etc.)
+    // Pool of device buffers released by per-call temporaries and frontiers
+    // (size -> pointers).  They are reused by later requests on this
+    // context instead of hipFree + hipMalloc: hipFree waits for the whole
+    // device and unmaps, hundreds of microseconds per edgeMap level.  A
+    // pooled buffer is only used on this context's stream, so stream order
+    // makes reuse safe.
+    std::multimap<uint64_t, void *> pool;
+    uint64_t pool_bytes = 0;
     double last_kernel_ms = 0;
     int last_kernel_launches = 0;
 };

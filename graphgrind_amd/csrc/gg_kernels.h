@@ -642,9 +642,9 @@ int reduce_sum_core(Ctx *c, const void *dev, int dtype, uint64_t count, double *
 inline void frontier_delete(Frontier *f)
 {
     if (!f) return;
-    buf_free(f->bytes);
-    buf_free(f->bitmap);
-    buf_free(f->list);
+    pool_give(f->ctx, f->bytes);
+    pool_give(f->ctx, f->bitmap);
+    pool_give(f->ctx, f->list);
     delete f;
 }
 

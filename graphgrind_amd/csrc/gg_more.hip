@@ -123,11 +123,11 @@ static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_ru
     GG_TRY(pull_view(g, &fwd));
     GG_TRY(csr_pull_view(g, &bwd));
     Buf sigma, level, dep, inv, gate;
-    int rc = buf_alloc(c, sigma, sizeof(V) * (uint64_t)n);
-    if (!rc) rc = buf_alloc(c, level, 4ull * n);
-    if (!rc) rc = buf_alloc(c, dep, sizeof(V) * (uint64_t)n);
-    if (!rc) rc = buf_alloc(c, inv, sizeof(V) * (uint64_t)n);
-    if (!rc) rc = buf_alloc(c, gate, sizeof(V) * (uint64_t)n);
+    int rc = pool_take(c, sigma, sizeof(V) * (uint64_t)n);
+    if (!rc) rc = pool_take(c, level, 4ull * n);
+    if (!rc) rc = pool_take(c, dep, sizeof(V) * (uint64_t)n);
+    if (!rc) rc = pool_take(c, inv, sizeof(V) * (uint64_t)n);
+    if (!rc) rc = pool_take(c, gate, sizeof(V) * (uint64_t)n);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -184,7 +184,7 @@ static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_ru
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     Buf *all[] = {&sigma, &level, &dep, &inv, &gate};
-    for (Buf *b : all) buf_free(*b);
+    for (Buf *b : all) pool_give(c, *b);
     return rc;
 }
 
@@ -270,9 +270,9 @@ extern "C" int gg_sssp(gg_graph_t g, uint32_t root, int32_t *dist_dev, gg_run_in
     CscView view;
     GG_TRY(pull_view(g, &view));
     Buf F[2], gate;
-    int rc = buf_alloc(c, F[0], (uint64_t)n + 64);
-    if (!rc) rc = buf_alloc(c, F[1], (uint64_t)n + 64);
-    if (!rc) rc = buf_alloc(c, gate, 4ull * n);
+    int rc = pool_take(c, F[0], (uint64_t)n + 64);
+    if (!rc) rc = pool_take(c, F[1], (uint64_t)n + 64);
+    if (!rc) rc = pool_take(c, gate, 4ull * n);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -313,8 +313,8 @@ extern "C" int gg_sssp(gg_graph_t g, uint32_t root, int32_t *dist_dev, gg_run_in
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    buf_free(F[0]);
-    buf_free(F[1]);
-    buf_free(gate);
+    pool_give(c, F[0]);
+    pool_give(c, F[1]);
+    pool_give(c, gate);
     return rc;
 }

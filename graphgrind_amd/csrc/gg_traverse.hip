@@ -657,12 +657,12 @@ int frontier_alloc(Graph *g, Frontier *f)
 
 static int ensure_bytes(Frontier *f)
 {
-    return buf_ensure(f->g->ctx, f->bytes, ((uint64_t)f->n + 64) & ~63ull);
+    return pool_ensure(f->ctx, f->bytes, ((uint64_t)f->n + 64) & ~63ull);
 }
 
 static int ensure_bitmap(Frontier *f)
 {
-    return buf_ensure(f->g->ctx, f->bitmap, 4ull * ((f->n + 31) / 32 + 1));
+    return pool_ensure(f->ctx, f->bitmap, 4ull * ((f->n + 31) / 32 + 1));
 }
 
 // Dense flags (+ bitmap) from whatever the frontier holds.
@@ -698,7 +698,7 @@ int frontier_make_sparse(Frontier *f)
     if (f->bit) {
         // bits() frontier: its flags are not set in the reference (toSparse
         // on it would read uninitialised memory); list every vertex
-        GG_TRY(buf_ensure(c, f->list, 4ull * f->n + 4));
+        GG_TRY(pool_ensure(c, f->list, 4ull * f->n + 4));
         GG_TRY(gg_iota(abi(c), f->list.p, GG_U32, f->n));
         f->sparse_valid = true;
         return GG_OK;
@@ -706,8 +706,8 @@ int frontier_make_sparse(Frontier *f)
     if (!f->dense_valid) return fail(GG_ERR_STATE, "frontier has no representation");
     uint32_t nb = blocks_for(f->n, kPackPer);
     Buf cnt, base;
-    GG_TRY(buf_alloc(c, cnt, 4ull * (nb + 1)));
-    GG_TRY(buf_alloc(c, base, 4ull * (nb + 1)));
+    GG_TRY(pool_take(c, cnt, 4ull * (nb + 1)));
+    GG_TRY(pool_take(c, base, 4ull * (nb + 1)));
     k_pack_count<<<nb, kBlock, 0, c->stream>>>((const uint8_t *)f->bytes.p, f->n, (uint32_t *)cnt.p);
     size_t tb = 0;
     GG_HIP(rocprim::exclusive_scan(nullptr, tb, (uint32_t *)cnt.p, (uint32_t *)base.p, 0u, (size_t)nb,
@@ -716,7 +716,7 @@ int frontier_make_sparse(Frontier *f)
     GG_HIP(rocprim::exclusive_scan(c->tmp.p, tb, (uint32_t *)cnt.p, (uint32_t *)base.p, 0u, (size_t)nb,
                                    rocprim::plus<uint32_t>(), c->stream));
     // capacity n whatever d_m says, so a stale count cannot overrun the list
-    GG_TRY(buf_ensure(c, f->list, 4ull * f->n + 4));
+    GG_TRY(pool_ensure(c, f->list, 4ull * f->n + 4));
     k_pack_write<<<nb, kBlock, 0, c->stream>>>((const uint8_t *)f->bytes.p, f->n,
                                                 (const uint32_t *)base.p, (uint32_t *)f->list.p);
     GG_HIP(hipGetLastError());
@@ -724,8 +724,8 @@ int frontier_make_sparse(Frontier *f)
     GG_HIP(hipMemcpyAsync(&last[0], (uint32_t *)base.p + (nb - 1), 4, hipMemcpyDeviceToHost, c->stream));
     GG_HIP(hipMemcpyAsync(&last[1], (uint32_t *)cnt.p + (nb - 1), 4, hipMemcpyDeviceToHost, c->stream));
     GG_TRY(sync(c));
-    buf_free(cnt);
-    buf_free(base);
+    pool_give(c, cnt);
+    pool_give(c, base);
     if ((uint64_t)last[0] + last[1] != f->d_m) // toSparse's "bad stored value of m" (ligra-numa.h:251-255)
         return fail(GG_ERR_STATE, "toSparse: stored count " + std::to_string(f->d_m) + " but " +
                                       std::to_string((uint64_t)last[0] + last[1]) + " flags set");
@@ -783,9 +783,9 @@ int frontier_stats_dense(Frontier *f)
 
 static void frontier_free(Frontier *f)
 {
-    buf_free(f->bytes);
-    buf_free(f->bitmap);
-    buf_free(f->list);
+    pool_give(f->ctx, f->bytes);
+    pool_give(f->ctx, f->bitmap);
+    pool_give(f->ctx, f->list);
 }
 
 // ============================================================================
@@ -837,7 +837,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         if ((rc = pull_view(g, &pv))) return bail(rc);
         switch (op->kind) {
         case GG_OP_PAGERANK: {
-            if ((rc = buf_alloc(c, tmpv, (f32 ? 4ull : 8ull) * n))) return bail(rc);
+            if ((rc = pool_take(c, tmpv, (f32 ? 4ull : 8ull) * n))) return bail(rc);
             if (f32) {
                 k_pr_contrib<float><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
                     (const float *)op->a, outdeg, F, bit, (float *)tmpv.p, n);
@@ -880,7 +880,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             // owned pull reads 4 bytes per edge where the COO lists read 8
             // and needs no atomics, so both branches run it.
             {
-                if ((rc = buf_alloc(c, tmpv, 4ull * n))) return bail(rc);
+                if ((rc = pool_take(c, tmpv, 4ull * n))) return bail(rc);
                 k_cc_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)op->b, F, bit,
                                                                             (uint32_t *)tmpv.p, n);
                 rc = launch_pull(c, pv, OpCCMin{(const uint32_t *)tmpv.p, (uint32_t *)op->a,
@@ -909,7 +909,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             // elsewhere): the sources are visited and the updated vertices
             // are not, so the copy changes no value but keeps the pull free
             // of aliasing between gathered and written entries
-            if ((rc = buf_alloc(c, tmpv, (f32 ? 4ull : 8ull) * n))) return bail(rc);
+            if ((rc = pool_take(c, tmpv, (f32 ? 4ull : 8ull) * n))) return bail(rc);
             if (f32) {
                 k_pathsum_gate<float><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const float *)op->a, F, bit,
                                                                                        (float *)tmpv.p, n);
@@ -926,7 +926,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             break;
         }
         case GG_OP_SSSP: {
-            if ((rc = buf_alloc(c, tmpv, 4ull * n))) return bail(rc);
+            if ((rc = pool_take(c, tmpv, 4ull * n))) return bail(rc);
             k_sp_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const int32_t *)op->a, F, bit,
                                                                         (int32_t *)tmpv.p, n);
             const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
@@ -937,7 +937,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         }
         default: return bail(fail(GG_ERR_INVALID, "edgeMap: unknown operator kind"));
         }
-        if (rc) { buf_free(tmpv); return bail(rc); }
+        if (rc) { pool_give(c, tmpv); return bail(rc); }
         GG_HIP(hipGetLastError());
         out->has_dense = true;
         out->dense_valid = true;
@@ -948,7 +948,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             out->out_edges = cn[CNT_OUT];
             if (ex) ex->scanned += op->kind == GG_OP_BFS ? cn[CNT_SCANNED] : m;
         }
-        buf_free(tmpv);
+        pool_give(c, tmpv);
         if ((rc = ensure_bitmap(out))) return bail(rc);
         if ((rc = bitmap_from_bytes(c, (uint8_t *)out->bytes.p, (uint32_t *)out->bitmap.p, n))) return bail(rc);
         *pout = out;
@@ -959,7 +959,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
     if ((rc = frontier_make_sparse(in))) return bail(rc);
     const uint32_t k = (uint32_t)in->d_m;
     Buf offs;
-    if ((rc = buf_alloc(c, offs, 4ull * (k + 1)))) return bail(rc);
+    if ((rc = pool_take(c, offs, 4ull * (k + 1)))) return bail(rc);
     {
         struct DegOf {
             const uint32_t *list, *outdeg;
@@ -972,16 +972,16 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         size_t tb = 0;
         GG_HIP(rocprim::exclusive_scan(nullptr, tb, it, (uint32_t *)offs.p, 0u, (size_t)k + 1,
                                        rocprim::plus<uint32_t>(), c->stream));
-        if ((rc = buf_ensure(c, c->tmp, tb + 16))) { buf_free(offs); return bail(rc); }
+        if ((rc = buf_ensure(c, c->tmp, tb + 16))) { pool_give(c, offs); return bail(rc); }
         GG_HIP(rocprim::exclusive_scan(c->tmp.p, tb, it, (uint32_t *)offs.p, 0u, (size_t)k + 1,
                                        rocprim::plus<uint32_t>(), c->stream));
     }
     uint32_t total32 = 0;
     GG_HIP(hipMemcpyAsync(&total32, (uint32_t *)offs.p + k, 4, hipMemcpyDeviceToHost, c->stream));
-    if ((rc = sync(c))) { buf_free(offs); return bail(rc); }
+    if ((rc = sync(c))) { pool_give(c, offs); return bail(rc); }
     const uint64_t total = total32;
-    if ((rc = buf_ensure(c, out->list, 4ull * (total ? total : 1)))) { buf_free(offs); return bail(rc); }
-    if ((rc = counters_zero(c))) { buf_free(offs); return bail(rc); }
+    if ((rc = pool_ensure(c, out->list, 4ull * (total ? total : 1)))) { pool_give(c, offs); return bail(rc); }
+    if ((rc = counters_zero(c))) { pool_give(c, offs); return bail(rc); }
     if (total) {
         PushArgs A;
         A.list = (const uint32_t *)in->list.p;
@@ -1010,14 +1010,14 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
     }
     if (ex) ex->scanned += total;
     unsigned long long cn[4];
-    if ((rc = counters_read(c, cn))) { buf_free(offs); return bail(rc); }
-    buf_free(offs);
+    if ((rc = counters_read(c, cn))) { pool_give(c, offs); return bail(rc); }
+    pool_give(c, offs);
     out->d_m = cn[CNT_CURSOR];
     out->out_edges = cn[CNT_OUT];
     if (remdups && out->d_m) {
         Buf flags, list2;
-        if ((rc = buf_alloc(c, flags, 4ull * n))) return bail(rc);
-        if ((rc = buf_alloc(c, list2, 4ull * out->d_m))) { buf_free(flags); return bail(rc); }
+        if ((rc = pool_take(c, flags, 4ull * n))) return bail(rc);
+        if ((rc = pool_take(c, list2, 4ull * out->d_m))) { pool_give(c, flags); return bail(rc); }
         fill_u32(c, flags.p, n, 0xffffffffu);
         counters_zero(c);
         k_dedup_mark<<<blocks_for(out->d_m, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)out->list.p,
@@ -1026,10 +1026,10 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             (const uint32_t *)out->list.p, out->d_m, (uint32_t *)flags.p, (const uint32_t *)g->outdeg.p,
             (uint32_t *)list2.p, (unsigned long long *)c->counters.p);
         GG_HIP(hipGetLastError());
-        if ((rc = counters_read(c, cn))) { buf_free(flags); buf_free(list2); return bail(rc); }
+        if ((rc = counters_read(c, cn))) { pool_give(c, flags); pool_give(c, list2); return bail(rc); }
         std::swap(out->list, list2);
-        buf_free(flags);
-        buf_free(list2);
+        pool_give(c, flags);
+        pool_give(c, list2);
         out->d_m = cn[CNT_CURSOR];
         out->out_edges = cn[CNT_OUT];
     }
@@ -1078,7 +1078,7 @@ int reduce_sum_core(Ctx *c, const void *dev, int dtype, uint64_t count, double *
 {
     const uint32_t np = 1024;
     Buf part;
-    GG_TRY(buf_alloc(c, part, 8ull * np));
+    GG_TRY(pool_take(c, part, 8ull * np));
     if (dtype == GG_F32)
         k_sum_part<float><<<np, kBlock, 0, c->stream>>>((const float *)dev, count, (double *)part.p);
     else
@@ -1086,7 +1086,7 @@ int reduce_sum_core(Ctx *c, const void *dev, int dtype, uint64_t count, double *
     k_sum_final<<<1, kBlock, 0, c->stream>>>((const double *)part.p, np, dev_out);
     GG_HIP(hipGetLastError());
     GG_TRY(sync(c));
-    buf_free(part);
+    pool_give(c, part);
     return GG_OK;
 }
 
@@ -1111,7 +1111,7 @@ extern "C" int gg_frontier_create(gg_graph_t g, int kind, uint32_t v, gg_frontie
         break;
     case GG_FRONTIER_SINGLE: {
         if (v >= g->n) { delete f; return fail(GG_ERR_INVALID, "gg_frontier_create: vertex out of range"); }
-        rc = buf_alloc(c, f->list, 8);
+        rc = pool_take(c, f->list, 8);
         if (!rc) rc = gg_memcpy_h2d(abi(c), f->list.p, &v, 4);
         uint32_t od = 0;
         if (!rc) rc = gg_memcpy_d2h(abi(c), &od, (uint32_t *)g->outdeg.p + v, 4);
@@ -1152,7 +1152,7 @@ extern "C" int gg_frontier_from_host(gg_graph_t g, const uint8_t *dense, const u
     } else {
         for (uint64_t i = 0; i < count; i++)
             if (sparse[i] >= g->n) { delete f; return fail(GG_ERR_INVALID, "gg_frontier_from_host: id out of range"); }
-        rc = buf_alloc(c, f->list, 4 * (count ? count : 1));
+        rc = pool_take(c, f->list, 4 * (count ? count : 1));
         if (!rc && count) rc = gg_memcpy_h2d(abi(c), f->list.p, sparse, 4 * count);
         f->d_m = count;
         f->sparse_valid = true;
@@ -1172,7 +1172,8 @@ extern "C" int gg_frontier_from_host(gg_graph_t g, const uint8_t *dense, const u
 extern "C" int gg_frontier_destroy(gg_frontier_t f)
 {
     if (!f) return fail(GG_ERR_INVALID, "gg_frontier_destroy: NULL");
-    (void)hipStreamSynchronize(f->ctx->stream);
+    // the buffers go back to the context's pool (or are freed if the
+    // context is gone); work queued on them finishes first by stream order
     frontier_free(f);
     delete f;
     return GG_OK;
@@ -1318,9 +1319,9 @@ extern "C" int gg_reduce_sum(gg_ctx_t c, const void *dev, int dtype, uint64_t co
     if (!c || !out || (count && !dev)) return fail(GG_ERR_INVALID, "gg_reduce_sum: NULL argument");
     if (dtype != GG_F64 && dtype != GG_F32) return fail(GG_ERR_INVALID, "gg_reduce_sum: F64 or F32 only");
     Buf r;
-    GG_TRY(buf_alloc(c, r, 8));
+    GG_TRY(pool_take(c, r, 8));
     int rc = reduce_sum_core(c, dev, dtype, count, (double *)r.p);
     if (!rc) rc = gg_memcpy_d2h(abi(c), out, r.p, 8);
-    buf_free(r);
+    pool_give(c, r);
     return rc;
 }
