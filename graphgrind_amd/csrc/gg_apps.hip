@@ -427,6 +427,7 @@ extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
     F->d_m = g->n;
     F->out_edges = g->m;
     EdgeMapExtra ex;
+    ex.cc_live = true;
     int rounds = 0, rc = GG_OK;
     while (F->d_m != 0) {
         gg_vop_t cp = {GG_VOP_CC_COPY, GG_U32, labels, prev.p, 0, 0};

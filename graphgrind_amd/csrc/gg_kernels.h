@@ -632,6 +632,9 @@ struct EdgeMapExtra {
     int32_t *levels = nullptr;
     int32_t level = 0;
     unsigned long long scanned = 0;
+    // CC dense rounds read the live labels instead of the round-start
+    // snapshot (Gauss-Seidel within a round): same fixed point, fewer rounds
+    bool cc_live = false;
 };
 
 int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, int remdups,

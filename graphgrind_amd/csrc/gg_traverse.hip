@@ -879,7 +879,16 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             // minimum prevID over active in-neighbours); the destination-
             // owned pull reads 4 bytes per edge where the COO lists read 8
             // and needs no atomics, so both branches run it.
-            {
+            if (ex && ex->cc_live) {
+                // fused Components loop: every in-neighbour's current label
+                // (a vertex outside the frontier holds a label its
+                // neighbours already took, so reading it changes nothing);
+                // labels only decrease, so a label read while its owner
+                // rewrites it is an old or a new upper bound, both valid
+                rc = launch_pull(c, pv, OpCCMin{(const uint32_t *)op->a, (uint32_t *)op->a,
+                                               (const uint32_t *)op->b, next, outdeg},
+                                 cnt, nullptr, nullptr);
+            } else {
                 if ((rc = pool_take(c, tmpv, 4ull * n))) return bail(rc);
                 k_cc_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)op->b, F, bit,
                                                                             (uint32_t *)tmpv.p, n);
