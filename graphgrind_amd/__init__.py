@@ -111,6 +111,8 @@ SIGNATURES = {
     "gg_graph_partition": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int]),
     "gg_graph_reorder": (C.c_int, [_VP, C.c_int]),
     "gg_graph_transpose": (C.c_int, [_VP]),
+    "gg_vebo_order": (C.c_int, [_VP, C.c_int, _VP, _VP]),
+    "gg_graph_relabel": (C.c_int, [_VP, _VP, C.POINTER(C.c_void_p)]),
     "gg_graph_device_view": (C.c_int, [_VP, C.c_int, C.POINTER(GraphView)]),
     "gg_graph_partition_starts": (C.c_int, [_VP, C.c_int, _VP, _IP]),
     "gg_graph_coo": (C.c_int, [_VP, _VP, _VP, _VP]),
@@ -324,6 +326,23 @@ class Graph:
         the reversed edges; a second call restores the graph."""
         check(lib().gg_graph_transpose(self.h))
         return self
+
+    def vebo_order(self, P):
+        """graphtools/VEBO.C order for P partitions: (new id of each vertex,
+        first new id of each partition), numpy arrays."""
+        new = np.empty(self.n, np.uint32)
+        starts = np.empty(P + 1, np.uint64)
+        check(lib().gg_vebo_order(self.h, P, new.ctypes.data, starts.ctypes.data))
+        return new, starts
+
+    def relabel(self, new_of_old):
+        """A new graph with vertex v renumbered new_of_old[v]."""
+        perm = np.ascontiguousarray(new_of_old, np.uint32)
+        if len(perm) != self.n:
+            raise ValueError("relabel: one new id per vertex")
+        h = C.c_void_p()
+        check(lib().gg_graph_relabel(self.h, perm.ctypes.data, C.byref(h)))
+        return Graph(self.ctx, h)
 
     def reorder(self, mode=1):
         """Internal hot-sources-first renumbering used by the pull kernels."""

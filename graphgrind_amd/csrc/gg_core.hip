@@ -737,6 +737,29 @@ static void graph_free(Graph *g)
 
 } // namespace gg
 
+namespace gg {
+int graph_from_device(Ctx *c, uint32_t n, uint64_t m, Buf &off, Buf &ids, Buf &w, bool symmetric, Graph **out)
+{
+    gg_graph_s *g = new gg_graph_s();
+    g->ctx = c;
+    g->n = n;
+    g->m = m;
+    g->symmetric = symmetric;
+    g->weighted = w.p != nullptr;
+    std::swap(g->off, off);
+    std::swap(g->ids, ids);
+    std::swap(g->w, w);
+    int rc = build_twin(g);
+    if (rc) {
+        graph_free(g);
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return GG_OK;
+}
+} // namespace gg
+
 extern "C" int gg_graph_create_csr(gg_ctx_t c, uint32_t n, uint64_t m, const uint32_t *off,
                                    const uint32_t *ids, const float *w, int symmetric,
                                    gg_graph_t *out)

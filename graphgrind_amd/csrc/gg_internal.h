@@ -249,6 +249,9 @@ struct Frontier {
 
 // ---- shared device helpers (defined in gg_core.hip) ------------------------
 int sched_build(Ctx *c, PullSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges);
+// A graph from a CSR already on the device (takes the buffers; w may be
+// empty): builds degrees, the in-edge twin and the pull schedule.
+int graph_from_device(Ctx *c, uint32_t n, uint64_t m, Buf &off, Buf &ids, Buf &w, bool symmetric, Graph **out);
 int fill_bytes(Ctx *c, void *p, uint64_t bytes, uint8_t v);
 int fill_u32(Ctx *c, void *p, uint64_t count, uint32_t v);
 int bitmap_from_bytes(Ctx *c, const uint8_t *bytes, uint32_t *bitmap, uint32_t n);

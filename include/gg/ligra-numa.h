@@ -703,6 +703,8 @@ struct timer {
         t0 = t1;
         return d;
     }
+    // seconds since start(), added to the total (gettime.h's stop())
+    double stop() { return next(); }
 };
 static timer _tm;
 #define startTime() _tm.start();

@@ -116,6 +116,22 @@ int gg_graph_create_arena(gg_ctx_t ctx, int count, const gg_graph_t *parts, gg_g
  * options) and a relabelled plan (gg_graph_reorder) is dropped. */
 int gg_graph_transpose(gg_graph_t g);
 
+/* VEBO order (graphtools/VEBO.C:56-321, the reference's offline reorder
+ * tool): the new id of every vertex (new_of_old, n entries, host) for P
+ * partitions from the graph's in-degrees, and the first new id of each
+ * partition (starts, P+1 entries, host, may be NULL).  Each partition gets
+ * about m/P in-edges and n/P vertices, its vertices in descending
+ * in-degree.  Where the reference's zero-degree placement would not place
+ * exactly the zero-degree vertices (its counts then overrun its arrays),
+ * they go one by one to the partition with the fewest vertices. */
+int gg_vebo_order(gg_graph_t g, int P, uint32_t *new_of_old, uint64_t *starts);
+
+/* The graph renumbered by new_of_old (a permutation of 0..n-1, host): new
+ * vertex new_of_old[v] gets v's out-edges, each list in its old order with
+ * its heads renumbered, weights carried (graphtools/VEBO.C:333-356).  *out
+ * is a new, unpartitioned graph on the same context. */
+int gg_graph_relabel(gg_graph_t g, const uint32_t *new_of_old, gg_graph_t *out);
+
 /* Internal vertex renumbering for the pull kernels (no reference
  * counterpart in the engine; the reference reorders offline with VEBO,
  * graphtools/VEBO.C, and '-o').  mode 1: vertices by out-degree descending,

@@ -98,6 +98,7 @@ def lib():
         L.ggo_sssp.argtypes = [C.c_uint32, u32, u32, f32, C.c_uint32, i64]
         L.ggo_prdelta.argtypes = [C.c_uint32, u32, u32, C.c_double, C.c_double, C.c_double, C.c_int, f64]
         L.ggo_prconv.argtypes = [C.c_uint32, u32, u32, C.c_double, C.c_double, C.c_int, f64]
+        L.ggo_vebo.argtypes = [C.c_uint32, u32, C.c_int, u32, P(np.uint64, flags="C_CONTIGUOUS")]
         _lib = L
     return _lib
 
@@ -358,6 +359,17 @@ def prconv(off, ids, damping=0.85, eps=1e-7, max_iter=100):
                          damping, eps, max_iter, p)
     assert r >= 0
     return p, r
+
+
+def vebo(indeg, P):
+    """graphtools/VEBO.C: (new id of each vertex, partition starts) for P
+    partitions."""
+    n = len(indeg)
+    out = np.empty(n, np.uint32)
+    starts = np.empty(P + 1, np.uint64)
+    r = lib().ggo_vebo(n, np.ascontiguousarray(indeg, np.uint32), P, out, starts)
+    assert r == 0, r
+    return out, starts
 
 
 def ref_available(app: str = "pagerank") -> bool:

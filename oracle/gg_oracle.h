@@ -177,6 +177,15 @@ int ggo_prdelta(uint32_t n, const uint32_t *off, const uint32_t *ids, double dam
  * p = ranks.  Returns the iteration count. */
 int ggo_prconv(uint32_t n, const uint32_t *off, const uint32_t *ids, double damping, double eps, int max_iter,
                double *p);
+/* graphtools/VEBO.C:84-321: the VEBO vertex order for P partitions from the
+ * in-degrees.  new_of_old[v] = new id of vertex v; starts (P+1 entries, may
+ * be NULL) = first new id of each partition.  Returns 0, or -1 on bad
+ * arguments.  When the reference's zero-degree placement would not place
+ * exactly the zero-degree vertices (its counts then overrun its arrays),
+ * the zero-degree vertices go one by one to the partition with the fewest
+ * vertices (the reference's own rule for fewer such vertices than
+ * partitions). */
+int ggo_vebo(uint32_t n, const uint32_t *indeg, int P, uint32_t *new_of_old, uint64_t *starts);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,12 @@
  *            Delta (PR_Vertex_F_FirstRound in round 1, PR_Vertex_F after,
  *            74-110) and stays active while |Delta| > eps2 * p; stop when
  *            sum |Delta| < eps.
+ *   ggo_vebo graphtools/VEBO.C:84-321: vertices sorted by in-degree
+ *            (descending, ties by id); degree by degree, vertices go to the
+ *            partition with the fewest edges (the lowest index on ties), a
+ *            run of (left)/P at once while the edge spread exceeds the
+ *            degree; zero-degree vertices then even out the vertex counts;
+ *            each partition's ids are given degree group by degree group.
  *   ggo_prconv app/PageRankConverage.C:227-310: add_factor = 0.85 p/outdeg
  *            pushed along out-edges (PR_F, 34-88), then p_next += (1 -
  *            sum p_next)/n (PR_Vertex_Csum), L1 = sum |p_curr - p_next|,
@@ -31,7 +37,11 @@
  * PageRankDelta with eps2 = 0 against PageRank power iteration (the two are
  * the same recurrence then) and with a huge eps2 against one power step;
  * PageRankConverage's first iteration in closed form and its converged
- * ranks against a direct solve of the dangling-corrected PageRank system.
+ * ranks against a direct solve of the dangling-corrected PageRank system;
+ * VEBO against a hand-worked order and its invariants (a permutation;
+ * in-degrees non-increasing inside each partition; partition edge counts
+ * within the largest degree of each other; vertex counts n/P or n/P + 1
+ * when there are enough zero-degree vertices).
  */
 #include <math.h>
 #include <stdint.h>
@@ -196,6 +206,127 @@ int ggo_prconv(uint32_t n, const uint32_t *off, const uint32_t *ids, double damp
     memcpy(p, cur, sizeof(double) * n);
     free(cur), free(nxt), free(add);
     return count;
+}
+
+typedef struct {
+    uint32_t v, d;
+} vebo_pair;
+
+static int vebo_cmp(const void *a, const void *b)
+{
+    const vebo_pair *x = (const vebo_pair *)a, *y = (const vebo_pair *)b;
+    if (x->d != y->d) return x->d > y->d ? -1 : 1; /* in-degree, high to low */
+    return x->v < y->v ? -1 : x->v > y->v;         /* then by id */
+}
+
+int ggo_vebo(uint32_t n, const uint32_t *indeg, int P, uint32_t *new_of_old, uint64_t *starts)
+{
+    if (!n || P < 1 || (uint32_t)P > n) return -1;
+    vebo_pair *chk = malloc(sizeof(vebo_pair) * n);
+    for (uint32_t v = 0; v < n; v++) chk[v] = (vebo_pair){v, indeg[v]};
+    qsort(chk, n, sizeof(vebo_pair), vebo_cmp);
+    /* degree groups, high to low (VEBO.C:95-144); the zero group counts only
+     * its first vertex, and only when it is the first group (no edges) */
+    uint32_t c = 1, czero = 0;
+    for (uint32_t i = 1; i < n; i++) {
+        if (chk[i].d != chk[i - 1].d) c++;
+        if (chk[i].d == 0) czero++;
+    }
+    uint32_t *gdeg = calloc(c, sizeof(uint32_t)), *gcnt = calloc(c, sizeof(uint32_t));
+    int64_t zgroup = -1;
+    {
+        uint32_t a = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            if (i && chk[i].d != chk[i - 1].d) a++;
+            if (chk[i].d == 0 && zgroup < 0) zgroup = a;
+            if (i == 0 || chk[i].d != 0) {
+                gcnt[a]++;
+                gdeg[a] = chk[i].d;
+            }
+        }
+    }
+    const uint32_t zg = zgroup < 0 ? 0u : (uint32_t)zgroup; /* degreeN[0] */
+    int64_t *edges = calloc((size_t)P, sizeof(int64_t));
+    int64_t *vsize = calloc((size_t)P, sizeof(int64_t));
+    int64_t *per = calloc((size_t)P * c, sizeof(int64_t)); /* perDeg[p][group] */
+    /* phase 1 (VEBO.C:179-216) */
+    for (uint32_t i = 0; i < c; i++) {
+        const int64_t deg = gdeg[i], same = gcnt[i];
+        for (int64_t k = 0; k < same;) {
+            int64_t mx = edges[0], mn = edges[0];
+            int mp = 0;
+            for (int j = 1; j < P; j++) {
+                if (edges[j] < mn) {
+                    mn = edges[j];
+                    mp = j;
+                }
+                if (edges[j] > mx) mx = edges[j];
+            }
+            int64_t take = 1;
+            if (mx - mn > deg) {
+                const int64_t remain = (same - k) / P;
+                if (remain > 1) take = remain;
+            }
+            edges[mp] += deg * take;
+            per[(size_t)mp * c + i] += take;
+            vsize[mp] += take;
+            k += take;
+        }
+    }
+    /* phase 2 (VEBO.C:235-284): zero-degree vertices */
+    const int64_t avg = n / (uint32_t)P, extra = n % (uint32_t)P, dzero = czero / (uint32_t)P;
+    int64_t *add = calloc((size_t)P, sizeof(int64_t));
+    int64_t placed = 0;
+    if (extra == 0 || dzero != 0) {
+        for (int j = 0; j < P; j++)
+            if (vsize[j] < avg) add[j] = avg - vsize[j];
+        if (extra != 0)
+            for (int64_t j = 0; j < extra; j++) add[j]++;
+        for (int j = 0; j < P; j++) placed += add[j];
+    }
+    if ((extra == 0 || dzero != 0) && placed == (int64_t)czero) {
+        for (int j = 0; j < P; j++) {
+            vsize[j] += add[j];
+            per[(size_t)j * c + zg] += add[j];
+        }
+    } else {
+        for (uint32_t z = 0; z < czero; z++) {
+            int64_t mnv = vsize[0];
+            int mp = 0;
+            for (int j = 1; j < P; j++)
+                if (vsize[j] < mnv) {
+                    mnv = vsize[j];
+                    mp = j;
+                }
+            vsize[mp]++;
+            per[(size_t)mp * c + zg]++;
+        }
+    }
+    /* ids: partition starts, then group by group, partition by partition,
+     * the next vertices of the sorted order (VEBO.C:289-321) */
+    int64_t *start = calloc((size_t)P, sizeof(int64_t));
+    for (int j = 1; j < P; j++) start[j] = start[j - 1] + vsize[j - 1];
+    int64_t *fill = calloc((size_t)P, sizeof(int64_t));
+    uint64_t real = 0;
+    int rc = 0;
+    for (uint32_t i = 0; i < c && !rc; i++)
+        for (int j = 0; j < P && !rc; j++)
+            for (int64_t k = 0; k < per[(size_t)j * c + i]; k++) {
+                if (real >= n) {
+                    rc = -2;
+                    break;
+                }
+                new_of_old[chk[real].v] = (uint32_t)(start[j] + fill[j]);
+                fill[j]++;
+                real++;
+            }
+    if (!rc && real != n) rc = -2;
+    if (!rc && starts) {
+        for (int j = 0; j < P; j++) starts[j] = (uint64_t)start[j];
+        starts[P] = n;
+    }
+    free(chk), free(gdeg), free(gcnt), free(edges), free(vsize), free(per), free(add), free(start), free(fill);
+    return rc;
 }
 
 int ggo_prdelta(uint32_t n, const uint32_t *off, const uint32_t *ids, double damping, double eps, double eps2,

@@ -1,7 +1,8 @@
 """Pins for the oracle's further apps (oracle/gg_oracle_apps.c): BC.C
 against textbook Brandes accumulation, BellmanFord.C against Dijkstra,
 PageRankDelta.C against PageRank power iteration and a one-step closed form,
-PageRankConverage.C against its first step in closed form and a direct solve."""
+PageRankConverage.C against its first step in closed form and a direct solve,
+graphtools/VEBO.C against a hand-worked order and its invariants."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -177,3 +178,40 @@ def test_prconv_converges_to_dangling_pagerank(scale, seed):
     assert it < 100
     assert abs(got.sum() - 1) < 1e-12
     assert np.abs(got - want).sum() < 1e-5
+
+
+def test_vebo_by_hand():
+    """n = 6, P = 2, in-degrees [3, 0, 1, 2, 0, 1] (worked through
+    graphtools/VEBO.C by hand): sorted order 0, 3, 2, 5, 1, 4; phase 1 gives
+    partition 0 the vertices of degree 3 and 1 (v0, v2), partition 1 those
+    of degree 2 and 1 (v3, v5); the two zero-degree vertices fill each
+    partition to n/P = 3; ids go group by group: p0 = [v0, v2, v1],
+    p1 = [v3, v5, v4]."""
+    new, starts = oracle.vebo(np.array([3, 0, 1, 2, 0, 1]), 2)
+    assert list(new) == [0, 2, 1, 3, 5, 4]
+    assert list(starts) == [0, 3, 6]
+
+
+@pytest.mark.parametrize("scale,sym,P", [(10, False, 16), (12, True, 64), (14, False, 384)])
+def test_vebo_invariants(scale, sym, P):
+    off, ids = oracle.rmat_csr(scale, 16, 3, symmetric=sym)
+    n = len(off) - 1
+    indeg = np.bincount(ids, minlength=n).astype(np.int64)
+    new, starts = oracle.vebo(indeg, P)
+    assert np.array_equal(np.sort(new), np.arange(n))  # a permutation
+    old_of_new = np.empty(n, np.int64)
+    old_of_new[new] = np.arange(n)
+    d_new = indeg[old_of_new]
+    st = starts.astype(np.int64)
+    assert st[0] == 0 and st[-1] == n and np.all(np.diff(st) >= 0)
+    e = np.array([d_new[st[j]:st[j + 1]].sum() for j in range(P)])
+    assert e.max() - e.min() <= indeg.max()  # edge-balanced to within one vertex
+    for j in range(P):
+        seg = d_new[st[j]:st[j + 1]]
+        # degree groups high to low, ties by old id: (-degree, id) increases
+        key = -seg * (n + 1) + old_of_new[st[j]:st[j + 1]]
+        assert np.all(np.diff(key) > 0)
+    zeros = int((indeg == 0).sum())
+    if zeros >= P:
+        sz = np.diff(st)
+        assert sz.max() - sz.min() <= 1 or sz.max() <= n // P + 1
