@@ -403,6 +403,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration under rocprofv3)")
+    ap.add_argument("--only-bfs-cc", action="store_true",
+                    help="only run the BFS + Components leg (for kernel traces of it)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every run of this build uses a single MI355X; "
@@ -413,6 +415,10 @@ def main():
     ctx = gg.Context(0)
     if a.calibrate_copy:
         copy_calibration(gg, ctx)
+        ctx.close()
+        return
+    if a.only_bfs_cc:
+        log(f"[bench] bfs/cc: {bfs_cc_bench(gg, ctx, a.scale, a.steps)}")
         ctx.close()
         return
     pr = pagerank_bench(gg, ctx, a.scale, a.steps, a.warmup, np.float64)

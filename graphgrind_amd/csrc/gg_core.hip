@@ -564,6 +564,20 @@ __global__ void k_rmat_keys(int scale, uint64_t nraw, uint64_t seed, int symmetr
     }
 }
 
+// Counts per high part of sorted keys without same-address atomics: the
+// last element of a run adds its end position, the first subtracts its
+// start, so cnt[h] ends as the run length (two atomics per distinct h; one
+// atomicAdd per element serialised on the hubs' counters).
+__device__ __forceinline__ void run_count(const uint64_t *__restrict__ keys, uint64_t m, int shift, uint64_t i,
+                                          uint64_t h, uint32_t *__restrict__ cnt)
+{
+    const bool first = i == 0 || (keys[i - 1] >> shift) != h;
+    const bool last = i + 1 == m || (keys[i + 1] >> shift) != h;
+    if (first && last) cnt[h] = 1u;
+    else if (last) atomicAdd(&cnt[h], (uint32_t)(i + 1));
+    else if (first) atomicSub(&cnt[h], (uint32_t)i);
+}
+
 __global__ void k_decode_keys(const uint64_t *__restrict__ keys, uint64_t m, int shift,
                               uint32_t *__restrict__ ids, uint32_t *__restrict__ deg)
 {
@@ -571,9 +585,9 @@ __global__ void k_decode_keys(const uint64_t *__restrict__ keys, uint64_t m, int
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t mask = (1ULL << shift) - 1;
     for (; i < m; i += stride) {
-        uint64_t k = keys[i];
+        const uint64_t k = keys[i];
         ids[i] = (uint32_t)(k & mask);
-        atomicAdd(&deg[k >> shift], 1u);
+        run_count(keys, m, shift, i, k >> shift, deg);
     }
 }
 
@@ -610,9 +624,9 @@ __global__ void k_csc_decode(const uint64_t *__restrict__ keys, uint64_t m, int 
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t mask = (1ULL << bits) - 1;
     for (; i < m; i += stride) {
-        uint64_t k = keys[i];
+        const uint64_t k = keys[i];
         tids[i] = (uint32_t)(k & mask);
-        atomicAdd(&indeg[k >> bits], 1u);
+        run_count(keys, m, bits, i, k >> bits, indeg);
     }
 }
 
