@@ -105,6 +105,14 @@ __device__ __forceinline__ bool bit_test(const uint32_t *bm, uint32_t v)
 // the staging phase together with the gathers (so the epilogue waits on no
 // memory); other operators are finished through finish(d, acc, stats).
 struct NoRow {};
+// Optional row_done(d): true when row d cannot change whatever its
+// in-neighbours hold (a CC label already 0).  The row-block pull then skips
+// the gathers of a work item whose rows are all done.
+template <class Op, class = void>
+struct has_row_done : std::false_type {};
+template <class Op>
+struct has_row_done<Op, std::void_t<decltype(std::declval<const Op &>().row_done(0u))>> : std::true_type {};
+
 template <class Op, class = void>
 struct has_row : std::false_type {};
 template <class Op>
@@ -405,7 +413,20 @@ __global__ __launch_bounds__(kBlock) void k_rows(Op op, const uint32_t *__restri
     if (op.skip()) return;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const RowWork w = work[blockIdx.x];
-    const uint32_t ne = w.e1 - w.e0;
+    uint32_t ne = w.e1 - w.e0;
+    if constexpr (has_row_done<Op>::value) {
+        // every row of the item done: no gathers; rows finish with the
+        // identity (a long row's chunk leaves the identity in its slot)
+        int done;
+        if (w.kind == kRowLong) {
+            done = op.row_done(w.r0);
+        } else {
+            bool d = true;
+            for (uint32_t i = tid; i < w.r1 - w.r0; i += kBlock) d = d && op.row_done(w.r0 + i);
+            done = __syncthreads_and(d);
+        }
+        if (done) ne = 0;
+    }
     const uint32_t nel = ne ? ne - 1 : 0u;
     // gathered values of this block's edges (clamped, unconditional loads:
     // a load under a divergent branch is waited for at the branch join; the
@@ -445,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void k_rows(Op op, const uint32_t *__restri
 #pragma unroll
     for (int k = 0; k < kRPT; k++) {
         const uint32_t i = tid + k * kBlock;
-        if (i < nr) s_rs[i] = __builtin_nontemporal_load(&toff[w.r0 + i]) - w.e0;
+        if (i < nr) s_rs[i] = ne ? __builtin_nontemporal_load(&toff[w.r0 + i]) - w.e0 : 0u;
     }
     if (tid == 0) s_rs[nr] = ne;
 #pragma unroll

@@ -157,6 +157,8 @@ struct OpCCMin {
     __device__ uint32_t identity() const { return 0xffffffffu; }
     __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a < b ? a : b; }
     __device__ uint32_t gather(uint32_t, uint32_t s) const { return g[s]; }
+    // label 0 is the least there is: such a row cannot change
+    __device__ bool row_done(uint32_t d) const { return ids[d] == 0u; }
     __device__ void finish(uint32_t d, uint32_t acc, LocalStats &st) const
     {
         const uint32_t cur = ids[d];
