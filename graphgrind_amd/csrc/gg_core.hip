@@ -68,6 +68,7 @@ constexpr uint64_t kPoolCap = 8ull << 30; // bytes kept per context
 
 void pool_drain(Ctx *c)
 {
+    std::lock_guard<std::mutex> lk(c->pool_mu);
     for (auto &kv : c->pool) (void)hipFree(kv.second);
     c->pool.clear();
     c->pool_bytes = 0;
@@ -77,13 +78,16 @@ int pool_take(Ctx *c, Buf &b, uint64_t bytes)
 {
     if (b.p) pool_give(c, b);
     if (bytes == 0) bytes = 8;
-    auto it = c->pool.lower_bound(bytes);
-    if (it != c->pool.end() && it->first <= std::max<uint64_t>(2 * bytes, bytes + (1ull << 20))) {
-        b.p = it->second;
-        b.bytes = it->first;
-        c->pool_bytes -= it->first;
-        c->pool.erase(it);
-        return GG_OK;
+    {
+        std::lock_guard<std::mutex> lk(c->pool_mu);
+        auto it = c->pool.lower_bound(bytes);
+        if (it != c->pool.end() && it->first <= std::max<uint64_t>(2 * bytes, bytes + (1ull << 20))) {
+            b.p = it->second;
+            b.bytes = it->first;
+            c->pool_bytes -= it->first;
+            c->pool.erase(it);
+            return GG_OK;
+        }
     }
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess && !c->pool.empty()) {
@@ -110,17 +114,20 @@ int pool_ensure(Ctx *c, Buf &b, uint64_t bytes)
 void pool_give(Ctx *c, Buf &b)
 {
     if (!b.p) return;
-    bool live;
     {
+        // the registry lock is held while pooling, so the context cannot be
+        // destroyed in between
         std::lock_guard<std::mutex> lk(g_ctx_mu);
-        live = c && g_live_ctx.count(c);
-    }
-    if (live && c->pool_bytes + b.bytes <= kPoolCap) {
-        c->pool.emplace(b.bytes, b.p);
-        c->pool_bytes += b.bytes;
-        b.p = nullptr;
-        b.bytes = 0;
-        return;
+        if (c && g_live_ctx.count(c)) {
+            std::lock_guard<std::mutex> pk(c->pool_mu);
+            if (c->pool_bytes + b.bytes <= kPoolCap) {
+                c->pool.emplace(b.bytes, b.p);
+                c->pool_bytes += b.bytes;
+                b.p = nullptr;
+                b.bytes = 0;
+                return;
+            }
+        }
     }
     buf_free(b);
 }

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -177,6 +178,7 @@ struct Ctx {
     // makes reuse safe.
     std::multimap<uint64_t, void *> pool;
     uint64_t pool_bytes = 0;
+    std::mutex pool_mu; // frontiers may be released from another host thread
     double last_kernel_ms = 0;
     int last_kernel_launches = 0;
 };
