@@ -311,7 +311,7 @@ def apps_bench(gg, ctx, scale, steps):
         _, _, info = gg.bc(g, 100, np.float32, out=sc)
         ts.append(info.ms)
     res = {"bc": {"scale": scale, "root": 100, "rounds": info.iters, "ms": float(np.median(ts)),
-                  "edges_read": int(info.edges_scanned),
+                  "paths": info.path_list(), "edges_read": int(info.edges_scanned),
                   "gbs": 8 * info.edges_scanned / float(np.median(ts)) / 1e6}}
     g.close()
     g = gg.Graph.rmat(ctx, scale - 2, 16, 4)

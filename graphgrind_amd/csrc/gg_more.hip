@@ -2,14 +2,16 @@
 // contributions (app/BC.C) and Bellman-Ford distances (app/BellmanFord.C),
 // each a loop of destination-owned pulls.
 //
-// BC from 'root' (BC.C:154-226).  Forward level L: every vertex not yet
-// visited sums NumPaths over its in-neighbours in level L-1 and, if any,
-// joins level L (BC_F 28-77; the sum starts at -0.0 so its sign says
-// whether a frontier in-neighbour exists).  Backward on the transpose,
-// deepest level first: a pull over the out-edge CSR sums Dependencies of
-// out-neighbours in level r+1 into every vertex not yet visited (BC_Back_F
-// 80-124), then level r adds inverseNumPaths (BC_Back_Vertex_F 141-152);
-// finally score = (Dependencies - inv) / inv (222), NaN where unreached.
+// BC from 'root' (BC.C:154-226) on the generic edgeMap with the path-sum
+// operator (GG_OP_PATHSUM).  Forward level L: every vertex not yet visited
+// adds NumPaths of its in-neighbours in level L-1 and, if it had none
+// before, joins level L (BC_F 28-77).  Backward on the transposed view,
+// deepest level first: Dependencies of level r+1 flow to every vertex not
+// yet visited (BC_Back_F 80-124), then level r adds inverseNumPaths
+// (BC_Back_Vertex_F 141-152); finally score = (Dependencies - inv) / inv
+// (222), NaN where unreached.  Each level takes the reference's branch by
+// the threshold m/20: the pull (fixed summation order) when the frontier is
+// large, the push (atomic adds, as BC_F::updateAtomic) when it is small.
 //
 // Bellman-Ford from 'root' (BellmanFord.C:100-140), integer edge lengths:
 // each round the vertices whose distance fell last round relax their
@@ -20,91 +22,57 @@
 // -(INT_MAX/2), as the reference does.
 #include <cstring>
 #include <limits>
+#include <vector>
 
 #include "gg_kernels.h"
 
 namespace gg {
 
 template <class V>
-__global__ void k_bc_gate(const int32_t *__restrict__ level, const V *__restrict__ val, int32_t want, V none,
-                          V *__restrict__ gate, uint32_t n)
-{
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) gate[v] = level[v] == want ? val[v] : none;
-}
-
-template <class V>
-struct OpBCFwd {
-    using acc_t = V;
-    const V *gate; // NumPaths of level L-1, -0.0 elsewhere
-    V *sigma;
-    int32_t *level;
-    int32_t L;
-    __device__ bool skip() const { return false; }
-    __device__ V identity() const { return -V(0); }
-    __device__ V combine(V a, V b) const { return a + b; }
-    __device__ V gather(uint32_t, uint32_t s) const { return gate[s]; }
-    struct row_t {
-        int32_t lvl;
-    };
-    __device__ row_t load_row(uint32_t d) const { return row_t{level[d]}; }
-    __device__ void store_row(uint32_t d, const row_t &q, V acc, LocalStats &st) const
-    {
-        if (q.lvl < 0 && !signbit(acc)) {
-            sigma[d] = acc; // NumPaths[d] was 0: NumPaths[d] += sum
-            level[d] = L;
-            st.active++;
-        }
-    }
-};
-
-template <class V>
-struct OpBCBack {
-    using acc_t = V;
-    const V *gate; // Dependencies of level r+1, 0 elsewhere
-    V *dep;
-    const V *inv;
-    const int32_t *level;
-    int32_t r;
-    __device__ bool skip() const { return false; }
-    __device__ V identity() const { return V(0); }
-    __device__ V combine(V a, V b) const { return a + b; }
-    __device__ V gather(uint32_t, uint32_t s) const { return gate[s]; }
-    struct row_t {
-        int32_t lvl;
-        V dep, inv;
-    };
-    __device__ row_t load_row(uint32_t d) const { return row_t{level[d], dep[d], inv[d]}; }
-    __device__ void store_row(uint32_t d, const row_t &q, V acc, LocalStats &) const
-    {
-        // visited so far in the backward sweep: levels deeper than r
-        if (q.lvl > r) return;
-        V x = q.dep + acc;
-        if (q.lvl == r) x = x + q.inv;
-        dep[d] = x;
-    }
-};
-
-template <class V>
-__global__ void k_bc_init(V *__restrict__ sigma, int32_t *__restrict__ level, V *__restrict__ dep, uint32_t root,
-                          uint32_t n)
+__global__ void k_bc_init(V *__restrict__ sigma, uint8_t *__restrict__ vis, int32_t *__restrict__ level,
+                          uint32_t root, uint32_t n)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    sigma[v] = v == root ? V(1) : V(0);
+    sigma[v] = v == root ? V(1) : V(0); // NumPaths (BC.C:167-172)
+    vis[v] = v == root;
     level[v] = v == root ? 0 : -1;
-    dep[v] = V(0);
 }
 
+// BC_Vertex_F (BC.C:127-137) over a level's output: mark visited and
+// record the level
+__global__ void k_bc_mark(const uint8_t *__restrict__ F, uint8_t *__restrict__ vis, int32_t *__restrict__ level,
+                          int32_t L, uint32_t n)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n && F[v]) {
+        vis[v] = 1;
+        level[v] = L;
+    }
+}
+
+// inverseNumPaths (BC.C:196-198), Dependencies and Visited reset (193, 202)
 template <class V>
-__global__ void k_bc_inv(const V *__restrict__ sigma, const int32_t *__restrict__ level, int32_t deepest,
-                         V *__restrict__ inv, V *__restrict__ dep, uint32_t n)
+__global__ void k_bc_inv(const V *__restrict__ sigma, V *__restrict__ inv, V *__restrict__ dep,
+                         uint8_t *__restrict__ vis, uint32_t n)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const V iv = V(1) / sigma[v];
-    inv[v] = iv;
-    if (level[v] == deepest) dep[v] = dep[v] + iv;
+    inv[v] = V(1) / sigma[v];
+    dep[v] = V(0);
+    vis[v] = 0;
+}
+
+// BC_Back_Vertex_F (BC.C:141-152) over a level's frontier
+template <class V>
+__global__ void k_bc_back_mark(const uint8_t *__restrict__ F, uint8_t *__restrict__ vis, V *__restrict__ dep,
+                               const V *__restrict__ inv, uint32_t n)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n && F[v]) {
+        vis[v] = 1;
+        dep[v] = dep[v] + inv[v];
+    }
 }
 
 template <class V>
@@ -114,56 +82,136 @@ __global__ void k_bc_final(const V *__restrict__ dep, const V *__restrict__ inv,
     if (v < n) score[v] = (dep[v] - inv[v]) / inv[v];
 }
 
+// The graph with its two edge directions swapped (GA.transpose()), sharing
+// g's arrays and schedules: edgeMap on it pushes along in-edges and pulls
+// over the out-edge CSR.  Unpartitioned, so its dense branch is the pull.
+// It owns nothing: release it with alias_release, never graph_free.
+static int transpose_alias(Graph *g, Graph &t)
+{
+    GG_TRY(ensure_csr_sched(g));
+    if (g->pull_schedule == 1) {
+        GG_TRY(ensure_rows(g));
+        GG_TRY(ensure_csr_rows(g));
+    }
+    t.ctx = g->ctx;
+    t.n = g->n;
+    t.m = g->m;
+    t.symmetric = g->symmetric;
+    t.weighted = g->weighted;
+    t.off = g->toff;
+    t.ids = g->tids;
+    t.w = g->tw;
+    t.toff = g->off;
+    t.tids = g->ids;
+    t.tw = g->w;
+    t.outdeg = g->indeg;
+    t.indeg = g->outdeg;
+    t.pull = g->csr_pull;
+    t.csr_pull = g->pull;
+    t.csr_pull_ok = true;
+    t.rows = g->csr_rows;
+    t.rows_ok = g->csr_rows_ok;
+    t.csr_rows = g->rows;
+    t.csr_rows_ok = g->rows_ok;
+    t.pull_schedule = g->pull_schedule;
+    t.edges_per_lane = g->edges_per_lane;
+    t.transposed = !g->transposed;
+    return GG_OK;
+}
+
+static void alias_release(Graph &t)
+{
+    Buf *bs[] = {&t.off, &t.ids, &t.w, &t.toff, &t.tids, &t.tw, &t.outdeg, &t.indeg};
+    for (Buf *b : bs) *b = Buf{};
+    t.pull = PullSched{};
+    t.csr_pull = PullSched{};
+    t.rows = RowSched{};
+    t.csr_rows = RowSched{};
+}
+
+// BC.C Compute (154-226) on the generic edgeMap: each level takes the
+// reference's branch (sparse push with atomic adds, or the destination-
+// owned pull) by its threshold m/20; the backward phase runs on the
+// transposed view.
 template <class V>
 static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_run_info *info)
 {
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
-    CscView fwd, bwd;
-    GG_TRY(pull_view(g, &fwd));
-    GG_TRY(csr_pull_view(g, &bwd));
-    Buf sigma, level, dep, inv, gate;
+    const int dt = sizeof(V) == 8 ? GG_F64 : GG_F32;
+    Buf sigma, vis, level, dep, inv;
     int rc = pool_take(c, sigma, sizeof(V) * (uint64_t)n);
+    if (!rc) rc = pool_take(c, vis, (uint64_t)n + 64);
     if (!rc) rc = pool_take(c, level, 4ull * n);
     if (!rc) rc = pool_take(c, dep, sizeof(V) * (uint64_t)n);
     if (!rc) rc = pool_take(c, inv, sizeof(V) * (uint64_t)n);
-    if (!rc) rc = pool_take(c, gate, sizeof(V) * (uint64_t)n);
+    std::vector<Frontier *> levels;
+    Graph tg;
+    bool alias = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
+    EdgeMapExtra ex;
     int round = 0;
     const uint32_t nb = blocks_for(n, kBlock);
     if (!rc) {
         (void)hipEventRecord(e0, c->stream);
-        k_bc_init<V><<<nb, kBlock, 0, c->stream>>>((V *)sigma.p, (int32_t *)level.p, (V *)dep.p, root, n);
-        unsigned long long *cnt = (unsigned long long *)c->counters.p;
-        // forward: one pull per level until a level finds no vertex
-        while (!rc) {
+        k_bc_init<V><<<nb, kBlock, 0, c->stream>>>((V *)sigma.p, (uint8_t *)vis.p, (int32_t *)level.p, root, n);
+        // partitioned_vertices::create(n, start, ...) (BC.C:175)
+        Frontier *F = new Frontier();
+        frontier_alloc(g, F);
+        uint32_t od = 0;
+        rc = pool_take(c, F->list, 8);
+        if (!rc && hipMemcpyAsync(F->list.p, &root, 4, hipMemcpyHostToDevice, c->stream)) rc = fail(GG_ERR_HIP, "gg_bc: root");
+        if (!rc && hipMemcpyAsync(&od, (uint32_t *)g->outdeg.p + root, 4, hipMemcpyDeviceToHost, c->stream))
+            rc = fail(GG_ERR_HIP, "gg_bc: outdeg");
+        if (!rc) rc = sync(c);
+        F->d_m = 1;
+        F->out_edges = od;
+        F->sparse_valid = true;
+        levels.push_back(F);
+        // forward (BC.C:183-191)
+        while (!rc && F->d_m != 0) {
             round++;
-            k_bc_gate<V><<<nb, kBlock, 0, c->stream>>>((const int32_t *)level.p, (const V *)sigma.p, round - 1,
-                                                       -V(0), (V *)gate.p, n);
-            rc = counters_zero(c);
-            if (!rc)
-                rc = launch_pull(c, fwd, OpBCFwd<V>{(const V *)gate.p, (V *)sigma.p, (int32_t *)level.p, round}, cnt,
-                                 nullptr, nullptr);
-            unsigned long long cn[4] = {0, 0, 0, 0};
-            if (!rc) rc = counters_read(c, cn);
-            if (!rc && info && round <= 64) info->paths[round - 1] = GG_PATH_CSC;
-            if (!rc && cn[CNT_ACTIVE] == 0) break;
-            if (!rc && round > (int)n) rc = fail(GG_ERR_STATE, "gg_bc: more levels than vertices");
-        }
-        // backward over the transpose, from the deepest level (round - 1)
-        if (!rc) {
-            k_bc_inv<V><<<nb, kBlock, 0, c->stream>>>((const V *)sigma.p, (const int32_t *)level.p, round - 1,
-                                                      (V *)inv.p, (V *)dep.p, n);
-            for (int r = round - 2; r >= 0 && !rc; r--) {
-                k_bc_gate<V><<<nb, kBlock, 0, c->stream>>>((const int32_t *)level.p, (const V *)dep.p, r + 1, V(0),
-                                                           (V *)gate.p, n);
-                rc = launch_pull(c, bwd,
-                                 OpBCBack<V>{(const V *)gate.p, (V *)dep.p, (const V *)inv.p,
-                                             (const int32_t *)level.p, r},
-                                 nullptr, nullptr, nullptr);
+            gg_op_t op = {GG_OP_PATHSUM, dt, sigma.p, vis.p};
+            Frontier *out = nullptr;
+            int path = 0;
+            if ((rc = edgemap_core(g, F, &op, -1, 0, &out, &path, &ex))) break;
+            if (info && round <= 64) info->paths[round - 1] = path;
+            levels.push_back(out);
+            if (out->d_m) {
+                if ((rc = frontier_make_dense(out))) break;
+                k_bc_mark<<<nb, kBlock, 0, c->stream>>>((const uint8_t *)out->bytes.p, (uint8_t *)vis.p,
+                                                        (int32_t *)level.p, round, n);
             }
+            F = out;
+            if (round > (int)n) rc = fail(GG_ERR_STATE, "gg_bc: more levels than vertices");
+        }
+        // backward on the transpose, deepest level first (BC.C:193-218)
+        if (!rc) {
+            k_bc_inv<V><<<nb, kBlock, 0, c->stream>>>((const V *)sigma.p, (V *)inv.p, (V *)dep.p, (uint8_t *)vis.p, n);
+            F = levels[round - 1];
+            rc = frontier_make_dense(F);
+            if (!rc)
+                k_bc_back_mark<V><<<nb, kBlock, 0, c->stream>>>((const uint8_t *)F->bytes.p, (uint8_t *)vis.p,
+                                                                (V *)dep.p, (const V *)inv.p, n);
+        }
+        if (!rc && round >= 2) {
+            rc = transpose_alias(g, tg);
+            alias = !rc;
+        }
+        for (int r = round - 2; r >= 0 && !rc; r--) {
+            gg_op_t op = {GG_OP_PATHSUM, dt, dep.p, vis.p};
+            Frontier *out = nullptr;
+            F->g = &tg;
+            rc = edgemap_core(&tg, F, &op, -1, 0, &out, nullptr, &ex);
+            F->g = g;
+            if (rc) break;
+            frontier_delete(out);
+            F = levels[r];
+            if ((rc = frontier_make_dense(F))) break;
+            k_bc_back_mark<V><<<nb, kBlock, 0, c->stream>>>((const uint8_t *)F->bytes.p, (uint8_t *)vis.p, (V *)dep.p,
+                                                            (const V *)inv.p, n);
         }
         if (!rc) {
             k_bc_final<V><<<nb, kBlock, 0, c->stream>>>((const V *)dep.p, (const V *)inv.p, score, n);
@@ -179,11 +227,13 @@ static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_ru
     if (!rc && info) {
         info->iters = round;
         info->ms = ms;
-        info->edges_scanned = (uint64_t)(2 * round - 1) * g->m;
+        info->edges_scanned = ex.scanned;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    Buf *all[] = {&sigma, &level, &dep, &inv, &gate};
+    if (alias) alias_release(tg);
+    for (Frontier *f : levels) frontier_delete(f);
+    Buf *all[] = {&sigma, &vis, &level, &dep, &inv};
     for (Buf *b : all) pool_give(c, *b);
     return rc;
 }
