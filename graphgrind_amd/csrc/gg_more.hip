@@ -13,13 +13,14 @@
 // the threshold m/20: the pull (fixed summation order) when the frontier is
 // large, the push (atomic adds, as BC_F::updateAtomic) when it is small.
 //
-// Bellman-Ford from 'root' (BellmanFord.C:100-140), integer edge lengths:
-// each round the vertices whose distance fell last round relax their
-// out-edges; a pull takes, per vertex, the minimum of dist[s] + w over
-// in-neighbours s in the frontier (read from a gated copy, so the round
-// is a Jacobi step with a deterministic result).  n rounds with a
-// non-empty frontier mean a negative cycle: every distance becomes
-// -(INT_MAX/2), as the reference does.
+// Bellman-Ford from 'root' (BellmanFord.C:100-140), integer edge lengths,
+// on the generic edgeMap with the relaxation operator (GG_OP_SSSP): each
+// round the vertices whose distance fell last round relax their out-edges,
+// by push (atomicMin, then the Visited CAS) or by the pull over a gated copy
+// of the frontier's distances, whichever the threshold m/20 picks.  Integer
+// minima do not depend on the order, so distances are exact either way.
+// n rounds with a non-empty frontier mean a negative cycle: every distance
+// becomes -(INT_MAX/2), as the reference does.
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -240,52 +241,21 @@ static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_ru
 
 // ---- Bellman-Ford ---------------------------------------------------------
 
-constexpr int32_t kDistInf = 2147483647 / 2;    // INT_MAX/2 (BellmanFord.C:105)
-constexpr int32_t kGateOff = std::numeric_limits<int32_t>::min();
+constexpr int32_t kDistInf = 2147483647 / 2; // INT_MAX/2 (BellmanFord.C:105)
 
-__global__ void k_sssp_gate(const int32_t *__restrict__ dist, const uint8_t *__restrict__ F, int32_t *__restrict__ gate,
-                            uint32_t n)
-{
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) gate[v] = F[v] ? dist[v] : kGateOff;
-}
-
-struct OpSSSP {
-    using acc_t = int32_t;
-    const int32_t *gate;
-    const float *tw; // lengths in in-edge order (integral values), NULL: 1
-    int32_t *dist;
-    uint8_t *next;
-    __device__ bool skip() const { return false; }
-    __device__ int32_t identity() const { return std::numeric_limits<int32_t>::max(); }
-    __device__ int32_t combine(int32_t a, int32_t b) const { return a < b ? a : b; }
-    __device__ int32_t gather(uint32_t e, uint32_t s) const
-    {
-        const int32_t gs = gate[s];
-        const int32_t w = tw ? (int32_t)__builtin_nontemporal_load(&tw[e]) : 1;
-        return gs == kGateOff ? std::numeric_limits<int32_t>::max() : gs + w;
-    }
-    struct row_t {
-        int32_t d;
-    };
-    __device__ row_t load_row(uint32_t v) const { return row_t{dist[v]}; }
-    __device__ void store_row(uint32_t v, const row_t &q, int32_t acc, LocalStats &st) const
-    {
-        const bool fell = acc < q.d;
-        if (fell) {
-            dist[v] = acc;
-            st.active++;
-        }
-        next[v] = fell;
-    }
-};
-
-__global__ void k_sssp_init(int32_t *__restrict__ dist, uint8_t *__restrict__ F, uint32_t root, uint32_t n)
+__global__ void k_sssp_init(int32_t *__restrict__ dist, int32_t *__restrict__ vis, uint32_t root, uint32_t n)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     dist[v] = v == root ? 0 : kDistInf;
-    F[v] = v == root;
+    vis[v] = 0;
+}
+
+// BF_Vertex_F (BellmanFord.C:94-103) over a round's output
+__global__ void k_sssp_unvisit(const uint8_t *__restrict__ F, int32_t *__restrict__ vis, uint32_t n)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n && F[v]) vis[v] = 0;
 }
 
 __global__ void k_sssp_negcycle(int32_t *__restrict__ dist, uint32_t n)
@@ -317,37 +287,51 @@ extern "C" int gg_sssp(gg_graph_t g, uint32_t root, int32_t *dist_dev, gg_run_in
     GG_HIP(hipSetDevice(c->device));
     if (info) std::memset(info, 0, sizeof(*info));
     const uint32_t n = g->n;
-    CscView view;
-    GG_TRY(pull_view(g, &view));
-    Buf F[2], gate;
-    int rc = pool_take(c, F[0], (uint64_t)n + 64);
-    if (!rc) rc = pool_take(c, F[1], (uint64_t)n + 64);
-    if (!rc) rc = pool_take(c, gate, 4ull * n);
+    Buf vis;
+    int rc = pool_take(c, vis, 4ull * n);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     const uint32_t nb = blocks_for(n, kBlock);
     int64_t round = 0;
+    EdgeMapExtra ex;
+    Frontier *F = nullptr;
     if (!rc) {
         (void)hipEventRecord(e0, c->stream);
-        k_sssp_init<<<nb, kBlock, 0, c->stream>>>(dist_dev, (uint8_t *)F[0].p, root, n);
-        unsigned long long *cnt = (unsigned long long *)c->counters.p;
-        const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
-        unsigned long long active = 1;
-        while (!rc && active) {
-            if (round == (int64_t)n) { // negative cycle (BellmanFord.C:116-122)
+        k_sssp_init<<<nb, kBlock, 0, c->stream>>>(dist_dev, (int32_t *)vis.p, root, n);
+        // partitioned_vertices::create(n, start, ...) (BellmanFord.C:121)
+        F = new Frontier();
+        frontier_alloc(g, F);
+        uint32_t od = 0;
+        rc = pool_take(c, F->list, 8);
+        if (!rc && hipMemcpyAsync(F->list.p, &root, 4, hipMemcpyHostToDevice, c->stream)) rc = fail(GG_ERR_HIP, "gg_sssp: root");
+        if (!rc && hipMemcpyAsync(&od, (uint32_t *)g->outdeg.p + root, 4, hipMemcpyDeviceToHost, c->stream))
+            rc = fail(GG_ERR_HIP, "gg_sssp: outdeg");
+        if (!rc) rc = sync(c);
+        F->d_m = 1;
+        F->out_edges = od;
+        F->sparse_valid = true;
+        // rounds (BellmanFord.C:124-139): each takes the reference's branch
+        // by the threshold m/20 (atomicMin push or the Jacobi pull)
+        while (!rc && F->d_m != 0) {
+            if (round == (int64_t)n) { // negative cycle (BellmanFord.C:126-132)
                 k_sssp_negcycle<<<nb, kBlock, 0, c->stream>>>(dist_dev, n);
                 break;
             }
-            const uint8_t *Fin = (const uint8_t *)F[round % 2].p;
-            uint8_t *Fout = (uint8_t *)F[(round + 1) % 2].p;
-            k_sssp_gate<<<nb, kBlock, 0, c->stream>>>(dist_dev, Fin, (int32_t *)gate.p, n);
-            rc = counters_zero(c);
-            if (!rc) rc = launch_pull(c, view, OpSSSP{(const int32_t *)gate.p, tw, dist_dev, Fout}, cnt, nullptr, nullptr);
-            unsigned long long cn[4] = {0, 0, 0, 0};
-            if (!rc) rc = counters_read(c, cn);
-            active = cn[CNT_ACTIVE];
-            if (!rc && info && round < 64) info->paths[round] = GG_PATH_CSC;
+            gg_op_t op = {GG_OP_SSSP, GG_I32, dist_dev, vis.p};
+            Frontier *out = nullptr;
+            int path = 0;
+            if ((rc = edgemap_core(g, F, &op, -1, 0, &out, &path, &ex))) break;
+            if (info && round < 64) info->paths[round] = path;
+            if (out->d_m) {
+                if ((rc = frontier_make_dense(out))) {
+                    frontier_delete(out);
+                    break;
+                }
+                k_sssp_unvisit<<<nb, kBlock, 0, c->stream>>>((const uint8_t *)out->bytes.p, (int32_t *)vis.p, n);
+            }
+            frontier_delete(F);
+            F = out;
             round++;
         }
         (void)hipEventRecord(e1, c->stream);
@@ -359,12 +343,11 @@ extern "C" int gg_sssp(gg_graph_t g, uint32_t root, int32_t *dist_dev, gg_run_in
     if (!rc && info) {
         info->iters = (int32_t)round;
         info->ms = ms;
-        info->edges_scanned = (uint64_t)round * g->m;
+        info->edges_scanned = ex.scanned;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    pool_give(c, F[0]);
-    pool_give(c, F[1]);
-    pool_give(c, gate);
+    frontier_delete(F);
+    pool_give(c, vis);
     return rc;
 }

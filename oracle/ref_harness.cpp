@@ -7,9 +7,12 @@
 // reference tree, and the reference's main() is renamed away with
 // -Dmain=gg_ref_app_main so this file supplies its own driver.  The per-app
 // loops below restate each app's Compute() (app/PageRank.C:102-151,
-// app/BFS.C:77-101, app/Components.C:101-126, app/SPMV.C:64-92) with the
-// app's own functors and the reference's own edgeMap/vertexMap, but keep the
-// result instead of freeing it.
+// app/BFS.C:77-101, app/Components.C:101-126, app/SPMV.C:64-92,
+// app/BC.C:154-226, app/BellmanFord.C:105-146, app/PageRankDelta.C:124-176,
+// app/PageRankConverage.C:238-310) with the app's own functors and the
+// reference's own edgeMap/vertexMap/vertexFilter, but keep the result
+// instead of freeing it.  BellmanFord is built with -DWEIGHTED (its
+// functor's update takes the edge length).
 //
 // Input: an in-memory graph, either read from a .ggb file written by the
 // tests (magic, n, m, flags, offsets, ids[, int weights]) or generated with
@@ -23,8 +26,16 @@
 #include "app/Components.C"
 #elif GG_APP == 4
 #include "app/SPMV.C"
+#elif GG_APP == 5
+#include "app/BC.C"
+#elif GG_APP == 6
+#include "app/BellmanFord.C"
+#elif GG_APP == 7
+#include "app/PageRankDelta.C"
+#elif GG_APP == 8
+#include "app/PageRankConverage.C"
 #else
-#error "GG_APP must be 1..4"
+#error "GG_APP must be 1..8"
 #endif
 #undef main
 
@@ -225,6 +236,176 @@ int run_app(GraphType &GA, long, std::vector<double> &res, long &iters)
     prevIDs.del();
     return 0;
 }
+#elif GG_APP == 5
+template <class GraphType>
+int run_app(GraphType &GA, long start, std::vector<double> &res, long &iters)
+{
+    const partitioner &part = GA.get_partitioner();
+    intT n = GA.n, m = GA.m;
+    mmap_ptr<fType> NumPaths;
+    NumPaths.part_allocate(part);
+    mmap_ptr<bool> Visited;
+    Visited.part_allocate(part);
+    map_vertexL(part, [&](intT j) { NumPaths[j] = 0.0; });
+    map_vertexL(part, [&](intT j) { Visited[j] = false; });
+    NumPaths[start] = 1.0;
+    Visited[start] = true;
+    partitioned_vertices Frontier = partitioned_vertices::create(n, start, GA.get_partition().V[start].getOutDegree());
+    vector<partitioned_vertices> Levels;
+    Levels.push_back(Frontier);
+    intT round = 0;
+    while (!Frontier.isEmpty()) {
+        round++;
+        partitioned_vertices output = edgeMap(GA, Frontier, BC_F(NumPaths, Visited), m / 20);
+        vertexMap(part, output, BC_Vertex_F(Visited));
+        Levels.push_back(output);
+        Frontier = output;
+    }
+    mmap_ptr<fType> Dependencies;
+    Dependencies.part_allocate(part);
+    map_vertexL(part, [&](intT j) { Dependencies[j] = 0.0; });
+    mmap_ptr<fType> inverseNumPaths;
+    inverseNumPaths = NumPaths;
+    map_vertexL(part, [&](intT j) { inverseNumPaths[j] = 1 / inverseNumPaths[j]; });
+    Levels[round].del();
+    map_vertexL(part, [&](intT j) { Visited[j] = false; });
+    Frontier = Levels[round - 1];
+    vertexMap(part, Frontier, BC_Back_Vertex_F(Visited, Dependencies, inverseNumPaths));
+    GA.transpose();
+    for (intT r = round - 2; r >= 0; r--) {
+        partitioned_vertices output = edgeMap(GA, Frontier, BC_Back_F(Dependencies, Visited), m / 20);
+        output.del();
+        Frontier.del();
+        Frontier = Levels[r];
+        vertexMap(part, Frontier, BC_Back_Vertex_F(Visited, Dependencies, inverseNumPaths));
+    }
+    Frontier.del();
+    map_vertexL(part, [&](intT j) { Dependencies[j] = (Dependencies[j] - inverseNumPaths[j]) / inverseNumPaths[j]; });
+    res.resize(n);
+    for (intT i = 0; i < n; i++) res[i] = (double)Dependencies[i];
+    iters = round;
+    inverseNumPaths.del();
+    Visited.del();
+    Dependencies.del();
+    return 0;
+}
+#elif GG_APP == 6
+template <class GraphType>
+int run_app(GraphType &GA, long start, std::vector<double> &res, long &iters)
+{
+    intT n = GA.n, m = GA.m;
+    const partitioner &part = GA.get_partitioner();
+    mmap_ptr<intT> ShortestPathLen;
+    ShortestPathLen.part_allocate(part);
+    map_vertexL(part, [&](intT j) { ShortestPathLen[j] = INT_MAX / 2; });
+    ShortestPathLen[start] = 0;
+    mmap_ptr<intT> Visited;
+    Visited.part_allocate(part);
+    map_vertexL(part, [&](intT j) { Visited[j] = 0; });
+    partitioned_vertices Frontier = partitioned_vertices::create(n, start, GA.get_partition().V[start].getOutDegree());
+    intT round = 0;
+    while (!Frontier.isEmpty()) {
+        if (round == n) {
+            map_vertexL(part, [&](intT j) { ShortestPathLen[j] = -(INT_MAX / 2); });
+            break;
+        }
+        partitioned_vertices output = edgeMap(GA, Frontier, BF_F(ShortestPathLen, Visited), m / 20);
+        vertexMap(part, output, BF_Vertex_F(Visited));
+        Frontier.del();
+        Frontier = output;
+        round++;
+    }
+    res.resize(n);
+    for (intT i = 0; i < n; i++) res[i] = (double)ShortestPathLen[i];
+    iters = round;
+    Frontier.del();
+    Visited.del();
+    ShortestPathLen.del();
+    return 0;
+}
+#elif GG_APP == 7
+template <class GraphType>
+int run_app(GraphType &GA, long, std::vector<double> &res, long &iters)
+{
+    typedef typename GraphType::vertex_type vertex;
+    const partitioner &part = GA.get_partitioner();
+    const double damping = 0.85, epsilon = 0.0000001, epsilon2 = 0.01;
+    intT n = GA.n, m = GA.m;
+    double one_over_n = 1 / (double)n;
+    mmap_ptr<double> p, nghSum, Delta;
+    p.part_allocate(part);
+    nghSum.part_allocate(part);
+    Delta.part_allocate(part);
+    map_vertexL(part, [&](intT j) { p[j] = 0.0; });
+    map_vertexL(part, [&](intT j) { Delta[j] = one_over_n; });
+    map_vertexL(part, [&](intT j) { nghSum[j] = 0.0; });
+    partitioned_vertices Frontier = partitioned_vertices::bits(part, n, m);
+    partitioned_vertices All = partitioned_vertices::bits(part, n, m);
+    intT round = 0;
+    while (1) {
+        round++;
+        partitioned_vertices output = edgeMap(GA, Frontier, PR_F<vertex>(GA.get_partition().V, Delta, nghSum), m / 20);
+        output.del();
+        partitioned_vertices active =
+            (round == 1) ? vertexFilter(GA, All, PR_Vertex_F_FirstRound(p, Delta, nghSum, damping, one_over_n, epsilon2))
+                         : vertexFilter(GA, All, PR_Vertex_F(p, Delta, nghSum, damping, epsilon2));
+        map_vertexL(part, [&](intT j) { nghSum[j] = fabs(Delta[j]); });
+        double L1_norm = sequence::plusReduce(nghSum.get(), n);
+        if (L1_norm < epsilon) { active.del(); break; }
+        vertexMap(part, All, PR_Vertex_Reset(nghSum));
+        Frontier.del();
+        Frontier = active;
+    }
+    res.assign(p.get(), p.get() + n);
+    iters = round;
+    Frontier.del();
+    All.del();
+    p.del();
+    Delta.del();
+    nghSum.del();
+    return 0;
+}
+#elif GG_APP == 8
+template <class GraphType>
+int run_app(GraphType &GA, long, std::vector<double> &res, long &iters)
+{
+    typedef typename GraphType::vertex_type vertex;
+    const partitioner &part = GA.get_partitioner();
+    graph<vertex> &WG = GA.get_partition();
+    intT n = GA.n, m = GA.m;
+    const double epsilon = 0.0000001;
+    double one_over_n = 1 / (double)n;
+    mmap_ptr<double> p_curr, p_next, edge_factor, add_factor;
+    p_curr.part_allocate(part);
+    p_next.part_allocate(part);
+    edge_factor.part_allocate(part);
+    add_factor.part_allocate(part);
+    map_vertexL(part, [&](intT j) { p_curr[j] = one_over_n; });
+    map_vertexL(part, [&](intT j) { p_next[j] = 0; });
+    map_vertexL(part, [&](intT j) { edge_factor[j] = (double)0.85 / (WG.V[j].getOutDegree()); });
+    int count = 0;
+    double L1_norm = 2;
+    partitioned_vertices Frontier = partitioned_vertices::bits(part, n, m);
+    while (count < MaxIter && L1_norm > epsilon) {
+        map_vertexL(part, [&](intT j) { add_factor[j] = (double)edge_factor[j] * p_curr[j]; });
+        partitioned_vertices output = edgeMap(GA, Frontier, PR_F<vertex>(p_curr, p_next, WG.V, add_factor), m / 20);
+        vertexMap(part, Frontier, PR_Vertex_Csum(p_next, sum(part, p_next, n, 1), n));
+        L1_norm = normdiff(part, p_curr, p_next, n);
+        vertexMap(part, Frontier, PR_Vertex_norm(p_next, sum(part, p_next, n, 0), n));
+        vertexMap(part, Frontier, PR_Vertex_Reset(p_curr));
+        count++;
+        swap(p_curr, p_next);
+        output.del();
+    }
+    res.assign(p_curr.get(), p_curr.get() + n);
+    iters = count;
+    Frontier.del();
+    p_curr.del();
+    p_next.del();
+    edge_factor.del();
+    add_factor.del();
+    return 0;
+}
 #else
 static std::vector<double> g_x;
 template <class GraphType>
@@ -316,7 +497,7 @@ int main(int argc, char **argv)
     int scale = 0, ef = 16;
     uint64_t seed = 1;
     bool sym = false;
-#if GG_APP == 4
+#if GG_APP == 4 || GG_APP == 6
     h.weighted = true;
 #endif
     for (int i = 1; i < argc; i++) {

@@ -59,8 +59,25 @@ def case(name, off, ids, symmetric, P, root, apps, wseed=None, x=None, store=Non
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def case_apps(name, off, ids, symmetric, P, root, apps, wseed=None):
+    """Fixtures of the further apps (BC.C, BellmanFord.C, PageRankDelta.C,
+    PageRankConverage.C) on one graph, single-threaded reference runs."""
+    out = {"off": off, "ids": ids, "symmetric": np.int64(symmetric), "P": np.int64(P),
+           "root": np.int64(root)}
+    for app in apps:
+        w = int_weights(wseed, len(ids)) if app == "bellmanford" else None
+        res, iters, _ = oracle.run_ref(app, off, ids, w=w, symmetric=symmetric, P=P, root=root, threads=1)
+        out[app] = res
+        out[f"{app}_iters"] = np.int64(iters)
+        if w is not None:
+            out["bellmanford_w"] = w
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
-    if not oracle.ref_available("pagerank"):
+    if not all(oracle.ref_available(a) for a in ("pagerank", "bc", "bellmanford", "prdelta", "prconv")):
         sys.exit("oracle/_ref not built (needs /root/reference): make -C oracle")
     apps_dir = ["pagerank", "bfs", "cc", "spmv"]
     case("sixnode_p2", SIX_OFF, SIX_IDS, False, 2, 0, apps_dir, wseed=7)
@@ -76,6 +93,14 @@ def main():
     case("rmat11_sym_p32", off, ids, True, 32, 100, ["bfs", "cc"])
     off, ids = oracle.rmat_csr(12, 8, 5)
     case("rmat12_dir_p64", off, ids, False, 64, 100, ["pagerank", "bfs"])
+    # further apps: BC on symmetric graphs (its GA.transpose() is then the
+    # identity in the reference too), the others on directed graphs
+    case_apps("apps_sixnode_p2", SIX_OFF, SIX_IDS, False, 2, 0, ["bellmanford", "prdelta", "prconv"], wseed=7)
+    case_apps("apps_sixnode_sym_p2", so, si, True, 2, 0, ["bc"])
+    off, ids = oracle.rmat_csr(10, 16, 2, symmetric=True)
+    case_apps("apps_rmat10_sym_p16", off, ids, True, 16, 3, ["bc"])
+    off, ids = oracle.rmat_csr(10, 16, 1)
+    case_apps("apps_rmat10_dir_p16", off, ids, False, 16, 0, ["bellmanford", "prdelta", "prconv"], wseed=3)
 
 
 if __name__ == "__main__":

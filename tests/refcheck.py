@@ -167,7 +167,18 @@ def rel_l1(a, b):
 
 
 def golden_cases():
-    return sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+    """Fixtures of the engine apps (partitions, COO lists, the four apps)."""
+    return sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                  if not os.path.basename(p).startswith("apps_"))
+
+
+def app_golden_cases(app=None):
+    """Fixtures of the further apps (BC, BellmanFord, PageRankDelta,
+    PageRankConverage), optionally only those holding 'app'."""
+    paths = sorted(glob.glob(os.path.join(GOLDEN, "apps_*.npz")))
+    if app is None:
+        return paths
+    return [p for p in paths if app in np.load(p, allow_pickle=False).files]
 
 
 def load_golden(path):

@@ -149,6 +149,21 @@ def test_frontend_pagerank_converage():
     assert R.rel_l1(got, want) < 1e-10
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("app", ["prdelta", "prconv"])
+def test_frontend_delta_and_converage_match_reference_golden(tmp_path, app):
+    """PageRankDelta.C / PageRankConverage.C loops through the front end,
+    reading the fixture's graph from a PBBS file, against the reference
+    engine's own run on it (tests/golden/apps_*.npz)."""
+    for path in R.app_golden_cases(app):
+        z = R.load_golden(path)
+        f = str(tmp_path / "g.adj")
+        write_pbbs(f, z["off"], z["ids"])
+        got, it, _ = _run("frontend_parity", app, ["-c", str(int(z["P"])), f])
+        assert it == int(z[f"{app}_iters"]), path
+        assert R.rel_l1(got, z[app]) < 1e-12, path
+
+
 # ---- input formats (IO.h:136-361 PBBS text, 365-656 Galois .gr) -------------
 
 def write_pbbs(path, off, ids, w=None):

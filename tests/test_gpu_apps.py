@@ -83,3 +83,35 @@ def test_sssp_negative_cycle(ctx):
     d, info = gg.sssp(g, 0)
     assert np.all(d.to_host() == -INF) and info.iters == 3
     g.close()
+
+
+# ---- against the reference engine's own runs (tests/golden/apps_*.npz) ----
+
+import refcheck as R  # noqa: E402
+
+
+@pytest.mark.parametrize("path", R.app_golden_cases("bc"), ids=lambda p: p.split("/")[-1])
+def test_bc_matches_reference_golden(ctx, path):
+    """The reference's BC.C computes in FP32; both of our precisions must
+    agree with it to FP32 rounding, with NaN exactly where it has NaN."""
+    z = R.load_golden(path)
+    want = z["bc"]
+    ok = ~np.isnan(want)
+    g = gg.Graph.from_csr(ctx, z["off"], z["ids"], symmetric=bool(z["symmetric"]))
+    for dt in (np.float32, np.float64):
+        sc, _, info = gg.bc(g, int(z["root"]), dt)
+        got = sc.to_host().astype(np.float64)
+        assert info.iters == int(z["bc_iters"])
+        assert np.array_equal(np.isnan(got), ~ok)
+        assert _rel(got[ok], want[ok]) < 1e-5, dt
+    g.close()
+
+
+@pytest.mark.parametrize("path", R.app_golden_cases("bellmanford"), ids=lambda p: p.split("/")[-1])
+def test_sssp_matches_reference_golden(ctx, path):
+    z = R.load_golden(path)
+    g = gg.Graph.from_csr(ctx, z["off"], z["ids"], w=z["bellmanford_w"].astype(np.float32),
+                          symmetric=bool(z["symmetric"]))
+    d, _ = gg.sssp(g, int(z["root"]))
+    assert np.array_equal(d.to_host().astype(np.int64), z["bellmanford"].astype(np.int64))
+    g.close()

@@ -9,6 +9,7 @@ import scipy.sparse as sp
 from scipy.sparse.csgraph import dijkstra
 
 import oracle
+import refcheck as R
 
 INF = 2147483647 // 2
 
@@ -215,3 +216,45 @@ def test_vebo_invariants(scale, sym, P):
     if zeros >= P:
         sz = np.diff(st)
         assert sz.max() - sz.min() <= 1 or sz.max() <= n // P + 1
+
+
+# ---- against the reference engine (tests/golden/apps_*.npz, written by
+# tests/golden/make_golden.py from oracle/_ref, the reference compiled) ----
+
+def _ids(p):
+    return p.split("/")[-1]
+
+
+@pytest.mark.parametrize("path", R.app_golden_cases("bc"), ids=_ids)
+def test_bc_matches_reference(path):
+    """The reference computes in FP32 (fType float); the oracle in FP64."""
+    z = R.load_golden(path)
+    want = z["bc"]
+    got, lv, _, rounds = oracle.bc(z["off"], z["ids"], int(z["root"]))
+    assert rounds == int(z["bc_iters"])
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert np.abs(got[ok] - want[ok]).sum() <= 1e-6 * max(np.abs(want[ok]).sum(), 1e-30)
+
+
+@pytest.mark.parametrize("path", R.app_golden_cases("bellmanford"), ids=_ids)
+def test_bellman_ford_matches_reference(path):
+    z = R.load_golden(path)
+    got, _ = oracle.sssp(z["off"], z["ids"], z["bellmanford_w"].astype(np.float32), int(z["root"]))
+    assert np.array_equal(got, z["bellmanford"].astype(np.int64))
+
+
+@pytest.mark.parametrize("path", R.app_golden_cases("prdelta"), ids=_ids)
+def test_prdelta_matches_reference(path):
+    z = R.load_golden(path)
+    got, rounds = oracle.prdelta(z["off"], z["ids"])
+    assert rounds == int(z["prdelta_iters"])
+    assert np.abs(got - z["prdelta"]).sum() <= 1e-12 * np.abs(z["prdelta"]).sum()
+
+
+@pytest.mark.parametrize("path", R.app_golden_cases("prconv"), ids=_ids)
+def test_prconv_matches_reference(path):
+    z = R.load_golden(path)
+    got, it = oracle.prconv(z["off"], z["ids"])
+    assert it == int(z["prconv_iters"])
+    assert np.abs(got - z["prconv"]).sum() <= 1e-12 * np.abs(z["prconv"]).sum()
