@@ -405,6 +405,8 @@ def main():
                     help="only run 1 GiB device copies (counter calibration under rocprofv3)")
     ap.add_argument("--only-bfs-cc", action="store_true",
                     help="only run the BFS + Components leg (for kernel traces of it)")
+    ap.add_argument("--only-adjoints", action="store_true",
+                    help="only run the SpMV and PageRank adjoint legs (for kernel traces of them)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every run of this build uses a single MI355X; "
@@ -419,6 +421,11 @@ def main():
         return
     if a.only_bfs_cc:
         log(f"[bench] bfs/cc: {bfs_cc_bench(gg, ctx, a.scale, a.steps)}")
+        ctx.close()
+        return
+    if a.only_adjoints:
+        log(f"[bench] spmv adjoint: {spmv_adjoint_bench(gg, ctx, a.scale, a.steps)}")
+        log(f"[bench] pagerank adjoint: {wpagerank_adjoint_bench(gg, ctx, a.scale - 2, a.steps)}")
         ctx.close()
         return
     pr = pagerank_bench(gg, ctx, a.scale, a.steps, a.warmup, np.float64)

@@ -394,6 +394,14 @@ public:
     static partitioned_vertices create(intT n, intT v, intT);
     static partitioned_vertices bits(const partitioner &, intT n, intT m);
     static partitioned_vertices empty();
+    // dense(n, part) / sparse(n): n rows, nothing active (132-159);
+    // boolean(n, flags, ...) / indice(n, ids, count, ...): from a bool
+    // array or an id list (183-208); the active count and out-degree sum are
+    // recomputed on the device
+    static partitioned_vertices dense(intT n, const partitioner &);
+    static partitioned_vertices sparse(intT n);
+    static partitioned_vertices boolean(intT n, mmap_ptr<bool> flags, intT, intT);
+    static partitioned_vertices indice(intT n, mmap_ptr<intT> ids, intT activeM, intT);
     void del()
     {
         if (h) gg_frontier_destroy(h);
@@ -438,6 +446,37 @@ inline partitioned_vertices partitioned_vertices::bits(const partitioner &, intT
 {
     gg_frontier_t f;
     gg_front::check(gg_frontier_create(gg_front::current_graph(), GG_FRONTIER_ALL, 0, &f), "bits");
+    return wrap(f);
+}
+inline partitioned_vertices partitioned_vertices::dense(intT n, const partitioner &)
+{
+    std::vector<uint8_t> none((size_t)std::max<intT>(n, 1), 0);
+    gg_frontier_t f;
+    gg_front::check(gg_frontier_from_host(gg_front::current_graph(), none.data(), nullptr, 0, &f), "dense");
+    return wrap(f);
+}
+inline partitioned_vertices partitioned_vertices::sparse(intT)
+{
+    gg_frontier_t f;
+    gg_front::check(gg_frontier_from_host(gg_front::current_graph(), nullptr, nullptr, 0, &f), "sparse");
+    return wrap(f);
+}
+inline partitioned_vertices partitioned_vertices::boolean(intT n, mmap_ptr<bool> flags, intT, intT)
+{
+    const bool *b = flags.get();
+    std::vector<uint8_t> d((size_t)std::max<intT>(n, 1), 0);
+    for (intT i = 0; i < n; i++) d[i] = b[i] ? 1 : 0;
+    gg_frontier_t f;
+    gg_front::check(gg_frontier_from_host(gg_front::current_graph(), d.data(), nullptr, 0, &f), "boolean");
+    return wrap(f);
+}
+inline partitioned_vertices partitioned_vertices::indice(intT, mmap_ptr<intT> ids, intT activeM, intT)
+{
+    const intT *s = ids.get();
+    std::vector<uint32_t> l((size_t)std::max<intT>(activeM, 0));
+    for (intT i = 0; i < activeM; i++) l[i] = (uint32_t)s[i];
+    gg_frontier_t f;
+    gg_front::check(gg_frontier_from_host(gg_front::current_graph(), nullptr, l.data(), l.size(), &f), "indice");
     return wrap(f);
 }
 inline partitioned_vertices partitioned_vertices::empty()

@@ -2,7 +2,7 @@
 // (the drop-in C++ API) that write their results to a file, so the tests can
 // compare the front end's device path with the oracle.
 //
-//   GG_APP=pagerank|bfs|cc|spmv|bc|bc32|sssp|prdelta|prconv GG_DUMP=out.bin
+//   GG_APP=pagerank|bfs|cc|spmv|bc|bc32|sssp|prdelta|prconv|factories GG_DUMP=out.bin
 //       frontend_parity[_w] <driver args>
 //
 // The functor structs follow the reference apps' interface (member names
@@ -374,6 +374,34 @@ void Compute(GraphType &GA, long start)
         for (intT i = 0; i < n; i++) out[i] = d[i];
         SP.del();
         Visited.del();
+    } else if (app == "factories") {
+        // partitioned_vertices factories (ligra-numa.h:103-208): out = the
+        // active count of dense(n), sparse(n), boolean(every third vertex),
+        // indice(vertices 0..9), then each frontier's out-edge sum
+        mmap_ptr<bool> flags;
+        flags.part_allocate(part);
+        map_vertexL(part, [&](intT j) { flags[j] = j % 3 == 0; });
+        mmap_ptr<intT> ids;
+        ids.part_allocate(part);
+        map_vertexL(part, [&](intT j) { ids[j] = j; });
+        partitioned_vertices a = partitioned_vertices::dense(n, part);
+        partitioned_vertices b = partitioned_vertices::sparse(n);
+        partitioned_vertices c = partitioned_vertices::boolean(n, flags, 0, 0);
+        partitioned_vertices d = partitioned_vertices::indice(n, ids, std::min<intT>(10, n), 0);
+        out.assign(n, 0.0);
+        partitioned_vertices *fs[4] = {&a, &b, &c, &d};
+        for (int k = 0; k < 4 && 2 * k + 1 < n; k++) {
+            out[2 * k] = fs[k]->numNonzeros();
+            out[2 * k + 1] = fs[k]->num_out_edges;
+            if (fs[k]->numRows() != n) out[2 * k] = -1;
+        }
+        iters = a.isEmpty() && b.isEmpty() && !c.isEmpty() && !d.isEmpty();
+        a.del();
+        b.del();
+        c.del();
+        d.del();
+        flags.del();
+        ids.del();
     } else if (app == "prconv") {
         // app/PageRankConverage.C:238-310
         graph<vertex> &WG = GA.get_partition();

@@ -164,6 +164,21 @@ def test_frontend_delta_and_converage_match_reference_golden(tmp_path, app):
         assert R.rel_l1(got, z[app]) < 1e-12, path
 
 
+@pytest.mark.gpu
+def test_frontend_frontier_factories():
+    """dense/sparse (empty, n rows), boolean (every third vertex) and
+    indice (vertices 0..9) frontiers from the front end."""
+    off, ids = oracle.rmat_csr(10, 8, 1)
+    n = len(off) - 1
+    deg = np.diff(off.astype(np.int64))
+    got, ok, _ = _run("frontend_parity", "factories", ["-c", "16", "-rmat", "10", "8", "1"])
+    assert ok == 1
+    assert list(got[:4]) == [0, 0, 0, 0]
+    third = np.arange(0, n, 3)
+    assert got[4] == len(third) and got[5] == deg[third].sum()
+    assert got[6] == 10 and got[7] == deg[:10].sum()
+
+
 # ---- input formats (IO.h:136-361 PBBS text, 365-656 Galois .gr) -------------
 
 def write_pbbs(path, off, ids, w=None):
