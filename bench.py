@@ -145,9 +145,9 @@ def spmv_adjoint_bench(gg, ctx, scale, steps, seed=3):
     f, b = float(np.median(fwd)), float(np.median(bwd))
     # FP32: per edge 4 B id + 4 B weight + 4 B gathered x; per row offset + y
     fbytes = 12 * m + 8 * n
-    # dx pass as the forward; dw pass: id + gathered gy + written gw per edge,
-    # x per row
-    bbytes = 12 * m + 8 * n + 12 * m + 8 * n
+    # one fused pass over the out-edge CSR: per edge 4 B id + 4 B weight +
+    # 4 B gathered gy + 4 B written gw; per row offset + x + gx
+    bbytes = 16 * m + 12 * n
     res = {"n": n, "m": m, "fwd_ms": f, "bwd_ms": b, "bwd_fwd_ratio": b / f,
            "fwd_gbs": fbytes / f / 1e6, "bwd_gbs": bbytes / b / 1e6}
     g.close()

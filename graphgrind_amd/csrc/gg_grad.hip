@@ -110,6 +110,29 @@ struct OpSpmvBwdX {
     __device__ void finish(uint32_t s, V acc, LocalStats &) const { gx[s] = acc; }
 };
 
+// Both outputs in one pass over the out-edge CSR (row-block schedule):
+// while staging edge e = (s -> d), gw[e] = x[s] gy[d] is written (coalesced,
+// in edge order) and w_e gy[d] is staged for gx[s] (the same products in the
+// same order as OpSpmvBwdX), so the random gy gathers are made once for both.
+template <class V>
+struct OpSpmvBwdFused {
+    using acc_t = V;
+    const float *w; // NULL: unit weights
+    const V *gy, *x;
+    V *gx, *gw;
+    __device__ bool skip() const { return false; }
+    __device__ V identity() const { return V(0); }
+    __device__ V combine(V a, V b) const { return a + b; }
+    __device__ V gather(uint32_t e, uint32_t d) const { return w ? (V)w[e] * gy[d] : gy[d]; }
+    __device__ V stage_edge(uint32_t e, uint32_t s, uint32_t d) const
+    {
+        const V g = gy[d];
+        __builtin_nontemporal_store(x[s] * g, &gw[e]);
+        return w ? (V)__builtin_nontemporal_load(&w[e]) * g : g;
+    }
+    __device__ void finish(uint32_t s, V acc, LocalStats &) const { gx[s] = acc; }
+};
+
 // gw[e] = x[s] gy[d]
 template <class V>
 struct EdgeSpmvGW {
@@ -694,14 +717,26 @@ extern "C" int gg_spmv_bwd(gg_graph_t g, int dtype, const void *x, const void *g
     (void)hipEventCreate(&k1);
     int rc = GG_OK;
     GG_HIP(hipEventRecord(e0, c->stream));
-    if (gx) {
+    // both outputs from one pass when the row-block schedule runs the pull
+    const bool fused = gx && gw && g->m && cv.rows;
+    if (fused) {
+        rc = dtype == GG_F64
+                 ? launch_pull(c, cv,
+                               OpSpmvBwdFused<double>{w, (const double *)gy, (const double *)x, (double *)gx,
+                                                      (double *)gw},
+                               nullptr, nullptr, nullptr, k0, k1)
+                 : launch_pull(c, cv,
+                               OpSpmvBwdFused<float>{w, (const float *)gy, (const float *)x, (float *)gx,
+                                                     (float *)gw},
+                               nullptr, nullptr, nullptr, k0, k1);
+    } else if (gx) {
         rc = dtype == GG_F64
                  ? launch_pull(c, cv, OpSpmvBwdX<double>{w, (const double *)gy, (double *)gx}, nullptr,
                                nullptr, nullptr, k0, k1)
                  : launch_pull(c, cv, OpSpmvBwdX<float>{w, (const float *)gy, (float *)gx}, nullptr,
                                nullptr, nullptr, k0, k1);
     }
-    if (!rc && gw && g->m) {
+    if (!rc && gw && g->m && !fused) {
         rc = dtype == GG_F64 ? launch_edge(c, g, EdgeSpmvGW<double>{(const double *)x, (const double *)gy, (double *)gw})
                              : launch_edge(c, g, EdgeSpmvGW<float>{(const float *)x, (const float *)gy, (float *)gw});
     }

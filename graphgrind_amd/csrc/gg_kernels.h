@@ -105,6 +105,17 @@ __device__ __forceinline__ bool bit_test(const uint32_t *bm, uint32_t v)
 // the staging phase together with the gathers (so the epilogue waits on no
 // memory); other operators are finished through finish(d, acc, stats).
 struct NoRow {};
+// Optional stage_edge(e, row, s): the value staged for edge e of row 'row'
+// with head s, in place of gather(e, s), for operators that need the row
+// while staging (the row-block pull then loads the item's row starts
+// first).  An operator may also write a per-edge output there, coalesced,
+// as the SpMV backward writes dL/dw.
+template <class Op, class = void>
+struct has_stage_edge : std::false_type {};
+template <class Op>
+struct has_stage_edge<Op, std::void_t<decltype(std::declval<const Op &>().stage_edge(0u, 0u, 0u))>>
+    : std::true_type {};
+
 // Optional row_done(d): true when row d cannot change whatever its
 // in-neighbours hold (a CC label already 0).  The row-block pull then skips
 // the gathers of a work item whose rows are all done.
@@ -437,8 +448,36 @@ __global__ __launch_bounds__(kBlock) void k_rows(Op op, const uint32_t *__restri
 #pragma unroll
         for (int k = 0; k < kPullIPT; k++)
             src[k] = __builtin_nontemporal_load(&tids[w.e0 + min(tid + k * kBlock, nel)]);
+        if constexpr (has_stage_edge<Op>::value) {
+            if (w.kind == kRowLong) {
 #pragma unroll
-        for (int k = 0; k < kPullIPT; k++) val[k] = op.gather(w.e0 + min(tid + k * kBlock, nel), src[k]);
+                for (int k = 0; k < kPullIPT; k++)
+                    val[k] = op.stage_edge(w.e0 + min(tid + k * kBlock, nel), w.r0, src[k]);
+            } else {
+                // row of each staged edge: the last row starting at or
+                // before it (empty rows share their start with the next)
+                const uint32_t nr0 = w.r1 - w.r0;
+                for (uint32_t i = tid; i < nr0; i += kBlock)
+                    s_rs[i] = __builtin_nontemporal_load(&toff[w.r0 + i]) - w.e0;
+                if (tid == 0) s_rs[nr0] = ne;
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < kPullIPT; k++) {
+                    const uint32_t j = min(tid + k * kBlock, nel);
+                    uint32_t lo = 0, hi = nr0;
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_rs[mid] <= j) lo = mid;
+                        else hi = mid;
+                    }
+                    val[k] = op.stage_edge(w.e0 + j, w.r0 + lo, src[k]);
+                }
+                __syncthreads(); // the row starts are written again below
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kPullIPT; k++) val[k] = op.gather(w.e0 + min(tid + k * kBlock, nel), src[k]);
+        }
     }
     if (w.kind == kRowLong) {
         A acc = op.identity();
