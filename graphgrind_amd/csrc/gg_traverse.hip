@@ -228,8 +228,17 @@ __global__ __launch_bounds__(kBlock) void k_bfs_pull(const uint32_t *__restrict_
         }
         next[d] = nx;
     }
+    // one counter add per work-group (not per wave): the striped counters
+    // take about 6.5e4 adds per level instead of 2.6e5
+    __shared__ unsigned long long s_scan[kBlock / 64];
     unsigned long long sc = wave_sum_u64(scanned);
-    if ((threadIdx.x & 63) == 0) cnt_add(counters, CNT_SCANNED, sc);
+    if ((threadIdx.x & 63) == 0) s_scan[threadIdx.x >> 6] = sc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < kBlock / 64; k++) t += s_scan[k];
+        cnt_add(counters, CNT_SCANNED, t);
+    }
     block_stats(st, counters);
 }
 
