@@ -464,11 +464,12 @@ struct OpSpmvY {
     const V *x;
     const float *tw;
     V *y;
+    const uint32_t *inv = nullptr; // relabelled twin: row d is vertex inv[d]
     __device__ bool skip() const { return false; }
     __device__ V identity() const { return V(0); }
     __device__ V combine(V a, V b) const { return a + b; }
     __device__ V gather(uint32_t e, uint32_t s) const { return tw ? x[s] * (V)tw[e] : x[s]; }
-    __device__ void finish(uint32_t d, V acc, LocalStats &) const { y[d] = acc; }
+    __device__ void finish(uint32_t d, V acc, LocalStats &) const { y[inv ? inv[d] : d] = acc; }
 };
 } // namespace gg
 
@@ -481,16 +482,39 @@ extern "C" int gg_spmv(gg_graph_t g, int dtype, const void *x, void *y, gg_run_i
     if (info) std::memset(info, 0, sizeof(*info));
     const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
     CscView pv;
-    GG_TRY(pull_view(g, &pv));
+    // With a renumbered plan the pull runs over the plan's relabelled twin
+    // (rows of similar in-degree together, hot sources together): x is
+    // permuted into the new numbering first and y scattered back per row.
+    const bool planned = g->plan.mode != 0 && g->m;
+    const uint32_t *inv = nullptr;
+    Buf xp;
+    if (planned) {
+        pv = plan_view(g);
+        if (g->weighted) tw = (const float *)g->plan.tw.p;
+        inv = (const uint32_t *)g->plan.inv.p;
+        GG_TRY(pool_take(c, xp, (dtype == GG_F64 ? 8ull : 4ull) * g->n));
+    } else {
+        GG_TRY(pull_view(g, &pv));
+    }
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    GG_HIP(hipEventRecord(e0, c->stream));
-    int rc = dtype == GG_F64
-                 ? launch_pull(c, pv, OpSpmvY<double>{(const double *)x, tw, (double *)y}, nullptr, nullptr, nullptr)
-                 : launch_pull(c, pv, OpSpmvY<float>{(const float *)x, tw, (float *)y}, nullptr, nullptr, nullptr);
-    GG_HIP(hipEventRecord(e1, c->stream));
+    (void)hipEventRecord(e0, c->stream);
+    int rc = GG_OK;
+    if (planned) {
+        rc = dtype == GG_F64 ? gather_idx(c, (const double *)x, inv, (double *)xp.p, g->n)
+                             : gather_idx(c, (const float *)x, inv, (float *)xp.p, g->n);
+        x = xp.p;
+    }
+    if (!rc)
+        rc = dtype == GG_F64
+                 ? launch_pull(c, pv, OpSpmvY<double>{(const double *)x, tw, (double *)y, inv}, nullptr, nullptr,
+                               nullptr)
+                 : launch_pull(c, pv, OpSpmvY<float>{(const float *)x, tw, (float *)y, inv}, nullptr, nullptr,
+                               nullptr);
+    (void)hipEventRecord(e1, c->stream);
     if (!rc) rc = sync(c);
+    pool_give(c, xp);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);

@@ -741,12 +741,16 @@ static void graph_free(Graph *g)
                    &g->indeg, &g->comp, &g->coo_src, &g->coo_dst, &g->coo_w, &g->pull.start_row,
                    &g->pull.start_edge, &g->pull.carry_row, &g->pull.carry_val, &g->pull.head_val,
                    &g->plan.perm, &g->plan.inv, &g->plan.toff, &g->plan.tids, &g->plan.tw, &g->plan.outdeg,
+                   &g->plan.coff, &g->plan.cids, &g->plan.cw, &g->plan.cdelta, &g->plan.csched.start_row,
+                   &g->plan.csched.start_edge, &g->plan.csched.carry_row, &g->plan.csched.carry_val,
+                   &g->plan.csched.head_val,
                    &g->plan.sched.start_row, &g->plan.sched.start_edge, &g->plan.sched.carry_row,
                    &g->plan.sched.carry_val, &g->plan.sched.head_val, &g->csr_pull.start_row,
                    &g->csr_pull.start_edge, &g->csr_pull.carry_row, &g->csr_pull.carry_val,
                    &g->csr_pull.head_val, &g->wsum};
     for (Buf *b : bufs) buf_free(*b);
     rows_free(g->plan.rows);
+    rows_free(g->plan.crows);
     rows_free(g->rows);
     rows_free(g->csr_rows);
     g->rows_ok = false;
@@ -1513,11 +1517,15 @@ extern "C" int gg_graph_reorder(gg_graph_t g, int mode)
     if (mode != 0 && mode != 1) return fail(GG_ERR_INVALID, "gg_graph_reorder: mode must be 0 or 1");
     Ctx *c = g->ctx;
     GG_HIP(hipSetDevice(c->device));
+    graph_exec_drop(g); // captured loops may hold the plan's arrays
     PullPlan &P = g->plan;
     Buf *pb[] = {&P.perm, &P.inv, &P.toff, &P.tids, &P.tw, &P.outdeg, &P.sched.start_row, &P.sched.start_edge,
-                 &P.sched.carry_row, &P.sched.carry_val, &P.sched.head_val};
+                 &P.sched.carry_row, &P.sched.carry_val, &P.sched.head_val, &P.coff, &P.cids, &P.cw, &P.cdelta,
+                 &P.csched.start_row, &P.csched.start_edge, &P.csched.carry_row, &P.csched.carry_val,
+                 &P.csched.head_val};
     for (Buf *b : pb) buf_free(*b);
     rows_free(P.rows);
+    rows_free(P.crows);
     P.mode = 0;
     if (mode == 0) return GG_OK;
     const uint32_t n = g->n;
@@ -1592,6 +1600,86 @@ extern "C" int gg_graph_reorder(gg_graph_t g, int mode)
 }
 
 namespace gg {
+// Edge e of source v (found by binary search in off) moves to
+// coff[perm[v]] + (e - off[v]) with its head renamed and its weight.
+__global__ void k_plan_csr_edges(const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
+                                 const float *__restrict__ w, const uint32_t *__restrict__ perm,
+                                 const uint32_t *__restrict__ coff, uint32_t n, uint64_t m,
+                                 uint32_t *__restrict__ cids, float *__restrict__ cw)
+{
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; e < m; e += stride) {
+        uint32_t lo = 0, hi = n; // largest v with off[v] <= e
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (off[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t to = coff[perm[lo]] + (uint32_t)(e - off[lo]);
+        cids[to] = perm[ids[e]];
+        if (w) cw[to] = w[e];
+    }
+}
+
+// cdelta[k] = off[inv[k]] - coff[k] (mod 2^32)
+__global__ void k_plan_csr_delta(const uint32_t *__restrict__ off, const uint32_t *__restrict__ inv,
+                                 const uint32_t *__restrict__ coff, uint32_t n, uint32_t *__restrict__ cdelta)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) cdelta[k] = off[inv[k]] - coff[k];
+}
+
+int plan_csr_view(Graph *g, CscView *v)
+{
+    PullPlan &P = g->plan;
+    if (!P.mode) return fail(GG_ERR_STATE, "no renumbered plan");
+    Ctx *c = g->ctx;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    if (!P.coff.p) {
+        Buf coff, cids, cw, cdelta;
+        int rc = buf_alloc(c, coff, 4ull * (n + 1));
+        if (!rc) rc = buf_alloc(c, cids, 4 * (m ? m : 1));
+        if (!rc && g->weighted) rc = buf_alloc(c, cw, 4 * (m ? m : 1));
+        if (!rc) rc = buf_alloc(c, cdelta, 4ull * n);
+        if (!rc) rc = scan_offsets(c, (const uint32_t *)P.outdeg.p, (uint32_t *)coff.p, n);
+        if (!rc && m) {
+            const uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+            k_plan_csr_edges<<<grid, kBlock, 0, c->stream>>>(
+                (const uint32_t *)g->off.p, (const uint32_t *)g->ids.p, (const float *)(g->weighted ? g->w.p : nullptr),
+                (const uint32_t *)P.perm.p, (const uint32_t *)coff.p, n, m, (uint32_t *)cids.p, (float *)cw.p);
+        }
+        if (!rc && n)
+            k_plan_csr_delta<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
+                (const uint32_t *)g->off.p, (const uint32_t *)P.inv.p, (const uint32_t *)coff.p, n,
+                (uint32_t *)cdelta.p);
+        if (!rc && hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "plan CSR kernels failed to launch");
+        if (!rc) rc = sched_build(c, P.csched, (const uint32_t *)coff.p, n, m);
+        if (!rc) rc = sync(c);
+        if (rc) {
+            Buf *bs[] = {&coff, &cids, &cw, &cdelta};
+            for (Buf *b : bs) buf_free(*b);
+            return rc;
+        }
+        P.coff = coff;
+        P.cids = cids;
+        P.cw = cw;
+        P.cdelta = cdelta;
+        coff = cids = cw = cdelta = Buf{};
+    }
+    if (g->pull_schedule == 1 && !P.crows.blocks && n)
+        GG_TRY(rows_build(c, P.crows, (const uint32_t *)P.coff.p, n, m, g->edges_per_lane));
+    *v = CscView{};
+    v->toff = (const uint32_t *)P.coff.p;
+    v->tids = (const uint32_t *)P.cids.p;
+    v->sched = &P.csched;
+    if (g->pull_schedule == 1) v->rows = &P.crows;
+    return GG_OK;
+}
+} // namespace gg
+
+namespace gg {
 // inv[perm[k]] = k
 __global__ void k_invert_perm(const uint32_t *__restrict__ perm, uint32_t *__restrict__ inv, uint64_t m)
 {
@@ -1658,6 +1746,7 @@ extern "C" int gg_graph_set_option(gg_graph_t g, int option, int64_t value)
             graph_exec_drop(g);
             rows_free(g->rows);
             rows_free(g->csr_rows);
+            rows_free(g->plan.crows); // rebuilt on first use
             g->rows_ok = false;
             g->csr_rows_ok = false;
         }

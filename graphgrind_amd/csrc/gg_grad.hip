@@ -103,23 +103,27 @@ struct OpSpmvBwdX {
     __device__ bool skip() const { return false; }
     __device__ V identity() const { return V(0); }
     __device__ V combine(V a, V b) const { return a + b; }
+    const uint32_t *inv = nullptr; // relabelled CSR: row s is vertex inv[s]
     __device__ V gather(uint32_t e, uint32_t d) const
     {
         return w ? (V)__builtin_nontemporal_load(&w[e]) * gy[d] : gy[d];
     }
-    __device__ void finish(uint32_t s, V acc, LocalStats &) const { gx[s] = acc; }
+    __device__ void finish(uint32_t s, V acc, LocalStats &) const { gx[inv ? inv[s] : s] = acc; }
 };
 
 // Both outputs in one pass over the out-edge CSR (row-block schedule):
 // while staging edge e = (s -> d), gw[e] = x[s] gy[d] is written (coalesced,
 // in edge order) and w_e gy[d] is staged for gx[s] (the same products in the
 // same order as OpSpmvBwdX), so the random gy gathers are made once for both.
+// Over the plan's relabelled CSR (x and gy in the new numbering), edge e of
+// row s is the caller's edge e + delta[s] and row s is vertex inv[s].
 template <class V>
 struct OpSpmvBwdFused {
     using acc_t = V;
     const float *w; // NULL: unit weights
     const V *gy, *x;
     V *gx, *gw;
+    const uint32_t *delta = nullptr, *inv = nullptr;
     __device__ bool skip() const { return false; }
     __device__ V identity() const { return V(0); }
     __device__ V combine(V a, V b) const { return a + b; }
@@ -127,10 +131,10 @@ struct OpSpmvBwdFused {
     __device__ V stage_edge(uint32_t e, uint32_t s, uint32_t d) const
     {
         const V g = gy[d];
-        __builtin_nontemporal_store(x[s] * g, &gw[e]);
+        __builtin_nontemporal_store(x[s] * g, &gw[delta ? e + delta[s] : e]);
         return w ? (V)__builtin_nontemporal_load(&w[e]) * g : g;
     }
-    __device__ void finish(uint32_t s, V acc, LocalStats &) const { gx[s] = acc; }
+    __device__ void finish(uint32_t s, V acc, LocalStats &) const { gx[inv ? inv[s] : s] = acc; }
 };
 
 // gw[e] = x[s] gy[d]
@@ -426,18 +430,73 @@ __global__ void k_copy(V *__restrict__ out, const V *__restrict__ in, uint32_t n
     if (i < n) out[i] = in[i];
 }
 
+// The numbering a weighted PageRank solve runs in: the graph's own, or the
+// renumbered plan (its relabelled in-edge twin for the forward, its
+// relabelled CSR for the adjoint: rows of similar degree together, hot
+// sources together), with vertex vectors permuted in and results back.
+struct WprSpace {
+    CscView view;
+    const float *w = nullptr;                       // weights in the view's edge order
+    const uint32_t *perm = nullptr, *inv = nullptr; // set when planned
+    bool planned() const { return perm != nullptr; }
+};
+
+static int wpr_space(Graph *g, bool adjoint, WprSpace *sp)
+{
+    *sp = WprSpace{};
+    if (g->plan.mode && g->m) {
+        if (adjoint) {
+            GG_TRY(plan_csr_view(g, &sp->view));
+            sp->w = (const float *)g->plan.cw.p;
+        } else {
+            sp->view = plan_view(g);
+            sp->w = (const float *)g->plan.tw.p;
+        }
+        sp->perm = (const uint32_t *)g->plan.perm.p;
+        sp->inv = (const uint32_t *)g->plan.inv.p;
+        return GG_OK;
+    }
+    if (adjoint) {
+        GG_TRY(csr_pull_view(g, &sp->view));
+        sp->w = (const float *)g->w.p;
+    } else {
+        GG_TRY(pull_view(g, &sp->view));
+        sp->w = (const float *)g->tw.p;
+    }
+    return GG_OK;
+}
+
+// The weight sums and one vertex vector x (may be NULL) in the space's
+// numbering: the graph's own arrays, or permuted copies in bs / bx (n
+// doubles / n values, allocated by the caller when planned).
+template <class V>
+static int wpr_inputs(Graph *g, const WprSpace &sp, const V *x, const Buf &bs, const Buf &bx, const double **S,
+                      const V **xs)
+{
+    *S = (const double *)g->wsum.p;
+    *xs = x;
+    if (!sp.planned()) return GG_OK;
+    GG_TRY(gather_idx(g->ctx, (const double *)g->wsum.p, sp.inv, (double *)bs.p, g->n));
+    *S = (const double *)bs.p;
+    if (x) {
+        GG_TRY(gather_idx(g->ctx, x, sp.inv, (V *)bx.p, g->n));
+        *xs = (const V *)bx.p;
+    }
+    return GG_OK;
+}
+
 template <class V, bool kV>
-static OpWPR<V, kV> wpr_op(Graph *g, const Buf *K, int it, V *r, const V *v, V vconst, double alpha,
-                           const int *done)
+static OpWPR<V, kV> wpr_op(const WprSpace &sp, const double *S, const Buf *K, int it, V *r, const V *v, V vconst,
+                           double alpha, const int *done)
 {
     OpWPR<V, kV> op;
-    op.tw = (const float *)g->tw.p;
+    op.tw = sp.w;
     op.cin = (const V *)K[(it - 1) % 2].p;
     op.cout = (V *)K[it % 2].p;
     op.r = r;
     op.v = v;
     op.vconst = vconst;
-    op.S = (const double *)g->wsum.p;
+    op.S = S;
     op.alpha = (V)alpha;
     op.beta = (V)(1 - alpha);
     op.done = done;
@@ -445,14 +504,15 @@ static OpWPR<V, kV> wpr_op(Graph *g, const Buf *K, int it, V *r, const V *v, V v
 }
 
 template <class V>
-static OpWPRAdj<V> adj_op(Graph *g, const Buf *L, int it, const V *u, double alpha, const int *done)
+static OpWPRAdj<V> adj_op(const WprSpace &sp, const double *S, const Buf *L, int it, const V *u, double alpha,
+                          const int *done)
 {
     OpWPRAdj<V> op;
-    op.w = (const float *)g->w.p;
+    op.w = sp.w;
     op.lin = (const V *)L[(it - 1) % 2].p;
     op.lout = (V *)L[it % 2].p;
     op.u = u;
-    op.S = (const double *)g->wsum.p;
+    op.S = S;
     op.alpha = (V)alpha;
     op.done = done;
     return op;
@@ -519,7 +579,7 @@ static void fixed_info(Ctx *c, float ms, int iters, uint64_t m, gg_run_info *inf
 // Exactly 'iters' forward iterations (no stop rule), captured once per
 // (buffers, parameters) and replayed as one hipGraph launch.
 template <class V>
-static int wpagerank_fixed(Graph *g, const CscView &view, double alpha, int iters, const V *v, V *r,
+static int wpagerank_fixed(Graph *g, const WprSpace &sp, double alpha, int iters, const V *v, V *r,
                            gg_run_info *info)
 {
     Ctx *c = g->ctx;
@@ -527,25 +587,36 @@ static int wpagerank_fixed(Graph *g, const CscView &view, double alpha, int iter
     const V vconst = (V)(1 / (double)n);
     GraphExec &X = g->fwd_exec;
     const uint64_t key[10] = {1, sizeof(V), (uint64_t)(uintptr_t)r, (uint64_t)(uintptr_t)v, dbits(alpha),
-                              (uint64_t)iters, 0, 0, 0, 0};
+                              (uint64_t)iters, sp.planned() ? 1u : 0u, 0, 0, 0};
     if (!X.exec || std::memcmp(X.key, key, sizeof key) != 0) {
         graph_exec_free(X);
+        // own: [0], [1] contributions, [2] flags; planned: [3] ranks,
+        // [4] weight sums, [5] teleport vector in the plan's numbering
         GG_TRY(buf_alloc(c, X.own[0], sizeof(V) * (uint64_t)n));
         GG_TRY(buf_alloc(c, X.own[1], sizeof(V) * (uint64_t)n));
         GG_TRY(buf_alloc(c, X.own[2], 64));
+        if (sp.planned()) {
+            GG_TRY(buf_alloc(c, X.own[3], sizeof(V) * (uint64_t)n));
+            GG_TRY(buf_alloc(c, X.own[4], 8ull * n));
+            if (v) GG_TRY(buf_alloc(c, X.own[5], sizeof(V) * (uint64_t)n));
+        }
         GG_TRY(fill_bytes(c, X.own[2].p, 64, 0));
         GG_TRY(sync(c));
         const int *done = (const int *)X.own[2].p;
         auto body = [&]() -> int {
-            k_wpr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(r, (V *)X.own[0].p, v, vconst,
-                                                                           (const double *)g->wsum.p, n);
+            const double *S;
+            const V *vv;
+            GG_TRY(wpr_inputs(g, sp, v, X.own[4], X.own[5], &S, &vv));
+            V *rr = sp.planned() ? (V *)X.own[3].p : r;
+            k_wpr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(rr, (V *)X.own[0].p, vv, vconst, S, n);
             for (int it = 1; it <= iters; it++) {
-                const int rc = v ? launch_pull(c, view, wpr_op<V, true>(g, X.own, it, r, v, vconst, alpha, done),
+                const int rc = v ? launch_pull(c, sp.view, wpr_op<V, true>(sp, S, X.own, it, rr, vv, vconst, alpha, done),
                                                nullptr, nullptr, nullptr)
-                                 : launch_pull(c, view, wpr_op<V, false>(g, X.own, it, r, v, vconst, alpha, done),
+                                 : launch_pull(c, sp.view, wpr_op<V, false>(sp, S, X.own, it, rr, vv, vconst, alpha, done),
                                                nullptr, nullptr, nullptr);
                 if (rc) return rc;
             }
+            if (sp.planned()) GG_TRY(gather_idx(c, (const V *)rr, sp.perm, r, n));
             return GG_OK;
         };
         const int rc = capture(c, body, X);
@@ -567,31 +638,47 @@ static int wpagerank_impl(Graph *g, double alpha, double eps, int max_iter, cons
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
     GG_TRY(ensure_wsum(g));
-    CscView view;
-    GG_TRY(pull_view(g, &view));
-    if (eps < 0) return wpagerank_fixed<V>(g, view, alpha, max_iter, v, r, info);
+    WprSpace sp;
+    GG_TRY(wpr_space(g, false, &sp));
+    if (eps < 0) return wpagerank_fixed<V>(g, sp, alpha, max_iter, v, r, info);
     SolveEvents E(max_iter);
-    GG_HIP(hipEventRecord(E.ev[0], c->stream));
-    Buf K[2];
+    Buf K[2], T[3]; // T (planned): ranks, weight sums, teleport vector in the plan's numbering
     int rc = pool_take(c, K[0], sizeof(V) * (uint64_t)n);
     if (!rc) rc = pool_take(c, K[1], sizeof(V) * (uint64_t)n);
+    if (!rc && sp.planned()) {
+        rc = pool_take(c, T[0], sizeof(V) * (uint64_t)n);
+        if (!rc) rc = pool_take(c, T[1], 8ull * n);
+        if (!rc && v) rc = pool_take(c, T[2], sizeof(V) * (uint64_t)n);
+    }
     const V vconst = (V)(1 / (double)n);
     if (!rc) {
-        k_wpr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(r, (V *)K[0].p, v, vconst,
-                                                                       (const double *)g->wsum.p, n);
+        (void)hipEventRecord(E.ev[0], c->stream);
+        const double *S = nullptr;
+        const V *vv = nullptr;
+        rc = wpr_inputs(g, sp, v, T[1], T[2], &S, &vv);
+        V *rr = sp.planned() ? (V *)T[0].p : r;
         int iters = 0;
-        auto make_v = [&](int it, const int *done) { return wpr_op<V, true>(g, K, it, r, v, vconst, alpha, done); };
-        auto make_u = [&](int it, const int *done) { return wpr_op<V, false>(g, K, it, r, v, vconst, alpha, done); };
-        rc = v ? run_fixed_point(g, view, max_iter, 0, eps, make_v, E, &iters, nullptr)
-               : run_fixed_point(g, view, max_iter, 0, eps, make_u, E, &iters, nullptr);
         if (!rc) {
-            GG_HIP(hipEventRecord(E.ev[1], c->stream));
+            k_wpr_init<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(rr, (V *)K[0].p, vv, vconst, S, n);
+            auto make_v = [&](int it, const int *done) {
+                return wpr_op<V, true>(sp, S, K, it, rr, vv, vconst, alpha, done);
+            };
+            auto make_u = [&](int it, const int *done) {
+                return wpr_op<V, false>(sp, S, K, it, rr, vv, vconst, alpha, done);
+            };
+            rc = v ? run_fixed_point(g, sp.view, max_iter, 0, eps, make_v, E, &iters, nullptr)
+                   : run_fixed_point(g, sp.view, max_iter, 0, eps, make_u, E, &iters, nullptr);
+        }
+        if (!rc && sp.planned()) rc = gather_idx(c, (const V *)rr, sp.perm, r, n);
+        if (!rc) {
+            (void)hipEventRecord(E.ev[1], c->stream);
             rc = sync(c);
         }
         if (!rc) record_times(c, E, iters, info, g->m);
     }
     pool_give(c, K[0]);
     pool_give(c, K[1]);
+    for (Buf &b : T) pool_give(c, b);
     return rc;
 }
 
@@ -618,7 +705,7 @@ static int adjoint_outputs(Graph *g, double alpha, const V *r, const V *u, const
 }
 
 template <class V>
-static int wpagerank_adjoint_fixed(Graph *g, const CscView &cv, double alpha, int iters, const V *r, const V *u,
+static int wpagerank_adjoint_fixed(Graph *g, const WprSpace &sp, double alpha, int iters, const V *r, const V *u,
                                    V *gw, V *gv, V *lam_out, gg_run_info *info)
 {
     Ctx *c = g->ctx;
@@ -626,20 +713,36 @@ static int wpagerank_adjoint_fixed(Graph *g, const CscView &cv, double alpha, in
     GraphExec &X = g->adj_exec;
     const uint64_t key[10] = {2, sizeof(V), (uint64_t)(uintptr_t)r, (uint64_t)(uintptr_t)u,
                               (uint64_t)(uintptr_t)gw, (uint64_t)(uintptr_t)gv, (uint64_t)(uintptr_t)lam_out,
-                              dbits(alpha), (uint64_t)iters, 0};
+                              dbits(alpha), (uint64_t)iters, sp.planned() ? 1u : 0u};
     if (!X.exec || std::memcmp(X.key, key, sizeof key) != 0) {
         graph_exec_free(X);
+        // own: [0], [1] adjoint iterates, [2] flags; planned: [3] weight
+        // sums, [4] seed vector u, [5] the solution in the caller's numbering
         GG_TRY(buf_alloc(c, X.own[0], sizeof(V) * (uint64_t)n));
         GG_TRY(buf_alloc(c, X.own[1], sizeof(V) * (uint64_t)n));
         GG_TRY(buf_alloc(c, X.own[2], 64));
+        if (sp.planned()) {
+            GG_TRY(buf_alloc(c, X.own[3], 8ull * n));
+            GG_TRY(buf_alloc(c, X.own[4], sizeof(V) * (uint64_t)n));
+            GG_TRY(buf_alloc(c, X.own[5], sizeof(V) * (uint64_t)n));
+        }
         GG_TRY(fill_bytes(c, X.own[2].p, 64, 0));
         GG_TRY(sync(c));
         const int *done = (const int *)X.own[2].p;
         auto body = [&]() -> int {
-            k_copy<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)X.own[0].p, u, n);
+            const double *S;
+            const V *uu;
+            GG_TRY(wpr_inputs(g, sp, u, X.own[3], X.own[4], &S, &uu));
+            k_copy<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)X.own[0].p, uu, n);
             for (int it = 1; it <= iters; it++)
-                GG_TRY(launch_pull(c, cv, adj_op<V>(g, X.own, it, u, alpha, done), nullptr, nullptr, nullptr));
-            return adjoint_outputs<V>(g, alpha, r, u, (const V *)X.own[iters % 2].p, gw, gv, lam_out);
+                GG_TRY(launch_pull(c, sp.view, adj_op<V>(sp, S, X.own, it, uu, alpha, done), nullptr, nullptr,
+                                   nullptr));
+            const V *lam = (const V *)X.own[iters % 2].p;
+            if (sp.planned()) {
+                GG_TRY(gather_idx(c, lam, sp.perm, (V *)X.own[5].p, n));
+                lam = (const V *)X.own[5].p;
+            }
+            return adjoint_outputs<V>(g, alpha, r, u, lam, gw, gv, lam_out);
         };
         const int rc = capture(c, body, X);
         if (rc) {
@@ -661,29 +764,45 @@ static int wpagerank_adjoint_impl(Graph *g, double alpha, double eps, int max_it
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
     GG_TRY(ensure_wsum(g));
-    GG_TRY(ensure_csr_sched(g));
-    CscView cv;
-    GG_TRY(csr_pull_view(g, &cv));
-    if (eps < 0) return wpagerank_adjoint_fixed<V>(g, cv, alpha, max_iter, r, u, gw, gv, lam_out, info);
+    GG_TRY(ensure_csr_sched(g)); // the dL/dw pass runs over the caller's CSR
+    WprSpace sp;
+    GG_TRY(wpr_space(g, true, &sp));
+    if (eps < 0) return wpagerank_adjoint_fixed<V>(g, sp, alpha, max_iter, r, u, gw, gv, lam_out, info);
     SolveEvents E(max_iter);
-    GG_HIP(hipEventRecord(E.ev[0], c->stream));
-    Buf L[2];
+    Buf L[2], T[3]; // T (planned): weight sums, u, the solution in the caller's numbering
     int rc = pool_take(c, L[0], sizeof(V) * (uint64_t)n);
     if (!rc) rc = pool_take(c, L[1], sizeof(V) * (uint64_t)n);
+    if (!rc && sp.planned()) {
+        rc = pool_take(c, T[0], 8ull * n);
+        if (!rc) rc = pool_take(c, T[1], sizeof(V) * (uint64_t)n);
+        if (!rc) rc = pool_take(c, T[2], sizeof(V) * (uint64_t)n);
+    }
     if (!rc) {
-        k_copy<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)L[0].p, u, n);
+        (void)hipEventRecord(E.ev[0], c->stream);
+        const double *S = nullptr;
+        const V *uu = nullptr;
+        rc = wpr_inputs(g, sp, u, T[0], T[1], &S, &uu);
         int iters = 0;
-        auto make = [&](int it, const int *done) { return adj_op<V>(g, L, it, u, alpha, done); };
-        rc = run_fixed_point(g, cv, max_iter, 1, eps, make, E, &iters, nullptr);
-        if (!rc) rc = adjoint_outputs<V>(g, alpha, r, u, (const V *)L[iters % 2].p, gw, gv, lam_out);
         if (!rc) {
-            GG_HIP(hipEventRecord(E.ev[1], c->stream));
+            k_copy<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((V *)L[0].p, uu, n);
+            auto make = [&](int it, const int *done) { return adj_op<V>(sp, S, L, it, uu, alpha, done); };
+            rc = run_fixed_point(g, sp.view, max_iter, 1, eps, make, E, &iters, nullptr);
+        }
+        const V *lam = (const V *)L[iters % 2].p;
+        if (!rc && sp.planned()) {
+            rc = gather_idx(c, lam, sp.perm, (V *)T[2].p, n);
+            lam = (const V *)T[2].p;
+        }
+        if (!rc) rc = adjoint_outputs<V>(g, alpha, r, u, lam, gw, gv, lam_out);
+        if (!rc) {
+            (void)hipEventRecord(E.ev[1], c->stream);
             rc = sync(c);
         }
         if (!rc) record_times(c, E, iters, info, g->m);
     }
     pool_give(c, L[0]);
     pool_give(c, L[1]);
+    for (Buf &b : T) pool_give(c, b);
     return rc;
 }
 
@@ -710,38 +829,71 @@ extern "C" int gg_spmv_bwd(gg_graph_t g, int dtype, const void *x, const void *g
     CscView cv;
     GG_TRY(csr_pull_view(g, &cv));
     const float *w = g->weighted ? (const float *)g->w.p : nullptr;
+    // both outputs from one pass when the row-block schedule runs the pull
+    const bool fused = gx && gw && g->m && cv.rows;
+    const bool f64 = dtype == GG_F64;
+    const size_t vs = f64 ? 8 : 4;
+    // With a renumbered plan the dL/dx pull runs over the plan's relabelled
+    // CSR, with x and gy permuted into the new numbering first and gx
+    // scattered back; the separate dL/dw pass stays on the caller's CSR.
+    const bool planned = g->plan.mode != 0 && g->m && gx;
+    const void *gy_in = gy, *x_in = x;
+    Buf gyp, xp;
+    const uint32_t *delta = nullptr, *inv = nullptr;
+    if (planned) {
+        GG_TRY(plan_csr_view(g, &cv));
+        if (g->weighted) w = (const float *)g->plan.cw.p;
+        inv = (const uint32_t *)g->plan.inv.p;
+        delta = (const uint32_t *)g->plan.cdelta.p;
+        GG_TRY(pool_take(c, gyp, vs * g->n));
+        if (fused) {
+            int rt = pool_take(c, xp, vs * g->n);
+            if (rt) {
+                pool_give(c, gyp);
+                return rt;
+            }
+        }
+    }
+    const bool fused_now = fused && (!planned || cv.rows);
     hipEvent_t e0, e1, k0, k1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventCreate(&k0);
     (void)hipEventCreate(&k1);
     int rc = GG_OK;
-    GG_HIP(hipEventRecord(e0, c->stream));
-    // both outputs from one pass when the row-block schedule runs the pull
-    const bool fused = gx && gw && g->m && cv.rows;
-    if (fused) {
-        rc = dtype == GG_F64
-                 ? launch_pull(c, cv,
+    (void)hipEventRecord(e0, c->stream);
+    if (planned) {
+        rc = f64 ? gather_idx(c, (const double *)gy_in, inv, (double *)gyp.p, g->n)
+                 : gather_idx(c, (const float *)gy_in, inv, (float *)gyp.p, g->n);
+        if (!rc && fused_now)
+            rc = f64 ? gather_idx(c, (const double *)x_in, inv, (double *)xp.p, g->n)
+                     : gather_idx(c, (const float *)x_in, inv, (float *)xp.p, g->n);
+        gy = gyp.p;
+        if (fused_now) x = xp.p;
+    }
+    if (!rc && fused_now) {
+        rc = f64 ? launch_pull(c, cv,
                                OpSpmvBwdFused<double>{w, (const double *)gy, (const double *)x, (double *)gx,
-                                                      (double *)gw},
+                                                      (double *)gw, delta, inv},
                                nullptr, nullptr, nullptr, k0, k1)
                  : launch_pull(c, cv,
                                OpSpmvBwdFused<float>{w, (const float *)gy, (const float *)x, (float *)gx,
-                                                     (float *)gw},
+                                                     (float *)gw, delta, inv},
                                nullptr, nullptr, nullptr, k0, k1);
-    } else if (gx) {
-        rc = dtype == GG_F64
-                 ? launch_pull(c, cv, OpSpmvBwdX<double>{w, (const double *)gy, (double *)gx}, nullptr,
+    } else if (!rc && gx) {
+        rc = f64 ? launch_pull(c, cv, OpSpmvBwdX<double>{w, (const double *)gy, (double *)gx, inv}, nullptr,
                                nullptr, nullptr, k0, k1)
-                 : launch_pull(c, cv, OpSpmvBwdX<float>{w, (const float *)gy, (float *)gx}, nullptr,
+                 : launch_pull(c, cv, OpSpmvBwdX<float>{w, (const float *)gy, (float *)gx, inv}, nullptr,
                                nullptr, nullptr, k0, k1);
     }
-    if (!rc && gw && g->m && !fused) {
-        rc = dtype == GG_F64 ? launch_edge(c, g, EdgeSpmvGW<double>{(const double *)x, (const double *)gy, (double *)gw})
-                             : launch_edge(c, g, EdgeSpmvGW<float>{(const float *)x, (const float *)gy, (float *)gw});
+    if (!rc && gw && g->m && !fused_now) {
+        rc = f64 ? launch_edge(c, g, EdgeSpmvGW<double>{(const double *)x_in, (const double *)gy_in, (double *)gw})
+                 : launch_edge(c, g, EdgeSpmvGW<float>{(const float *)x_in, (const float *)gy_in, (float *)gw});
     }
-    GG_HIP(hipEventRecord(e1, c->stream));
+    (void)hipEventRecord(e1, c->stream);
     if (!rc) rc = sync(c);
+    pool_give(c, gyp);
+    pool_give(c, xp);
     float ms = 0, kms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     if (gx) (void)hipEventElapsedTime(&kms, k0, k1);

@@ -108,6 +108,14 @@ struct PullPlan {
     Buf outdeg;          // uint32[n] in the new numbering
     PullSched sched;
     RowSched rows;
+    // Out-edge CSR in the new numbering, built on first use by the
+    // transpose-side kernels: row perm[v] holds v's out-edges in their
+    // original order (heads renamed), so the caller's edge e of source v sits
+    // at coff[perm[v]] + (e - off[v]); cdelta[perm[v]] = off[v] - coff[perm[v]]
+    // (mod 2^32) maps it back.
+    Buf coff, cids, cw, cdelta;
+    PullSched csched;
+    RowSched crows;
 };
 
 // In-edge arrays + schedule a pull runs over: the row-block schedule when
@@ -138,7 +146,7 @@ struct GraphExec {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t key[10] = {};
-    Buf own[4];
+    Buf own[6];
 };
 void graph_exec_free(GraphExec &x);
 
@@ -248,6 +256,10 @@ struct Frontier {
     Buf bitmap; // uint32[(n+31)/32]
     Buf list;   // uint32[cap]
 };
+
+// The relabelled out-edge CSR of the graph's plan (built on first use), as
+// a pull view with the graph's schedule kind; requires a plan.
+int plan_csr_view(Graph *g, CscView *v);
 
 // ---- shared device helpers (defined in gg_core.hip) ------------------------
 int sched_build(Ctx *c, PullSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges);

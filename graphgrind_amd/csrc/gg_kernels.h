@@ -1,6 +1,7 @@
 // gg_kernels.h -- device kernel templates shared by the traversal and app
 // translation units.  Included by .hip files only.
 #pragma once
+#include <algorithm>
 #include <type_traits>
 
 #include "gg_internal.h"
@@ -41,6 +42,27 @@ __device__ __forceinline__ void cnt_add(unsigned long long *counters, int which,
 // Sums the stripes into counters[0..2] and clears them (one work-group of
 // kStripes threads).
 __global__ void k_counters_fold(unsigned long long *counters);
+
+// dst[i] = src[idx[i]] for i < count (a permutation of a vertex vector, or
+// ids renamed through a permutation).
+template <class T>
+__global__ void k_gather_idx(const T *__restrict__ src, const uint32_t *__restrict__ idx, T *__restrict__ dst,
+                             uint64_t count)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) dst[i] = src[__builtin_nontemporal_load(&idx[i])];
+}
+
+template <class T>
+inline int gather_idx(Ctx *c, const T *src, const uint32_t *idx, T *dst, uint64_t count)
+{
+    if (!count) return GG_OK;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((count + kBlock - 1) / kBlock, 1u << 20);
+    k_gather_idx<T><<<grid, kBlock, 0, c->stream>>>(src, idx, dst, count);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 {

@@ -98,6 +98,68 @@ def test_spmv_bwd_ragged_rows(ctx):
     g.close()
 
 
+@pytest.mark.parametrize("scale", [12, 17])
+@pytest.mark.parametrize("schedule", [1, 0])
+def test_spmv_renumbered_plan(ctx, scale, schedule):
+    """With the hot-sources-first plan, the forward pulls over the relabelled
+    in-edge twin and the backward over the relabelled CSR (x, gy permuted in,
+    y, gx scattered back, dL/dw written at the caller's edge positions).
+    Only the summation order changes: oracle bars as without the plan, dL/dw
+    exact in FP64, bitwise run to run, gx alone == gx of the fused pass, and
+    dropping the plan restores the plain results bit for bit."""
+    print(f"spmv plan scale={scale} schedule={schedule}", flush=True)
+    g, off, ids, w = _weighted(ctx, scale, 3)
+    g.set_option(gg.OPT_PULL_SCHEDULE, schedule)
+    n = g.n
+    x = oracle.uniform_pm1(3, n)
+    gy = np.random.default_rng(scale).standard_normal(n)
+    want_y = oracle.spmv_csr(off, ids, w, x)
+    want_gx, want_gw = oracle.spmv_bwd(off, ids, w, x, gy)
+    for dt, tol in ((np.float64, TOL64), (np.float32, TOL32)):
+        xd = gg.DeviceArray.from_host(ctx, x.astype(dt))
+        gyd = gg.DeviceArray.from_host(ctx, gy.astype(dt))
+        outs = []
+        for mode in (0, 1, 1, 0):
+            g.reorder(mode)
+            y, _ = gg.spmv(g, xd)
+            gx, gw, _ = gg.spmv_bwd(g, xd, gyd)
+            gx2, _, _ = gg.spmv_bwd(g, None, gyd, want_gw=False)
+            outs.append([a.to_host() for a in (y, gx, gw, gx2)])
+        for a, b in zip(outs[0], outs[3]):
+            assert np.array_equal(a, b), dt
+        for a, b in zip(outs[1], outs[2]):
+            assert np.array_equal(a, b), dt
+        y1, gx1, gw1, gx21 = outs[1]
+        assert np.array_equal(gx1, gx21)
+        assert _rel(y1, want_y) < tol and _rel(gx1, want_gx) < tol
+        if dt == np.float64:
+            assert np.array_equal(gw1[: g.m], want_gw)
+        else:
+            assert np.array_equal(gw1[: g.m], outs[0][2][: g.m])  # same FP32 products
+    g.close()
+
+
+def test_spmv_plan_ragged_rows(ctx):
+    """The relabelled CSR's edge-position map across empty rows, a row far
+    longer than a work item and a tail of isolated vertices."""
+    n = 5000
+    src = np.concatenate([np.zeros(4000, np.int64), np.full(3000, 7), np.arange(10, 2000)])
+    dst = np.concatenate([np.arange(1, 4001), np.arange(100, 3100), np.arange(11, 2001) % n])
+    off, ids = oracle.csr_from_edges(n, src, dst)
+    w = oracle.weights(9, len(ids))
+    g = gg.Graph.from_csr(ctx, off, ids, w=w).reorder(1)
+    x = oracle.uniform_pm1(5, n)
+    gy = oracle.uniform_pm1(6, n)
+    gx_w, gw_w = oracle.spmv_bwd(off, ids, w, x, gy)
+    xd = gg.DeviceArray.from_host(ctx, x)
+    gx, gw, _ = gg.spmv_bwd(g, xd, gg.DeviceArray.from_host(ctx, gy))
+    assert _rel(gx.to_host(), gx_w) < TOL64
+    assert np.array_equal(gw.to_host()[: g.m], gw_w)
+    y, _ = gg.spmv(g, xd)
+    assert _rel(y.to_host(), oracle.spmv_csr(off, ids, w, x)) < TOL64
+    g.close()
+
+
 def test_spmv_bwd_euler_identities_scale24(ctx):
     """At the full SpMV size (R-MAT scale 24, seed 3, FP32): y is linear in
     x and in w, so x.gx = w.gw = gy.y for any gy."""
@@ -119,10 +181,14 @@ def test_spmv_bwd_euler_identities_scale24(ctx):
 
 # --------------------------------------------------------- weighted PR --
 
+@pytest.mark.parametrize("plan", [0, 1])
 @pytest.mark.parametrize("scale", [8, 14, 18])
-def test_wpagerank_fp64_vs_oracle(ctx, scale):
-    print(f"wpagerank fp64 scale={scale}", flush=True)
+def test_wpagerank_fp64_vs_oracle(ctx, scale, plan):
+    """plan 1: the solve runs in the hot-sources-first numbering (relabelled
+    twin, vectors permuted in and out); only the summation order changes."""
+    print(f"wpagerank fp64 scale={scale} plan={plan}", flush=True)
     g, off, ids, w = _weighted(ctx, scale, 4)
+    g.reorder(plan)
     want, it, _ = oracle.wpagerank(off, ids, w, ALPHA, 1e-7, 100)
     r, info = gg.wpagerank(g, np.float64, ALPHA, 1e-7, 100)
     assert info.iters == it
@@ -130,8 +196,10 @@ def test_wpagerank_fp64_vs_oracle(ctx, scale):
     g.close()
 
 
-def test_wpagerank_personalised_and_fixed_iterations(ctx):
+@pytest.mark.parametrize("plan", [0, 1])
+def test_wpagerank_personalised_and_fixed_iterations(ctx, plan):
     g, off, ids, w = _weighted(ctx, 12, 4)
+    g.reorder(plan)
     n = g.n
     v = np.zeros(n)
     v[[3, 99, 1234, 4000]] = 0.25
@@ -150,12 +218,15 @@ def _targets(n, k, seed):
     return np.random.default_rng(seed).choice(n, min(k, n), replace=False)
 
 
+@pytest.mark.parametrize("plan", [0, 1])
 @pytest.mark.parametrize("scale", [8, 14, 18])
-def test_wpagerank_adjoint_fp64_vs_oracle(ctx, scale):
+def test_wpagerank_adjoint_fp64_vs_oracle(ctx, scale, plan):
     """Gradients w.r.t. the edge weights and the teleport vector, at the
-    GPU's own forward output, against the FP64 oracle."""
-    print(f"wpagerank adjoint fp64 scale={scale}", flush=True)
+    GPU's own forward output, against the FP64 oracle (plan 1: both solves
+    in the hot-sources-first numbering, over the relabelled twin and CSR)."""
+    print(f"wpagerank adjoint fp64 scale={scale} plan={plan}", flush=True)
     g, off, ids, w = _weighted(ctx, scale, 4)
+    g.reorder(plan)
     n = g.n
     r, _ = gg.wpagerank(g, np.float64, ALPHA, 1e-7, 100)
     rh = r.to_host()
@@ -176,13 +247,15 @@ def test_wpagerank_adjoint_fp64_vs_oracle(ctx, scale):
     g.close()
 
 
-def test_wpagerank_adjoint_identities_scale22(ctx):
+@pytest.mark.parametrize("plan", [0, 1])
+def test_wpagerank_adjoint_identities_scale22(ctx, plan):
     """The differentiable-PageRank shape (R-MAT scale 22, ef 16, seed 4,
     4096 targets, FP64): Euler's identities hold at any size --
     v.dL/dv = 2L (r is linear in v) and, per source, sum_e w_e dL/dw_e = 0
     (P does not change when one source's out-weights are scaled)."""
-    print("wpagerank identities scale=22", flush=True)
+    print(f"wpagerank identities scale=22 plan={plan}", flush=True)
     g = gg.Graph.rmat(ctx, 22, 16, 4, weighted=True)
+    g.reorder(plan)
     n, m = g.n, g.m
     r, finfo = gg.wpagerank(g, np.float64, ALPHA, 1e-7, 100)
     rh = r.to_host()
@@ -203,6 +276,45 @@ def test_wpagerank_adjoint_identities_scale22(ctx):
     # exact at the fixed point; the stopped adjoint leaves a defect of the
     # order of its last step (largest, relatively, for one-edge sources)
     assert np.abs(per_src).sum() <= 1e-5 * scale.sum()
+    g.close()
+
+
+@pytest.mark.parametrize("dt,tol", [(np.float64, TOL64), (np.float32, TOL32)])
+def test_wpagerank_fixed_loops_with_plan(ctx, dt, tol):
+    """The captured fixed-iteration loops (eps < 0) in the plan's numbering:
+    against the oracle and the unplanned loops, replayed twice (bitwise the
+    same), and re-captured after the plan is dropped."""
+    g, off, ids, w = _weighted(ctx, 13, 4)
+    n = g.n
+    v = np.zeros(n)
+    v[[5, 77, 2048]] = 1 / 3
+    want, it, _ = oracle.wpagerank(off, ids, w, ALPHA, 0.0, 25, v=v)
+    vd = gg.DeviceArray.from_host(ctx, v.astype(dt))
+    u = np.zeros(n)
+    T = _targets(n, 64, 13)
+    u[T] = 1.0
+    ud = gg.DeviceArray.from_host(ctx, u.astype(dt))
+    res = {}
+    for plan in (1, 0):
+        g.reorder(plan)
+        outs = []
+        for _ in range(2):
+            r, info = gg.wpagerank(g, dt, ALPHA, -1.0, 25, v=vd)
+            assert info.iters == 25 == it
+            gw, gv, lam, ainfo = gg.wpagerank_adjoint(g, r, ud, ALPHA, -1.0, 25, want_lam=True)
+            assert ainfo.iters == 25
+            outs.append([a.to_host() for a in (r, gw, gv, lam)])
+        for a, b in zip(*outs):
+            assert np.array_equal(a, b)
+        assert _rel(outs[0][0], want) < tol
+        res[plan] = outs[0]
+    lam_w, gw_w, gv_w, _ = oracle.wpagerank_adjoint(off, ids, w, res[0][0].astype(np.float64), u, ALPHA, 0.0, 25)
+    for plan in (0, 1):
+        r_, gw_, gv_, lam_ = res[plan]
+        assert _rel(r_, res[0][0]) < tol
+        assert _rel(lam_, lam_w) < tol and _rel(gv_, gv_w) < tol
+        if dt == np.float64:  # gw = coef (lam_d - mu_s) cancels: FP32 is not held to 1e-5
+            assert _rel(gw_[: g.m], gw_w) < tol
     g.close()
 
 
