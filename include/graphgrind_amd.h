@@ -136,7 +136,12 @@ int gg_graph_relabel(gg_graph_t g, const uint32_t *new_of_old, gg_graph_t *out);
  * counterpart in the engine; the reference reorders offline with VEBO,
  * graphtools/VEBO.C, and '-o').  mode 1: vertices by out-degree descending,
  * so the sources most edges read sit together in L2; mode 0 drops it.
- * Kernels that use it map results back to the caller's numbering. */
+ * Used, when present, by gg_pagerank, gg_spmv, gg_spmv_bwd, gg_wpagerank
+ * and gg_wpagerank_adjoint, which permute their vectors in and map results
+ * back to the caller's numbering (only the FP summation order changes).
+ * Costs about 3 extra copies of the edge ids and weights in device memory
+ * (the relabelled in-edge twin, plus a relabelled out-edge CSR built on the
+ * first transpose-side call).  Drops the graph's captured loops. */
 int gg_graph_reorder(gg_graph_t g, int mode);
 
 /* Raw device addresses of a graph's layout, for interop (no reference
