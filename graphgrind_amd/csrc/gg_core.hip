@@ -138,6 +138,15 @@ int sync(Ctx *c)
     return GG_OK;
 }
 
+int read_back(Ctx *c, void *host, const void *dev, size_t bytes)
+{
+    if (bytes > kPinBytes) return fail(GG_ERR_INVALID, "read_back: more than the pinned buffer holds");
+    GG_HIP(hipMemcpyAsync(c->hpin, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    GG_TRY(sync(c));
+    std::memcpy(host, c->hpin, bytes);
+    return GG_OK;
+}
+
 // ---- small kernels -----------------------------------------------------------
 
 __global__ void k_fill_u8(uint8_t *p, uint64_t n, uint8_t v)
@@ -417,7 +426,12 @@ extern "C" int gg_ctx_create(int device, gg_ctx_t *out)
     (void)hipEventCreate(&c->ev[1]);
     int rc = buf_alloc(c, c->counters, kCounterBytes);
     if (!rc) rc = fill_bytes(c, c->counters.p, kCounterBytes, 0);
+    if (!rc && hipHostMalloc(&c->hpin, kPinBytes, hipHostMallocDefault) != hipSuccess) {
+        c->hpin = nullptr;
+        rc = fail(GG_ERR_HIP, "gg_ctx_create: hipHostMalloc of the read-back buffer failed");
+    }
     if (rc) {
+        buf_free(c->counters);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return rc;
@@ -442,7 +456,7 @@ extern "C" int gg_ctx_destroy(gg_ctx_t c)
     pool_drain(c);
     buf_free(c->counters);
     buf_free(c->tmp);
-    buf_free(c->host_pin);
+    if (c->hpin) (void)hipHostFree(c->hpin);
     (void)hipEventDestroy(c->ev[0]);
     (void)hipEventDestroy(c->ev[1]);
     (void)hipStreamDestroy(c->stream);

@@ -383,8 +383,7 @@ static int run_fixed_point(Graph *g, const CscView &view, int max_iter, int rela
             }
             if (rc) break;
             GG_HIP(hipGetLastError());
-            GG_HIP(hipMemcpyAsync(hflags, flags.p, 8, hipMemcpyDeviceToHost, c->stream));
-            rc = sync(c);
+            rc = read_back(c, hflags, flags.p, 8);
             if (!rc && hflags[0]) {
                 done_iter = hflags[1];
                 break;

@@ -168,6 +168,11 @@ int pool_ensure(Ctx *c, Buf &b, uint64_t bytes);
 void pool_give(Ctx *c, Buf &b);
 void pool_drain(Ctx *c);
 int counters_zero(Ctx *c);                           // totals and stripes
+// Copies 'bytes' (<= kPinBytes) from device memory to host memory through
+// the context's pinned buffer and waits for the stream: a read-back into
+// pageable memory goes through a driver staging copy and takes longer.
+constexpr size_t kPinBytes = 256;
+int read_back(Ctx *c, void *host, const void *dev, size_t bytes);
 int counters_read(Ctx *c, unsigned long long out[4]); // folds the stripes first
 
 // ---- context ----------------------------------------------------------------
@@ -176,7 +181,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     Buf counters;   // uint64 counters: totals + stripes (kCounterBytes)
-    Buf host_pin;   // unused placeholder for pinned staging
+    void *hpin = nullptr; // kPinBytes of pinned host memory for small read-backs
     Buf tmp;        // grow-only temporary (sort space etc.)
     // Pool of device buffers released by per-call temporaries and frontiers
     // (size -> pointers).  They are reused by later requests on this

@@ -732,9 +732,11 @@ int frontier_make_sparse(Frontier *f)
                                                 (const uint32_t *)base.p, (uint32_t *)f->list.p);
     GG_HIP(hipGetLastError());
     uint32_t last[2] = {0, 0};
-    GG_HIP(hipMemcpyAsync(&last[0], (uint32_t *)base.p + (nb - 1), 4, hipMemcpyDeviceToHost, c->stream));
-    GG_HIP(hipMemcpyAsync(&last[1], (uint32_t *)cnt.p + (nb - 1), 4, hipMemcpyDeviceToHost, c->stream));
+    GG_HIP(hipMemcpyAsync(c->hpin, (uint32_t *)base.p + (nb - 1), 4, hipMemcpyDeviceToHost, c->stream));
+    GG_HIP(hipMemcpyAsync((uint32_t *)c->hpin + 1, (uint32_t *)cnt.p + (nb - 1), 4, hipMemcpyDeviceToHost,
+                          c->stream));
     GG_TRY(sync(c));
+    std::memcpy(last, c->hpin, 8);
     pool_give(c, cnt);
     pool_give(c, base);
     if ((uint64_t)last[0] + last[1] != f->d_m) // toSparse's "bad stored value of m" (ligra-numa.h:251-255)
@@ -771,8 +773,7 @@ int counters_read(Ctx *c, unsigned long long out[4])
 {
     k_counters_fold<<<1, kStripes, 0, c->stream>>>((unsigned long long *)c->counters.p);
     GG_HIP(hipGetLastError());
-    GG_HIP(hipMemcpyAsync(out, c->counters.p, 32, hipMemcpyDeviceToHost, c->stream));
-    return sync(c);
+    return read_back(c, out, c->counters.p, 32);
 }
 
 
@@ -997,8 +998,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
                                        rocprim::plus<uint32_t>(), c->stream));
     }
     uint32_t total32 = 0;
-    GG_HIP(hipMemcpyAsync(&total32, (uint32_t *)offs.p + k, 4, hipMemcpyDeviceToHost, c->stream));
-    if ((rc = sync(c))) { pool_give(c, offs); return bail(rc); }
+    if ((rc = read_back(c, &total32, (uint32_t *)offs.p + k, 4))) { pool_give(c, offs); return bail(rc); }
     const uint64_t total = total32;
     if ((rc = pool_ensure(c, out->list, 4ull * (total ? total : 1)))) { pool_give(c, offs); return bail(rc); }
     if ((rc = counters_zero(c))) { pool_give(c, offs); return bail(rc); }
