@@ -377,6 +377,7 @@ extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *p
     if (rc0) { frontier_delete(F); return rc0; }
     F->d_m = 1;
     F->out_edges = od;
+    F->deg_src = g->outdeg.p;
     F->sparse_valid = true;
     EdgeMapExtra ex;
     ex.levels = levels;

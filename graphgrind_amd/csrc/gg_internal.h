@@ -257,6 +257,11 @@ struct Frontier {
     bool sparse_valid = false; // list holds the set
     uint64_t d_m = 0;
     uint64_t out_edges = 0;
+    // The out-degree array out_edges was summed from over exactly the
+    // members (list entries, duplicates included) when the engine made it;
+    // NULL when unknown.  A sparse push over the same degrees sizes itself
+    // from out_edges instead of reading the scan total back.
+    const void *deg_src = nullptr;
     Buf bytes;  // uint8[n]
     Buf bitmap; // uint32[(n+31)/32]
     Buf list;   // uint32[cap]

@@ -169,6 +169,7 @@ static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_ru
         if (!rc) rc = sync(c);
         F->d_m = 1;
         F->out_edges = od;
+        F->deg_src = g->outdeg.p;
         F->sparse_valid = true;
         levels.push_back(F);
         // forward (BC.C:183-191)
@@ -310,6 +311,7 @@ extern "C" int gg_sssp(gg_graph_t g, uint32_t root, int32_t *dist_dev, gg_run_in
         if (!rc) rc = sync(c);
         F->d_m = 1;
         F->out_edges = od;
+        F->deg_src = g->outdeg.p;
         F->sparse_valid = true;
         // rounds (BellmanFord.C:124-139): each takes the reference's branch
         // by the threshold m/20 (atomicMin push or the Jacobi pull)

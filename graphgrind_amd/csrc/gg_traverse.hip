@@ -381,8 +381,11 @@ __global__ __launch_bounds__(kBlock) void k_push(PushArgs A)
 {
     __shared__ uint32_t s_off[kPushLds];
     __shared__ uint32_t s_lo, s_hi;
+    // the edge range is the host's total clamped to the scan's own
+    const uint64_t total = min(A.total, (uint64_t)A.offs[A.k]);
     const uint64_t t0 = (uint64_t)blockIdx.x * kPushTile;
-    const uint64_t t1 = min(t0 + (uint64_t)kPushTile, A.total);
+    if (t0 >= total) return; // whole work-group: no barrier is skipped by part of it
+    const uint64_t t1 = min(t0 + (uint64_t)kPushTile, total);
     if (threadIdx.x == 0) {
         s_lo = upper_idx(A.offs, 0, A.k, t0);
         s_hi = upper_idx(A.offs, 0, A.k, t1 - 1) + 1;
@@ -967,6 +970,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             if ((rc = counters_read(c, cn))) return bail(rc);
             out->d_m = cn[CNT_ACTIVE];
             out->out_edges = cn[CNT_OUT];
+            out->deg_src = g->outdeg.p;
             if (ex) ex->scanned += op->kind == GG_OP_BFS ? cn[CNT_SCANNED] : m;
         }
         pool_give(c, tmpv);
@@ -997,9 +1001,15 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         GG_HIP(rocprim::exclusive_scan(c->tmp.p, tb, it, (uint32_t *)offs.p, 0u, (size_t)k + 1,
                                        rocprim::plus<uint32_t>(), c->stream));
     }
-    uint32_t total32 = 0;
-    if ((rc = read_back(c, &total32, (uint32_t *)offs.p + k, 4))) { pool_give(c, offs); return bail(rc); }
-    const uint64_t total = total32;
+    // The scan total is the frontier's out-degree sum.  When the engine
+    // summed it over these very entries and degrees, no read-back is
+    // needed; k_push clamps its range to the device total either way.
+    uint64_t total = in->out_edges;
+    if (in->deg_src != g->outdeg.p || total > 0xffffffffull) {
+        uint32_t total32 = 0;
+        if ((rc = read_back(c, &total32, (uint32_t *)offs.p + k, 4))) { pool_give(c, offs); return bail(rc); }
+        total = total32;
+    }
     if ((rc = pool_ensure(c, out->list, 4ull * (total ? total : 1)))) { pool_give(c, offs); return bail(rc); }
     if ((rc = counters_zero(c))) { pool_give(c, offs); return bail(rc); }
     if (total) {
@@ -1034,6 +1044,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
     pool_give(c, offs);
     out->d_m = cn[CNT_CURSOR];
     out->out_edges = cn[CNT_OUT];
+    out->deg_src = g->outdeg.p;
     if (remdups && out->d_m) {
         Buf flags, list2;
         if ((rc = pool_take(c, flags, 4ull * n))) return bail(rc);
