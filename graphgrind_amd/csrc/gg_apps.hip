@@ -327,6 +327,24 @@ static int pagerank_generic(Graph *g, int max_iter, double damping, double eps, 
     return rc;
 }
 
+// BFS start state (app/BFS.C:86-92): Parents[i] = -1 except the root's
+// (itself), levels -1 except the root's 0, the one-entry frontier list, and
+// the root's out-degree copied out for the frontier's statistics.
+__global__ void k_bfs_start(int32_t *__restrict__ parents, int32_t *__restrict__ levels, uint32_t n,
+                            uint32_t root, uint32_t *__restrict__ list, const uint32_t *__restrict__ outdeg,
+                            uint32_t *__restrict__ root_deg)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool r = i == root;
+    parents[i] = r ? (int32_t)root : -1;
+    if (levels) levels[i] = r ? 0 : -1;
+    if (r) {
+        list[0] = root;
+        *root_deg = outdeg[root];
+    }
+}
+
 } // namespace gg
 
 using namespace gg;
@@ -359,21 +377,20 @@ extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *p
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     GG_HIP(hipEventRecord(e0, c->stream));
-    // app/BFS.C:86-92
-    GG_TRY(fill_u32(c, parents, g->n, 0xffffffffu));
-    if (levels) GG_TRY(fill_u32(c, levels, g->n, 0xffffffffu));
-    int32_t r = (int32_t)root, zero = 0;
-    GG_HIP(hipMemcpyAsync(parents + root, &r, 4, hipMemcpyHostToDevice, c->stream));
-    if (levels) GG_HIP(hipMemcpyAsync(levels + root, &zero, 4, hipMemcpyHostToDevice, c->stream));
-    // partitioned_vertices::create(n, root, outdeg(root)) (ligra-numa.h:103-116)
+    // app/BFS.C:86-92 and partitioned_vertices::create(n, root,
+    // outdeg(root)) (ligra-numa.h:103-116) in one kernel; the list holds the
+    // root and, after it, the root's out-degree for the read-back
     Frontier *F = new Frontier();
     frontier_alloc(g, F);
     uint32_t od = 0;
     int rc0 = pool_take(c, F->list, 8);
-    if (!rc0 && hipMemcpyAsync(F->list.p, &root, 4, hipMemcpyHostToDevice, c->stream)) rc0 = fail(GG_ERR_HIP, "root upload");
-    if (!rc0 && hipMemcpyAsync(&od, (uint32_t *)g->outdeg.p + root, 4, hipMemcpyDeviceToHost, c->stream))
-        rc0 = fail(GG_ERR_HIP, "outdeg readback");
-    if (!rc0) rc0 = sync(c);
+    if (!rc0) {
+        k_bfs_start<<<blocks_for(g->n, kBlock), kBlock, 0, c->stream>>>(
+            parents, levels, g->n, root, (uint32_t *)F->list.p, (const uint32_t *)g->outdeg.p,
+            (uint32_t *)F->list.p + 1);
+        if (hipGetLastError() != hipSuccess) rc0 = fail(GG_ERR_HIP, "gg_bfs: start kernel");
+    }
+    if (!rc0) rc0 = read_back(c, &od, (uint32_t *)F->list.p + 1, 4);
     if (rc0) { frontier_delete(F); return rc0; }
     F->d_m = 1;
     F->out_edges = od;
