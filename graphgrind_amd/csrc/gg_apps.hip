@@ -74,12 +74,18 @@ struct OpPRFused {
     }
 };
 
+// The rank vector of the last executed iteration (known on the device from
+// the stop flags) into the caller's array and numbering: ranks[v] =
+// R[it % 3][perm ? perm[v] : v], it = the stopping iteration or max_iter.
 template <class V>
-__global__ void k_unpermute(V *__restrict__ out, const V *__restrict__ in, const uint32_t *__restrict__ perm,
-                            uint32_t n)
+__global__ void k_pr_result(V *__restrict__ out, const V *r0, const V *r1, const V *r2,
+                            const uint32_t *__restrict__ perm, const int *__restrict__ flags, int max_iter, uint32_t n)
 {
-    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) out[v] = in[perm[v]];
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int it = flags[0] ? flags[1] : max_iter;
+    const V *in = it % 3 == 0 ? r0 : it % 3 == 1 ? r1 : r2;
+    out[v] = in[perm ? perm[v] : v];
 }
 
 template <class V>
@@ -208,16 +214,21 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
                                                  eps, it, done, done_iter, cnt,
                                                  (unsigned long long *)B.stats.p, (double *)B.l1h.p);
     }
-    // the rank vector in the caller's numbering: the buffer of the last
-    // executed iteration is known only on the device (break flag), so copy
-    // after reading it; with a relabelled plan this is the un-permute
+    // the rank vector in the caller's numbering (un-permuted with a plan);
+    // the kernel picks the last executed iteration's buffer from the flags
+    if (!rc)
+        k_pr_result<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
+            (V *)ranks, (const V *)B.R[0].p, (const V *)B.R[1].p, (const V *)B.R[2].p,
+            use_plan ? (const uint32_t *)g->plan.perm.p : nullptr, done, max_iter, n);
     GG_HIP(hipEventRecord(ev[1], c->stream));
     GG_HIP(hipGetLastError());
-    if (!rc) rc = sync(c);
     int hflags[2] = {0, 0};
     std::vector<unsigned long long> st(2ull * (max_iter + 1));
-    if (!rc) rc = gg_memcpy_d2h(abi(c), hflags, B.flags.p, 8);
-    if (!rc) rc = gg_memcpy_d2h(abi(c), st.data(), B.stats.p, 16ull * max_iter);
+    if (!rc) rc = read_back(c, hflags, B.flags.p, 8);
+    if (!rc) {
+        const size_t sb = 16ull * max_iter;
+        rc = sb <= kPinBytes ? read_back(c, st.data(), B.stats.p, sb) : gg_memcpy_d2h(abi(c), st.data(), B.stats.p, sb);
+    }
     int iters = hflags[0] ? hflags[1] : max_iter;
     if (!rc) {
         float ms = 0, kms = 0, tot = 0;
@@ -247,16 +258,8 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
         }
         if (info && iters >= 1)
             info->paths[0] = g->P && !g->part_vertex ? GG_PATH_COO : GG_PATH_CSC;
-        if (!*needs_generic) {
-            if (use_plan) {
-                k_unpermute<V><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
-                    (V *)ranks, (const V *)B.R[iters % 3].p, (const uint32_t *)g->plan.perm.p, n);
-                GG_HIP(hipGetLastError());
-                rc = sync(c);
-            } else {
-                rc = gg_memcpy_d2d(abi(c), ranks, B.R[iters % 3].p, sizeof(V) * (uint64_t)n);
-            }
-        }
+        // with needs_generic set, the caller reruns the generic loop, which
+        // overwrites 'ranks'
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     return rc;
