@@ -428,32 +428,69 @@ extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *p
     return rc;
 }
 
-extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
+namespace gg {
+// The graph in its plan's numbering, sharing the plan's arrays: the
+// relabelled in-edge twin with its schedules and the relabelled out-edge
+// CSR (built on first use), degrees in the new numbering, the same
+// partition settings.  It owns nothing: release it with plan_alias_release.
+static int plan_alias(Graph *g, Graph &t)
 {
-    if (!g || !labels) return fail(GG_ERR_INVALID, "gg_cc: NULL argument");
-    Ctx *c = g->ctx;
-    GG_HIP(hipSetDevice(c->device));
-    if (info) std::memset(info, 0, sizeof(*info));
-    Buf prev;
-    GG_TRY(pool_take(c, prev, 4ull * g->n));
-    hipEvent_t e0, e1;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    GG_HIP(hipEventRecord(e0, c->stream));
-    GG_TRY(gg_iota(abi(c), labels, GG_U32, g->n)); // Components.C:113
+    CscView cv;
+    GG_TRY(plan_csr_view(g, &cv));
+    PullPlan &P = g->plan;
+    if (g->pull_schedule == 1 && !P.rows.blocks) return fail(GG_ERR_STATE, "plan has no row-block schedule");
+    t.ctx = g->ctx;
+    t.n = g->n;
+    t.m = g->m;
+    t.symmetric = g->symmetric;
+    t.off = P.coff;
+    t.ids = P.cids;
+    t.toff = P.toff;
+    t.tids = P.tids;
+    t.outdeg = P.outdeg;
+    t.pull = P.sched;
+    t.rows = P.rows;
+    t.rows_ok = P.rows.blocks != 0;
+    t.csr_pull = P.csched;
+    t.csr_pull_ok = true;
+    t.csr_rows = P.crows;
+    t.csr_rows_ok = P.crows.blocks != 0;
+    t.P = g->P;
+    t.part_vertex = g->part_vertex;
+    t.pull_schedule = g->pull_schedule;
+    t.edges_per_lane = g->edges_per_lane;
+    return GG_OK;
+}
+
+static void plan_alias_release(Graph &t)
+{
+    Buf *bs[] = {&t.off, &t.ids, &t.toff, &t.tids, &t.outdeg};
+    for (Buf *b : bs) *b = Buf{};
+    t.pull = PullSched{};
+    t.csr_pull = PullSched{};
+    t.rows = RowSched{};
+    t.csr_rows = RowSched{};
+}
+
+// Components.C's loop (101-126) from the labels already in 'labels': copy
+// to prev, edgeMap with the minimum-label functor, until the frontier is
+// empty; the dense rounds read the live labels.
+static int cc_loop(Graph *g, uint32_t *labels, uint32_t *prev, gg_run_info *info)
+{
     Frontier *F = new Frontier();
     frontier_alloc(g, F);
     F->bit = true;
     F->has_dense = true;
     F->d_m = g->n;
     F->out_edges = g->m;
+    F->deg_src = g->outdeg.p;
     EdgeMapExtra ex;
     ex.cc_live = true;
     int rounds = 0, rc = GG_OK;
     while (F->d_m != 0) {
-        gg_vop_t cp = {GG_VOP_CC_COPY, GG_U32, labels, prev.p, 0, 0};
+        gg_vop_t cp = {GG_VOP_CC_COPY, GG_U32, labels, prev, 0, 0};
         if ((rc = vertexmap_core(g, F, &cp))) break;
-        gg_op_t op = {GG_OP_CC, GG_U32, labels, prev.p};
+        gg_op_t op = {GG_OP_CC, GG_U32, labels, prev};
         Frontier *out = nullptr;
         int path = 0;
         if ((rc = edgemap_core(g, F, &op, -1, 0, &out, &path, &ex))) break;
@@ -462,19 +499,63 @@ extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
         frontier_delete(F);
         F = out;
     }
-    GG_HIP(hipEventRecord(e1, c->stream));
+    frontier_delete(F);
+    if (info) {
+        info->iters = rounds;
+        info->edges_scanned = ex.scanned;
+    }
+    return rc;
+}
+} // namespace gg
+
+extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
+{
+    if (!g || !labels) return fail(GG_ERR_INVALID, "gg_cc: NULL argument");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    if (info) std::memset(info, 0, sizeof(*info));
+    // With a renumbered plan the loop runs on the plan's relabelled arrays
+    // (the dense rounds gather labels of hot vertices from a narrow range);
+    // label values stay the caller's vertex ids, so the minimum of each
+    // component, the fixed point, is the same, and positions are mapped
+    // back at the end.
+    const bool planned = g->plan.mode != 0 && g->m;
+    Buf prev, lp;
+    GG_TRY(pool_take(c, prev, 4ull * g->n));
+    if (planned) {
+        int r = pool_take(c, lp, 4ull * g->n);
+        if (r) {
+            pool_give(c, prev);
+            return r;
+        }
+    }
+    Graph t;
+    int rc = planned ? plan_alias(g, t) : GG_OK;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, c->stream);
+    if (!rc) {
+        if (planned) { // Components.C:113 IDs[i] = i, in the new numbering
+            if (hipMemcpyAsync(lp.p, g->plan.inv.p, 4ull * g->n, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                rc = fail(GG_ERR_HIP, "gg_cc: initial labels");
+            if (!rc) rc = cc_loop(&t, (uint32_t *)lp.p, (uint32_t *)prev.p, info);
+            if (!rc) rc = gather_idx(c, (const uint32_t *)lp.p, (const uint32_t *)g->plan.perm.p, labels, g->n);
+        } else {
+            rc = gg_iota(abi(c), labels, GG_U32, g->n); // Components.C:113
+            if (!rc) rc = cc_loop(g, labels, (uint32_t *)prev.p, info);
+        }
+    }
+    (void)hipEventRecord(e1, c->stream);
     if (!rc) rc = sync(c);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    frontier_delete(F);
+    if (planned) plan_alias_release(t);
     pool_give(c, prev);
-    if (info) {
-        info->iters = rounds;
-        info->ms = ms;
-        info->edges_scanned = ex.scanned;
-    }
+    pool_give(c, lp);
+    if (info) info->ms = ms;
     return rc;
 }
 

@@ -270,13 +270,16 @@ def test_bfs_matches_reference_golden(ctx, path):
     g.close()
 
 
+@pytest.mark.parametrize("plan", [0, 1])
 @pytest.mark.parametrize("scale,sym", [(12, True), (16, True), (14, False)])
-def test_cc_labels_exact(ctx, scale, sym):
-    print(f"cc scale={scale} sym={sym}", flush=True)
+def test_cc_labels_exact(ctx, scale, sym, plan):
+    """plan 1: the loop runs on the hot-sources-first renumbering with the
+    caller's ids as label values; the fixed point is the same."""
+    print(f"cc scale={scale} sym={sym} plan={plan}", flush=True)
     off, ids = oracle.rmat_csr(scale, 16, 2, symmetric=sym)
     og = oracle.Graph(off, ids, symmetric=sym, P=64)
     want, rounds, _ = og.cc()
-    g = _dev_graph(ctx, off, ids, sym).partition(64)
+    g = _dev_graph(ctx, off, ids, sym).partition(64).reorder(plan)
     lab, info = gg.components(g)
     assert np.array_equal(lab.to_host(), want)
     if sym:
@@ -289,8 +292,10 @@ def test_cc_labels_exact(ctx, scale, sym):
 def test_cc_matches_reference_golden(ctx, path):
     z = R.load_golden(path)
     g = _dev_graph(ctx, z["off"], z["ids"], bool(z["symmetric"])).partition(int(z["P"]))
-    lab, _ = gg.components(g)
-    assert np.array_equal(lab.to_host(), z["cc_edge"].astype(np.int64))
+    for plan in (0, 1, 0):
+        g.reorder(plan)
+        lab, _ = gg.components(g)
+        assert np.array_equal(lab.to_host(), z["cc_edge"].astype(np.int64)), plan
     g.close()
 
 
