@@ -368,66 +368,6 @@ extern "C" int gg_pagerank(gg_graph_t g, int dtype, int max_iter, double damping
                            : pagerank_generic<float>(g, max_iter, damping, eps, ranks_dev, info);
 }
 
-extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *parents, int32_t *levels,
-                      gg_run_info *info)
-{
-    if (!g || !parents) return fail(GG_ERR_INVALID, "gg_bfs: NULL argument");
-    if (root >= g->n) return fail(GG_ERR_INVALID, "gg_bfs: root out of range");
-    Ctx *c = g->ctx;
-    GG_HIP(hipSetDevice(c->device));
-    if (info) std::memset(info, 0, sizeof(*info));
-    hipEvent_t e0, e1;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    GG_HIP(hipEventRecord(e0, c->stream));
-    // app/BFS.C:86-92 and partitioned_vertices::create(n, root,
-    // outdeg(root)) (ligra-numa.h:103-116) in one kernel; the list holds the
-    // root and, after it, the root's out-degree for the read-back
-    Frontier *F = new Frontier();
-    frontier_alloc(g, F);
-    uint32_t od = 0;
-    int rc0 = pool_take(c, F->list, 8);
-    if (!rc0) {
-        k_bfs_start<<<blocks_for(g->n, kBlock), kBlock, 0, c->stream>>>(
-            parents, levels, g->n, root, (uint32_t *)F->list.p, (const uint32_t *)g->outdeg.p,
-            (uint32_t *)F->list.p + 1);
-        if (hipGetLastError() != hipSuccess) rc0 = fail(GG_ERR_HIP, "gg_bfs: start kernel");
-    }
-    if (!rc0) rc0 = read_back(c, &od, (uint32_t *)F->list.p + 1, 4);
-    if (rc0) { frontier_delete(F); return rc0; }
-    F->d_m = 1;
-    F->out_edges = od;
-    F->deg_src = g->outdeg.p;
-    F->sparse_valid = true;
-    EdgeMapExtra ex;
-    ex.levels = levels;
-    int lv = 0, rc = GG_OK;
-    while (F->d_m != 0) {
-        ex.level = lv + 1;
-        gg_op_t op = {GG_OP_BFS, GG_I32, parents, nullptr};
-        Frontier *out = nullptr;
-        int path = 0;
-        if ((rc = edgemap_core(g, F, &op, threshold, 0, &out, &path, &ex))) break;
-        if (info && lv < 64) info->paths[lv] = path;
-        lv++;
-        frontier_delete(F);
-        F = out;
-    }
-    GG_HIP(hipEventRecord(e1, c->stream));
-    if (!rc) rc = sync(c);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    frontier_delete(F);
-    if (info) {
-        info->iters = lv;
-        info->ms = ms;
-        info->edges_scanned = ex.scanned;
-    }
-    return rc;
-}
-
 namespace gg {
 // The graph in its plan's numbering, sharing the plan's arrays: the
 // relabelled in-edge twin with its schedules and the relabelled out-edge
@@ -506,7 +446,117 @@ static int cc_loop(Graph *g, uint32_t *labels, uint32_t *prev, gg_run_info *info
     }
     return rc;
 }
+// BFS.C's Compute (77-101) on g from 'root': the start state, then edgeMap
+// levels until the frontier is empty.
+static int bfs_loop(Graph *g, uint32_t root, int64_t threshold, int32_t *parents, int32_t *levels,
+                    gg_run_info *info)
+{
+    Ctx *c = g->ctx;
+    // app/BFS.C:86-92 and partitioned_vertices::create(n, root,
+    // outdeg(root)) (ligra-numa.h:103-116) in one kernel; the list holds the
+    // root and, after it, the root's out-degree for the read-back
+    Frontier *F = new Frontier();
+    frontier_alloc(g, F);
+    uint32_t od = 0;
+    int rc0 = pool_take(c, F->list, 8);
+    if (!rc0) {
+        k_bfs_start<<<blocks_for(g->n, kBlock), kBlock, 0, c->stream>>>(
+            parents, levels, g->n, root, (uint32_t *)F->list.p, (const uint32_t *)g->outdeg.p,
+            (uint32_t *)F->list.p + 1);
+        if (hipGetLastError() != hipSuccess) rc0 = fail(GG_ERR_HIP, "gg_bfs: start kernel");
+    }
+    if (!rc0) rc0 = read_back(c, &od, (uint32_t *)F->list.p + 1, 4);
+    if (rc0) { frontier_delete(F); return rc0; }
+    F->d_m = 1;
+    F->out_edges = od;
+    F->deg_src = g->outdeg.p;
+    F->sparse_valid = true;
+    EdgeMapExtra ex;
+    ex.levels = levels;
+    int lv = 0, rc = GG_OK;
+    while (F->d_m != 0) {
+        ex.level = lv + 1;
+        gg_op_t op = {GG_OP_BFS, GG_I32, parents, nullptr};
+        Frontier *out = nullptr;
+        int path = 0;
+        if ((rc = edgemap_core(g, F, &op, threshold, 0, &out, &path, &ex))) break;
+        if (info && lv < 64) info->paths[lv] = path;
+        lv++;
+        frontier_delete(F);
+        F = out;
+    }
+    frontier_delete(F);
+    if (info) {
+        info->iters = lv;
+        info->edges_scanned = ex.scanned;
+    }
+    return rc;
+}
+
+// Parents and levels of a BFS run in the plan's numbering, in the caller's:
+// parents[v] = inv[pp[perm[v]]] (or -1), levels[v] = lp[perm[v]].
+__global__ void k_bfs_unmap(const int32_t *__restrict__ pp, const int32_t *__restrict__ lp,
+                            const uint32_t *__restrict__ perm, const uint32_t *__restrict__ inv,
+                            int32_t *__restrict__ parents, int32_t *__restrict__ levels, uint32_t n)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint32_t k = perm[v];
+    const int32_t p = pp[k];
+    parents[v] = p < 0 ? -1 : (int32_t)inv[p];
+    if (levels) levels[v] = lp[k];
+}
 } // namespace gg
+
+extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *parents, int32_t *levels,
+                      gg_run_info *info)
+{
+    if (!g || !parents) return fail(GG_ERR_INVALID, "gg_bfs: NULL argument");
+    if (root >= g->n) return fail(GG_ERR_INVALID, "gg_bfs: root out of range");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    if (info) std::memset(info, 0, sizeof(*info));
+    // With a renumbered plan the levels run on the plan's relabelled arrays
+    // (hot vertices first in every in-list, so bottom-up scans meet the
+    // frontier sooner) and parents and levels are mapped back at the end.
+    const bool planned = g->plan.mode != 0 && g->m;
+    Buf pp, lp;
+    Graph t;
+    int rc = GG_OK;
+    if (planned) {
+        rc = pool_take(c, pp, 4ull * g->n);
+        if (!rc && levels) rc = pool_take(c, lp, 4ull * g->n);
+        if (!rc) rc = plan_alias(g, t);
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, c->stream);
+    if (!rc && planned) {
+        uint32_t r2 = 0;
+        rc = read_back(c, &r2, (const uint32_t *)g->plan.perm.p + root, 4);
+        if (!rc) rc = bfs_loop(&t, r2, threshold, (int32_t *)pp.p, levels ? (int32_t *)lp.p : nullptr, info);
+        if (!rc) {
+            k_bfs_unmap<<<blocks_for(g->n, kBlock), kBlock, 0, c->stream>>>(
+                (const int32_t *)pp.p, (const int32_t *)lp.p, (const uint32_t *)g->plan.perm.p,
+                (const uint32_t *)g->plan.inv.p, parents, levels, g->n);
+            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_bfs: map-back kernel");
+        }
+    } else if (!rc) {
+        rc = bfs_loop(g, root, threshold, parents, levels, info);
+    }
+    (void)hipEventRecord(e1, c->stream);
+    if (!rc) rc = sync(c);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (planned) plan_alias_release(t);
+    pool_give(c, pp);
+    pool_give(c, lp);
+    if (info) info->ms = ms;
+    return rc;
+}
 
 extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
 {

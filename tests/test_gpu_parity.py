@@ -241,13 +241,18 @@ def test_pagerank_generic_edgemap_loop(ctx):
 
 # ------------------------------------------------------------ BFS and CC --
 
+@pytest.mark.parametrize("plan", [0, 1])
 @pytest.mark.parametrize("scale,sym,root", [(12, True, 100), (16, True, 100), (16, False, 3)])
-def test_bfs_levels_exact(ctx, scale, sym, root):
-    print(f"bfs scale={scale} sym={sym}", flush=True)
+def test_bfs_levels_exact(ctx, scale, sym, root, plan):
+    """plan 1: the levels run on the hot-sources-first renumbering and
+    parents and levels are mapped back; the branch taken at every level is
+    the oracle's either way (the frontier statistics do not depend on the
+    numbering)."""
+    print(f"bfs scale={scale} sym={sym} plan={plan}", flush=True)
     off, ids = oracle.rmat_csr(scale, 16, 2, symmetric=sym)
     og = oracle.Graph(off, ids, symmetric=sym, P=64)
     opar, lv, tr = og.bfs(root)
-    g = _dev_graph(ctx, off, ids, sym).partition(64)
+    g = _dev_graph(ctx, off, ids, sym).partition(64).reorder(plan)
     p, l, info = gg.bfs(g, root)
     par = p.to_host()
     lvl = R.check_parents(off, ids, root, par)
@@ -257,16 +262,29 @@ def test_bfs_levels_exact(ctx, scale, sym, root):
     g.close()
 
 
+def test_bfs_plan_without_levels(ctx):
+    """The plan's map-back when the caller asks for parents only."""
+    off, ids = oracle.rmat_csr(13, 8, 7, symmetric=True)
+    g = _dev_graph(ctx, off, ids, True).reorder(1)
+    p, l, _ = gg.bfs(g, 5, levels=False)
+    assert l is None
+    R.check_parents(off, ids, 5, p.to_host())
+    g.close()
+
+
 @pytest.mark.parametrize("path", [p for p in R.golden_cases() if "bfs_edge" in np.load(p).files],
                          ids=lambda p: p.split("/")[-1])
 def test_bfs_matches_reference_golden(ctx, path):
     z = R.load_golden(path)
     root = int(z["root"])
     g = _dev_graph(ctx, z["off"], z["ids"], bool(z["symmetric"])).partition(int(z["P"]))
-    p, l, info = gg.bfs(g, root)
-    lvl = R.check_parents(z["off"], z["ids"], root, p.to_host())
-    R.check_parents(z["off"], z["ids"], root, z["bfs_edge"].astype(np.int64), lvl)
-    assert info.iters == int(z["bfs_edge_iters"])
+    for plan in (0, 1):
+        g.reorder(plan)
+        p, l, info = gg.bfs(g, root)
+        lvl = R.check_parents(z["off"], z["ids"], root, p.to_host())
+        assert np.array_equal(l.to_host(), lvl)
+        R.check_parents(z["off"], z["ids"], root, z["bfs_edge"].astype(np.int64), lvl)
+        assert info.iters == int(z["bfs_edge_iters"])
     g.close()
 
 
