@@ -303,12 +303,15 @@ def batched_bench(gg, ctx, count=512, seed=5, iters=30, msbfs_scale=22):
     return res
 
 
-def apps_bench(gg, ctx, scale, steps):
+def apps_bench(gg, ctx, scale, steps, plan=True):
     """The further apps on the same engine: BC.C from root 100 (FP32, the
     reference's fType) on the symmetric R-MAT graph of the BFS leg, and
     BellmanFord.C from vertex 0 on a directed R-MAT graph (seed 4) with
-    integer lengths 1..19."""
+    integer lengths 1..19; both on the hot-sources-first plan (untimed
+    preprocessing) unless plan is False."""
     g = gg.Graph.rmat(ctx, scale, 16, 2, symmetric=True)
+    if plan:
+        g.reorder(1)
     sc = gg.DeviceArray(ctx, g.n, np.float32)
     gg.bc(g, 100, np.float32, out=sc)
     ts = []
@@ -322,6 +325,8 @@ def apps_bench(gg, ctx, scale, steps):
     g = gg.Graph.rmat(ctx, scale - 2, 16, 4)
     w = (1 + (np.arange(g.m, dtype=np.uint64) * np.uint64(2654435761)) % np.uint64(19)).astype(np.float32)
     g.set_weights(w)
+    if plan:
+        g.reorder(1)
     d = gg.DeviceArray(ctx, g.n, np.int32)
     gg.sssp(g, 0, out=d)
     ts = []

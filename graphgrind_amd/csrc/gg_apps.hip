@@ -369,49 +369,6 @@ extern "C" int gg_pagerank(gg_graph_t g, int dtype, int max_iter, double damping
 }
 
 namespace gg {
-// The graph in its plan's numbering, sharing the plan's arrays: the
-// relabelled in-edge twin with its schedules and the relabelled out-edge
-// CSR (built on first use), degrees in the new numbering, the same
-// partition settings.  It owns nothing: release it with plan_alias_release.
-static int plan_alias(Graph *g, Graph &t)
-{
-    CscView cv;
-    GG_TRY(plan_csr_view(g, &cv));
-    PullPlan &P = g->plan;
-    if (g->pull_schedule == 1 && !P.rows.blocks) return fail(GG_ERR_STATE, "plan has no row-block schedule");
-    t.ctx = g->ctx;
-    t.n = g->n;
-    t.m = g->m;
-    t.symmetric = g->symmetric;
-    t.off = P.coff;
-    t.ids = P.cids;
-    t.toff = P.toff;
-    t.tids = P.tids;
-    t.outdeg = P.outdeg;
-    t.pull = P.sched;
-    t.rows = P.rows;
-    t.rows_ok = P.rows.blocks != 0;
-    t.csr_pull = P.csched;
-    t.csr_pull_ok = true;
-    t.csr_rows = P.crows;
-    t.csr_rows_ok = P.crows.blocks != 0;
-    t.P = g->P;
-    t.part_vertex = g->part_vertex;
-    t.pull_schedule = g->pull_schedule;
-    t.edges_per_lane = g->edges_per_lane;
-    return GG_OK;
-}
-
-static void plan_alias_release(Graph &t)
-{
-    Buf *bs[] = {&t.off, &t.ids, &t.toff, &t.tids, &t.outdeg};
-    for (Buf *b : bs) *b = Buf{};
-    t.pull = PullSched{};
-    t.csr_pull = PullSched{};
-    t.rows = RowSched{};
-    t.csr_rows = RowSched{};
-}
-
 // Components.C's loop (101-126) from the labels already in 'labels': copy
 // to prev, edgeMap with the minimum-label functor, until the frontier is
 // empty; the dense rounds read the live labels.

@@ -755,7 +755,8 @@ static void graph_free(Graph *g)
                    &g->indeg, &g->comp, &g->coo_src, &g->coo_dst, &g->coo_w, &g->pull.start_row,
                    &g->pull.start_edge, &g->pull.carry_row, &g->pull.carry_val, &g->pull.head_val,
                    &g->plan.perm, &g->plan.inv, &g->plan.toff, &g->plan.tids, &g->plan.tw, &g->plan.outdeg,
-                   &g->plan.coff, &g->plan.cids, &g->plan.cw, &g->plan.cdelta, &g->plan.csched.start_row,
+                   &g->plan.coff, &g->plan.cids, &g->plan.cw, &g->plan.cdelta, &g->plan.indeg,
+                   &g->plan.csched.start_row,
                    &g->plan.csched.start_edge, &g->plan.csched.carry_row, &g->plan.csched.carry_val,
                    &g->plan.csched.head_val,
                    &g->plan.sched.start_row, &g->plan.sched.start_edge, &g->plan.sched.carry_row,
@@ -1068,6 +1069,8 @@ static int weights_changed(gg_graph_t g)
         int rc = gg_graph_partition(g, g->P, g->part_vertex, g->relabel, g->hilbert);
         if (rc) return rc;
     }
+    // the renumbered plan holds its own copies of the weights: rebuild it
+    if (g->plan.mode) GG_TRY(gg_graph_reorder(g, g->plan.mode));
     return sync(c);
 }
 } // namespace gg
@@ -1534,7 +1537,7 @@ extern "C" int gg_graph_reorder(gg_graph_t g, int mode)
     graph_exec_drop(g); // captured loops may hold the plan's arrays
     PullPlan &P = g->plan;
     Buf *pb[] = {&P.perm, &P.inv, &P.toff, &P.tids, &P.tw, &P.outdeg, &P.sched.start_row, &P.sched.start_edge,
-                 &P.sched.carry_row, &P.sched.carry_val, &P.sched.head_val, &P.coff, &P.cids, &P.cw, &P.cdelta,
+                 &P.sched.carry_row, &P.sched.carry_val, &P.sched.head_val, &P.coff, &P.cids, &P.cw, &P.cdelta, &P.indeg,
                  &P.csched.start_row, &P.csched.start_edge, &P.csched.carry_row, &P.csched.carry_val,
                  &P.csched.head_val};
     for (Buf *b : pb) buf_free(*b);
@@ -1652,12 +1655,16 @@ int plan_csr_view(Graph *g, CscView *v)
     const uint32_t n = g->n;
     const uint64_t m = g->m;
     if (!P.coff.p) {
-        Buf coff, cids, cw, cdelta;
+        Buf coff, cids, cw, cdelta, indeg;
         int rc = buf_alloc(c, coff, 4ull * (n + 1));
         if (!rc) rc = buf_alloc(c, cids, 4 * (m ? m : 1));
         if (!rc && g->weighted) rc = buf_alloc(c, cw, 4 * (m ? m : 1));
         if (!rc) rc = buf_alloc(c, cdelta, 4ull * n);
+        if (!rc) rc = buf_alloc(c, indeg, 4ull * n);
         if (!rc) rc = scan_offsets(c, (const uint32_t *)P.outdeg.p, (uint32_t *)coff.p, n);
+        if (!rc && n)
+            k_diff<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)P.toff.p,
+                                                                     (uint32_t *)indeg.p, n);
         if (!rc && m) {
             const uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
             k_plan_csr_edges<<<grid, kBlock, 0, c->stream>>>(
@@ -1672,7 +1679,7 @@ int plan_csr_view(Graph *g, CscView *v)
         if (!rc) rc = sched_build(c, P.csched, (const uint32_t *)coff.p, n, m);
         if (!rc) rc = sync(c);
         if (rc) {
-            Buf *bs[] = {&coff, &cids, &cw, &cdelta};
+            Buf *bs[] = {&coff, &cids, &cw, &cdelta, &indeg};
             for (Buf *b : bs) buf_free(*b);
             return rc;
         }
@@ -1680,7 +1687,7 @@ int plan_csr_view(Graph *g, CscView *v)
         P.cids = cids;
         P.cw = cw;
         P.cdelta = cdelta;
-        coff = cids = cw = cdelta = Buf{};
+        P.indeg = indeg;
     }
     if (g->pull_schedule == 1 && !P.crows.blocks && n)
         GG_TRY(rows_build(c, P.crows, (const uint32_t *)P.coff.p, n, m, g->edges_per_lane));
@@ -1691,6 +1698,54 @@ int plan_csr_view(Graph *g, CscView *v)
     if (g->pull_schedule == 1) v->rows = &P.crows;
     return GG_OK;
 }
+// The graph in its plan's numbering, sharing the plan's arrays: the
+// relabelled in-edge twin with its schedules and the relabelled out-edge
+// CSR (built on first use), degrees in the new numbering, the same
+// partition settings, weights when the graph has them.  It owns nothing:
+// release it with plan_alias_release.
+int plan_alias(Graph *g, Graph &t)
+{
+    CscView cv;
+    GG_TRY(plan_csr_view(g, &cv));
+    PullPlan &P = g->plan;
+    if (g->pull_schedule == 1 && !P.rows.blocks) return fail(GG_ERR_STATE, "plan has no row-block schedule");
+    t.ctx = g->ctx;
+    t.n = g->n;
+    t.m = g->m;
+    t.symmetric = g->symmetric;
+    t.off = P.coff;
+    t.ids = P.cids;
+    t.toff = P.toff;
+    t.tids = P.tids;
+    t.outdeg = P.outdeg;
+    t.indeg = P.indeg;
+    t.weighted = g->weighted;
+    t.w = P.cw;
+    t.tw = P.tw;
+    t.pull = P.sched;
+    t.rows = P.rows;
+    t.rows_ok = P.rows.blocks != 0;
+    t.csr_pull = P.csched;
+    t.csr_pull_ok = true;
+    t.csr_rows = P.crows;
+    t.csr_rows_ok = P.crows.blocks != 0;
+    t.P = g->P;
+    t.part_vertex = g->part_vertex;
+    t.pull_schedule = g->pull_schedule;
+    t.edges_per_lane = g->edges_per_lane;
+    return GG_OK;
+}
+
+void plan_alias_release(Graph &t)
+{
+    Buf *bs[] = {&t.off, &t.ids, &t.w, &t.toff, &t.tids, &t.tw, &t.outdeg, &t.indeg};
+    for (Buf *b : bs) *b = Buf{};
+    t.pull = PullSched{};
+    t.csr_pull = PullSched{};
+    t.rows = RowSched{};
+    t.csr_rows = RowSched{};
+}
+
 } // namespace gg
 
 namespace gg {

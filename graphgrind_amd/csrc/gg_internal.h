@@ -114,6 +114,7 @@ struct PullPlan {
     // at coff[perm[v]] + (e - off[v]); cdelta[perm[v]] = off[v] - coff[perm[v]]
     // (mod 2^32) maps it back.
     Buf coff, cids, cw, cdelta;
+    Buf indeg; // uint32[n] in-degrees in the new numbering
     PullSched csched;
     RowSched crows;
 };
@@ -270,6 +271,11 @@ struct Frontier {
 // The relabelled out-edge CSR of the graph's plan (built on first use), as
 // a pull view with the graph's schedule kind; requires a plan.
 int plan_csr_view(Graph *g, CscView *v);
+// The graph in its plan's numbering as a Graph sharing the plan's arrays
+// (both edge directions, degrees, weights, schedules); it owns nothing:
+// release it with plan_alias_release, never graph_free.
+int plan_alias(Graph *g, Graph &t);
+void plan_alias_release(Graph &t);
 
 // ---- shared device helpers (defined in gg_core.hip) ------------------------
 int sched_build(Ctx *c, PullSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges);

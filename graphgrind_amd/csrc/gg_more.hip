@@ -77,10 +77,32 @@ __global__ void k_bc_back_mark(const uint8_t *__restrict__ F, uint8_t *__restric
 }
 
 template <class V>
-__global__ void k_bc_final(const V *__restrict__ dep, const V *__restrict__ inv, V *__restrict__ score, uint32_t n)
+__global__ void k_bc_final(const V *__restrict__ dep, const V *__restrict__ inv, V *__restrict__ score,
+                           const int32_t *__restrict__ level, int32_t *__restrict__ levels_out,
+                           const uint32_t *__restrict__ to_caller, uint32_t n)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) score[v] = (dep[v] - inv[v]) / inv[v];
+    if (v >= n) return;
+    const uint32_t o = to_caller ? to_caller[v] : v; // the caller's id of v
+    score[o] = (dep[v] - inv[v]) / inv[v];
+    if (levels_out) levels_out[o] = level[v];
+}
+
+// One-entry frontier list {root} with the root's out-degree after it (for
+// the read-back of the frontier's statistics).
+__global__ void k_root_list(uint32_t root, const uint32_t *__restrict__ outdeg, uint32_t *__restrict__ list)
+{
+    if (threadIdx.x == 0) {
+        list[0] = root;
+        list[1] = outdeg[root];
+    }
+}
+
+static int root_frontier(Ctx *c, Graph *g, uint32_t root, Frontier *F)
+{
+    k_root_list<<<1, 64, 0, c->stream>>>(root, (const uint32_t *)g->outdeg.p, (uint32_t *)F->list.p);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
 }
 
 // The graph with its two edge directions swapped (GA.transpose()), sharing
@@ -134,8 +156,11 @@ static void alias_release(Graph &t)
 // reference's branch (sparse push with atomic adds, or the destination-
 // owned pull) by its threshold m/20; the backward phase runs on the
 // transposed view.
+// On a plan alias, 'root' is in the plan's numbering and to_caller (the
+// plan's inverse permutation) maps scores and levels back.
 template <class V>
-static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_run_info *info)
+static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_run_info *info,
+                   const uint32_t *to_caller)
 {
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
@@ -163,10 +188,8 @@ static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_ru
         frontier_alloc(g, F);
         uint32_t od = 0;
         rc = pool_take(c, F->list, 8);
-        if (!rc && hipMemcpyAsync(F->list.p, &root, 4, hipMemcpyHostToDevice, c->stream)) rc = fail(GG_ERR_HIP, "gg_bc: root");
-        if (!rc && hipMemcpyAsync(&od, (uint32_t *)g->outdeg.p + root, 4, hipMemcpyDeviceToHost, c->stream))
-            rc = fail(GG_ERR_HIP, "gg_bc: outdeg");
-        if (!rc) rc = sync(c);
+        if (!rc) rc = root_frontier(c, g, root, F);
+        if (!rc) rc = read_back(c, &od, (uint32_t *)F->list.p + 1, 4);
         F->d_m = 1;
         F->out_edges = od;
         F->deg_src = g->outdeg.p;
@@ -216,9 +239,8 @@ static int bc_impl(Graph *g, uint32_t root, V *score, int32_t *levels_out, gg_ru
                                                             (const V *)inv.p, n);
         }
         if (!rc) {
-            k_bc_final<V><<<nb, kBlock, 0, c->stream>>>((const V *)dep.p, (const V *)inv.p, score, n);
-            if (levels_out)
-                (void)hipMemcpyAsync(levels_out, level.p, 4ull * n, hipMemcpyDeviceToDevice, c->stream);
+            k_bc_final<V><<<nb, kBlock, 0, c->stream>>>((const V *)dep.p, (const V *)inv.p, score,
+                                                         (const int32_t *)level.p, levels_out, to_caller, n);
             if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_bc: launch");
         }
         (void)hipEventRecord(e1, c->stream);
@@ -276,17 +298,32 @@ extern "C" int gg_bc(gg_graph_t g, uint32_t root, int dtype, void *scores_dev, i
     if (dtype != GG_F64 && dtype != GG_F32) return fail(GG_ERR_INVALID, "gg_bc: dtype must be F64 or F32");
     GG_HIP(hipSetDevice(g->ctx->device));
     if (info) std::memset(info, 0, sizeof(*info));
-    return dtype == GG_F64 ? bc_impl<double>(g, root, (double *)scores_dev, levels_dev, info)
-                           : bc_impl<float>(g, root, (float *)scores_dev, levels_dev, info);
+    // With a renumbered plan both phases run on the plan's relabelled
+    // arrays; scores and levels are written back in the caller's numbering.
+    Graph t;
+    uint32_t r = root;
+    const uint32_t *to_caller = nullptr;
+    Graph *run = g;
+    if (g->plan.mode != 0 && g->m) {
+        GG_TRY(plan_alias(g, t));
+        int rc = read_back(g->ctx, &r, (const uint32_t *)g->plan.perm.p + root, 4);
+        if (rc) {
+            plan_alias_release(t);
+            return rc;
+        }
+        to_caller = (const uint32_t *)g->plan.inv.p;
+        run = &t;
+    }
+    const int rc = dtype == GG_F64 ? bc_impl<double>(run, r, (double *)scores_dev, levels_dev, info, to_caller)
+                                   : bc_impl<float>(run, r, (float *)scores_dev, levels_dev, info, to_caller);
+    if (run == &t) plan_alias_release(t);
+    return rc;
 }
 
-extern "C" int gg_sssp(gg_graph_t g, uint32_t root, int32_t *dist_dev, gg_run_info *info)
+// BellmanFord.C Compute (108-142) on g from 'root'.
+static int sssp_impl(Graph *g, uint32_t root, int32_t *dist_dev, gg_run_info *info)
 {
-    if (!g || !dist_dev) return fail(GG_ERR_INVALID, "gg_sssp: NULL argument");
-    if (root >= g->n) return fail(GG_ERR_INVALID, "gg_sssp: root out of range");
     Ctx *c = g->ctx;
-    GG_HIP(hipSetDevice(c->device));
-    if (info) std::memset(info, 0, sizeof(*info));
     const uint32_t n = g->n;
     Buf vis;
     int rc = pool_take(c, vis, 4ull * n);
@@ -305,10 +342,8 @@ extern "C" int gg_sssp(gg_graph_t g, uint32_t root, int32_t *dist_dev, gg_run_in
         frontier_alloc(g, F);
         uint32_t od = 0;
         rc = pool_take(c, F->list, 8);
-        if (!rc && hipMemcpyAsync(F->list.p, &root, 4, hipMemcpyHostToDevice, c->stream)) rc = fail(GG_ERR_HIP, "gg_sssp: root");
-        if (!rc && hipMemcpyAsync(&od, (uint32_t *)g->outdeg.p + root, 4, hipMemcpyDeviceToHost, c->stream))
-            rc = fail(GG_ERR_HIP, "gg_sssp: outdeg");
-        if (!rc) rc = sync(c);
+        if (!rc) rc = root_frontier(c, g, root, F);
+        if (!rc) rc = read_back(c, &od, (uint32_t *)F->list.p + 1, 4);
         F->d_m = 1;
         F->out_edges = od;
         F->deg_src = g->outdeg.p;
@@ -351,5 +386,29 @@ extern "C" int gg_sssp(gg_graph_t g, uint32_t root, int32_t *dist_dev, gg_run_in
     (void)hipEventDestroy(e1);
     frontier_delete(F);
     pool_give(c, vis);
+    return rc;
+}
+
+extern "C" int gg_sssp(gg_graph_t g, uint32_t root, int32_t *dist_dev, gg_run_info *info)
+{
+    if (!g || !dist_dev) return fail(GG_ERR_INVALID, "gg_sssp: NULL argument");
+    if (root >= g->n) return fail(GG_ERR_INVALID, "gg_sssp: root out of range");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    if (info) std::memset(info, 0, sizeof(*info));
+    if (g->plan.mode == 0 || g->m == 0) return sssp_impl(g, root, dist_dev, info);
+    // With a renumbered plan the rounds run on the plan's relabelled arrays
+    // (weights carried) and the distances are mapped back.
+    Graph t;
+    Buf dp;
+    uint32_t r = 0;
+    int rc = plan_alias(g, t);
+    if (!rc) rc = pool_take(c, dp, 4ull * g->n);
+    if (!rc) rc = read_back(c, &r, (const uint32_t *)g->plan.perm.p + root, 4);
+    if (!rc) rc = sssp_impl(&t, r, (int32_t *)dp.p, info);
+    if (!rc) rc = gather_idx(c, (const int32_t *)dp.p, (const uint32_t *)g->plan.perm.p, dist_dev, g->n);
+    if (!rc) rc = sync(c);
+    plan_alias_release(t);
+    pool_give(c, dp);
     return rc;
 }

@@ -160,6 +160,26 @@ def test_spmv_plan_ragged_rows(ctx):
     g.close()
 
 
+def test_plan_follows_new_weights(ctx):
+    """Weights set after the plan was built (on an unweighted graph, then
+    changed again) reach the plan's copies: SpMV and weighted PageRank
+    with the plan match the oracle on the new weights."""
+    off, ids = oracle.rmat_csr(11, 8, 12)
+    n, m = len(off) - 1, len(ids)
+    g = gg.Graph.from_csr(ctx, off, ids).reorder(1)
+    x = oracle.uniform_pm1(4, n)
+    xd = gg.DeviceArray.from_host(ctx, x)
+    for seed in (21, 22):
+        w = oracle.weights(seed, m)
+        g.set_weights(w)
+        y, _ = gg.spmv(g, xd)
+        assert _rel(y.to_host(), oracle.spmv_csr(off, ids, w, x)) < TOL64, seed
+        want, it, _ = oracle.wpagerank(off, ids, w, ALPHA, 1e-7, 100)
+        r, info = gg.wpagerank(g, np.float64, ALPHA, 1e-7, 100)
+        assert info.iters == it and _rel(r.to_host(), want) < TOL64, seed
+    g.close()
+
+
 def test_spmv_bwd_euler_identities_scale24(ctx):
     """At the full SpMV size (R-MAT scale 24, seed 3, FP32): y is linear in
     x and in w, so x.gx = w.gw = gy.y for any gy."""
