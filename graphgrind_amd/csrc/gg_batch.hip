@@ -50,16 +50,32 @@ struct OpMSBFS {
     }
 };
 
+// perm (may be NULL): the renumbered plan's new id of each caller's vertex
 __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_t n, uint32_t *__restrict__ fin,
-                             uint32_t *__restrict__ seen, int32_t *__restrict__ levels)
+                             uint32_t *__restrict__ seen, int32_t *__restrict__ levels,
+                             const uint32_t *__restrict__ perm)
 {
     // one thread per source; sources may repeat, so bits are ORed atomically
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsrc) return;
-    const uint32_t v = src[k];
+    const uint32_t v = perm ? perm[src[k]] : src[k];
     atomicOr(&fin[v], 1u << k);
     atomicOr(&seen[v], 1u << k);
     levels[(uint64_t)k * n + v] = 0;
+}
+
+// levels[k][v] = lp[k][perm[v]]: per-source levels from the plan's
+// numbering into the caller's
+__global__ void k_msbfs_unmap(const int32_t *__restrict__ lp, const uint32_t *__restrict__ perm, int nsrc,
+                              uint32_t n, int32_t *__restrict__ levels)
+{
+    const uint64_t total = (uint64_t)nsrc * n;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint64_t k = i / n;
+        const uint32_t v = (uint32_t)(i - k * n);
+        levels[i] = lp[k * n + perm[v]];
+    }
 }
 
 } // namespace gg
@@ -76,13 +92,20 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
     GG_HIP(hipSetDevice(c->device));
     if (info) std::memset(info, 0, sizeof(*info));
     const uint32_t n = g->n;
+    // With a renumbered plan the levels run over the plan's relabelled twin
+    // and the per-source levels are mapped back at the end.
+    const bool planned = g->plan.mode != 0 && g->m;
     CscView view;
-    GG_TRY(pull_view(g, &view));
-    Buf F[2], seen, src;
+    if (planned) view = plan_view(g);
+    else GG_TRY(pull_view(g, &view));
+    const uint32_t *perm = planned ? (const uint32_t *)g->plan.perm.p : nullptr;
+    Buf F[2], seen, src, lp;
     int rc = pool_take(c, F[0], 4ull * n);
     if (!rc) rc = pool_take(c, F[1], 4ull * n);
     if (!rc) rc = pool_take(c, seen, 4ull * n);
     if (!rc) rc = pool_take(c, src, 4ull * nsrc);
+    if (!rc && planned) rc = pool_take(c, lp, 4ull * nsrc * n);
+    int32_t *lv = planned ? (int32_t *)lp.p : levels;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -91,12 +114,12 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
         (void)hipEventRecord(e0, c->stream);
         rc = fill_u32(c, F[0].p, n, 0);
         if (!rc) rc = fill_u32(c, seen.p, n, 0);
-        if (!rc) rc = gg_fill(abi(c), levels, GG_I32, (uint64_t)nsrc * n, -1.0);
+        if (!rc) rc = gg_fill(abi(c), lv, GG_I32, (uint64_t)nsrc * n, -1.0);
         if (!rc && hipMemcpyAsync(src.p, sources, 4ull * nsrc, hipMemcpyHostToDevice, c->stream) != hipSuccess)
             rc = fail(GG_ERR_HIP, "gg_msbfs: upload sources");
         if (!rc) {
             k_msbfs_init<<<1, 64, 0, c->stream>>>((const uint32_t *)src.p, nsrc, n, (uint32_t *)F[0].p,
-                                                  (uint32_t *)seen.p, levels);
+                                                  (uint32_t *)seen.p, lv, perm);
             if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_msbfs: init");
         }
         unsigned long long *cnt = (unsigned long long *)c->counters.p;
@@ -104,7 +127,7 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
         while (!rc && level < (int)n) {
             level++;
             OpMSBFS op{(const uint32_t *)F[(level - 1) % 2].p, (uint32_t *)F[level % 2].p, (uint32_t *)seen.p,
-                       levels, n, level};
+                       lv, n, level};
             rc = counters_zero(c);
             if (!rc) rc = launch_pull(c, view, op, cnt, nullptr, nullptr);
             unsigned long long cn[4] = {0, 0, 0, 0};
@@ -112,6 +135,12 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
             const unsigned long long active = cn[CNT_ACTIVE];
             if (!rc && info && level <= 64) info->paths[level - 1] = GG_PATH_CSC;
             if (!rc && active == 0) break;
+        }
+        if (!rc && planned) {
+            const uint64_t total = (uint64_t)nsrc * n;
+            const uint32_t grid = (uint32_t)std::min<uint64_t>((total + kBlock - 1) / kBlock, 1u << 20);
+            k_msbfs_unmap<<<grid, kBlock, 0, c->stream>>>((const int32_t *)lp.p, perm, nsrc, n, levels);
+            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_msbfs: map-back");
         }
         (void)hipEventRecord(e1, c->stream);
         if (!rc) rc = sync(c);
@@ -129,6 +158,7 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
     pool_give(c, F[1]);
     pool_give(c, seen);
     pool_give(c, src);
+    pool_give(c, lp);
     return rc;
 }
 

@@ -125,8 +125,11 @@ def test_batched_ppr_and_adjoint_vs_oracle(ctx, dt, tol):
         p.close()
 
 
-def test_msbfs_levels_exact(ctx):
-    g = gg.Graph.rmat(ctx, 14, 16, 2, symmetric=True)
+@pytest.mark.parametrize("plan", [0, 1])
+def test_msbfs_levels_exact(ctx, plan):
+    """plan 1: the levels run on the hot-sources-first renumbering (sources
+    mapped in, per-source levels mapped back)."""
+    g = gg.Graph.rmat(ctx, 14, 16, 2, symmetric=True).reorder(plan)
     off, ids, _ = g.csr()
     n = g.n
     deg = np.diff(off.astype(np.int64))
@@ -141,8 +144,9 @@ def test_msbfs_levels_exact(ctx):
     g.close()
 
 
-def test_msbfs_directed_and_limits(ctx):
-    g = gg.Graph.rmat(ctx, 12, 8, 3)
+@pytest.mark.parametrize("plan", [0, 1])
+def test_msbfs_directed_and_limits(ctx, plan):
+    g = gg.Graph.rmat(ctx, 12, 8, 3).reorder(plan)
     off, ids, _ = g.csr()
     src = [0, 5, 17]
     lv, _ = gg.msbfs(g, src)
@@ -162,7 +166,7 @@ def test_msbfs_scale22_against_single_source(ctx):
     single-source engine's levels on a sample of sources, and the level
     structure of every source (neighbours differ by at most one level)."""
     print("msbfs scale=22", flush=True)
-    g = gg.Graph.rmat(ctx, 22, 16, 2, symmetric=True)
+    g = gg.Graph.rmat(ctx, 22, 16, 2, symmetric=True).reorder(1)
     off, ids, _ = g.csr()
     n = g.n
     deg = np.diff(off.astype(np.int64))
