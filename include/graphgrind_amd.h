@@ -136,12 +136,15 @@ int gg_graph_relabel(gg_graph_t g, const uint32_t *new_of_old, gg_graph_t *out);
  * counterpart in the engine; the reference reorders offline with VEBO,
  * graphtools/VEBO.C, and '-o').  mode 1: vertices by out-degree descending,
  * so the sources most edges read sit together in L2; mode 0 drops it.
- * Used, when present, by gg_pagerank, gg_spmv, gg_spmv_bwd, gg_wpagerank
- * and gg_wpagerank_adjoint, which permute their vectors in and map results
- * back to the caller's numbering (only the FP summation order changes).
+ * Used, when present, by gg_pagerank, gg_bfs, gg_cc, gg_spmv, gg_spmv_bwd,
+ * gg_wpagerank, gg_wpagerank_adjoint, gg_bc, gg_sssp and gg_msbfs, which
+ * map their inputs in and results back to the caller's numbering (integer
+ * results are the same, BFS parents stay a valid tree, only the FP
+ * summation order changes); gg_edge_map and gg_vertex_map ignore it.
  * Costs about 3 extra copies of the edge ids and weights in device memory
  * (the relabelled in-edge twin, plus a relabelled out-edge CSR built on the
- * first transpose-side call).  Drops the graph's captured loops. */
+ * first call that needs it).  Rebuilt when the weights change; drops the
+ * graph's captured loops. */
 int gg_graph_reorder(gg_graph_t g, int mode);
 
 /* Raw device addresses of a graph's layout, for interop (no reference
