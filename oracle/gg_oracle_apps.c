@@ -11,8 +11,14 @@
  *            first: Dependencies[d] += Dependencies[s] for s in level r+1
  *            and d not yet visited (BC_Back_F, 80-124), then level r adds
  *            inverseNumPaths (BC_Back_Vertex_F, 141-152); finally
- *            (Dependencies - inv) / inv (222).  fType is float in the
- *            reference; this restatement computes in FP64.
+ *            (Dependencies - inv) / inv (222).  A backward step from a
+ *            level whose forward out-degree sum is 0 is skipped, as the
+ *            reference's edgeMap returns empty() on the Levels' stored
+ *            counts (ligra-numa.h:946-947): on a directed graph whose
+ *            deepest level has no out-edges, that level's dependencies do
+ *            not reach the level above (textbook Brandes would add them).
+ *            fType is float in the reference; this restatement computes in
+ *            FP64.
  *   ggo_sssp app/BellmanFord.C: distances by relaxation rounds over the
  *            frontier of changed vertices (integer weights).
  *   ggo_prdelta app/PageRankDelta.C:124-176: each round the active
@@ -102,7 +108,14 @@ int ggo_bc(uint32_t n, const uint32_t *off, const uint32_t *ids, uint32_t root, 
             dep[v] += inv[v];
         }
     for (int r = round - 2; r >= 0; r--) {
-        for (uint32_t s = 0; s < n; s++) {
+        /* edgeMap returns empty() when the frontier's stored out-degree sum
+         * is 0 (ligra-numa.h:946-947); the Levels keep the sums of the
+         * forward graph, so after GA.transpose() the deepest level is not
+         * propagated when none of its vertices has a forward out-edge */
+        uint64_t fwd_out = 0;
+        for (uint32_t s = 0; s < n; s++)
+            if (level[s] == r + 1) fwd_out += off[s + 1] - off[s];
+        for (uint32_t s = 0; s < n && fwd_out; s++) {
             if (level[s] != r + 1) continue;
             for (uint32_t e = toff[s]; e < toff[s + 1]; e++) {
                 const uint32_t d = tids[e];

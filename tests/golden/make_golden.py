@@ -59,14 +59,15 @@ def case(name, off, ids, symmetric, P, root, apps, wseed=None, x=None, store=Non
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
-def case_apps(name, off, ids, symmetric, P, root, apps, wseed=None):
+def case_apps(name, off, ids, symmetric, P, root, apps, wseed=None, part_vertex=False):
     """Fixtures of the further apps (BC.C, BellmanFord.C, PageRankDelta.C,
     PageRankConverage.C) on one graph, single-threaded reference runs."""
     out = {"off": off, "ids": ids, "symmetric": np.int64(symmetric), "P": np.int64(P),
-           "root": np.int64(root)}
+           "root": np.int64(root), "part_vertex": np.int64(part_vertex)}
     for app in apps:
         w = int_weights(wseed, len(ids)) if app == "bellmanford" else None
-        res, iters, _ = oracle.run_ref(app, off, ids, w=w, symmetric=symmetric, P=P, root=root, threads=1)
+        res, iters, _ = oracle.run_ref(app, off, ids, w=w, symmetric=symmetric, P=P, root=root, threads=1,
+                                       part_vertex=part_vertex)
         out[app] = res
         out[f"{app}_iters"] = np.int64(iters)
         if w is not None:
@@ -97,6 +98,12 @@ def main():
     # identity in the reference too), the others on directed graphs
     case_apps("apps_sixnode_p2", SIX_OFF, SIX_IDS, False, 2, 0, ["bellmanford", "prdelta", "prconv"], wseed=7)
     case_apps("apps_sixnode_sym_p2", so, si, True, 2, 0, ["bc"])
+    # BC on a directed graph, vertex partitions (no COO lists, whose
+    # direction the reference's transpose() keeps): from root 100 the deepest
+    # level has no out-edges, so its stored out-degree sum is 0 and the
+    # reference's backward phase starts one level up
+    d_off, d_ids = oracle.rmat_csr(10, 16, 1)
+    case_apps("apps_rmat10_dir_vertex_p16", d_off, d_ids, False, 16, 100, ["bc"], part_vertex=True)
     off, ids = oracle.rmat_csr(10, 16, 2, symmetric=True)
     case_apps("apps_rmat10_sym_p16", off, ids, True, 16, 3, ["bc"])
     off, ids = oracle.rmat_csr(10, 16, 1)

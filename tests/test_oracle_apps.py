@@ -14,9 +14,13 @@ import refcheck as R
 INF = 2147483647 // 2
 
 
-def brandes_single_source(off, ids, root):
+def brandes_single_source(off, ids, root, stored_counts=True):
     """Textbook single-source Brandes: BFS path counts sigma, then
-    dependencies delta accumulated from the deepest level up."""
+    dependencies delta accumulated from the deepest level up.  With
+    stored_counts, the reference's edgeMap rule on the Levels' stored
+    out-degree sums (ligra-numa.h:946-947) is applied: a level whose
+    vertices have no out-edges passes nothing up (only the deepest level of
+    a directed graph can be such a level)."""
     n = len(off) - 1
     dist = np.full(n, -1, np.int64)
     sigma = np.zeros(n)
@@ -42,7 +46,11 @@ def brandes_single_source(off, ids, root):
         for w in ids[off[v]:off[v + 1]]:
             if dist[v] >= 0 and dist[w] == dist[v] + 1:
                 preds.setdefault(int(w), []).append(v)
+    deepest = dist.max()
+    dead = stored_counts and all(off[v + 1] == off[v] for v in np.flatnonzero(dist == deepest))
     for w in reversed(order):
+        if dead and dist[w] == deepest:
+            continue
         for v in preds.get(int(w), []):
             delta[v] += sigma[v] / sigma[w] * (1 + delta[w])
     return delta, dist, sigma
@@ -63,13 +71,32 @@ def test_bc_matches_brandes(scale, ef, seed, sym, root):
 
 
 def test_bc_tiny_by_hand():
-    # diamond 0->1,0->2,1->3,2->3 plus a tail 3->4: two shortest paths to 3
-    off, ids = oracle.csr_from_edges(5, [0, 0, 1, 2, 3], [1, 2, 3, 3, 4])
+    # diamond 0->1,0->2,1->3,2->3 plus a tail 3->4 and an edge 4->0 back to
+    # the root: two shortest paths to 3
+    off, ids = oracle.csr_from_edges(5, [0, 0, 1, 2, 3, 4], [1, 2, 3, 3, 4, 0])
     score, level, sigma, _ = oracle.bc(off, ids, 0)
     assert list(level) == [0, 1, 1, 2, 3]
     assert list(sigma) == [1, 1, 1, 2, 2]
-    # delta(3) = 1 (vertex 4); delta(1) = delta(2) = 1/2 * (1 + 1) = 1; delta(0) = 1+1 + ... = 4
+    # delta(3) = 1 (vertex 4); delta(1) = delta(2) = 1/2 * (1 + 1) = 1; delta(0) = 2 * (1 + 1) = 4
     assert np.allclose(score, [4.0, 1.0, 1.0, 1.0, 0.0])
+
+
+def test_bc_deepest_level_without_out_edges():
+    """The same diamond without 4->0: the deepest level {4} has no
+    out-edges, so its stored out-degree sum is 0 and the reference's first
+    backward edgeMap returns empty(): 3 gets no dependency from 4
+    (textbook Brandes would give [4, 1, 1, 1, 0]).  The compiled reference
+    gives [0, .5, .5, 0, 0] here: its root also has no in-edge, and after
+    GA.transpose() its compressed CSC still lists only vertices with
+    original in-edges, so the dense backward step to the root skips it.
+    Here every structure follows the transpose, so the root keeps its
+    dependencies: [3, .5, .5, 0, 0]."""
+    off, ids = oracle.csr_from_edges(5, [0, 0, 1, 2, 3], [1, 2, 3, 3, 4])
+    score, level, sigma, rounds = oracle.bc(off, ids, 0)
+    assert list(level) == [0, 1, 1, 2, 3] and rounds == 4
+    assert np.allclose(score, [3.0, 0.5, 0.5, 0.0, 0.0])
+    textbook, _, _ = brandes_single_source(off, ids, 0, stored_counts=False)
+    assert np.allclose(textbook, [4.0, 1.0, 1.0, 1.0, 0.0])
 
 
 @pytest.mark.parametrize("scale,seed,sym", [(8, 3, False), (10, 4, True)])
