@@ -1824,6 +1824,10 @@ extern "C" int gg_graph_set_option(gg_graph_t g, int option, int64_t value)
             return rows_build(g->ctx, g->plan.rows, (const uint32_t *)g->plan.toff.p, g->n, g->m,
                               g->edges_per_lane);
         return GG_OK;
+    case GG_OPT_DENSE_COO:
+        if (value != 0 && value != 1) return fail(GG_ERR_INVALID, "gg_graph_set_option: dense COO must be 0 or 1");
+        g->dense_coo = (int)value;
+        return GG_OK;
     default:
         return fail(GG_ERR_INVALID, "gg_graph_set_option: unknown option " + std::to_string(option));
     }
@@ -1835,6 +1839,7 @@ extern "C" int gg_graph_get_option(gg_graph_t g, int option, int64_t *value)
     switch (option) {
     case GG_OPT_PULL_SCHEDULE: *value = g->pull_schedule; return GG_OK;
     case GG_OPT_ROW_EDGES_PER_LANE: *value = g->edges_per_lane; return GG_OK;
+    case GG_OPT_DENSE_COO: *value = g->dense_coo; return GG_OK;
     default: return fail(GG_ERR_INVALID, "gg_graph_get_option: unknown option " + std::to_string(option));
     }
 }

@@ -240,6 +240,7 @@ struct Graph {
     // tuning (gg_graph_set_option)
     int pull_schedule = 1;  // 0 merge-path tiles, 1 row blocks where built
     int edges_per_lane = 8; // row blocks: target edges per lane of a row group
+    int dense_coo = 0;      // 1: the dense COO branch streams the COO lists
 };
 // Drops captured loops (their addresses or schedules changed).
 void graph_exec_drop(Graph *g);

@@ -377,6 +377,107 @@ __device__ __forceinline__ uint32_t upper_idx(const uint32_t *a, uint32_t lo, ui
     return lo;
 }
 
+// One update of edge (v -> ngh) by the functor of A.kind (the apps'
+// updateAtomic: BFS_F, CC_F, PR_F, SPMV_F, BF_F, BC_F), w[pos] its weight;
+// returns whether ngh joins the output frontier.  Shared by the sparse push
+// and the COO traversal.
+__device__ __forceinline__ bool edge_update(const PushArgs &A, uint32_t v, uint32_t ngh, uint64_t pos)
+{
+    bool emit = false;
+    switch (A.kind) {
+    case PUSH_BFS: {
+        int32_t *par = (int32_t *)A.a;
+        if (par[ngh] == -1 && atomicCAS(&par[ngh], -1, (int32_t)v) == -1) {
+            emit = true;
+            if (A.levels) A.levels[ngh] = A.level;
+        }
+        break;
+    }
+    case PUSH_CC: {
+        uint32_t *ids = (uint32_t *)A.a;
+        const uint32_t *prev = (const uint32_t *)A.b;
+        const uint32_t val = prev[v];
+        if (val < ids[ngh]) {
+            const uint32_t old = atomicMin(&ids[ngh], val);
+            emit = val < old && old == prev[ngh];
+        }
+        break;
+    }
+    case PUSH_PATHSUM: { // BC_F::updateAtomic: an add; first update emits
+        const uint8_t *vis = (const uint8_t *)A.b;
+        if (!vis[ngh]) {
+            if (A.f32) {
+                float *x = (float *)A.a;
+                emit = atomicAdd(&x[ngh], x[v]) == 0.0f;
+            } else {
+                double *x = (double *)A.a;
+                emit = atomicAdd(&x[ngh], x[v]) == 0.0;
+            }
+        }
+        break;
+    }
+    case PUSH_SSSP: { // BF_F::updateAtomic: writeMin && CAS(Visited, 0, 1)
+        int32_t *sp = (int32_t *)A.a;
+        int32_t *vis = (int32_t *)A.b;
+        const int32_t nd = sp[v] + (A.w ? (int32_t)A.w[pos] : 1);
+        if (nd < sp[ngh]) {
+            const int32_t old = atomicMin(&sp[ngh], nd);
+            if (nd < old) emit = atomicCAS(&vis[ngh], 0, 1) == 0;
+        }
+        break;
+    }
+    case PUSH_PR:
+        if (A.f32) {
+            const float *pc = (const float *)A.a;
+            atomicAdd(&((float *)A.b)[ngh], pc[v] / (float)A.outdeg[v]);
+        } else {
+            const double *pc = (const double *)A.a;
+            atomicAdd(&((double *)A.b)[ngh], pc[v] / (double)A.outdeg[v]);
+        }
+        emit = true;
+        break;
+    default: { // SPMV
+        const float wv = A.w ? A.w[pos] : 1.0f;
+        if (A.f32) atomicAdd(&((float *)A.b)[ngh], ((const float *)A.a)[v] * wv);
+        else atomicAdd(&((double *)A.b)[ngh], ((const double *)A.a)[v] * (double)wv);
+        emit = true;
+        break;
+    }
+    }
+    return emit;
+}
+
+// edgeMapDense over the COO partitions (ligra-numa.h:803-866): the
+// partitions' Hilbert-ordered edge lists, back to back, in blocks of
+// kCooTile edges staged in LDS by coalesced loads; each edge whose source is
+// in the frontier is updated by the functor's atomic form, and a destination
+// it reports joins the dense output.
+constexpr int kCooTile = kBlock * 4;
+
+__global__ __launch_bounds__(kBlock) void k_coo(PushArgs A, const uint32_t *__restrict__ src,
+                                                const uint32_t *__restrict__ dst, uint64_t m,
+                                                const uint32_t *__restrict__ bm, int bit, uint8_t *__restrict__ next)
+{
+    __shared__ uint32_t s_src[kCooTile], s_dst[kCooTile];
+    const uint64_t t0 = (uint64_t)blockIdx.x * kCooTile;
+    for (int i = threadIdx.x; i < kCooTile; i += kBlock) {
+        const uint64_t e = t0 + (uint64_t)i;
+        if (e < m) {
+            s_src[i] = __builtin_nontemporal_load(&src[e]);
+            s_dst[i] = __builtin_nontemporal_load(&dst[e]);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kCooTile; i += kBlock) {
+        const uint64_t e = t0 + (uint64_t)i;
+        if (e >= m) break;
+        const uint32_t s = s_src[i];
+        if (!bit && !bit_test(bm, s)) continue;
+        const uint32_t d = s_dst[i];
+        if (edge_update(A, s, d, e)) next[d] = 1;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_push(PushArgs A)
 {
     __shared__ uint32_t s_off[kPushLds];
@@ -408,66 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_push(PushArgs A)
             const uint32_t v = A.list[kk];
             const uint32_t pos = A.off[v] + (uint32_t)(e - A.offs[kk]);
             ngh = A.ids[pos];
-            switch (A.kind) {
-            case PUSH_BFS: {
-                int32_t *par = (int32_t *)A.a;
-                if (par[ngh] == -1 && atomicCAS(&par[ngh], -1, (int32_t)v) == -1) {
-                    emit = true;
-                    if (A.levels) A.levels[ngh] = A.level;
-                }
-                break;
-            }
-            case PUSH_CC: {
-                uint32_t *ids = (uint32_t *)A.a;
-                const uint32_t *prev = (const uint32_t *)A.b;
-                const uint32_t val = prev[v];
-                if (val < ids[ngh]) {
-                    const uint32_t old = atomicMin(&ids[ngh], val);
-                    emit = val < old && old == prev[ngh];
-                }
-                break;
-            }
-            case PUSH_PATHSUM: { // BC_F::updateAtomic: an add; first update emits
-                const uint8_t *vis = (const uint8_t *)A.b;
-                if (!vis[ngh]) {
-                    if (A.f32) {
-                        float *x = (float *)A.a;
-                        emit = atomicAdd(&x[ngh], x[v]) == 0.0f;
-                    } else {
-                        double *x = (double *)A.a;
-                        emit = atomicAdd(&x[ngh], x[v]) == 0.0;
-                    }
-                }
-                break;
-            }
-            case PUSH_SSSP: { // BF_F::updateAtomic: writeMin && CAS(Visited, 0, 1)
-                int32_t *sp = (int32_t *)A.a;
-                int32_t *vis = (int32_t *)A.b;
-                const int32_t nd = sp[v] + (A.w ? (int32_t)A.w[pos] : 1);
-                if (nd < sp[ngh]) {
-                    const int32_t old = atomicMin(&sp[ngh], nd);
-                    if (nd < old) emit = atomicCAS(&vis[ngh], 0, 1) == 0;
-                }
-                break;
-            }
-            case PUSH_PR:
-                if (A.f32) {
-                    const float *pc = (const float *)A.a;
-                    atomicAdd(&((float *)A.b)[ngh], pc[v] / (float)A.outdeg[v]);
-                } else {
-                    const double *pc = (const double *)A.a;
-                    atomicAdd(&((double *)A.b)[ngh], pc[v] / (double)A.outdeg[v]);
-                }
-                emit = true;
-                break;
-            default: { // SPMV
-                const float wv = A.w ? A.w[pos] : 1.0f;
-                if (A.f32) atomicAdd(&((float *)A.b)[ngh], ((const float *)A.a)[v] * wv);
-                else atomicAdd(&((double *)A.b)[ngh], ((const double *)A.a)[v] * (double)wv);
-                emit = true;
-                break;
-            }
-            }
+            emit = edge_update(A, v, ngh, pos);
         }
         // wave-ballot compaction of the emitted destinations
         const uint64_t mask = __ballot(emit);
@@ -840,6 +882,37 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         if ((rc = ensure_bytes(out))) return bail(rc);
         const bool coo = (int64_t)(in->d_m + in->out_edges) > dense_thr && g->P && !g->part_vertex;
         if (path) *path = coo ? GG_PATH_COO : GG_PATH_CSC;
+        if (coo && g->dense_coo && g->coo_src.p && m && (!g->weighted || g->coo_w.p)) {
+            // the COO traversal, on request (GG_OPT_DENSE_COO)
+            if ((rc = fill_bytes(c, out->bytes.p, n, 0))) return bail(rc);
+            PushArgs A{};
+            A.outdeg = (const uint32_t *)g->outdeg.p;
+            A.w = (g->weighted && op->kind != GG_OP_SUM) ? (const float *)g->coo_w.p : nullptr;
+            A.kind = op->kind == GG_OP_BFS ? PUSH_BFS
+                     : op->kind == GG_OP_CC ? PUSH_CC
+                     : op->kind == GG_OP_PAGERANK ? PUSH_PR
+                     : op->kind == GG_OP_SSSP ? PUSH_SSSP
+                     : op->kind == GG_OP_PATHSUM ? PUSH_PATHSUM
+                                                  : PUSH_SPMV;
+            A.f32 = f32;
+            A.a = op->a;
+            A.b = op->b;
+            A.levels = ex ? ex->levels : nullptr;
+            A.level = ex ? ex->level : 0;
+            k_coo<<<blocks_for(m, kCooTile), kBlock, 0, c->stream>>>(
+                A, (const uint32_t *)g->coo_src.p, (const uint32_t *)g->coo_dst.p, m,
+                (const uint32_t *)in->bitmap.p, in->bit ? 1 : 0, (uint8_t *)out->bytes.p);
+            GG_HIP(hipGetLastError());
+            out->has_dense = true;
+            out->dense_valid = true;
+            if ((rc = frontier_stats_dense(out))) return bail(rc);
+            out->deg_src = g->outdeg.p;
+            if (ex) ex->scanned += m;
+            if ((rc = ensure_bitmap(out))) return bail(rc);
+            if ((rc = bitmap_from_bytes(c, (uint8_t *)out->bytes.p, (uint32_t *)out->bitmap.p, n))) return bail(rc);
+            *pout = out;
+            return GG_OK;
+        }
         if ((rc = counters_zero(c))) return bail(rc);
         unsigned long long *cnt = (unsigned long long *)c->counters.p;
         const int bit = in->bit ? 1 : 0;

@@ -352,6 +352,14 @@ int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t *levels_de
  * choices of the device pull, set per graph; the autotuner sweeps them. */
 #define GG_OPT_PULL_SCHEDULE 0      /* 0 merge-path tiles, 1 row blocks (default) */
 #define GG_OPT_ROW_EDGES_PER_LANE 1 /* row blocks: target edges per lane of a row group (8) */
+/* The kernel of edgeMap's dense COO branch (ligra-numa.h:953-961, taken when
+ * d_m + out-degree > m/2 on an edge-partitioned graph): 0 (default) the
+ * destination-owned pull over the in-edge twin, which computes the same
+ * values with a fixed summation order; 1 the COO traversal: the Hilbert-
+ * ordered partition edge lists (edgeMapDense, ligra-numa.h:803-866) streamed
+ * in LDS-staged blocks with the functors' atomic updates, so floating-point
+ * sums are in no fixed order. */
+#define GG_OPT_DENSE_COO 2
 int gg_graph_set_option(gg_graph_t g, int option, int64_t value);
 int gg_graph_get_option(gg_graph_t g, int option, int64_t *value);
 /* Autotuner: times 'trials' pulls of a probe operator (one gathered value

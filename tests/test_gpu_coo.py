@@ -1,0 +1,140 @@
+"""The COO traversal of edgeMap's dense COO branch (GG_OPT_DENSE_COO = 1:
+the Hilbert-ordered partition edge lists streamed with the functors' atomic
+updates, ligra-numa.h:803-866) against the default destination-owned pull
+on the same inputs, one edgeMap step per operator, and the apps run whole
+with it.  Bars: output frontiers and integer results exact; FP sums within
+1e-12 relative (the atomic order is not fixed); path counts exact (integer
+sums in FP64)."""
+import numpy as np
+import pytest
+
+import graphgrind_amd as gg
+import oracle
+import refcheck as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gg.Context(0)
+    yield c
+    c.close()
+
+
+def _graph(ctx, scale, seed, sym=False, weighted=False, P=16):
+    off, ids = oracle.rmat_csr(scale, 16, seed, symmetric=sym)
+    w = oracle.weights(seed, len(ids)) if weighted else None
+    g = gg.Graph.from_csr(ctx, off, ids, w=w, symmetric=sym).partition(P)
+    return g, off, ids, w
+
+
+def _both(g, front, kind, make):
+    """edge_map under the pull (option 0) and the COO traversal (option 1);
+    'make' returns fresh (a, b) device arrays; returns per option the output
+    flags, the path taken and the arrays' host values."""
+    res = []
+    for opt in (0, 1):
+        g.set_option(gg.OPT_DENSE_COO, opt)
+        a, b = make()
+        F = gg.Frontier.from_dense(g, front) if front is not None else gg.Frontier.bits(g)
+        out, path = gg.edge_map(g, F, kind, a, b)
+        assert path == gg.PATH_COO, path
+        res.append((out.dense(), a.to_host(), None if b is None else b.to_host(), out.info()))
+    g.set_option(gg.OPT_DENSE_COO, 0)
+    return res
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
+
+
+@pytest.mark.parametrize("frac", [None, 0.7])
+@pytest.mark.parametrize("kind", ["pagerank", "spmv", "sum"])
+def test_coo_fp_sums(ctx, kind, frac):
+    g, off, ids, w = _graph(ctx, 13, 3, weighted=(kind == "spmv"))
+    n = g.n
+    rng = np.random.default_rng(5)
+    front = None if frac is None else (rng.random(n) < frac).astype(np.uint8)
+    x = rng.random(n)
+    y0 = rng.random(n)
+    k = {"pagerank": gg.OP_PAGERANK, "spmv": gg.OP_SPMV, "sum": gg.OP_SUM}[kind]
+    (f0, _, b0, i0), (f1, _, b1, i1) = _both(
+        g, front, k, lambda: (gg.DeviceArray.from_host(ctx, x), gg.DeviceArray.from_host(ctx, y0)))
+    assert np.array_equal(f0, f1)
+    assert i0 == i1
+    assert _rel(b1, b0) < 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("sym", [True, False])
+def test_coo_cc_bfs_pathsum(ctx, sym):
+    g, off, ids, _ = _graph(ctx, 12, 4, sym=sym)
+    n = g.n
+    rng = np.random.default_rng(9)
+    front = (rng.random(n) < 0.8).astype(np.uint8)
+    # Components.C one round: labels and the changed set exact
+    lab0 = rng.permutation(n).astype(np.uint32)
+    (f0, a0, _, _), (f1, a1, _, _) = _both(
+        g, front, gg.OP_CC, lambda: (gg.DeviceArray.from_host(ctx, lab0), gg.DeviceArray.from_host(ctx, lab0)))
+    assert np.array_equal(a0, a1) and np.array_equal(f0, f1)
+    # BFS.C one level: the same newly visited set, every new parent a frontier
+    # in-neighbour
+    par0 = np.where(rng.random(n) < 0.3, np.arange(n), -1).astype(np.int32)
+    (f0, p0, _, _), (f1, p1, _, _) = _both(g, front, gg.OP_BFS, lambda: (gg.DeviceArray.from_host(ctx, par0), None))
+    assert np.array_equal(f0, f1) and np.array_equal(p0 >= 0, p1 >= 0)
+    new = np.flatnonzero(f1)
+    src = np.repeat(np.arange(n), np.diff(off))
+    edges = set(zip(src.tolist(), ids.tolist()))
+    for d in new[:2000]:
+        assert front[p1[d]] and (int(p1[d]), int(d)) in edges
+    # BC.C forward step: path counts (integers in FP64) exact
+    vis = (rng.random(n) < 0.3).astype(np.uint8) | front
+    x0 = np.where(front.astype(bool), rng.integers(1, 9, n), 0).astype(np.float64)
+    (f0, x0a, _, _), (f1, x1a, _, _) = _both(
+        g, front, gg.OP_PATHSUM, lambda: (gg.DeviceArray.from_host(ctx, x0), gg.DeviceArray.from_host(ctx, vis)))
+    assert np.array_equal(f0, f1) and np.array_equal(x0a, x1a)
+    g.close()
+
+
+def test_coo_sssp_whole(ctx):
+    """BellmanFord.C with the COO traversal: one round may relax in another
+    order than the pull (the reference's COO loop reads live distances), so
+    the check is on the converged distances, exact."""
+    off, ids = oracle.rmat_csr(12, 16, 6)
+    wl = (1 + (np.arange(len(ids)) * 2654435761 % 19)).astype(np.float32)
+    g = gg.Graph.from_csr(ctx, off, ids, w=wl).partition(16)
+    want, _ = oracle.sssp(off, ids, wl, 0)
+    for opt in (0, 1):
+        g.set_option(gg.OPT_DENSE_COO, opt)
+        d, _ = gg.sssp(g, 0)
+        assert np.array_equal(d.to_host().astype(np.int64), want), opt
+    g.close()
+
+
+@pytest.mark.parametrize("sym", [True, False])
+def test_coo_apps_whole(ctx, sym):
+    """BFS, Components, BC and Bellman-Ford on a partitioned graph with the
+    COO traversal on its COO levels: the oracle's answers."""
+    off, ids = oracle.rmat_csr(14, 16, 2, symmetric=sym)
+    g = gg.Graph.from_csr(ctx, off, ids, symmetric=sym).partition(64)
+    g.set_option(gg.OPT_DENSE_COO, 1)
+    og = oracle.Graph(off, ids, symmetric=sym, P=64)
+    _, lv, tr = og.bfs(100)
+    p, l, info = gg.bfs(g, 100)
+    R.check_parents(off, ids, 100, p.to_host())
+    assert np.array_equal(l.to_host(), R.bfs_levels(off, ids, 100)) and info.iters == lv
+    want_cc, _, _ = og.cc()
+    lab, cinfo = gg.components(g)
+    assert np.array_equal(lab.to_host(), want_cc)
+    want_bc, wl, _, rounds = oracle.bc(off, ids, 100)
+    sc, blv, binfo = gg.bc(g, 100, np.float64)
+    reached = wl >= 0
+    got = sc.to_host()
+    assert np.array_equal(blv.to_host(), wl) and binfo.iters == rounds
+    assert _rel(got[reached], want_bc[reached]) < 1e-12
+    paths = info.path_list() + cinfo.path_list() + binfo.path_list()
+    assert "coo" in paths  # the COO traversal ran
+    g.close()
