@@ -408,10 +408,10 @@ __device__ __forceinline__ bool edge_update(const PushArgs &A, uint32_t v, uint3
         if (!vis[ngh]) {
             if (A.f32) {
                 float *x = (float *)A.a;
-                emit = atomicAdd(&x[ngh], x[v]) == 0.0f;
+                emit = unsafeAtomicAdd(&x[ngh], x[v]) == 0.0f;
             } else {
                 double *x = (double *)A.a;
-                emit = atomicAdd(&x[ngh], x[v]) == 0.0;
+                emit = unsafeAtomicAdd(&x[ngh], x[v]) == 0.0;
             }
         }
         break;
@@ -429,17 +429,17 @@ __device__ __forceinline__ bool edge_update(const PushArgs &A, uint32_t v, uint3
     case PUSH_PR:
         if (A.f32) {
             const float *pc = (const float *)A.a;
-            atomicAdd(&((float *)A.b)[ngh], pc[v] / (float)A.outdeg[v]);
+            unsafeAtomicAdd(&((float *)A.b)[ngh], pc[v] / (float)A.outdeg[v]);
         } else {
             const double *pc = (const double *)A.a;
-            atomicAdd(&((double *)A.b)[ngh], pc[v] / (double)A.outdeg[v]);
+            unsafeAtomicAdd(&((double *)A.b)[ngh], pc[v] / (double)A.outdeg[v]);
         }
         emit = true;
         break;
     default: { // SPMV
         const float wv = A.w ? A.w[pos] : 1.0f;
-        if (A.f32) atomicAdd(&((float *)A.b)[ngh], ((const float *)A.a)[v] * wv);
-        else atomicAdd(&((double *)A.b)[ngh], ((const double *)A.a)[v] * (double)wv);
+        if (A.f32) unsafeAtomicAdd(&((float *)A.b)[ngh], ((const float *)A.a)[v] * wv);
+        else unsafeAtomicAdd(&((double *)A.b)[ngh], ((const double *)A.a)[v] * (double)wv);
         emit = true;
         break;
     }
