@@ -30,7 +30,8 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("plan", [0, 1])
 @pytest.mark.parametrize("scale,ef,seed,sym,root", [(8, 8, 2, True, 3), (14, 16, 2, True, 100),
-                                                    (12, 8, 5, False, 1), (16, 16, 1, False, 7)])
+                                                    (12, 8, 5, False, 1), (16, 16, 1, False, 7),
+                                                    (14, 16, 2, False, 100)])  # deepest level without out-edges
 def test_bc_vs_oracle(ctx, scale, ef, seed, sym, root, plan):
     """plan 1: both phases on the hot-sources-first renumbering, scores and
     levels written back in the caller's numbering."""
