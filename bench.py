@@ -424,8 +424,8 @@ def main():
                     help="only run the SpMV and PageRank adjoint legs (for kernel traces of them)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every run of this build uses a single MI355X; "
-              "multi-GPU sharding is a design (see DESIGN.md), not a measured path.", file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1): the engine runs on a single MI355X; "
+              "sharding across GPUs is designed but not built or measured.", file=sys.stderr)
         sys.exit(2)
     import graphgrind_amd as gg
 
