@@ -51,21 +51,32 @@ def _rel(a, b):
     return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
 
 
+@pytest.mark.parametrize("dt,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])
 @pytest.mark.parametrize("frac", [None, 0.7])
 @pytest.mark.parametrize("kind", ["pagerank", "spmv", "sum"])
-def test_coo_fp_sums(ctx, kind, frac):
+def test_coo_fp_sums(ctx, kind, frac, dt, tol):
     g, off, ids, w = _graph(ctx, 13, 3, weighted=(kind == "spmv"))
     n = g.n
     rng = np.random.default_rng(5)
     front = None if frac is None else (rng.random(n) < frac).astype(np.uint8)
-    x = rng.random(n)
-    y0 = rng.random(n)
+    x = rng.random(n).astype(dt)
+    y0 = rng.random(n).astype(dt)
     k = {"pagerank": gg.OP_PAGERANK, "spmv": gg.OP_SPMV, "sum": gg.OP_SUM}[kind]
     (f0, _, b0, i0), (f1, _, b1, i1) = _both(
         g, front, k, lambda: (gg.DeviceArray.from_host(ctx, x), gg.DeviceArray.from_host(ctx, y0)))
     assert np.array_equal(f0, f1)
     assert i0 == i1
-    assert _rel(b1, b0) < 1e-12
+    assert _rel(b1, b0) < tol
+    g.close()
+
+
+def test_coo_option_values(ctx):
+    g, _, _, _ = _graph(ctx, 8, 1)
+    assert g.get_option(gg.OPT_DENSE_COO) == 0
+    g.set_option(gg.OPT_DENSE_COO, 1)
+    assert g.get_option(gg.OPT_DENSE_COO) == 1
+    with pytest.raises(gg.GGError):
+        g.set_option(gg.OPT_DENSE_COO, 2)
     g.close()
 
 
