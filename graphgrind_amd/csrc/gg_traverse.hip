@@ -454,6 +454,39 @@ __device__ __forceinline__ bool edge_update(const PushArgs &A, uint32_t v, uint3
 // it reports joins the dense output.
 constexpr int kCooTile = kBlock * 4;
 
+// The sum functors (PR_F, SPMV_F) in the COO traversal: the lanes of a wave
+// whose destination is the first active lane's combine their terms with a
+// wave sum and make one atomic add, the others add their own.  Hub
+// destinations fill whole waves of a Hilbert-ordered block, and their
+// same-address atomics would otherwise serialise.  Called by every lane.
+template <class V>
+__device__ __forceinline__ void coo_sum_wave(const PushArgs &A, bool act, uint32_t s, uint32_t d, uint64_t e,
+                                             uint8_t *__restrict__ next)
+{
+    const uint64_t am = __ballot(act);
+    if (!am) return;
+    const int lane = threadIdx.x & 63;
+    const int lead = __ffsll((unsigned long long)am) - 1;
+    const uint32_t dl = __shfl(d, lead, 64);
+    V term = V(0);
+    if (act) {
+        const V *a = (const V *)A.a;
+        term = A.kind == PUSH_PR ? a[s] / (V)A.outdeg[s] : a[s] * (V)(A.w ? A.w[e] : 1.0f);
+    }
+    const bool mine = act && d == dl;
+    V tot = mine ? term : V(0);
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
+    V *b = (V *)A.b;
+    if (lane == lead) {
+        unsafeAtomicAdd(&b[dl], tot);
+        next[dl] = 1;
+    }
+    if (act && !mine) {
+        unsafeAtomicAdd(&b[d], term);
+        next[d] = 1;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_coo(PushArgs A, const uint32_t *__restrict__ src,
                                                 const uint32_t *__restrict__ dst, uint64_t m,
                                                 const uint32_t *__restrict__ bm, int bit, uint8_t *__restrict__ next)
@@ -468,13 +501,23 @@ __global__ __launch_bounds__(kBlock) void k_coo(PushArgs A, const uint32_t *__re
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kCooTile; i += kBlock) {
+    const bool sums = A.kind == PUSH_PR || A.kind == PUSH_SPMV;
+    for (int it = 0; it < kCooTile / kBlock; it++) {
+        const int i = it * kBlock + threadIdx.x;
         const uint64_t e = t0 + (uint64_t)i;
-        if (e >= m) break;
-        const uint32_t s = s_src[i];
-        if (!bit && !bit_test(bm, s)) continue;
-        const uint32_t d = s_dst[i];
-        if (edge_update(A, s, d, e)) next[d] = 1;
+        bool act = e < m;
+        uint32_t s = 0, d = 0;
+        if (act) {
+            s = s_src[i];
+            d = s_dst[i];
+            act = bit || bit_test(bm, s);
+        }
+        if (sums) { // wave-uniform branch: the ballot below needs every lane
+            if (A.f32) coo_sum_wave<float>(A, act, s, d, e, next);
+            else coo_sum_wave<double>(A, act, s, d, e, next);
+        } else if (act && edge_update(A, s, d, e)) {
+            next[d] = 1;
+        }
     }
 }
 
