@@ -690,25 +690,36 @@ inline bool CAS(ET *ptr, ET oldv, ET newv)
         return __sync_bool_compare_and_swap(ptr, oldv, newv);
     }
 }
+// The current value is read once, atomically: a plain read may be split by
+// the compiler into one load for the arithmetic and another for the CAS's
+// expected bits, and a write between the two then slips through the CAS.
+template <class ET>
+inline ET atomic_peek(ET *a)
+{
+    ET v;
+    __atomic_load(a, &v, __ATOMIC_RELAXED);
+    return v;
+}
 template <class ET>
 inline bool writeMin(ET *a, ET b)
 {
     ET c;
     bool r = 0;
-    do c = *a;
+    do c = atomic_peek(a);
     while (c > b && !(r = CAS(a, c, b)));
     return r;
 }
+// writeAdd (utils.h:468-477): a CAS loop of the element's own width, so
+// 4-byte types touch only their own 4 bytes
 template <class ET>
 inline void writeAdd(ET *a, ET b)
 {
+    static_assert(sizeof(ET) == 4 || sizeof(ET) == 8, "writeAdd: 4- or 8-byte element");
     ET o, n;
     do {
-        o = *a;
+        o = atomic_peek(a);
         n = o + b;
-    } while (!__sync_bool_compare_and_swap(reinterpret_cast<uint64_t *>(a),
-                                          *reinterpret_cast<uint64_t *>(&o),
-                                          *reinterpret_cast<uint64_t *>(&n)));
+    } while (!CAS(a, o, n));
 }
 
 namespace sequence {

@@ -31,6 +31,19 @@ def test_reference_app_compiles_unchanged(app):
     assert r.returncode == 0, r.stderr[-2000:]
 
 
+def test_host_atomics_under_threads():
+    """CAS / writeMin / writeAdd of the front end (utils.h:423-477) on float,
+    int, unsigned, double and long from 8 OpenMP threads: exact sums and
+    minima, and the neighbouring elements untouched (including a target at
+    the end of its array)."""
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "apps"), "_build/atomics"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run([os.path.join(BUILD, "atomics")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count(" ok") == 5, r.stdout
+
+
 def _run(exe, app, args):
     with tempfile.TemporaryDirectory() as td:
         out = os.path.join(td, "out.bin")
