@@ -177,21 +177,39 @@ def _ptr(a):
     return None if a is None else a.ctypes.data
 
 
-def _dev(a):
+def _dev(a, count=None, dtype=None, ctx=None, name="array"):
     """(device pointer, numpy dtype) of a DeviceArray, or of a contiguous
     device tensor exposing data_ptr() (a torch tensor on a HIP device; the
-    caller orders its own stream before the call)."""
+    caller orders its own stream before the call).
+
+    With 'count' the array must hold at least that many elements, with
+    'dtype' it must have that element type, and with 'ctx' it must live on
+    that context's device.  A mismatch raises ValueError before any device
+    work, since the library writes count * itemsize bytes through the
+    pointer."""
     if a is None:
         return None, None
     if isinstance(a, DeviceArray):
-        return a.ptr, a.dtype
-    if hasattr(a, "data_ptr"):
+        ptr, dt, size = a.ptr, a.dtype, a.count
+        if ptr is None:
+            raise ValueError(f"{name}: freed DeviceArray")
+        if ctx is not None and a.ctx.device != ctx.device:
+            raise ValueError(f"{name}: on device {a.ctx.device}, the graph is on device {ctx.device}")
+    elif hasattr(a, "data_ptr"):
         if getattr(a.device, "type", None) != "cuda":
-            raise ValueError("device tensor required")
+            raise ValueError(f"{name}: device tensor required")
         if not a.is_contiguous():
-            raise ValueError("contiguous tensor required")
-        return a.data_ptr(), np.dtype(str(a.dtype).replace("torch.", ""))
-    raise TypeError(f"not a device array: {type(a)!r}")
+            raise ValueError(f"{name}: contiguous tensor required")
+        if ctx is not None and a.device.index is not None and a.device.index != ctx.device:
+            raise ValueError(f"{name}: tensor on device {a.device.index}, the graph is on device {ctx.device}")
+        ptr, dt, size = a.data_ptr(), np.dtype(str(a.dtype).replace("torch.", "")), a.numel()
+    else:
+        raise TypeError(f"{name}: not a device array: {type(a)!r}")
+    if dtype is not None and dt != np.dtype(dtype):
+        raise ValueError(f"{name}: dtype {dt}, expected {np.dtype(dtype)}")
+    if count is not None and size < count:
+        raise ValueError(f"{name}: {size} elements, at least {count} required")
+    return ptr, dt
 
 
 # ----------------------------------------------------------------- memory --
@@ -203,6 +221,7 @@ class Context:
         h = C.c_void_p()
         check(lib().gg_ctx_create(device, C.byref(h)))
         self.h = h
+        self.device = int(device)
 
     def close(self):
         if self.h:
@@ -398,9 +417,7 @@ class Graph:
 
     def set_weights_dev(self, w):
         """Edge weights from a device array (m FP32 values, CSR order)."""
-        p, dt = _dev(w)
-        if dt != np.float32:
-            raise ValueError("weights are FP32")
+        p, dt = _dev(w, self.m, np.float32, self.ctx, "weights")
         check(lib().gg_graph_set_weights_dev(self.h, p))
         self.weighted = True
 
@@ -499,11 +516,26 @@ class Frontier:
             pass
 
 
+# element types of each edge operator's arrays a and b ("float": FP64 or
+# FP32, b the same as a), as include/graphgrind_amd.h documents them
+_EDGE_OP_TYPES = {OP_PAGERANK: ("float", "float"), OP_BFS: (np.int32, None), OP_CC: (np.uint32, np.uint32),
+                  OP_SPMV: ("float", "float"), OP_SSSP: (np.int32, np.int32), OP_PATHSUM: ("float", np.uint8),
+                  OP_SUM: ("float", "float")}
+
+
 def edge_map(g: Graph, frontier: Frontier, kind, a: DeviceArray, b: DeviceArray = None,
              threshold=-1, remdups=False):
     """edgeMap(GA, frontier, F, threshold, DENSE, remDups) -> (output, path)."""
-    dtype = DT_OF[a.dtype]
-    op = Op(kind, dtype, a.ptr, None if b is None else b.ptr)
+    ap, adt = _dev(a, g.n, None, g.ctx, "a")
+    want_a, want_b = _EDGE_OP_TYPES.get(kind, (None, None))
+    if want_a == "float":
+        if adt not in (np.float64, np.float32):
+            raise ValueError(f"a: FP64 or FP32 for operator {kind}, not {adt}")
+    elif want_a is not None and adt != want_a:
+        raise ValueError(f"a: dtype {adt}, operator {kind} takes {np.dtype(want_a)}")
+    bp, _ = _dev(b, g.n, adt if want_b == "float" else want_b, g.ctx, "b")
+    dtype = DT_OF[adt]
+    op = Op(kind, dtype, ap, bp)
     h = C.c_void_p()
     path = C.c_int()
     check(lib().gg_edgemap(g.h, frontier.h, C.byref(op), threshold, int(remdups), C.byref(h),
@@ -513,13 +545,21 @@ def edge_map(g: Graph, frontier: Frontier, kind, a: DeviceArray, b: DeviceArray 
 
 def vertex_map(g: Graph, frontier, kind, a: DeviceArray, b: DeviceArray = None, c0=0.0, c1=0.0):
     """vertexMap(part, frontier, F); frontier None = map_vertexL over all."""
-    op = VOp(kind, DT_OF[a.dtype], a.ptr, None if b is None else b.ptr, c0, c1)
+    ap, adt = _dev(a, g.n, None, g.ctx, "a")
+    if adt not in DT_OF:
+        raise ValueError(f"a: unsupported dtype {adt}")
+    bp, _ = _dev(b, g.n, adt, g.ctx, "b")
+    op = VOp(kind, DT_OF[adt], ap, bp, c0, c1)
     check(lib().gg_vertexmap(g.h, None if frontier is None else frontier.h, C.byref(op)))
 
 
 def reduce_sum(ctx: Context, a: DeviceArray):
     out = C.c_double()
-    check(lib().gg_reduce_sum(ctx.h, a.ptr, DT_OF[a.dtype], a.count, C.byref(out)))
+    ap, adt = _dev(a, None, None, ctx, "a")
+    if adt not in DT_OF:
+        raise ValueError(f"a: unsupported dtype {adt}")
+    check(lib().gg_reduce_sum(ctx.h, ap, DT_OF[adt], a.count if isinstance(a, DeviceArray) else a.numel(),
+                              C.byref(out)))
     return out.value
 
 
@@ -527,37 +567,49 @@ def reduce_sum(ctx: Context, a: DeviceArray):
 
 def pagerank(g: Graph, dtype=np.float64, max_iter=10, damping=0.85, eps=1e-7, out=None):
     """app/PageRank.C Compute(): returns (ranks DeviceArray, RunInfo)."""
-    r = out if out is not None else DeviceArray(g.ctx, g.n, dtype)
+    dt = np.dtype(dtype)
+    if dt not in (np.float64, np.float32):
+        raise ValueError(f"pagerank: FP64 or FP32, not {dt}")
+    r = out if out is not None else DeviceArray(g.ctx, g.n, dt)
+    rp, _ = _dev(r, g.n, dt, g.ctx, "out")
     info = RunInfo()
-    check(lib().gg_pagerank(g.h, DT_OF[np.dtype(dtype)], max_iter, damping, eps, r.ptr,
-                            C.byref(info)))
+    check(lib().gg_pagerank(g.h, DT_OF[dt], max_iter, damping, eps, rp, C.byref(info)))
     return r, info
 
 
 def bfs(g: Graph, root, threshold=-1, levels=True, parents=None, level_out=None):
     """app/BFS.C Compute(): returns (parents, levels|None, RunInfo)."""
+    # the caller's arrays are checked before anything is allocated
+    _dev(parents, g.n, np.int32, g.ctx, "parents")
+    _dev(level_out, g.n, np.int32, g.ctx, "levels")
     p = parents if parents is not None else DeviceArray(g.ctx, g.n, np.int32)
     lv = level_out if level_out is not None else (DeviceArray(g.ctx, g.n, np.int32) if levels else None)
+    pp, _ = _dev(p, g.n, np.int32, g.ctx, "parents")
+    lp, _ = _dev(lv, g.n, np.int32, g.ctx, "levels")
     info = RunInfo()
-    check(lib().gg_bfs(g.h, root, threshold, p.ptr, None if lv is None else lv.ptr, C.byref(info)))
+    check(lib().gg_bfs(g.h, root, threshold, pp, lp, C.byref(info)))
     return p, lv, info
 
 
 def components(g: Graph, out=None):
     """app/Components.C Compute(): returns (labels, RunInfo)."""
     lab = out if out is not None else DeviceArray(g.ctx, g.n, np.uint32)
+    lp, _ = _dev(lab, g.n, np.uint32, g.ctx, "labels")
     info = RunInfo()
-    check(lib().gg_cc(g.h, lab.ptr, C.byref(info)))
+    check(lib().gg_cc(g.h, lp, C.byref(info)))
     return lab, info
 
 
 def spmv(g: Graph, x, y=None):
     """y[d] = sum_{s->d} w * x[s] over the in-edge twin (app/SPMV.C).  x and y
     are DeviceArrays or device tensors."""
-    xp, dt = _dev(x)
+    xp, dt = _dev(x, g.n, None, g.ctx, "x")
+    if dt not in (np.float64, np.float32):
+        raise ValueError(f"x: FP64 or FP32, not {dt}")
     yy = y if y is not None else DeviceArray(g.ctx, g.n, dt)
+    yp, _ = _dev(yy, g.n, dt, g.ctx, "y")
     info = RunInfo()
-    check(lib().gg_spmv(g.h, DT_OF[dt], xp, _dev(yy)[0], C.byref(info)))
+    check(lib().gg_spmv(g.h, DT_OF[dt], xp, yp, C.byref(info)))
     return yy, info
 
 
@@ -567,17 +619,18 @@ def spmv_bwd(g: Graph, x, gy, gx=None, gw=None, want_gx=True, want_gw=True):
     """Backward of spmv for dL/dy = gy: gx[s] = sum_e w_e gy[d] (pull over
     the out-edge CSR), gw[e] = x[s] gy[d] in CSR order.  Arrays are
     DeviceArrays or device tensors of one float dtype; returns (gx, gw, info)."""
-    gp, dt = _dev(gy)
-    xp, xdt = _dev(x)
-    if xdt is not None and xdt != dt:
-        raise ValueError("x and gy differ in dtype")
+    gp, dt = _dev(gy, g.n, None, g.ctx, "gy")
+    if dt not in (np.float64, np.float32):
+        raise ValueError(f"gy: FP64 or FP32, not {dt}")
+    xp, _ = _dev(x, g.n, dt, g.ctx, "x")
     if want_gx and gx is None:
         gx = DeviceArray(g.ctx, g.n, dt)
     if want_gw and gw is None:
         gw = DeviceArray(g.ctx, max(g.m, 1), dt)
+    gxp = _dev(gx, g.n, dt, g.ctx, "gx")[0] if want_gx else None
+    gwp = _dev(gw, g.m, dt, g.ctx, "gw")[0] if want_gw else None
     info = RunInfo()
-    check(lib().gg_spmv_bwd(g.h, DT_OF[dt], xp, gp, _dev(gx)[0] if want_gx else None,
-                            _dev(gw)[0] if want_gw else None, C.byref(info)))
+    check(lib().gg_spmv_bwd(g.h, DT_OF[dt], xp, gp, gxp, gwp, C.byref(info)))
     return (gx if want_gx else None), (gw if want_gw else None), info
 
 
@@ -585,9 +638,13 @@ def wpagerank(g: Graph, dtype=np.float64, alpha=0.85, eps=1e-7, max_iter=100, v=
     """Row-normalised weighted PageRank with all vertices active (teleport v,
     uniform when None) until the L1 step is below eps; returns (r, info)."""
     dt = np.dtype(dtype)
+    if dt not in (np.float64, np.float32):
+        raise ValueError(f"wpagerank: FP64 or FP32, not {dt}")
     r = out if out is not None else DeviceArray(g.ctx, g.n, dt)
+    vp, _ = _dev(v, g.n, dt, g.ctx, "v")
+    rp, _ = _dev(r, g.n, dt, g.ctx, "out")
     info = RunInfo()
-    check(lib().gg_wpagerank(g.h, DT_OF[dt], alpha, eps, max_iter, _dev(v)[0], _dev(r)[0], C.byref(info)))
+    check(lib().gg_wpagerank(g.h, DT_OF[dt], alpha, eps, max_iter, vp, rp, C.byref(info)))
     return r, info
 
 
@@ -595,17 +652,22 @@ def wpagerank_adjoint(g: Graph, r, u, alpha=0.85, eps=1e-7, max_iter=100, gw=Non
                       want_gw=True, want_gv=True, want_lam=False):
     """Implicit adjoint of wpagerank at r for dL/dr = u.  Returns
     (gw per edge in CSR order, gv = dL/dv, lambda, info)."""
-    rp, dt = _dev(r)
+    rp, dt = _dev(r, g.n, None, g.ctx, "r")
+    if dt not in (np.float64, np.float32):
+        raise ValueError(f"r: FP64 or FP32, not {dt}")
+    up, _ = _dev(u, g.n, dt, g.ctx, "u")
     if want_gw and gw is None:
         gw = DeviceArray(g.ctx, max(g.m, 1), dt)
     if want_gv and gv is None:
         gv = DeviceArray(g.ctx, g.n, dt)
     if want_lam and lam is None:
         lam = DeviceArray(g.ctx, g.n, dt)
+    gwp = _dev(gw, g.m, dt, g.ctx, "gw")[0] if want_gw else None
+    gvp = _dev(gv, g.n, dt, g.ctx, "gv")[0] if want_gv else None
+    lamp = _dev(lam, g.n, dt, g.ctx, "lam")[0] if want_lam else None
     info = RunInfo()
-    check(lib().gg_wpagerank_adjoint(g.h, DT_OF[dt], alpha, eps, max_iter, rp, _dev(u)[0],
-                                     _dev(gw)[0] if want_gw else None, _dev(gv)[0] if want_gv else None,
-                                     _dev(lam)[0] if want_lam else None, C.byref(info)))
+    check(lib().gg_wpagerank_adjoint(g.h, DT_OF[dt], alpha, eps, max_iter, rp, up, gwp, gvp, lamp,
+                                     C.byref(info)))
     return (gw if want_gw else None), (gv if want_gv else None), (lam if want_lam else None), info
 
 
@@ -616,7 +678,9 @@ def bc(g: Graph, root, dtype=np.float32, out=None, levels=True):
     sc = out if out is not None else DeviceArray(g.ctx, g.n, dtype)
     lv = DeviceArray(g.ctx, g.n, np.int32) if levels else None
     info = RunInfo()
-    p, dt = _dev(sc)
+    p, dt = _dev(sc, g.n, None, g.ctx, "out")
+    if dt not in (np.float64, np.float32):
+        raise ValueError(f"bc: FP64 or FP32, not {dt}")
     check(lib().gg_bc(g.h, root, DT_OF[dt], p, None if lv is None else lv.ptr, C.byref(info)))
     return sc, lv, info
 
@@ -625,7 +689,7 @@ def sssp(g: Graph, root, out=None):
     """app/BellmanFord.C from root with integer lengths: (dist int32, RunInfo)."""
     d = out if out is not None else DeviceArray(g.ctx, g.n, np.int32)
     info = RunInfo()
-    check(lib().gg_sssp(g.h, root, _dev(d)[0], C.byref(info)))
+    check(lib().gg_sssp(g.h, root, _dev(d, g.n, np.int32, g.ctx, "out")[0], C.byref(info)))
     return d, info
 
 
@@ -638,7 +702,8 @@ def msbfs(g: Graph, sources, out=None):
     src = np.ascontiguousarray(sources, np.uint32)
     lv = out if out is not None else DeviceArray(g.ctx, len(src) * g.n, np.int32)
     info = RunInfo()
-    check(lib().gg_msbfs(g.h, len(src), src.ctypes.data, _dev(lv)[0], C.byref(info)))
+    lvp, _ = _dev(lv, len(src) * g.n, np.int32, g.ctx, "out")
+    check(lib().gg_msbfs(g.h, len(src), src.ctypes.data, lvp, C.byref(info)))
     return lv, info
 
 
