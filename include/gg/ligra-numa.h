@@ -50,6 +50,7 @@
 #include <map>
 #include <string>
 #include <type_traits>
+#include <typeinfo>
 #include <utility>
 #include <vector>
 
@@ -517,32 +518,218 @@ constexpr int dtype_of()
     return std::is_same<T, float>::value ? GG_F32 : GG_F64;
 }
 
+// ---- functor self-checks ------------------------------------------------------------
+// Functors are recognised by their data members.  Before the first device
+// launch for a functor type, its own update()/updateAtomic()/cond() (edgeMap)
+// or operator() (vertexMap) run on a copy whose arrays point at small host
+// scratch arrays with known values, and the results are compared with what
+// the chosen device operator computes.  A recognised edgeMap functor whose
+// body computes something else stops the program with a message (there is
+// no host path for edgeMap); a vertexMap functor that fails runs on the host.
+template <class F, class = void>
+struct has_update3 : std::false_type {};
+template <class F>
+struct has_update3<F, std::void_t<decltype(std::declval<F &>().update(intT(0), intT(0), intE(0)))>>
+    : std::true_type {};
+template <class F, class = void>
+struct has_update2 : std::false_type {};
+template <class F>
+struct has_update2<F, std::void_t<decltype(std::declval<F &>().update(intT(0), intT(0)))>> : std::true_type {};
+template <class F, class = void>
+struct has_atomic3 : std::false_type {};
+template <class F>
+struct has_atomic3<F, std::void_t<decltype(std::declval<F &>().updateAtomic(intT(0), intT(0), intE(0)))>>
+    : std::true_type {};
+template <class F, class = void>
+struct has_atomic2 : std::false_type {};
+template <class F>
+struct has_atomic2<F, std::void_t<decltype(std::declval<F &>().updateAtomic(intT(0), intT(0)))>>
+    : std::true_type {};
+template <class F, class = void>
+struct has_cond : std::false_type {};
+template <class F>
+struct has_cond<F, std::void_t<decltype(std::declval<F &>().cond(intT(0)))>> : std::true_type {};
+
+// update / updateAtomic with the edge weight when the functor takes one
+// (the device operators use weight 1 on unweighted graphs)
+template <class F>
+bool call_edge(F &f, bool atomic, intT s, intT d, intE w)
+{
+    if (atomic) {
+        if constexpr (has_atomic3<F>::value) return f.updateAtomic(s, d, w);
+        else if constexpr (has_atomic2<F>::value) return f.updateAtomic(s, d);
+    }
+    if constexpr (has_update3<F>::value) return f.update(s, d, w);
+    else if constexpr (has_update2<F>::value) return f.update(s, d);
+    else die("edgeMap: functor has no update(s, d[, w])");
+}
+template <class F>
+constexpr intE weight_used()
+{
+    return has_update3<F>::value ? intE(4) : intE(1);
+}
+template <class F>
+bool call_cond(F &f, intT d)
+{
+    if constexpr (has_cond<F>::value) return f.cond(d);
+    else return true;
+}
+template <class P>
+using elem_of = typename std::remove_cv<typename std::remove_pointer<P>::type>::type;
+
+[[noreturn]] inline void functor_mismatch(const char *fname, const char *what)
+{
+    std::cerr << "edgeMap: functor " << fname << " has the data members of the device operator " << what
+              << " but its update computes something else; the device engine cannot run it\n";
+    std::exit(1);
+}
+
+// One scenario per operator: source 0, destinations 1..3, both the plain
+// and the atomic update.  Values are exact in binary floating point.
+template <int K, class F>
+bool edge_op_agrees(const F &f)
+{
+    for (int atomic = 0; atomic < 2; atomic++) {
+        F g = f;
+        if constexpr (K == GG_OP_BFS) {
+            using T = elem_of<decltype(g.Parents)>;
+            T P[4] = {T(0), T(-1), T(5), T(-1)};
+            g.Parents = P;
+            if (!call_cond(g, 1) || call_cond(g, 2)) return false;
+            if (!call_edge(g, atomic, 0, 1, 1) || P[1] != T(0)) return false;
+            if (call_edge(g, atomic, 0, 2, 1) || P[2] != T(5)) return false;
+        } else if constexpr (K == GG_OP_CC) {
+            using T = elem_of<decltype(g.IDs)>;
+            T I[4] = {T(2), T(9), T(1), T(9)}, Pv[4] = {T(2), T(9), T(1), T(7)};
+            g.IDs = I;
+            g.prevIDs = Pv;
+            if (!call_edge(g, atomic, 0, 1, 1) || I[1] != T(2)) return false;
+            if (call_edge(g, atomic, 0, 2, 1) || I[2] != T(1)) return false;
+            if (call_edge(g, atomic, 0, 3, 1) || I[3] != T(2)) return false;
+        } else if constexpr (K == GG_OP_SUM) {
+            using T = elem_of<decltype(g.p_next)>;
+            T A[4] = {T(2.5), T(0), T(0), T(0)}, N[4] = {T(0), T(1), T(0), T(0)};
+            g.add_factor = A;
+            g.p_next = N;
+            call_edge(g, atomic, 0, 1, 1);
+            if (N[1] != T(3.5)) return false;
+        } else if constexpr (K == GG_OP_PAGERANK && has_Delta<F>::value) {
+            // PageRankDelta's PR_F: nghSum[d] += Delta[s]/outdeg(s)
+            using T = elem_of<decltype(g.Delta)>;
+            using VT = elem_of<decltype(g.V)>;
+            T D[4] = {T(-6), T(0), T(0), T(0)}, N[4] = {T(0), T(1), T(0), T(0)};
+            VT V[4];
+            V[0].setOutDegree(3);
+            g.V = V;
+            g.Delta = D;
+            g.nghSum = N;
+            call_edge(g, atomic, 0, 1, 1);
+            if (N[1] != T(-1)) return false;
+        } else if constexpr (K == GG_OP_PAGERANK) {
+            // PR_F: p_next[d] += p_curr[s]/outdeg(s)
+            using T = elem_of<decltype(g.p_next)>;
+            using VT = elem_of<decltype(g.V)>;
+            T C[4] = {T(-6), T(0), T(0), T(0)}, N[4] = {T(0), T(1), T(0), T(0)};
+            VT V[4];
+            V[0].setOutDegree(3);
+            g.V = V;
+            g.p_curr = C;
+            g.p_next = N;
+            call_edge(g, atomic, 0, 1, 1);
+            if (N[1] != T(-1)) return false;
+        } else if constexpr (K == GG_OP_SPMV) {
+            // SPMV_F: p_next[d] += p_curr[s] * w
+            using T = elem_of<decltype(g.p_next)>;
+            T C[4] = {T(1.5), T(0), T(0), T(0)}, N[4] = {T(0), T(1), T(0), T(0)};
+            g.p_curr = C;
+            g.p_next = N;
+            call_edge(g, atomic, 0, 1, weight_used<F>());
+            if (N[1] != T(1) + T(1.5) * T(weight_used<F>())) return false;
+        } else if constexpr (K == GG_OP_SSSP) {
+            using T = elem_of<decltype(g.ShortestPathLen)>;
+            using U = elem_of<decltype(g.Visited)>;
+            const intE w = weight_used<F>();
+            T L[4] = {T(2), T(10), T(2), T(10)};
+            U Vis[4] = {U(1), U(0), U(0), U(1)};
+            g.ShortestPathLen = L;
+            g.Visited = Vis;
+            if (!call_edge(g, atomic, 0, 1, w) || L[1] != T(2 + w) || Vis[1] != U(1)) return false;
+            if (call_edge(g, atomic, 0, 2, w) || L[2] != T(2)) return false;
+            if (call_edge(g, atomic, 0, 3, w) || L[3] != T(2 + w)) return false;
+        } else if constexpr (K == GG_OP_PATHSUM) {
+            using U = elem_of<decltype(g.Visited)>;
+            U Vis[4] = {U(1), U(0), U(0), U(0)};
+            g.Visited = Vis;
+            auto run = [&](auto *A) {
+                using T = std::remove_pointer_t<decltype(A)>;
+                A[0] = T(2);
+                A[1] = T(3);
+                A[2] = T(0);
+                A[3] = T(0);
+                if (call_cond(g, 0) || !call_cond(g, 1)) return false;
+                if (call_edge(g, atomic, 0, 1, 1) || A[1] != T(5)) return false;
+                if (!call_edge(g, atomic, 0, 2, 1) || A[2] != T(2)) return false;
+                return true;
+            };
+            if constexpr (has_NumPaths<F>::value) {
+                elem_of<decltype(g.NumPaths)> A[4];
+                g.NumPaths = A;
+                if (!run(A)) return false;
+            } else {
+                elem_of<decltype(g.Dependencies)> A[4];
+                g.Dependencies = A;
+                if (!run(A)) return false;
+            }
+        } else {
+            return false;
+        }
+    }
+    return true;
+}
+
+template <int K, class F>
+void check_edge_functor(const F &f)
+{
+    static bool checked = false;
+    if (checked) return;
+    if (!edge_op_agrees<K>(f)) {
+        static const char *names[] = {"PAGERANK", "BFS", "CC", "SPMV", "SSSP", "PATHSUM", "SUM"};
+        functor_mismatch(typeid(F).name(), names[K]);
+    }
+    checked = true;
+}
+
 template <class F>
 gg_op_t edge_op(F &f)
 {
     gg_op_t op{};
     if constexpr (has_Parents<F>::value) {
+        check_edge_functor<GG_OP_BFS>(f);
         op.kind = GG_OP_BFS;
         op.dtype = GG_I32;
         op.a = device_of(f.Parents, true);
     } else if constexpr (has_IDs<F>::value && has_prevIDs<F>::value) {
+        check_edge_functor<GG_OP_CC>(f);
         op.kind = GG_OP_CC;
         op.dtype = GG_U32;
         op.a = device_of(f.IDs, true);
         op.b = device_of(f.prevIDs, false);
     } else if constexpr (has_p_next<F>::value && has_add_factor<F>::value) {
         // PR_F of app/PageRankConverage.C:34-88: p_next[d] += add_factor[s]
+        check_edge_functor<GG_OP_SUM>(f);
         op.kind = GG_OP_SUM;
         op.dtype = dtype_of<decltype(f.add_factor)>();
         op.a = device_of(f.add_factor, false);
         op.b = device_of(f.p_next, true);
     } else if constexpr (has_p_curr<F>::value && has_p_next<F>::value) {
         op.kind = has_V<F>::value ? GG_OP_PAGERANK : GG_OP_SPMV;
+        check_edge_functor<has_V<F>::value ? GG_OP_PAGERANK : GG_OP_SPMV>(f);
         op.dtype = dtype_of<decltype(f.p_curr)>();
         op.a = device_of(f.p_curr, false);
         op.b = device_of(f.p_next, true);
     } else if constexpr (has_V<F>::value && has_Delta<F>::value && has_nghSum<F>::value) {
         // PR_F of app/PageRankDelta.C:27-72: nghSum[d] += Delta[s]/outdeg(s)
+        check_edge_functor<GG_OP_PAGERANK>(f);
         op.kind = GG_OP_PAGERANK;
         op.dtype = dtype_of<decltype(f.Delta)>();
         op.a = device_of(f.Delta, false);
@@ -550,6 +737,7 @@ gg_op_t edge_op(F &f)
     } else if constexpr (has_ShortestPathLen<F>::value && has_Visited<F>::value) {
         // BF_F of app/BellmanFord.C:28-92
         static_assert(sizeof(*f.ShortestPathLen) == 4 && sizeof(*f.Visited) == 4, "BF_F: 32-bit intT arrays");
+        check_edge_functor<GG_OP_SSSP>(f);
         op.kind = GG_OP_SSSP;
         op.dtype = GG_I32;
         op.a = device_of(f.ShortestPathLen, true);
@@ -557,6 +745,7 @@ gg_op_t edge_op(F &f)
     } else if constexpr ((has_NumPaths<F>::value || has_Dependencies<F>::value) && has_Visited<F>::value) {
         // BC_F / BC_Back_F of app/BC.C:28-124: a[d] += a[s] for unvisited d
         static_assert(sizeof(*f.Visited) == 1, "BC_F: bool Visited");
+        check_edge_functor<GG_OP_PATHSUM>(f);
         op.kind = GG_OP_PATHSUM;
         if constexpr (has_NumPaths<F>::value) {
             op.dtype = dtype_of<decltype(f.NumPaths)>();
@@ -570,6 +759,49 @@ gg_op_t edge_op(F &f)
         static_assert(sizeof(F) == 0, "edgeMap: functor not recognised by the device engine");
     }
     return op;
+}
+} // namespace gg_front
+
+namespace gg_front {
+// vertexMap's self-check (see the edgeMap one above): the functor's own
+// operator() on a copy over scratch arrays must do what the device
+// operator does, else the functor runs on the host.  Checked once per type.
+template <class F>
+bool vertex_op_agrees(const F &add, int kind)
+{
+    static int verdict = -1;
+    if (verdict >= 0) return verdict != 0;
+    bool ok = false;
+    F g = add;
+    if constexpr (has_damping<F>::value && has_addedConstant<F>::value && has_p_next<F>::value) {
+        if (kind == GG_VOP_PAGERANK) {
+            using T = elem_of<decltype(g.p_next)>;
+            T N[2] = {T(2), T(3)};
+            g.p_next = N;
+            g(0);
+            const double want = g.damping * 2.0 + g.addedConstant;
+            ok = std::fabs((double)N[0] - want) <= 1e-6 * std::fabs(want) && N[1] == T(3);
+        }
+    } else if constexpr (has_IDs<F>::value && has_prevIDs<F>::value) {
+        if (kind == GG_VOP_CC_COPY) {
+            using T = elem_of<decltype(g.IDs)>;
+            T I[2] = {T(7), T(8)}, Pv[2] = {T(1), T(2)};
+            g.IDs = I;
+            g.prevIDs = Pv;
+            g(0);
+            ok = Pv[0] == T(7) && I[0] == T(7) && Pv[1] == T(2);
+        }
+    } else if constexpr (has_p_curr<F>::value) {
+        if (kind == GG_VOP_RESET) {
+            using T = elem_of<decltype(g.p_curr)>;
+            T C[2] = {T(5), T(6)};
+            g.p_curr = C;
+            g(0);
+            ok = C[0] == T(0) && C[1] == T(6);
+        }
+    }
+    verdict = ok ? 1 : 0;
+    return ok;
 }
 } // namespace gg_front
 
@@ -616,7 +848,7 @@ void vertexMap(const partitioner &part, partitioned_vertices V, F add)
     } else {
         device = false;
     }
-    if (device) {
+    if (device && vertex_op_agrees(add, op.kind)) {
         check(gg_vertexmap(current_graph(), V.h, &op), "vertexMap");
         return;
     }

@@ -44,6 +44,47 @@ def test_host_atomics_under_threads():
     assert r.stdout.count(" ok") == 5, r.stdout
 
 
+def _build_functor_check(tmp, app=None):
+    """tests/cpp/functor_check.cpp, alone or behind a reference app source
+    fed to the compiler unchanged (as apps/Makefile builds the apps)."""
+    exe = os.path.join(tmp, "fc_" + (app or "neg"))
+    src = open(os.path.join(ROOT, "tests", "cpp", "functor_check.cpp")).read()
+    flags = []
+    if app:
+        src = open(os.path.join(REF_APP, app + ".C")).read() + "\n" + src
+        flags = ["-DGG_FRONT_NO_DRIVER", "-DCHECK_" + app.upper()]
+        if app in ("SPMV", "BellmanFord"):
+            flags.append("-DWEIGHTED")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-fopenmp", "-w"] + flags +
+                       ["-x", "c++", "-", "-I.", "-I..", "-o", exe,
+                        "-L" + os.path.join(ROOT, "graphgrind_amd"), "-lgraphgrind_amd",
+                        "-Wl,-rpath," + os.path.join(ROOT, "graphgrind_amd"),
+                        "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"],
+                       input=src, cwd=os.path.join(ROOT, "include", "gg"), capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return exe
+
+
+def test_functor_self_check_refuses_other_bodies(tmp_path):
+    """Functors with PR_F / BFS_F / CC_F / PR_Vertex_Reset members but other
+    update bodies are refused by the front end's self-check (edgeMap stops
+    with a message, vertexMap runs them on the host)."""
+    r = subprocess.run([_build_functor_check(str(tmp_path))], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("refused") == 4, r.stdout
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_APP), reason="reference sources not present")
+@pytest.mark.parametrize("app", ["PageRank", "BFS", "Components", "SPMV", "BC", "BellmanFord", "PageRankDelta",
+                                 "PageRankConverage"])
+def test_functor_self_check_accepts_reference_functors(tmp_path, app):
+    """The reference apps' own functors (their sources compiled unchanged)
+    pass the self-check of the device operator they are mapped to."""
+    r = subprocess.run([_build_functor_check(str(tmp_path), app)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "refused" not in r.stdout and "agrees" in r.stdout, r.stdout
+
+
 def _run(exe, app, args):
     with tempfile.TemporaryDirectory() as td:
         out = os.path.join(td, "out.bin")
