@@ -618,15 +618,25 @@ static fr_t edge_map(const ggo_graph *g, const fr_t *Fin, const op_t *f,
         out.n = g->n;
         out.d = (uint8_t *)calloc((size_t)g->n, 1);
         out.has_dense = 1;
+        /* Partitions run in parallel, as the reference's map_partitionL
+         * does.  Each partition owns a destination range and sums its
+         * edges in order, so values do not depend on the thread count.
+         * The exception is CC's COO path, whose update reads IDs[s] of
+         * other partitions while they are written (Gauss-Seidel): it stays
+         * sequential so its round count is reproducible. */
+        const int par = !(f->kind == OP_CC);
         if (mcount + tod > dense_thr && !g->part_vertex) {
+#pragma omp parallel for schedule(dynamic, 1) if (par)
             for (int p = 0; p < g->P; p++) dense_coo(g, f, F.d, F.bit, out.d, p);
             if (tr) tr->path = GGO_PATH_COO;
         } else {
+#pragma omp parallel for schedule(dynamic, 1)
             for (int p = 0; p < g->P; p++)
                 dense_csc(g, f, F.d, F.bit, out.d, g->csc_starts[p], g->csc_starts[p + 1]);
             if (tr) tr->path = GGO_PATH_CSC;
         }
         int64_t dm = 0, oe = 0;
+#pragma omp parallel for reduction(+ : dm, oe) schedule(static)
         for (int64_t i = 0; i < (int64_t)g->n; i++)
             if (out.d[i]) { dm++; oe += outdeg(g, i); }
         out.d_m = dm;
@@ -663,9 +673,11 @@ static void vertex_map(const ggo_graph *g, const fr_t *F, int kind, double *a, d
                        int64_t *ia, int64_t *ib, double damping, double added)
 {
     if (F->has_dense) {
+#pragma omp parallel for schedule(static)
         for (int64_t j = 0; j < (int64_t)g->n; j++)
             if (F->bit || F->d[j]) vertex_apply(kind, j, a, b, ia, ib, damping, added);
     } else {
+#pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < F->d_m; i++)
             vertex_apply(kind, F->s[i], a, b, ia, ib, damping, added);
     }
@@ -698,6 +710,7 @@ double ggo_plus_reduce(const double *a, uint64_t n)
     uint64_t l = n ? 1 + (n - 1) / 1024 : 1;
     if (l <= 1) return reduce_serial(a, 0, n);
     double *sums = (double *)malloc((size_t)l * 8);
+#pragma omp parallel for schedule(static)
     for (uint64_t i = 0; i < l; i++) {
         uint64_t s = i * 1024, e = s + 1024 < n ? s + 1024 : n;
         sums[i] = reduce_serial(a, s, e);
@@ -728,6 +741,7 @@ int ggo_pagerank(const ggo_graph *g, int max_iter, double *out, int *iters,
         fr_t output = edge_map(g, &F, &f, (int64_t)(g->m / 20), trace ? &trace[count - 1] : NULL, &err);
         if (err) { fr_free(&F); free(pc); free(pn); return err; }
         vertex_map(g, &F, VM_PR, pc, pn, NULL, NULL, damping, (1 - damping) * (1 / (double)n));
+#pragma omp parallel for schedule(static)
         for (int64_t j = 0; j < n; j++) pc[j] = fabs(pc[j] - pn[j]); /* 136 */
         double L1 = ggo_plus_reduce(pc, (uint64_t)n);
         if (L1 < epsilon) { broke = 1; fr_free(&output); break; }
