@@ -555,18 +555,19 @@ struct has_cond<F, std::void_t<decltype(std::declval<F &>().cond(intT(0)))>> : s
 template <class F>
 bool call_edge(F &f, bool atomic, intT s, intT d, intE w)
 {
-    if (atomic) {
+    // a functor with only one of the two forms is checked through it twice
+    if (atomic || !(has_update3<F>::value || has_update2<F>::value)) {
         if constexpr (has_atomic3<F>::value) return f.updateAtomic(s, d, w);
         else if constexpr (has_atomic2<F>::value) return f.updateAtomic(s, d);
     }
     if constexpr (has_update3<F>::value) return f.update(s, d, w);
     else if constexpr (has_update2<F>::value) return f.update(s, d);
-    else die("edgeMap: functor has no update(s, d[, w])");
+    else die("edgeMap: functor has neither update(s, d[, w]) nor updateAtomic(s, d[, w])");
 }
 template <class F>
 constexpr intE weight_used()
 {
-    return has_update3<F>::value ? intE(4) : intE(1);
+    return (has_update3<F>::value || has_atomic3<F>::value) ? intE(4) : intE(1);
 }
 template <class F>
 bool call_cond(F &f, intT d)

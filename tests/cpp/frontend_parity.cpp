@@ -65,6 +65,8 @@ template <class vertex>
 struct WeightedStep {  // SPMV_F interface (app/SPMV.C:7-48)
     double *p_curr, *p_next;
     WeightedStep(double *c, double *x) : p_curr(c), p_next(x) {}
+    bool update(intT s, intT d, intE w) { p_next[d] += p_curr[s] * w; return true; }
+    bool updateAtomic(intT s, intT d, intE w) { writeAdd(&p_next[d], p_curr[s] * w); return true; }
     bool cond(intT) { return true; }
 };
 
@@ -103,6 +105,10 @@ struct MarkBack {  // BC_Back_Vertex_F (app/BC.C:141-152): runs on the host
 struct RelaxStep {  // BF_F interface (app/BellmanFord.C:28-92)
     intT *ShortestPathLen, *Visited;
     RelaxStep(intT *a, intT *v) : ShortestPathLen(a), Visited(v) {}
+    bool updateAtomic(intT s, intT d, intE w)
+    {
+        return writeMin(&ShortestPathLen[d], ShortestPathLen[s] + w) && CAS(&Visited[d], intT(0), intT(1));
+    }
     bool cond(intT) { return true; }
 };
 
@@ -117,6 +123,8 @@ struct DeltaStep {  // PR_F interface (app/PageRankDelta.C:27-72)
     vertex *V;
     double *Delta, *nghSum;
     DeltaStep(vertex *v, double *d, double *s) : V(v), Delta(d), nghSum(s) {}
+    bool update(intT s, intT d) { nghSum[d] += Delta[s] / V[s].getOutDegree(); return true; }
+    bool updateAtomic(intT s, intT d) { writeAdd(&nghSum[d], Delta[s] / V[s].getOutDegree()); return true; }
     bool cond(intT) { return true; }
 };
 
@@ -159,6 +167,8 @@ struct FactorStep {  // PR_F interface of app/PageRankConverage.C:34-88
     vertex *V;
     double *add_factor;
     FactorStep(double *c, double *x, vertex *v, double *a) : p_curr(c), p_next(x), V(v), add_factor(a) {}
+    bool update(intT s, intT d) { p_next[d] += add_factor[s]; return true; }
+    bool updateAtomic(intT s, intT d) { writeAdd(&p_next[d], add_factor[s]); return true; }
     bool cond(intT) { return true; }
 };
 

@@ -104,6 +104,23 @@ int main()
 #elif defined(CHECK_PAGERANKCONVERAGE)
     expect<GG_OP_SUM>("PageRankConverage.C PR_F",
                       PR_F<asymmetricVertex>(nullptr, nullptr, nullptr, nullptr), true);
+#elif defined(CHECK_FRONTEND_PARITY)
+    // this repository's own parity program (tests/cpp/frontend_parity.cpp)
+#ifndef WEIGHTED
+    expect<GG_OP_PAGERANK>("RankStep", RankStep<asymmetricVertex>(nullptr, nullptr, nullptr), true);
+    expect<GG_OP_BFS>("ParentStep", ParentStep(nullptr), true);
+    expect<GG_OP_CC>("LabelStep", LabelStep(nullptr, nullptr), true);
+    expect<GG_OP_PATHSUM>("PathStep", PathStep<double>(nullptr, nullptr), true);
+    expect<GG_OP_PATHSUM>("DepStep", DepStep<float>(nullptr, nullptr), true);
+    expect<GG_OP_PAGERANK>("DeltaStep", DeltaStep<asymmetricVertex>(nullptr, nullptr, nullptr), true);
+    expect<GG_OP_SUM>("FactorStep", FactorStep<asymmetricVertex>(nullptr, nullptr, nullptr, nullptr), true);
+    expect_v("RankDamp", RankDamp(nullptr, nullptr, 0.85, 1000), GG_VOP_PAGERANK, true);
+    expect_v("RankReset", RankReset(nullptr), GG_VOP_RESET, true);
+    expect_v("LabelCopy", LabelCopy(nullptr, nullptr), GG_VOP_CC_COPY, true);
+#else
+    expect<GG_OP_SPMV>("WeightedStep", WeightedStep<asymmetricVertex>(nullptr, nullptr), true);
+    expect<GG_OP_SSSP>("RelaxStep", RelaxStep(nullptr, nullptr), true);
+#endif
 #else
     expect<GG_OP_PAGERANK>("DoubledRank", DoubledRank<asymmetricVertex>{nullptr, nullptr, nullptr}, false);
     expect<GG_OP_BFS>("AnyParent", AnyParent{nullptr}, false);

@@ -50,7 +50,10 @@ def _build_functor_check(tmp, app=None):
     exe = os.path.join(tmp, "fc_" + (app or "neg"))
     src = open(os.path.join(ROOT, "tests", "cpp", "functor_check.cpp")).read()
     flags = []
-    if app:
+    if app and app.startswith("frontend_parity"):
+        src = open(os.path.join(ROOT, "tests", "cpp", "frontend_parity.cpp")).read() + "\n" + src
+        flags = ["-DGG_FRONT_NO_DRIVER", "-DCHECK_FRONTEND_PARITY"] + (["-DWEIGHTED=1"] if app.endswith("_w") else [])
+    elif app:
         src = open(os.path.join(REF_APP, app + ".C")).read() + "\n" + src
         flags = ["-DGG_FRONT_NO_DRIVER", "-DCHECK_" + app.upper()]
         if app in ("SPMV", "BellmanFord"):
@@ -72,6 +75,15 @@ def test_functor_self_check_refuses_other_bodies(tmp_path):
     r = subprocess.run([_build_functor_check(str(tmp_path))], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("refused") == 4, r.stdout
+
+
+@pytest.mark.parametrize("prog", ["frontend_parity", "frontend_parity_w"])
+def test_functor_self_check_accepts_parity_program_functors(tmp_path, prog):
+    """The functors of this repository's GPU parity program pass the
+    self-check (they run on the GPU in the -m gpu tests)."""
+    r = subprocess.run([_build_functor_check(str(tmp_path), prog)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "refused" not in r.stdout and "agrees" in r.stdout, r.stdout
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_APP), reason="reference sources not present")
