@@ -121,6 +121,7 @@ SIGNATURES = {
     "gg_frontier_destroy": (C.c_int, [_VP]),
     "gg_frontier_info": (C.c_int, [_VP, _U64P, _U64P, _IP, _IP, _U32P]),
     "gg_frontier_to_dense": (C.c_int, [_VP]),
+    "gg_frontier_set_bit": (C.c_int, [_VP, C.c_int]),
     "gg_frontier_to_sparse": (C.c_int, [_VP]),
     "gg_frontier_copy_dense": (C.c_int, [_VP, _VP]),
     "gg_frontier_copy_sparse": (C.c_int, [_VP, _VP, C.c_uint64, _U64P]),
@@ -486,6 +487,11 @@ class Frontier:
     def isEmpty(self):
         return self.numNonzeros() == 0
 
+    def set_bit(self, bit=True):
+        """The public 'bit' field (all vertices active, stored d_m and
+        num_out_edges kept), as app/PageRankBit.C:145 sets it."""
+        check(lib().gg_frontier_set_bit(self.h, int(bool(bit))))
+
     def to_dense(self):
         check(lib().gg_frontier_to_dense(self.h))
 
@@ -551,6 +557,16 @@ def vertex_map(g: Graph, frontier, kind, a: DeviceArray, b: DeviceArray = None, 
     bp, _ = _dev(b, g.n, adt, g.ctx, "b")
     op = VOp(kind, DT_OF[adt], ap, bp, c0, c1)
     check(lib().gg_vertexmap(g.h, None if frontier is None else frontier.h, C.byref(op)))
+
+
+def vertex_filter(g: Graph, frontier, flags):
+    """vertexFilter(GA, V, F) (ligra-numa.h:1053-1082) with the filter's
+    answers given as device flags (uint8, n): the members of 'frontier'
+    whose flag is non-zero, as a new dense frontier."""
+    fp, _ = _dev(flags, g.n, np.uint8, g.ctx, "flags")
+    h = C.c_void_p()
+    check(lib().gg_vertexfilter(g.h, frontier.h, fp, C.byref(h)))
+    return Frontier(g, h)
 
 
 def reduce_sum(ctx: Context, a: DeviceArray):

@@ -769,6 +769,9 @@ int frontier_make_dense(Frontier *f)
 {
     Ctx *c = f->g->ctx;
     if (f->bit) {
+        // flags already held (a frontier whose bit was set later) stay: the
+        // dense kernels take 'bit' and do not read them
+        if (f->dense_valid) return GG_OK;
         GG_TRY(ensure_bytes(f));
         GG_TRY(fill_bytes(c, f->bytes.p, f->n, 1));
         GG_TRY(ensure_bitmap(f));
@@ -794,7 +797,7 @@ int frontier_make_sparse(Frontier *f)
 {
     Ctx *c = f->g->ctx;
     if (f->sparse_valid) return GG_OK;
-    if (f->bit) {
+    if (f->bit && !f->dense_valid) {
         // bits() frontier: its flags are not set in the reference (toSparse
         // on it would read uninitialised memory); list every vertex
         GG_TRY(pool_ensure(c, f->list, 4ull * f->n + 4));
@@ -1336,6 +1339,24 @@ extern "C" int gg_frontier_info(gg_frontier_t f, uint64_t *d_m, uint64_t *out_ed
     if (bit) *bit = f->bit;
     if (n) *n = f->n;
     return GG_OK;
+}
+
+extern "C" int gg_frontier_set_bit(gg_frontier_t f, int bit)
+{
+    if (!f) return fail(GG_ERR_INVALID, "gg_frontier_set_bit: NULL");
+    if (f->n == 0) return fail(GG_ERR_STATE, "gg_frontier_set_bit: empty() frontier has no rows");
+    if (f->bit == (bit != 0)) return GG_OK;
+    if (bit) {
+        // dense branches now treat every vertex as active; the stored
+        // members (flags or list) stay, as the reference's d does, for a
+        // sparse branch chosen from the stored d_m / num_out_edges
+        f->bit = true;
+        return GG_OK;
+    }
+    // clearing the flag: the members become every vertex explicitly
+    GG_TRY(frontier_make_dense(f));
+    f->bit = false;
+    return sync(f->g->ctx);
 }
 
 extern "C" int gg_frontier_to_dense(gg_frontier_t f)

@@ -421,6 +421,15 @@ public:
         gg_front::check(gg_frontier_to_sparse(h), "toSparse");
         refresh();
     }
+    // Apps may write the public field 'bit' (app/PageRankBit.C:145 sets it
+    // on an edgeMap output); the engine's frontier follows it before use.
+    void push_bit() const
+    {
+        if (!h || numVertices == 0) return;
+        int b = 0;
+        gg_front::check(gg_frontier_info(h, nullptr, nullptr, nullptr, &b, nullptr), "gg_frontier_info");
+        if ((b != 0) != bit) gg_front::check(gg_frontier_set_bit(h, bit ? 1 : 0), "bit");
+    }
     std::vector<uint8_t> members() const
     {
         std::vector<uint8_t> d((size_t)std::max<intT>(numVertices, 0));
@@ -816,6 +825,7 @@ partitioned_vertices edgeMap(partitioned_graph<vertex> GA, partitioned_vertices 
                              intT threshold = -1, char option = DENSE, bool remDups = false)
 {
     (void)option;
+    Localfrontier.push_bit();
     gg_op_t op = gg_front::edge_op(f);
     gg_frontier_t out = nullptr;
     int path = 0;
@@ -828,6 +838,7 @@ template <class F>
 void vertexMap(const partitioner &part, partitioned_vertices V, F add)
 {
     using namespace gg_front;
+    V.push_bit();
     gg_vop_t op{};
     bool device = true;
     if constexpr (has_damping<F>::value && has_addedConstant<F>::value && has_p_next<F>::value) {

@@ -198,6 +198,12 @@ int gg_frontier_destroy(gg_frontier_t f);
 /* numNonzeros (268-271), num_out_edges, has_dense, bit, numRows. */
 int gg_frontier_info(gg_frontier_t f, uint64_t *d_m, uint64_t *out_edges, int *is_dense,
                      int *bit, uint32_t *n);
+/* The public field 'bit' (ligra-numa.h:91-101), written by apps such as
+ * app/PageRankBit.C:145 ('output.bit = true'): with bit set every vertex
+ * counts as active and the flags are not read; d_m and num_out_edges keep
+ * their stored values, which edgeMap's branch choice still reads
+ * (ligra-numa.h:929-957).  Not allowed on an empty() frontier. */
+int gg_frontier_set_bit(gg_frontier_t f, int bit);
 /* toDense (228-244) / toSparse (246-260; list ascending like packIndex). */
 int gg_frontier_to_dense(gg_frontier_t f);
 int gg_frontier_to_sparse(gg_frontier_t f);

@@ -64,7 +64,7 @@ def lib():
         L.ggo_round_up_pow2.argtypes = [C.c_uint64]
         L.ggo_graph_new.restype = C.c_void_p
         L.ggo_graph_new.argtypes = [C.c_uint32, C.c_uint64, u32, u32, C.c_void_p, C.c_int,
-                                    C.c_int, C.c_int, C.c_int]
+                                    C.c_int, C.c_int, C.c_int, C.c_int]
         L.ggo_graph_free.argtypes = [C.c_void_p]
         L.ggo_graph_starts.argtypes = [C.c_void_p, C.c_int, u64]
         L.ggo_graph_ncompressed.restype = C.c_uint64
@@ -74,6 +74,7 @@ def lib():
         L.ggo_graph_coo.restype = C.c_int64
         L.ggo_graph_coo.argtypes = [C.c_void_p, C.c_int, u32, u32]
         L.ggo_pagerank.argtypes = [C.c_void_p, C.c_int, f64, C.POINTER(C.c_int), C.c_void_p]
+        L.ggo_pagerank_ex.argtypes = [C.c_void_p, C.c_int, C.c_int, f64, C.POINTER(C.c_int), C.c_void_p]
         L.ggo_bfs.argtypes = [C.c_void_p, C.c_uint32, i32, C.c_void_p, C.c_int]
         L.ggo_cc.argtypes = [C.c_void_p, u32, C.c_void_p, C.c_int]
         L.ggo_spmv.argtypes = [C.c_void_p, C.c_void_p, f64, C.c_void_p]
@@ -186,7 +187,7 @@ class Graph:
     """Oracle graph: CSR + CSC twin + P partitions (+ Hilbert COO lists)."""
 
     def __init__(self, off, ids, w=None, symmetric=False, P=384, part_vertex=False,
-                 hilbert=True):
+                 hilbert=True, relabel=False):
         self.off = np.ascontiguousarray(off, np.uint32)
         self.ids = np.ascontiguousarray(ids, np.uint32)
         self.n = len(self.off) - 1
@@ -197,7 +198,7 @@ class Graph:
         wp = None if self._w is None else self._w.ctypes.data
         self.h = L.ggo_graph_new(self.n, self.m, self.off, self.ids if self.m else
                                  np.zeros(1, np.uint32), wp, int(symmetric), P,
-                                 int(part_vertex), int(hilbert))
+                                 int(part_vertex), int(hilbert), int(relabel))
         if not self.h:
             raise ValueError(f"oracle graph build failed: {L.ggo_graph_error()}")
 
@@ -223,11 +224,13 @@ class Graph:
         lib().ggo_graph_coo(self.h, p, s, d)
         return s[:cnt], d[:cnt]
 
-    def pagerank(self, max_iter=10):
+    def pagerank(self, max_iter=10, bit_frontiers=False):
+        """PageRank.C (bit_frontiers: PageRankBit.C); (ranks, iters, trace)."""
         out = np.empty(self.n, np.float64)
         it = C.c_int(0)
         tr = (Trace * max_iter)()
-        rc = lib().ggo_pagerank(self.h, max_iter, out, C.byref(it), C.cast(tr, C.c_void_p))
+        rc = lib().ggo_pagerank_ex(self.h, max_iter, int(bit_frontiers), out, C.byref(it),
+                                   C.cast(tr, C.c_void_p))
         assert rc == 0, rc
         return out, it.value, [tr[i] for i in range(it.value)]
 
@@ -389,10 +392,13 @@ def write_ggb(path, off, ids, w=None, symmetric=False):
 
 
 def run_ref(app: str, off, ids, w=None, symmetric=False, P=384, part_vertex=False,
-            root=0, x=None, with_partitions=False, threads=None):
+            root=0, x=None, with_partitions=False, threads=None, relabel=False, trace=False):
     """Run oracle/_ref/ref_<app> (the reference engine) on a graph.
 
-    Returns (result float64[n], iters, partitions-or-None)."""
+    relabel builds the partitioned graph with partition_relabel (the driver's
+    -o); with trace the frontier's (d_m, num_out_edges) before each edgeMap
+    is returned as an int64 array of pairs.  Returns (result float64[n],
+    iters, partitions-or-None[, trace])."""
     exe = os.path.join(REF_DIR, f"ref_{app}")
     n = len(off) - 1
     with tempfile.TemporaryDirectory() as td:
@@ -404,6 +410,11 @@ def run_ref(app: str, off, ids, w=None, symmetric=False, P=384, part_vertex=Fals
                "vertex" if part_vertex else "edge", "-r", str(root)]
         if with_partitions:
             cmd += ["-p", pp]
+        if relabel:
+            cmd += ["-R"]
+        tp = os.path.join(td, "trace.bin")
+        if trace:
+            cmd += ["-t", tp]
         if x is not None:
             xp = os.path.join(td, "x.bin")
             np.ascontiguousarray(x, np.float64).tofile(xp)
@@ -431,4 +442,6 @@ def run_ref(app: str, off, ids, w=None, symmetric=False, P=384, part_vertex=Fals
                     lists.append(sd)
             parts = (coo_starts, csc_starts, lists)
         assert len(res) == n
+        if trace:
+            return res, iters, parts, np.fromfile(tp, np.int64).reshape(-1, 2)
         return res, iters, parts

@@ -269,7 +269,7 @@ void ggo_graph_free(ggo_graph *g)
 
 ggo_graph *ggo_graph_new(uint32_t n, uint64_t m, const uint32_t *off,
                          const uint32_t *ids, const double *w, int symmetric,
-                         int P, int part_vertex, int hilbert)
+                         int P, int part_vertex, int hilbert, int relabel)
 {
     g_last_error = 0;
     if (n == 0 || P < 1 || off[n] != m) { g_last_error = -1; return NULL; }
@@ -298,7 +298,7 @@ ggo_graph *ggo_graph_new(uint32_t n, uint64_t m, const uint32_t *off,
 
     /* COO partitioner sizes + starts (graph-numa.h:651-655) */
     uint64_t *sizes = (uint64_t *)malloc((size_t)P * 8);
-    int rc = ggo_partition_sizes(n, indeg, m, P, part_vertex ? 1 : 0, 0, sizes);
+    int rc = ggo_partition_sizes(n, indeg, m, P, part_vertex ? 1 : 0, relabel, sizes);
     if (rc < 0) { g_last_error = rc; free(sizes); free(indeg); ggo_graph_free(g); return NULL; }
     g_last_error = rc;
     g->coo_starts = (uint64_t *)malloc(((size_t)P + 1) * 8);
@@ -358,7 +358,7 @@ ggo_graph *ggo_graph_new(uint32_t n, uint64_t m, const uint32_t *off,
     uint64_t c = 0;
     for (uint32_t v = 0; v < n; v++)
         if (indeg[v]) { g->comp[c] = v; cdeg[c] = indeg[v]; c++; }
-    rc = ggo_partition_sizes(nc, cdeg, m, P, part_vertex ? 1 : 0, 0, sizes);
+    rc = ggo_partition_sizes(nc, cdeg, m, P, part_vertex ? 1 : 0, relabel, sizes);
     free(cdeg);
     free(indeg);
     if (rc < 0) { g_last_error = rc - 10; free(sizes); ggo_graph_free(g); return NULL; }
@@ -727,7 +727,15 @@ double ggo_plus_reduce(const double *a, uint64_t n)
 int ggo_pagerank(const ggo_graph *g, int max_iter, double *out, int *iters,
                  ggo_trace *trace)
 {
-    /* app/PageRank.C:102-151 */
+    return ggo_pagerank_ex(g, max_iter, 0, out, iters, trace);
+}
+
+int ggo_pagerank_ex(const ggo_graph *g, int max_iter, int bit_frontiers, double *out, int *iters,
+                    ggo_trace *trace)
+{
+    /* app/PageRank.C:102-151; bit_frontiers: app/PageRankBit.C:102-150,
+     * whose loop marks each edgeMap output all-active (output.bit = true,
+     * 145) while its stored d_m / num_out_edges still steer the branch */
     int64_t n = g->n;
     const double damping = 0.85, epsilon = 0.0000001;
     double one_over_n = 1 / (double)n;
@@ -749,6 +757,7 @@ int ggo_pagerank(const ggo_graph *g, int max_iter, double *out, int *iters,
         double *t = pc; pc = pn; pn = t;
         fr_free(&F);
         F = output;
+        if (bit_frontiers && F.n) F.bit = 1;
     }
     /* the x of the last executed iteration */
     memcpy(out, broke ? pn : pc, (size_t)n * 8);

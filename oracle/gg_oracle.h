@@ -77,11 +77,12 @@ uint64_t ggo_round_up_pow2(uint64_t v);
 /* ---- engine ------------------------------------------------------------ */
 typedef struct ggo_graph ggo_graph;
 /* Copies off/ids (and weights when w != NULL, CSR order).  Builds the CSC
- * twin, P partitions (mode 0 edge / 1 vertex), COO lists (edge mode only,
- * Hilbert order when hilbert != 0 else CSR order). */
+ * twin, P partitions (mode 0 edge / 1 vertex; relabel != 0: the driver's -o,
+ * partition_relabel in graph-numa.h:1180-1194, 1290-1305), COO lists (edge
+ * mode only, Hilbert order when hilbert != 0 else CSR order). */
 ggo_graph *ggo_graph_new(uint32_t n, uint64_t m, const uint32_t *off,
                          const uint32_t *ids, const double *w, int symmetric,
-                         int P, int part_vertex, int hilbert);
+                         int P, int part_vertex, int hilbert, int relabel);
 void ggo_graph_free(ggo_graph *g);
 int ggo_graph_error(void); /* status of the last ggo_graph_new (<0 error, bit0/bit1: COO/CSC sizing underran) */
 /* COO partition starts (P+1 vertex ids) and CSC partition starts (P+1
@@ -108,6 +109,10 @@ typedef struct {
 /* app/PageRank.C:102-151.  out = x of the last executed iteration. */
 int ggo_pagerank(const ggo_graph *g, int max_iter, double *out, int *iters,
                  ggo_trace *trace);
+/* ggo_pagerank with bit_frontiers != 0: app/PageRankBit.C (every edgeMap
+ * output marked all-active). */
+int ggo_pagerank_ex(const ggo_graph *g, int max_iter, int bit_frontiers, double *out, int *iters,
+                    ggo_trace *trace);
 /* app/BFS.C:77-101.  parents: -1 unreached.  Returns number of levels. */
 int ggo_bfs(const ggo_graph *g, uint32_t root, int32_t *parents,
             ggo_trace *trace, int max_trace);

@@ -9,10 +9,17 @@
 // loops below restate each app's Compute() (app/PageRank.C:102-151,
 // app/BFS.C:77-101, app/Components.C:101-126, app/SPMV.C:64-92,
 // app/BC.C:154-226, app/BellmanFord.C:105-146, app/PageRankDelta.C:124-176,
-// app/PageRankConverage.C:238-310) with the app's own functors and the
+// app/PageRankConverage.C:238-310, app/PageRankBit.C:102-150) with the app's own functors and the
 // reference's own edgeMap/vertexMap/vertexFilter, but keep the result
 // instead of freeing it.  BellmanFord is built with -DWEIGHTED (its
 // functor's update takes the edge length).
+//
+// Options beyond the reference driver's: -R builds the partitioned graph
+// with partition_relabel (the driver's -o), -t FILE records the frontier's
+// public d_m and num_out_edges (ligra-numa.h:91-272) before every edgeMap of
+// PageRank, PageRankBit, BFS and Components (int64 pairs), and -p FILE
+// writes the partition starts and COO lists.  Built with -DEDGES_HILBERT=0
+// (ref_pagerank_csr) the COO lists are in the reference's CSR order.
 //
 // Input: an in-memory graph, either read from a .ggb file written by the
 // tests (magic, n, m, flags, offsets, ids[, int weights]) or generated with
@@ -34,8 +41,10 @@
 #include "app/PageRankDelta.C"
 #elif GG_APP == 8
 #include "app/PageRankConverage.C"
+#elif GG_APP == 9
+#include "app/PageRankBit.C"
 #else
-#error "GG_APP must be 1..8"
+#error "GG_APP must be 1..9"
 #endif
 #undef main
 
@@ -46,6 +55,17 @@
 #include "gg_oracle.h"
 
 namespace {
+
+// -t: (d_m, num_out_edges) of the frontier each edgeMap is called with
+std::vector<int64_t> g_trace;
+bool g_tracing = false;
+inline void trace_frontier(const partitioned_vertices &F)
+{
+    if (g_tracing) {
+        g_trace.push_back((int64_t)F.d_m);
+        g_trace.push_back((int64_t)F.num_out_edges);
+    }
+}
 
 struct HostGraph {
     uint64_t n = 0, m = 0;
@@ -147,7 +167,9 @@ wholeGraph<vertex> to_whole(const HostGraph &h)
     return G;
 }
 
-#if GG_APP == 1
+#if GG_APP == 1 || GG_APP == 9
+// PageRank.C:102-151; PageRankBit.C:102-150 differs only in marking each
+// edgeMap output as all-active (output.bit = true)
 template <class GraphType>
 int run_app(GraphType &GA, long, std::vector<double> &res, long &iters)
 {
@@ -166,6 +188,7 @@ int run_app(GraphType &GA, long, std::vector<double> &res, long &iters)
     bool broke = false;
     while (count < MaxIter) {
         count++;
+        trace_frontier(Frontier);
         partitioned_vertices output = edgeMap(GA, Frontier, PR_F<vertex>(p_curr, p_next, WG.V), m / 20);
         vertexMap(part, Frontier, PR_Vertex_F(p_curr, p_next, 0.85, n));
         map_vertexL(part, [&](intT j) { p_curr[j] = fabs(p_curr[j] - p_next[j]); });
@@ -174,6 +197,9 @@ int run_app(GraphType &GA, long, std::vector<double> &res, long &iters)
         vertexMap(part, Frontier, PR_Vertex_Reset(p_curr));
         swap(p_curr, p_next);
         Frontier.del();
+#if GG_APP == 9
+        output.bit = true; // app/PageRankBit.C:145
+#endif
         Frontier = output;
     }
     double *r = broke ? p_next.get() : p_curr.get();
@@ -197,6 +223,7 @@ int run_app(GraphType &GA, long start, std::vector<double> &res, long &iters)
     partitioned_vertices Frontier = partitioned_vertices::create(n, start, GA.get_partition().V[start].getOutDegree());
     long levels = 0;
     while (!Frontier.isEmpty()) {
+        trace_frontier(Frontier);
         partitioned_vertices output = edgeMap(GA, Frontier, BFS_F(Parents), m / 20);
         Frontier.del();
         Frontier = output;
@@ -223,6 +250,7 @@ int run_app(GraphType &GA, long, std::vector<double> &res, long &iters)
     long rounds = 0;
     while (!Frontier.isEmpty()) {
         vertexMap(part, Frontier, CC_Vertex_F(IDs, prevIDs));
+        trace_frontier(Frontier);
         partitioned_vertices output = edgeMap(GA, Frontier, CC_F(IDs, prevIDs), m / 20);
         Frontier.del();
         Frontier = output;
@@ -435,11 +463,11 @@ int run_app(GraphType &GA, long, std::vector<double> &res, long &iters)
 #endif
 
 template <class vertex>
-int drive(HostGraph &h, int P, bool part_vertex, long start, int rounds, const char *out_path,
-          const char *part_path)
+int drive(HostGraph &h, int P, bool part_vertex, bool relabel, long start, int rounds, const char *out_path,
+          const char *part_path, const char *trace_path)
 {
     wholeGraph<vertex> G = to_whole<vertex>(h);
-    partitioned_graph<vertex> PG(G, P, false, part_vertex, false);
+    partitioned_graph<vertex> PG(G, P, false, part_vertex, relabel);
     if (part_path) {
         // partition boundaries and COO lists, for pinning the restatement
         FILE *f = fopen(part_path, "wb");
@@ -463,6 +491,7 @@ int drive(HostGraph &h, int P, bool part_vertex, long start, int rounds, const c
     long iters = 0;
     std::vector<double> times;
     for (int r = 0; r < rounds; r++) {
+        g_tracing = trace_path && r == 0;
         auto t0 = std::chrono::steady_clock::now();
         run_app(PG, start, res, iters);
         auto t1 = std::chrono::steady_clock::now();
@@ -473,6 +502,11 @@ int drive(HostGraph &h, int P, bool part_vertex, long start, int rounds, const c
            (unsigned long long)h.m, iters);
     for (size_t i = 0; i < times.size(); i++) printf("%s%.6f", i ? ", " : "", times[i]);
     printf("], \"threads\": %d}\n", omp_get_max_threads());
+    if (trace_path) {
+        FILE *f = fopen(trace_path, "wb");
+        fwrite(g_trace.data(), 8, g_trace.size(), f);
+        fclose(f);
+    }
     if (out_path) {
         FILE *f = fopen(out_path, "wb");
         int64_t it = iters;
@@ -493,7 +527,8 @@ int main(int argc, char **argv)
     int P = 384, rounds = 1;
     bool part_vertex = false;
     long start = 100;
-    const char *in = nullptr, *out = nullptr, *part = nullptr, *xfile = nullptr;
+    const char *in = nullptr, *out = nullptr, *part = nullptr, *xfile = nullptr, *trace = nullptr;
+    bool relabel = false;
     int scale = 0, ef = 16;
     uint64_t seed = 1;
     bool sym = false;
@@ -512,6 +547,8 @@ int main(int argc, char **argv)
         else if (a == "-p") part = next();
         else if (a == "-x") xfile = next();
         else if (a == "-s") sym = true;
+        else if (a == "-R") relabel = true;
+        else if (a == "-t") trace = next();
         else if (a == "-rmat") { scale = atoi(next()); ef = atoi(next()); seed = strtoull(next(), 0, 10); }
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
@@ -520,7 +557,7 @@ int main(int argc, char **argv)
     } else if (scale > 0) {
         if (!gen_rmat(scale, ef, seed, sym, h)) { fprintf(stderr, "generator failed\n"); return 1; }
     } else {
-        fprintf(stderr, "usage: %s [-c P] [-v edge|vertex] [-r root] [-rounds R] (-i g.ggb | -rmat S EF SEED [-s]) [-o out] [-p part.bin]\n", argv[0]);
+        fprintf(stderr, "usage: %s [-c P] [-v edge|vertex] [-R] [-r root] [-rounds R] (-i g.ggb | -rmat S EF SEED [-s]) [-o out] [-p part.bin] [-t trace.bin]\n", argv[0]);
         return 2;
     }
 #if GG_APP == 4
@@ -534,6 +571,6 @@ int main(int argc, char **argv)
     (void)xfile;
 #endif
     if (start < 0 || (uint64_t)start >= h.n) start = 0;
-    if (h.symmetric) return drive<symmetricVertex>(h, P, part_vertex, start, rounds, out, part);
-    return drive<asymmetricVertex>(h, P, part_vertex, start, rounds, out, part);
+    if (h.symmetric) return drive<symmetricVertex>(h, P, part_vertex, relabel, start, rounds, out, part, trace);
+    return drive<asymmetricVertex>(h, P, part_vertex, relabel, start, rounds, out, part, trace);
 }

@@ -2,7 +2,7 @@
 // (the drop-in C++ API) that write their results to a file, so the tests can
 // compare the front end's device path with the oracle.
 //
-//   GG_APP=pagerank|bfs|cc|spmv|bc|bc32|sssp|prdelta|prconv|factories GG_DUMP=out.bin
+//   GG_APP=pagerank|prbit|bfs|cc|spmv|bc|bc32|sssp|prdelta|prconv|factories GG_DUMP=out.bin
 //       frontend_parity[_w] <driver args>
 //
 // The functor structs follow the reference apps' interface (member names
@@ -291,7 +291,8 @@ void Compute(GraphType &GA, long start)
     const std::string app = getenv("GG_APP") ? getenv("GG_APP") : "pagerank";
     std::vector<double> out(n);
     long iters = 0;
-    if (app == "pagerank") {
+    if (app == "pagerank" || app == "prbit") {
+        // prbit: app/PageRankBit.C, which marks each output all-active
         graph<vertex> &WG = GA.get_partition();
         mmap_ptr<double> c, x;
         c.part_allocate(part);
@@ -314,6 +315,7 @@ void Compute(GraphType &GA, long start)
             vertexMap(part, F, RankReset(c));
             swap(c, x);
             F.del();
+            if (app == "prbit") next.bit = true; // app/PageRankBit.C:145
             F = next;
         }
         const double *r = broke ? x.get() : c.get();

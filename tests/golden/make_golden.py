@@ -5,7 +5,8 @@ and partitioned_graph, compiled from /root/reference by oracle/Makefile) on
 small graphs and stores inputs and outputs as .npz fixtures.  Run in the
 build container, where /root/reference exists:
 
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py        # all fixtures
+    make -C oracle && python tests/golden/make_golden.py ext    # ext_*.npz only
 
 Inputs: the reference's own 'sixnode' graph (its only shipped fixture, kept
 here as data) and seeded R-MAT graphs from the shared generator.
@@ -77,6 +78,56 @@ def case_apps(name, off, ids, symmetric, P, root, apps, wseed=None, part_vertex=
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def case_ext(name, off, ids, symmetric, P, root, trace_apps):
+    """Further reference runs on one graph (ext_<name>.npz): partition
+    starts and COO lists with partition_relabel (the driver's -o), COO lists
+    in the reference's CSR order (EDGES_HILBERT=0), PageRankBit.C's ranks,
+    and the frontier's (d_m, num_out_edges) before every edgeMap of the
+    listed apps."""
+    out = {"off": off, "ids": ids, "symmetric": np.int64(symmetric), "P": np.int64(P),
+           "root": np.int64(root)}
+    for part_vertex in (False, True):
+        tag = "vertex" if part_vertex else "edge"
+        _, _, (cs, ss, lists) = oracle.run_ref("pagerank", off, ids, symmetric=symmetric, P=P,
+                                               part_vertex=part_vertex, root=root, with_partitions=True,
+                                               threads=1, relabel=True)
+        out[f"coo_starts_{tag}_relabel"] = cs
+        out[f"csc_starts_{tag}_relabel"] = ss
+        if lists:
+            out[f"coo_count_{tag}_relabel"] = np.array([len(l) for l in lists], np.int64)
+            out[f"coo_edges_{tag}_relabel"] = np.concatenate(lists)
+        if not symmetric:
+            res, iters, _ = oracle.run_ref("prbit", off, ids, P=P, part_vertex=part_vertex, root=root, threads=1)
+            out[f"prbit_{tag}"] = res
+            out[f"prbit_{tag}_iters"] = np.int64(iters)
+        for app in trace_apps:
+            _, iters, _, tr = oracle.run_ref(app, off, ids, symmetric=symmetric, P=P, part_vertex=part_vertex,
+                                             root=root, threads=1, trace=True)
+            out[f"trace_{app}_{tag}"] = tr
+            out[f"trace_{app}_{tag}_iters"] = np.int64(iters)
+    _, _, (cs, ss, lists) = oracle.run_ref("pagerank_csr", off, ids, symmetric=symmetric, P=P, root=root,
+                                           with_partitions=True, threads=1)
+    out["coo_count_edge_csr"] = np.array([len(l) for l in lists], np.int64)
+    out["coo_edges_edge_csr"] = np.concatenate(lists)
+    path = os.path.join(HERE, f"ext_{name}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def main_ext():
+    if not all(oracle.ref_available(a) for a in ("pagerank", "prbit", "pagerank_csr", "bfs", "cc")):
+        sys.exit("oracle/_ref not built (needs /root/reference): make -C oracle")
+    case_ext("sixnode_p2", SIX_OFF, SIX_IDS, False, 2, 0, ["pagerank", "bfs"])
+    so, si = oracle.csr_from_edges(6, np.repeat(np.arange(6), np.diff(SIX_OFF)), SIX_IDS, symmetric=True)
+    case_ext("sixnode_sym_p2", so, si, True, 2, 0, ["bfs", "cc"])
+    off, ids = oracle.rmat_csr(10, 16, 1)
+    case_ext("rmat10_dir_p16", off, ids, False, 16, 100, ["pagerank", "bfs"])
+    off, ids = oracle.rmat_csr(11, 16, 2, symmetric=True)
+    case_ext("rmat11_sym_p32", off, ids, True, 32, 100, ["bfs", "cc"])
+    off, ids = oracle.rmat_csr(12, 8, 5)
+    case_ext("rmat12_dir_p64", off, ids, False, 64, 100, ["pagerank", "bfs"])
+
+
 def main():
     if not all(oracle.ref_available(a) for a in ("pagerank", "bc", "bellmanford", "prdelta", "prconv")):
         sys.exit("oracle/_ref not built (needs /root/reference): make -C oracle")
@@ -111,4 +162,5 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    # 'ext' writes only the ext_*.npz fixtures (the others stay as committed)
+    main_ext() if sys.argv[1:] == ["ext"] else main()

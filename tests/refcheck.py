@@ -169,7 +169,17 @@ def rel_l1(a, b):
 def golden_cases():
     """Fixtures of the engine apps (partitions, COO lists, the four apps)."""
     return sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith("apps_"))
+                  if not os.path.basename(p).startswith(("apps_", "ext_")))
+
+
+def ext_golden_cases(key=None):
+    """Further reference runs (ext_*.npz: -o relabel partitions, CSR-order
+    COO lists, PageRankBit.C, per-step frontier traces), optionally only
+    those holding 'key'."""
+    paths = sorted(glob.glob(os.path.join(GOLDEN, "ext_*.npz")))
+    if key is None:
+        return paths
+    return [p for p in paths if key in np.load(p, allow_pickle=False).files]
 
 
 def app_golden_cases(app=None):

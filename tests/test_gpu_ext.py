@@ -1,0 +1,254 @@
+"""GPU parity on the further reference runs (tests/golden/ext_*.npz, written
+by the reference engine compiled from its own sources) and on entry points
+the apps reach only indirectly:
+
+- '-o' relabel partitioning (graph-numa.h:1180-1194, 1290-1305): partition
+  starts bit-exact; COO lists with the reference's Hilbert key sequence and
+  edges (the order of equal keys, std::sort's, is not pinned);
+- EDGES_HILBERT=0: COO lists in the reference's CSR order, edge for edge;
+- PageRankBit.C (the app source's loop through the drop-in front end, and
+  the same loop on the Python mirror with Frontier.set_bit): ranks within
+  1e-12 of the reference's;
+- the frontier's d_m and num_out_edges before every edgeMap of PageRank.C,
+  BFS.C and Components.C loops run on gg_edgemap, equal to the reference's;
+- remDuplicates (ligra-numa.h:317-348) on a push whose functor reports a
+  destination once per in-edge: each id once, the same set, the counts of
+  the set;
+- vertexFilter's device entry (1053-1082): members of the frontier whose
+  flag is set, with their count and out-degree sum.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import graphgrind_amd as gg
+import oracle
+import refcheck as R
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(ROOT, "apps", "_build")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gg.Context(0)
+    yield c
+    c.close()
+
+
+def _graph(ctx, z):
+    return gg.Graph.from_csr(ctx, z["off"], z["ids"], symmetric=bool(z["symmetric"]))
+
+
+def _split(counts, edges):
+    out, o = [], 0
+    for c in counts:
+        out.append(edges[o:o + int(c)])
+        o += int(c)
+    return out
+
+
+@pytest.mark.parametrize("path", R.ext_golden_cases(), ids=lambda p: p.split("/")[-1])
+def test_relabel_partitions_match_reference(ctx, path):
+    z = np.load(path)
+    P = int(z["P"])
+    print(f"relabel partitions {os.path.basename(path)}", flush=True)
+    for pv in (False, True):
+        tag = "vertex" if pv else "edge"
+        g = _graph(ctx, z).partition(P, by_vertex=pv, relabel=True)
+        assert np.array_equal(g.starts(0)[0], z[f"coo_starts_{tag}_relabel"])
+        assert np.array_equal(g.starts(1)[0], z[f"csc_starts_{tag}_relabel"])
+        if not pv:
+            src, dst, po = g.coo()
+            refs = _split(z["coo_count_edge_relabel"], z["coo_edges_edge_relabel"])
+            for p, ref in enumerate(refs):
+                mine = np.stack([src[po[p]:po[p + 1]], dst[po[p]:po[p + 1]]], 1)
+                side = oracle.lib().ggo_round_up_pow2(len(ref))
+                key = lambda e: [oracle.hilbert_xy2d(side, int(a), int(b)) for a, b in e]  # noqa: E731
+                assert key(mine) == key(ref), p
+                assert sorted(map(tuple, mine.tolist())) == sorted(map(tuple, ref.tolist())), p
+        g.close()
+
+
+@pytest.mark.parametrize("path", R.ext_golden_cases(), ids=lambda p: p.split("/")[-1])
+def test_csr_order_coo_matches_reference(ctx, path):
+    z = np.load(path)
+    P = int(z["P"])
+    g = _graph(ctx, z).partition(P, hilbert=False)
+    src, dst, po = g.coo()
+    for p, ref in enumerate(_split(z["coo_count_edge_csr"], z["coo_edges_edge_csr"])):
+        mine = np.stack([src[po[p]:po[p + 1]], dst[po[p]:po[p + 1]]], 1)
+        assert np.array_equal(mine, ref), p
+    g.close()
+
+
+def _pagerank_loop(ctx, g, bit_frontiers=False, trace=None):
+    """PageRank.C (PageRankBit.C with bit_frontiers) on the edgeMap /
+    vertexMap entry points; 'trace' collects (d_m, num_out_edges) of the
+    frontier before each edgeMap."""
+    n = g.n
+    pc = gg.DeviceArray(ctx, n, np.float64).fill(1.0 / n)
+    pn = gg.DeviceArray(ctx, n, np.float64).fill(0.0)
+    F = gg.Frontier.bits(g)
+    broke, it = False, 0
+    while it < 10:
+        it += 1
+        if trace is not None:
+            i = F.info()
+            trace.append((i["d_m"], i["out_edges"]))
+        out, _ = gg.edge_map(g, F, gg.OP_PAGERANK, pc, pn, threshold=g.m // 20)
+        gg.vertex_map(g, F, gg.VOP_PAGERANK, pc, pn, 0.85, (1 - 0.85) * (1 / n))
+        gg.vertex_map(g, None, gg.VOP_ABSDIFF, pc, pn)
+        if gg.reduce_sum(ctx, pc) < 1e-7:
+            broke = True
+            out.close()
+            break
+        gg.vertex_map(g, F, gg.VOP_RESET, pc)
+        pc, pn = pn, pc
+        F.close()
+        if bit_frontiers:
+            out.set_bit(True)  # app/PageRankBit.C:145
+        F = out
+    F.close()
+    return (pn if broke else pc).to_host(), it
+
+
+@pytest.mark.parametrize("path", R.ext_golden_cases("prbit_edge"), ids=lambda p: p.split("/")[-1])
+def test_pagerankbit_loop_matches_reference(ctx, path):
+    z = np.load(path)
+    P = int(z["P"])
+    for pv in (False, True):
+        tag = "vertex" if pv else "edge"
+        g = _graph(ctx, z).partition(P, by_vertex=pv)
+        r, it = _pagerank_loop(ctx, g, bit_frontiers=True)
+        assert it == int(z[f"prbit_{tag}_iters"])
+        assert R.rel_l1(r, z[f"prbit_{tag}"]) < 1e-12
+        g.close()
+
+
+@pytest.mark.parametrize("scale,ef,seed,P", [(10, 16, 1, 16), (12, 8, 5, 64)])
+def test_pagerankbit_through_front_end(scale, ef, seed, P):
+    """PageRankBit.C's loop written against the drop-in front end (its
+    'output.bit = true' reaches the engine's frontier)."""
+    name = f"ext_rmat{scale}_dir_p{P}.npz"
+    z = np.load(os.path.join(ROOT, "tests", "golden", name))
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "out.bin")
+        env = dict(os.environ, GG_APP="prbit", GG_DUMP=out)
+        r = subprocess.run([os.path.join(BUILD, "frontend_parity"), "-rounds", "1", "-c", str(P), "-rmat",
+                            str(scale), str(ef), str(seed)], env=env, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr[-2000:]
+        raw = np.fromfile(out, np.float64)
+    assert int(raw[:1].view(np.int64)[0]) == int(z["prbit_edge_iters"])
+    assert R.rel_l1(raw[1:], z["prbit_edge"]) < 1e-12
+
+
+@pytest.mark.parametrize("path", R.ext_golden_cases(), ids=lambda p: p.split("/")[-1])
+def test_frontier_trace_matches_reference(ctx, path):
+    z = np.load(path)
+    P, root = int(z["P"]), int(z["root"])
+    n = len(z["off"]) - 1
+    for pv in (False, True):
+        tag = "vertex" if pv else "edge"
+        g = _graph(ctx, z).partition(P, by_vertex=pv)
+        if f"trace_pagerank_{tag}" in z.files:
+            tr = []
+            _, it = _pagerank_loop(ctx, g, trace=tr)
+            assert it == int(z[f"trace_pagerank_{tag}_iters"])
+            assert np.array_equal(np.array(tr, np.int64).reshape(-1, 2), z[f"trace_pagerank_{tag}"])
+        if f"trace_bfs_{tag}" in z.files:
+            par = np.full(n, -1, np.int32)
+            par[root] = root
+            Pd = gg.DeviceArray.from_host(ctx, par)
+            F = gg.Frontier.single(g, root)
+            tr = []
+            while F.info()["d_m"]:
+                i = F.info()
+                tr.append((i["d_m"], i["out_edges"]))
+                out, _ = gg.edge_map(g, F, gg.OP_BFS, Pd, threshold=g.m // 20)
+                F.close()
+                F = out
+            F.close()
+            assert len(tr) == int(z[f"trace_bfs_{tag}_iters"])
+            assert np.array_equal(np.array(tr, np.int64).reshape(-1, 2), z[f"trace_bfs_{tag}"])
+        if f"trace_cc_{tag}" in z.files:
+            ids0 = np.arange(n, dtype=np.uint32)
+            A = gg.DeviceArray.from_host(ctx, ids0)
+            B = gg.DeviceArray.from_host(ctx, ids0)
+            F = gg.Frontier.bits(g)
+            tr = []
+            while F.info()["d_m"]:
+                gg.vertex_map(g, F, gg.VOP_CC_COPY, A, B)
+                i = F.info()
+                tr.append((i["d_m"], i["out_edges"]))
+                out, _ = gg.edge_map(g, F, gg.OP_CC, A, B, threshold=g.m // 20)
+                F.close()
+                F = out
+            F.close()
+            assert len(tr) == int(z[f"trace_cc_{tag}_iters"])
+            assert np.array_equal(np.array(tr, np.int64).reshape(-1, 2), z[f"trace_cc_{tag}"])
+        g.close()
+
+
+def test_remdups_push(ctx):
+    """A sparse push with the PageRank operator reports each destination once
+    per in-edge from the frontier; remDuplicates keeps each id once."""
+    off, ids = oracle.rmat_csr(12, 16, 1)
+    n = len(off) - 1
+    g = gg.Graph.from_csr(ctx, off, ids).partition(64)
+    rng = np.random.default_rng(3)
+    front = np.sort(rng.choice(np.flatnonzero(np.diff(off) > 0), 40, replace=False)).astype(np.uint32)
+    dst = ids[R.expand_ranges(off[front].astype(np.int64), off[front + 1].astype(np.int64))]
+    want = np.unique(dst)
+    assert len(dst) > len(want)  # the case has duplicates
+    outdeg = np.diff(off.astype(np.int64))
+    for remdups in (False, True):
+        a = gg.DeviceArray(ctx, n, np.float64).fill(1.0 / n)
+        b = gg.DeviceArray(ctx, n, np.float64).fill(0.0)
+        F = gg.Frontier.from_sparse(g, front)
+        out, path = gg.edge_map(g, F, gg.OP_PAGERANK, a, b, remdups=remdups)
+        assert path == gg.PATH_SPARSE
+        lst = out.sparse()
+        info = out.info()
+        if remdups:
+            assert len(np.unique(lst)) == len(lst)
+            assert np.array_equal(np.sort(lst), want)
+            assert info["d_m"] == len(want) and info["out_edges"] == int(outdeg[want].sum())
+        else:
+            assert np.array_equal(np.sort(lst), np.sort(dst))
+            assert info["d_m"] == len(dst)
+        out.close()
+        F.close()
+    g.close()
+
+
+@pytest.mark.parametrize("form", ["dense", "sparse", "bits"])
+def test_vertexfilter_device(ctx, form):
+    off, ids = oracle.rmat_csr(12, 16, 2, symmetric=True)
+    n = len(off) - 1
+    g = gg.Graph.from_csr(ctx, off, ids, symmetric=True).partition(64)
+    rng = np.random.default_rng(7)
+    members = rng.random(n) < 0.4
+    flags = (rng.random(n) < 0.5).astype(np.uint8)
+    if form == "dense":
+        F = gg.Frontier.from_dense(g, members.astype(np.uint8))
+    elif form == "sparse":
+        F = gg.Frontier.from_sparse(g, np.flatnonzero(members).astype(np.uint32))
+    else:
+        F = gg.Frontier.bits(g)
+        members = np.ones(n, bool)
+    fd = gg.DeviceArray.from_host(ctx, flags)
+    out = gg.vertex_filter(g, F, fd)
+    want = members & (flags != 0)
+    assert np.array_equal(out.dense().astype(bool), want)
+    info = out.info()
+    assert info["d_m"] == int(want.sum())
+    assert info["out_edges"] == int(np.diff(off.astype(np.int64))[want].sum())
+    out.close()
+    F.close()
+    g.close()

@@ -271,3 +271,108 @@ def test_plus_reduce_blocked_order():
     # serial left-to-right inside each 1024-element block, then over blocks
     blk = [sum(a[i * 1024:(i + 1) * 1024].tolist()) for i in range(5)]
     assert oracle.plus_reduce(a) == sum(blk)
+
+
+# ------------------------------------- further reference runs (ext_*.npz) --
+
+def _ext_cases():
+    return R.ext_golden_cases()
+
+
+def _coo_lists(og, P):
+    return [np.stack(og.coo(p), 1) for p in range(P)]
+
+
+def _split(counts, edges):
+    out, o = [], 0
+    for c in counts:
+        out.append(edges[o:o + int(c)])
+        o += int(c)
+    return out
+
+
+@pytest.mark.parametrize("path", _ext_cases(), ids=lambda p: p.split("/")[-1])
+def test_relabel_partitions_match_reference(path):
+    """-o (partition_relabel, graph-numa.h:1180-1194, 1290-1305): partition
+    starts exact.  The COO lists hold the same edges with the same Hilbert key
+    sequence; the order of edges with equal keys is std::sort's (unstable)
+    and not pinned: relabel's small partitions give curve sides below the
+    ids, where keys fold together."""
+    z = np.load(path)
+    off, ids, sym, P = z["off"], z["ids"], bool(z["symmetric"]), int(z["P"])
+    for pv in (False, True):
+        tag = "vertex" if pv else "edge"
+        og = oracle.Graph(off, ids, symmetric=sym, P=P, part_vertex=pv, relabel=True)
+        assert np.array_equal(og.starts(0), z[f"coo_starts_{tag}_relabel"])
+        assert np.array_equal(og.starts(1), z[f"csc_starts_{tag}_relabel"])
+        if pv:
+            continue
+        for p, (mine, ref) in enumerate(zip(_coo_lists(og, P), _split(z["coo_count_edge_relabel"],
+                                                                          z["coo_edges_edge_relabel"]))):
+            side = oracle.lib().ggo_round_up_pow2(len(ref))
+            key = lambda e: [oracle.hilbert_xy2d(side, int(a), int(b)) for a, b in e]  # noqa: E731
+            assert key(mine) == key(ref), p
+            assert sorted(map(tuple, mine.tolist())) == sorted(map(tuple, ref.tolist())), p
+
+
+@pytest.mark.parametrize("path", _ext_cases(), ids=lambda p: p.split("/")[-1])
+def test_csr_order_coo_matches_reference(path):
+    """EDGES_HILBERT=0: the COO lists in the reference's CSR order
+    (graph-numa.h:338-350, 491-494), edge for edge."""
+    z = np.load(path)
+    off, ids, sym, P = z["off"], z["ids"], bool(z["symmetric"]), int(z["P"])
+    og = oracle.Graph(off, ids, symmetric=sym, P=P, hilbert=False)
+    for p, (mine, ref) in enumerate(zip(_coo_lists(og, P), _split(z["coo_count_edge_csr"], z["coo_edges_edge_csr"]))):
+        assert np.array_equal(mine, ref), p
+
+
+@pytest.mark.parametrize("path", R.ext_golden_cases("prbit_edge"),
+                         ids=lambda p: p.split("/")[-1])
+def test_pagerankbit_matches_reference_and_power_iteration(path):
+    """app/PageRankBit.C: the reference's ranks, and (with every vertex active
+    from the second iteration on and x reset before each pass) textbook
+    power iteration x' = 0.85 A^T D^-1 x + 0.15/n from x = 1/n with the
+    same L1 stop, dangling mass dropped."""
+    z = np.load(path)
+    off, ids, P = z["off"], z["ids"], int(z["P"])
+    for pv in (False, True):
+        tag = "vertex" if pv else "edge"
+        r, it, _ = oracle.Graph(off, ids, P=P, part_vertex=pv).pagerank(10, bit_frontiers=True)
+        assert it == int(z[f"prbit_{tag}_iters"])
+        assert R.rel_l1(r, z[f"prbit_{tag}"]) < 1e-12
+    n = len(off) - 1
+    A = R.to_scipy(off, ids).tocsr()
+    od = np.diff(off.astype(np.int64))
+    c = np.full(n, 1.0 / n)
+    for _ in range(10):
+        x = 0.85 * (A.T @ np.where(od > 0, c / np.maximum(od, 1), 0.0)) + 0.15 / n
+        done = np.abs(c - x).sum() < 1e-7
+        c = x
+        if done:
+            break
+    assert R.rel_l1(c, z["prbit_edge"]) < 1e-12
+
+
+@pytest.mark.parametrize("path", _ext_cases(), ids=lambda p: p.split("/")[-1])
+def test_frontier_trace_matches_reference(path):
+    """The frontier's d_m and num_out_edges before each edgeMap (the public
+    fields of partitioned_vertices, ligra-numa.h:91-272), recorded inside
+    the reference's own PageRank.C / BFS.C / Components.C loops."""
+    z = np.load(path)
+    off, ids, sym, P, root = z["off"], z["ids"], bool(z["symmetric"]), int(z["P"]), int(z["root"])
+    for pv in (False, True):
+        tag = "vertex" if pv else "edge"
+        og = oracle.Graph(off, ids, symmetric=sym, P=P, part_vertex=pv)
+        for app in ("pagerank", "bfs", "cc"):
+            key = f"trace_{app}_{tag}"
+            if key not in z.files:
+                continue
+            if app == "pagerank":
+                _, it, tr = og.pagerank(10)
+            elif app == "bfs":
+                _, it, tr = og.bfs(root)
+            else:
+                _, it, tr = og.cc()
+            assert it == int(z[key + "_iters"]), (app, tag)
+            mine = np.array([[t.in_active, t.in_out_edges] for t in tr], np.int64).reshape(-1, 2)
+            assert np.array_equal(mine, z[key]), (app, tag)
