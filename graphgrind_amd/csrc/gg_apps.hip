@@ -164,6 +164,12 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
     CscView view;
     if (use_plan) view = plan_view(g);
     else GG_TRY(pull_view(g, &view));
+    if (!use_plan && g->dense_coo && g->coo_src.p && g->P && !g->part_vertex && m) {
+        // GG_OPT_DENSE_COO: every iteration takes the COO branch, whose
+        // in-lists keep each destination's edges in COO (Hilbert) order
+        GG_TRY(ensure_coo_pull(g));
+        view.tids = (const uint32_t *)g->coo_tids.p;
+    }
     const uint32_t *outdeg = use_plan ? (const uint32_t *)g->plan.outdeg.p : (const uint32_t *)g->outdeg.p;
     const uint32_t nblk = view.blocks();
     const uint32_t nfix = view.fix_blocks();

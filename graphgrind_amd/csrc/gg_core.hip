@@ -752,7 +752,8 @@ static int build_twin(Graph *g)
 static void graph_free(Graph *g)
 {
     Buf *bufs[] = {&g->off, &g->ids, &g->w, &g->toff, &g->tids, &g->tw, &g->perm, &g->outdeg,
-                   &g->indeg, &g->comp, &g->coo_src, &g->coo_dst, &g->coo_w, &g->pull.start_row,
+                   &g->indeg, &g->comp, &g->coo_src, &g->coo_dst, &g->coo_w, &g->coo_tids, &g->coo_tw,
+                   &g->pull.start_row,
                    &g->pull.start_edge, &g->pull.carry_row, &g->pull.carry_val, &g->pull.head_val,
                    &g->plan.perm, &g->plan.inv, &g->plan.toff, &g->plan.tids, &g->plan.tw, &g->plan.outdeg,
                    &g->plan.coff, &g->plan.cids, &g->plan.cw, &g->plan.cdelta, &g->plan.indeg,
@@ -1389,6 +1390,9 @@ extern "C" int gg_graph_partition(gg_graph_t g, int P, int by_vertex, int relabe
     buf_free(g->coo_src);
     buf_free(g->coo_dst);
     buf_free(g->coo_w);
+    buf_free(g->coo_tids);
+    buf_free(g->coo_tw);
+    g->coo_pull_ok = false;
     g->coo_eoff.assign((size_t)P + 1, 0);
     if (by_vertex || m == 0) return GG_OK;
     std::vector<uint32_t> toff_h((size_t)n + 1);
@@ -1460,6 +1464,73 @@ extern "C" int gg_graph_partition(gg_graph_t g, int P, int by_vertex, int relabe
     if (e != hipSuccess) return fail(GG_ERR_HIP, "k_coo_emit");
     return rc;
 }
+
+namespace gg {
+__global__ void k_coo_rows(const uint32_t *__restrict__ order, const uint32_t *__restrict__ csrc,
+                           const float *__restrict__ cw, uint64_t m, uint32_t *__restrict__ tids,
+                           float *__restrict__ tw)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < m; i += stride) {
+        const uint32_t k = order[i];
+        tids[i] = csrc[k];
+        if (cw) tw[i] = cw[k];
+    }
+}
+
+// The COO lists regrouped by destination: a stable radix sort of the COO
+// positions by destination keeps each destination's edges in COO order,
+// and the rows line up with the twin's offsets (a row's length is its
+// in-degree either way).
+int ensure_coo_pull(Graph *g)
+{
+    if (g->coo_pull_ok) return GG_OK;
+    if (!g->coo_src.p || !g->m) return fail(GG_ERR_STATE, "COO traversal: graph has no COO partitions");
+    Ctx *c = g->ctx;
+    const uint64_t m = g->m;
+    unsigned bits = 1;
+    while (bits < 32 && (1ull << bits) < (uint64_t)g->n) bits++;
+    Buf k0, k1, v0, v1;
+    auto cleanup = [&]() {
+        Buf *bs[] = {&k0, &k1, &v0, &v1};
+        for (Buf *b : bs) buf_free(*b);
+    };
+    int rc = buf_alloc(c, k0, 4 * m);
+    if (!rc) rc = buf_alloc(c, k1, 4 * m);
+    if (!rc) rc = buf_alloc(c, v0, 4 * m);
+    if (!rc) rc = buf_alloc(c, v1, 4 * m);
+    if (!rc) rc = buf_alloc(c, g->coo_tids, 4 * m);
+    if (!rc && g->coo_w.p) rc = buf_alloc(c, g->coo_tw, 4 * m);
+    if (rc) { cleanup(); return rc; }
+    if (hipMemcpyAsync(k0.p, g->coo_dst.p, 4 * m, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+        cleanup();
+        return fail(GG_ERR_HIP, "COO traversal: key copy");
+    }
+    if ((rc = gg_iota(abi(c), v0.p, GG_U32, m))) { cleanup(); return rc; }
+    size_t tb = 0;
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, tb, (uint32_t *)k0.p, (uint32_t *)k1.p, (uint32_t *)v0.p,
+                                             (uint32_t *)v1.p, (size_t)m, 0, bits, c->stream);
+    if (e == hipSuccess) rc = buf_ensure(c, c->tmp, tb + 16);
+    if (e == hipSuccess && !rc)
+        e = rocprim::radix_sort_pairs(c->tmp.p, tb, (uint32_t *)k0.p, (uint32_t *)k1.p, (uint32_t *)v0.p,
+                                      (uint32_t *)v1.p, (size_t)m, 0, bits, c->stream);
+    if (e != hipSuccess || rc) {
+        cleanup();
+        return rc ? rc : fail(GG_ERR_HIP, std::string("COO traversal sort: ") + hipGetErrorString(e));
+    }
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((m + kBlock - 1) / kBlock, 1u << 20);
+    k_coo_rows<<<grid, kBlock, 0, c->stream>>>((const uint32_t *)v1.p, (const uint32_t *)g->coo_src.p,
+                                                (const float *)g->coo_w.p, m, (uint32_t *)g->coo_tids.p,
+                                                (float *)g->coo_tw.p);
+    e = hipGetLastError();
+    rc = sync(c);
+    cleanup();
+    if (e != hipSuccess) return fail(GG_ERR_HIP, "k_coo_rows");
+    if (!rc) g->coo_pull_ok = true;
+    return rc;
+}
+} // namespace gg
 
 extern "C" int gg_graph_partition_starts(gg_graph_t g, int which, uint64_t *starts, int *underran)
 {

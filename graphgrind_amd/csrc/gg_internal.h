@@ -219,6 +219,10 @@ struct Graph {
     uint64_t ncomp = 0;
     Buf coo_src, coo_dst; // uint32[m] (edge mode only)
     Buf coo_w;            // float[m] when weighted
+    // the in-edge twin's rows with each destination's edges in COO (Hilbert)
+    // order: sources and weights, built on first use by the COO traversal
+    Buf coo_tids, coo_tw;
+    bool coo_pull_ok = false;
     // merge-path schedule over the in-edge twin
     PullSched pull;
     PullPlan plan;
@@ -244,6 +248,9 @@ struct Graph {
 };
 // Drops captured loops (their addresses or schedules changed).
 void graph_exec_drop(Graph *g);
+// The COO-ordered in-lists (coo_tids / coo_tw), built on first use from the
+// COO partitions; requires edge partitions.
+int ensure_coo_pull(Graph *g);
 // Row blocks of the in-edge twin / of the out-edge CSR (built on first use).
 int ensure_rows(Graph *g);
 int ensure_csr_rows(Graph *g);

@@ -447,80 +447,6 @@ __device__ __forceinline__ bool edge_update(const PushArgs &A, uint32_t v, uint3
     return emit;
 }
 
-// edgeMapDense over the COO partitions (ligra-numa.h:803-866): the
-// partitions' Hilbert-ordered edge lists, back to back, in blocks of
-// kCooTile edges staged in LDS by coalesced loads; each edge whose source is
-// in the frontier is updated by the functor's atomic form, and a destination
-// it reports joins the dense output.
-constexpr int kCooTile = kBlock * 4;
-
-// The sum functors (PR_F, SPMV_F) in the COO traversal: the lanes of a wave
-// whose destination is the first active lane's combine their terms with a
-// wave sum and make one atomic add, the others add their own.  Hub
-// destinations fill whole waves of a Hilbert-ordered block, and their
-// same-address atomics would otherwise serialise.  Called by every lane.
-template <class V>
-__device__ __forceinline__ void coo_sum_wave(const PushArgs &A, bool act, uint32_t s, uint32_t d, uint64_t e,
-                                             uint8_t *__restrict__ next)
-{
-    const uint64_t am = __ballot(act);
-    if (!am) return;
-    const int lane = threadIdx.x & 63;
-    const int lead = __ffsll((unsigned long long)am) - 1;
-    const uint32_t dl = __shfl(d, lead, 64);
-    V term = V(0);
-    if (act) {
-        const V *a = (const V *)A.a;
-        term = A.kind == PUSH_PR ? a[s] / (V)A.outdeg[s] : a[s] * (V)(A.w ? A.w[e] : 1.0f);
-    }
-    const bool mine = act && d == dl;
-    V tot = mine ? term : V(0);
-    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
-    V *b = (V *)A.b;
-    if (lane == lead) {
-        unsafeAtomicAdd(&b[dl], tot);
-        next[dl] = 1;
-    }
-    if (act && !mine) {
-        unsafeAtomicAdd(&b[d], term);
-        next[d] = 1;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_coo(PushArgs A, const uint32_t *__restrict__ src,
-                                                const uint32_t *__restrict__ dst, uint64_t m,
-                                                const uint32_t *__restrict__ bm, int bit, uint8_t *__restrict__ next)
-{
-    __shared__ uint32_t s_src[kCooTile], s_dst[kCooTile];
-    const uint64_t t0 = (uint64_t)blockIdx.x * kCooTile;
-    for (int i = threadIdx.x; i < kCooTile; i += kBlock) {
-        const uint64_t e = t0 + (uint64_t)i;
-        if (e < m) {
-            s_src[i] = __builtin_nontemporal_load(&src[e]);
-            s_dst[i] = __builtin_nontemporal_load(&dst[e]);
-        }
-    }
-    __syncthreads();
-    const bool sums = A.kind == PUSH_PR || A.kind == PUSH_SPMV;
-    for (int it = 0; it < kCooTile / kBlock; it++) {
-        const int i = it * kBlock + threadIdx.x;
-        const uint64_t e = t0 + (uint64_t)i;
-        bool act = e < m;
-        uint32_t s = 0, d = 0;
-        if (act) {
-            s = s_src[i];
-            d = s_dst[i];
-            act = bit || bit_test(bm, s);
-        }
-        if (sums) { // wave-uniform branch: the ballot below needs every lane
-            if (A.f32) coo_sum_wave<float>(A, act, s, d, e, next);
-            else coo_sum_wave<double>(A, act, s, d, e, next);
-        } else if (act && edge_update(A, s, d, e)) {
-            next[d] = 1;
-        }
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void k_push(PushArgs A)
 {
     __shared__ uint32_t s_off[kPushLds];
@@ -928,36 +854,20 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         if ((rc = ensure_bytes(out))) return bail(rc);
         const bool coo = (int64_t)(in->d_m + in->out_edges) > dense_thr && g->P && !g->part_vertex;
         if (path) *path = coo ? GG_PATH_COO : GG_PATH_CSC;
-        if (coo && g->dense_coo && g->coo_src.p && m && (!g->weighted || g->coo_w.p)) {
-            // the COO traversal, on request (GG_OPT_DENSE_COO)
-            if ((rc = fill_bytes(c, out->bytes.p, n, 0))) return bail(rc);
-            PushArgs A{};
-            A.outdeg = (const uint32_t *)g->outdeg.p;
-            A.w = (g->weighted && op->kind != GG_OP_SUM) ? (const float *)g->coo_w.p : nullptr;
-            A.kind = op->kind == GG_OP_BFS ? PUSH_BFS
-                     : op->kind == GG_OP_CC ? PUSH_CC
-                     : op->kind == GG_OP_PAGERANK ? PUSH_PR
-                     : op->kind == GG_OP_SSSP ? PUSH_SSSP
-                     : op->kind == GG_OP_PATHSUM ? PUSH_PATHSUM
-                                                  : PUSH_SPMV;
-            A.f32 = f32;
-            A.a = op->a;
-            A.b = op->b;
-            A.levels = ex ? ex->levels : nullptr;
-            A.level = ex ? ex->level : 0;
-            k_coo<<<blocks_for(m, kCooTile), kBlock, 0, c->stream>>>(
-                A, (const uint32_t *)g->coo_src.p, (const uint32_t *)g->coo_dst.p, m,
-                (const uint32_t *)in->bitmap.p, in->bit ? 1 : 0, (uint8_t *)out->bytes.p);
-            GG_HIP(hipGetLastError());
-            out->has_dense = true;
-            out->dense_valid = true;
-            if ((rc = frontier_stats_dense(out))) return bail(rc);
-            out->deg_src = g->outdeg.p;
-            if (ex) ex->scanned += m;
-            if ((rc = ensure_bitmap(out))) return bail(rc);
-            if ((rc = bitmap_from_bytes(c, (uint8_t *)out->bytes.p, (uint32_t *)out->bitmap.p, n))) return bail(rc);
-            *pout = out;
-            return GG_OK;
+        // In-edge lists the pull reads: the twin (in-lists by source id) or,
+        // on the COO branch with GG_OPT_DENSE_COO, the same rows with each
+        // destination's edges in its COO partition's Hilbert order, the
+        // order in which the reference's PART96 loop adds them
+        // (ligra-numa.h:803-866); either way one owner per destination.
+        const uint32_t *tids_v = (const uint32_t *)g->tids.p;
+        const float *tw_v = (const float *)g->tw.p;
+        CscView pv;
+        if ((rc = pull_view(g, &pv))) return bail(rc);
+        if (coo && g->dense_coo && g->coo_src.p && m) {
+            if ((rc = ensure_coo_pull(g))) return bail(rc);
+            tids_v = (const uint32_t *)g->coo_tids.p;
+            tw_v = (const float *)g->coo_tw.p;
+            pv.tids = tids_v;
         }
         if ((rc = counters_zero(c))) return bail(rc);
         unsigned long long *cnt = (unsigned long long *)c->counters.p;
@@ -967,8 +877,6 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         uint8_t *next = (uint8_t *)out->bytes.p;
         const uint32_t *outdeg = (const uint32_t *)g->outdeg.p;
         Buf tmpv;
-        CscView pv;
-        if ((rc = pull_view(g, &pv))) return bail(rc);
         switch (op->kind) {
         case GG_OP_PAGERANK: {
             if ((rc = pool_take(c, tmpv, (f32 ? 4ull : 8ull) * n))) return bail(rc);
@@ -987,7 +895,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
         }
         case GG_OP_SPMV:
         case GG_OP_SUM: {
-            const float *tw = (g->weighted && op->kind == GG_OP_SPMV) ? (const float *)g->tw.p : nullptr;
+            const float *tw = (g->weighted && op->kind == GG_OP_SPMV) ? tw_v : nullptr;
             if (bit) {
                 if (f32)
                     rc = launch_pull(c, pv, OpSpmvBit<float>{(const float *)op->a, tw, (float *)op->b, next, outdeg,
@@ -1041,7 +949,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             // reference's own choice depends on its thread schedule).
             {
                 k_bfs_pull<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
-                    (const uint32_t *)g->toff.p, (const uint32_t *)g->tids.p, bm, bit, (int32_t *)op->a,
+                    (const uint32_t *)g->toff.p, tids_v, bm, bit, (int32_t *)op->a,
                     ex ? ex->levels : nullptr, ex ? ex->level : 0, next, outdeg, n, cnt);
             }
             rc = GG_OK;
@@ -1072,7 +980,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             if ((rc = pool_take(c, tmpv, 4ull * n))) return bail(rc);
             k_sp_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const int32_t *)op->a, F, bit,
                                                                         (int32_t *)tmpv.p, n);
-            const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
+            const float *tw = g->weighted ? tw_v : nullptr;
             rc = launch_pull(c, pv, OpSPRelax{(const int32_t *)tmpv.p, tw, (int32_t *)op->a, (int32_t *)op->b, next,
                                               outdeg},
                              cnt, nullptr, nullptr);

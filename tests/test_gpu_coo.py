@@ -1,10 +1,15 @@
-"""The COO traversal of edgeMap's dense COO branch (GG_OPT_DENSE_COO = 1:
-the Hilbert-ordered partition edge lists streamed with the functors' atomic
-updates, ligra-numa.h:803-866) against the default destination-owned pull
-on the same inputs, one edgeMap step per operator, and the apps run whole
-with it.  Bars: output frontiers and integer results exact; FP sums within
-1e-12 relative (the atomic order is not fixed); path counts exact (integer
-sums in FP64)."""
+"""The COO traversal of edgeMap's dense COO branch (GG_OPT_DENSE_COO = 1,
+ligra-numa.h:803-866): each destination's edges in its COO partition's
+Hilbert order, the order the reference's PART96 loop adds them in, with one
+owner per destination (no atomics, a fixed summation order).
+
+Checked against the oracle, not against the default pull: one edgeMap step
+per operator against the FP64 sum over the frontier's in-edges (numpy), the
+PageRank.C loop (generic edgeMap and the fused loop) and SPMV.C against the
+oracle's COO run, BFS/CC/BC/Bellman-Ford whole against the oracle.  Results
+are bitwise equal run to run.  Bars: output frontiers and integer results
+exact; FP64 within 1e-12 and FP32 within 1e-5 relative L1.
+"""
 import numpy as np
 import pytest
 
@@ -54,7 +59,11 @@ def _rel(a, b):
 @pytest.mark.parametrize("dt,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])
 @pytest.mark.parametrize("frac", [None, 0.7])
 @pytest.mark.parametrize("kind", ["pagerank", "spmv", "sum"])
-def test_coo_fp_sums(ctx, kind, frac, dt, tol):
+def test_coo_fp_sums_vs_fp64(ctx, kind, frac, dt, tol):
+    """One dense COO step: y[d] += sum over active in-neighbours s of
+    x[s]/outdeg(s) (PR_F), x[s]*w (SPMV_F) or x[s] (PageRankConverage's
+    PR_F), against that sum in FP64; the output frontier is the set of
+    destinations with an active in-neighbour; a second run is bitwise equal."""
     g, off, ids, w = _graph(ctx, 13, 3, weighted=(kind == "spmv"))
     n = g.n
     rng = np.random.default_rng(5)
@@ -62,12 +71,91 @@ def test_coo_fp_sums(ctx, kind, frac, dt, tol):
     x = rng.random(n).astype(dt)
     y0 = rng.random(n).astype(dt)
     k = {"pagerank": gg.OP_PAGERANK, "spmv": gg.OP_SPMV, "sum": gg.OP_SUM}[kind]
-    (f0, _, b0, i0), (f1, _, b1, i1) = _both(
-        g, front, k, lambda: (gg.DeviceArray.from_host(ctx, x), gg.DeviceArray.from_host(ctx, y0)))
-    assert np.array_equal(f0, f1)
-    assert i0 == i1
-    assert _rel(b1, b0) < tol
+    g.set_option(gg.OPT_DENSE_COO, 1)
+    got = []
+    for _ in range(2):
+        a, b = gg.DeviceArray.from_host(ctx, x), gg.DeviceArray.from_host(ctx, y0)
+        F = gg.Frontier.from_dense(g, front) if front is not None else gg.Frontier.bits(g)
+        out, path = gg.edge_map(g, F, k, a, b)
+        assert path == gg.PATH_COO
+        got.append((out.dense(), b.to_host()))
+        out.close()
+        F.close()
+    assert np.array_equal(got[0][0], got[1][0]) and np.array_equal(got[0][1], got[1][1])
+    src = np.repeat(np.arange(n), np.diff(off.astype(np.int64)))
+    act = np.ones(len(ids), bool) if front is None else front[src].astype(bool)
+    od = np.diff(off.astype(np.int64)).astype(np.float64)
+    xs = x.astype(np.float64)[src]
+    term = xs / od[src] if kind == "pagerank" else xs * (w.astype(np.float64) if kind == "spmv" else 1.0)
+    want = y0.astype(np.float64) + np.bincount(ids[act], term[act], minlength=n)
+    flags = np.zeros(n, np.uint8)
+    flags[ids[act]] = 1
+    assert np.array_equal(got[0][0], flags)
+    assert _rel(got[0][1], want) < tol
     g.close()
+
+
+@pytest.mark.parametrize("scale,P", [(14, 64), (18, 384)])
+def test_coo_pagerank_loop_vs_oracle(ctx, scale, P):
+    """PageRank.C with every dense step on the COO traversal, through the
+    generic edgeMap/vertexMap loop and the fused loop, against the oracle's
+    COO run (its PART96 sequential adds in Hilbert order); bitwise equal run
+    to run."""
+    off, ids = oracle.rmat_csr(scale, 16, 1)
+    n = len(off) - 1
+    og = oracle.Graph(off, ids, P=P)
+    want, it, tr = og.pagerank(10)
+    g = gg.Graph.from_csr(ctx, off, ids).partition(P)
+    g.set_option(gg.OPT_DENSE_COO, 1)
+    runs = []
+    for _ in range(2):
+        r, info = gg.pagerank(g, np.float64)
+        assert info.iters == it
+        runs.append(r.to_host())
+    assert np.array_equal(runs[0], runs[1])
+    assert R.rel_l1(runs[0], want) < 1e-12
+    r32, _ = gg.pagerank(g, np.float32)
+    assert R.rel_l1(r32.to_host(), want) < 1e-5
+    pc = gg.DeviceArray(ctx, n, np.float64).fill(1.0 / n)
+    pn = gg.DeviceArray(ctx, n, np.float64).fill(0.0)
+    F = gg.Frontier.bits(g)
+    paths = []
+    for _ in range(10):
+        out, path = gg.edge_map(g, F, gg.OP_PAGERANK, pc, pn, threshold=len(ids) // 20)
+        paths.append(path)
+        gg.vertex_map(g, F, gg.VOP_PAGERANK, pc, pn, 0.85, (1 - 0.85) * (1 / n))
+        gg.vertex_map(g, None, gg.VOP_ABSDIFF, pc, pn)
+        gg.vertex_map(g, F, gg.VOP_RESET, pc)
+        pc, pn = pn, pc
+        F.close()
+        F = out
+    F.close()
+    assert paths == [t.path for t in tr] and gg.PATH_COO in paths
+    assert R.rel_l1(pc.to_host(), want) < 1e-12
+    g.close()
+
+
+def test_coo_spmv_vs_reference_golden(ctx):
+    """SPMV.C (integer weights, x = 1/n or the fixture's x) on the COO
+    traversal against the reference engine's own result."""
+    for path in R.golden_cases():
+        z = R.load_golden(path)
+        if "spmv_edge" not in z:
+            continue
+        w = z["spmv_w"].astype(np.float32)
+        g = gg.Graph.from_csr(ctx, z["off"], z["ids"], w=w).partition(int(z["P"]))
+        g.set_option(gg.OPT_DENSE_COO, 1)
+        n = g.n
+        x = z["spmv_x"] if "spmv_x" in z else np.full(n, 1.0 / n)
+        xd = gg.DeviceArray.from_host(ctx, x.astype(np.float64))
+        y = gg.DeviceArray(ctx, n, np.float64).fill(0.0)
+        F = gg.Frontier.bits(g)
+        out, pth = gg.edge_map(g, F, gg.OP_SPMV, xd, y)
+        assert pth == gg.PATH_COO
+        assert R.rel_l1(y.to_host(), z["spmv_edge"]) < 1e-12, path
+        out.close()
+        F.close()
+        g.close()
 
 
 def test_coo_option_values(ctx):
