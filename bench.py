@@ -48,6 +48,14 @@ def pagerank_bench(gg, ctx, scale, steps, warmup, dtype, P=384):
     g.reorder(1)  # hot-sources-first numbering for the pull (untimed preprocessing)
     log(f"[bench] pagerank graph n={g.n} m={g.m} built+partitioned in {time.perf_counter() - t0:.1f}s")
     out = gg.DeviceArray(ctx, g.n, dtype)
+    # the first call on the graph allocates the loop's rank, contribution and
+    # frontier buffers (kept for later calls): the "including per-call
+    # allocation and initialisation" figure
+    ctx.sync()
+    tf = time.perf_counter()
+    gg.pagerank(g, dtype, out=out)
+    ctx.sync()
+    first_ms = (time.perf_counter() - tf) * 1e3
     for _ in range(warmup):
         gg.pagerank(g, dtype, out=out)
     ctx.sync()
@@ -73,7 +81,8 @@ def pagerank_bench(gg, ctx, scale, steps, warmup, dtype, P=384):
     res = {
         "kernel": f"{kern}<OpPRFused<{vname}>>",
         "n": n, "m": m, "step_ms": wall * 1e3, "event_ms": float(np.mean(ev_ms)),
-        "gteps": 10 * m / wall / 1e9,
+        "gteps": 10 * m / wall / 1e9, "first_call_ms": first_ms,
+        "median_event_ms": float(np.median(ev_ms)),
         "kernel_ms": avg_kernel * 1e3,
         "algo_bytes_per_launch": algo_bytes,
         "achieved_gbs": algo_bytes / avg_kernel / 1e9,
@@ -344,36 +353,101 @@ def apps_bench(gg, ctx, scale, steps, plan=True):
     return res
 
 
-def cpu_baseline(scale=20, rounds=3):
+def dropin_bench(scale, rounds=3):
+    """The reference's own app sources compiled unchanged against the drop-in
+    front end (apps/_build/<App>), timed by their own driver ('Running : t'
+    per round, ligra-numa.h:1155-1157), with the front end's per-round
+    HBM<->host copy report (GG_FRONT_STATS): PageRank.C on config 1 (directed
+    R-MAT 20) and on the headline graph, BFS.C and Components.C on symmetric
+    R-MAT 'scale' (root 100), SPMV.C on directed R-MAT 'scale' (seed 3)."""
+    runs = [("PageRank", ["-c", "384", "-rmat", "20", "16", "1"], "config 1, R-MAT 20"),
+            ("PageRank", ["-c", "384", "-rmat", str(scale), "16", "1"], f"R-MAT {scale}"),
+            ("BFS", ["-s", "-c", "384", "-r", "100", "-rmat", str(scale), "16", "2"], f"symmetric R-MAT {scale}"),
+            ("Components", ["-s", "-c", "384", "-rmat", str(scale), "16", "2"], f"symmetric R-MAT {scale}"),
+            ("SPMV", ["-c", "384", "-rmat", str(scale), "16", "3"], f"R-MAT {scale}")]
+    out = []
+    for app, args, what in runs:
+        exe = os.path.join(ROOT, "apps", "_build", app)
+        if not os.path.exists(exe):
+            continue
+        env = dict(os.environ, GG_FRONT_STATS="1")
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "-rounds", str(rounds)] + args, capture_output=True, text=True, env=env,
+                           timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            log(f"[bench] drop-in {app} failed: {r.stderr[-400:]}")
+            continue
+        times = [float(l.split(":")[1]) * 1e3 for l in r.stdout.splitlines() if l.startswith("Running")]
+        stats = [l for l in r.stdout.splitlines() if l.startswith("gg_front: round")]
+        rec = {"app": f"app/{app}.C", "graph": what, "round_ms": times,
+               "median_round_ms": float(np.median(times)) if times else None, "wall_s": wall}
+        if stats:
+            # the last round's coherence traffic: copies each way, host loops
+            last = stats[-1].split(":", 2)[2]
+            rec["last_round_host_traffic"] = last.strip()
+        out.append(rec)
+        log(f"[bench] drop-in {app} ({what}): {rec}")
+    return out
+
+
+def host_cpu_record():
+    """What the CPU baseline ran on: logical CPUs, the CPUs this process may
+    use, the OpenMP thread setting and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"nproc": usable, "cpu_count": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "model": model}
+
+
+def cpu_baseline(scale=24, rounds=3):
     """The reference engine (oracle/_ref, compiled from the reference's own
-    sources with its OpenMP branch) on a bounded sample: PageRank.C on the
-    R-MAT scale-20 graph, 384 partitions, timed as the reference times
-    Compute()."""
+    sources with its OpenMP branch) on the headline workload: PageRank.C on
+    directed R-MAT scale 24 (ef 16, seed 1), 384 partitions, 10 iterations,
+    timed as the reference times Compute() (ligra-numa.h:1155-1157), on the
+    host threads this process is given (OMP_NUM_THREADS where set, else every
+    CPU in its affinity mask)."""
     import oracle
 
+    hw = host_cpu_record()
     exe = os.path.join(oracle.REF_DIR, "ref_pagerank")
-    threads = os.environ.get("OMP_NUM_THREADS") or str(min(os.cpu_count() or 1, 16))
+    threads = os.environ.get("OMP_NUM_THREADS") or str(hw["nproc"] or 1)
     env = dict(os.environ, OMP_NUM_THREADS=threads)
     if os.path.exists(exe):
+        t0 = time.perf_counter()
         r = subprocess.run([exe, "-rmat", str(scale), "16", "1", "-c", "384", "-rounds", str(rounds)],
-                           capture_output=True, text=True, env=env, timeout=600)
+                           capture_output=True, text=True, env=env, timeout=900)
+        wall = time.perf_counter() - t0
         if r.returncode == 0:
             line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
             d = json.loads(line)
             t = float(np.mean(d["times"]))
             return {"value": 10 * d["m"] / t / 1e9, "unit": "GTEPS", "cores": int(d["threads"]),
-                    "kind": "reference",
-                    "sample": f"reference PageRank.C (OpenMP build) on R-MAT scale {scale} ef 16 seed 1, "
-                              f"n={d['n']} m={d['m']}, 384 partitions, 10 iterations, mean of {rounds} rounds "
-                              f"({t * 1e3:.1f} ms/round)"}
+                    "kind": "reference", "host": hw,
+                    "round_ms": [x * 1e3 for x in d["times"]], "median_round_ms": float(np.median(d["times"])) * 1e3,
+                    "sample": f"reference PageRank.C (OpenMP build of its own engine, NUMA off, PART96, Hilbert COO) "
+                              f"on directed R-MAT scale {scale} ef 16 seed 1, n={d['n']} m={d['m']}, "
+                              f"384 partitions, 10 iterations, mean of {rounds} rounds ({t * 1e3:.1f} ms/round), "
+                              f"{d['threads']} OpenMP threads; {wall:.0f} s including graph build and partitioning"}
         log(f"[bench] reference harness failed: {r.stderr[-500:]}")
-    # fallback: the single-threaded C restatement
+    # fallback: the single-threaded C restatement on a smaller graph
     off, ids = oracle.rmat_csr(18, 16, 1)
     og = oracle.Graph(off, ids, P=384)
     t1 = time.perf_counter()
     og.pagerank(10)
     t = time.perf_counter() - t1
-    return {"value": 10 * len(ids) / t / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+    return {"value": 10 * len(ids) / t / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port", "host": hw,
             "sample": f"C restatement, R-MAT scale 18, 10 iterations ({t * 1e3:.0f} ms)"}
 
 
@@ -416,6 +490,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched leg")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the reference app sources run through the drop-in front end")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration under rocprofv3)")
     ap.add_argument("--only-bfs-cc", action="store_true",
@@ -467,7 +543,9 @@ def main():
             log(f"[bench] batched: {bb}")
             extras["batched"] = bb
     ctx.close()
-    cpu = None if a.no_cpu else cpu_baseline()
+    if not a.no_extras and not a.no_dropin:
+        extras["dropin_apps"] = dropin_bench(a.scale)
+    cpu = None if a.no_cpu else cpu_baseline(a.scale)
     line = {
         "metric": "GTEPS & % HBM peak (PageRank/BFS, RMAT-24); adjoint/forward time ratio",
         "value": pr["gteps"],
@@ -489,6 +567,10 @@ def main():
                      "traffic": traffic_from_profile(pr["kernel"], pr["n"], pr["m"]),
                      "kernel": pr["kernel"], "kernel_ms": pr["kernel_ms"],
                      "algo_bytes_per_launch": pr["algo_bytes_per_launch"]},
+        "timing": {"mean_step_ms": pr["step_ms"], "median_compute_event_ms": pr["median_event_ms"],
+                   "first_call_ms_incl_allocation": pr["first_call_ms"],
+                   "note": "a step is one Compute(): initialisation + 10 iterations, buffers kept between "
+                           "calls; the first call on the graph also allocates them"},
         "cpu_baseline": cpu,
         "extras": extras,
     }

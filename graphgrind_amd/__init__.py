@@ -93,6 +93,8 @@ SIGNATURES = {
     "gg_version": (C.c_char_p, []),
     "gg_malloc": (C.c_int, [_VP, C.c_uint64, C.POINTER(C.c_void_p)]),
     "gg_free": (C.c_int, [_VP, _VP]),
+    "gg_host_alloc": (C.c_int, [_VP, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "gg_host_free": (C.c_int, [_VP, _VP]),
     "gg_memcpy_h2d": (C.c_int, [_VP, _VP, _VP, C.c_uint64]),
     "gg_memcpy_d2h": (C.c_int, [_VP, _VP, _VP, C.c_uint64]),
     "gg_memcpy_d2d": (C.c_int, [_VP, _VP, _VP, C.c_uint64]),

@@ -482,6 +482,27 @@ extern "C" int gg_malloc(gg_ctx_t c, uint64_t bytes, void **dev)
     return GG_OK;
 }
 
+extern "C" int gg_host_alloc(gg_ctx_t c, uint64_t bytes, void **host)
+{
+    if (!c || !host) return fail(GG_ERR_INVALID, "gg_host_alloc: NULL argument");
+    GG_HIP(hipSetDevice(c->device));
+    hipError_t e = hipHostMalloc(host, bytes ? bytes : 8, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *host = nullptr;
+        return fail(GG_ERR_OOM, "gg_host_alloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    return GG_OK;
+}
+
+extern "C" int gg_host_free(gg_ctx_t c, void *host)
+{
+    if (!c) return fail(GG_ERR_INVALID, "gg_host_free: NULL ctx");
+    if (!host) return GG_OK;
+    GG_HIP(hipStreamSynchronize(c->stream));
+    GG_HIP(hipHostFree(host));
+    return GG_OK;
+}
+
 extern "C" int gg_free(gg_ctx_t c, void *dev)
 {
     if (!c) return fail(GG_ERR_INVALID, "gg_free: NULL ctx");

@@ -112,10 +112,32 @@ inline std::map<const void *, Buffer *> &registry()
     return r;
 }
 
+// What keeping the host shadows coherent costs: copies each way and the time
+// spent in host loops, printed by the driver after each round when the
+// environment sets GG_FRONT_STATS (see report_traffic).
+struct Traffic {
+    uint64_t down_bytes = 0, up_bytes = 0, downs = 0, ups = 0;
+    double down_s = 0, up_s = 0, host_s = 0;
+};
+inline Traffic &traffic()
+{
+    static Traffic t;
+    return t;
+}
+inline double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 inline void to_host(Buffer *b)
 {
     if (b->state == Buffer::DEVICE) {
+        const double t0 = now_s();
         check(gg_memcpy_d2h(context(), b->host, b->dev, b->bytes), "download");
+        Traffic &t = traffic();
+        t.down_s += now_s() - t0;
+        t.down_bytes += b->bytes;
+        t.downs++;
         b->state = Buffer::BOTH;
     }
 }
@@ -123,19 +145,44 @@ inline void to_host(Buffer *b)
 inline void to_device(Buffer *b)
 {
     if (b->state == Buffer::HOST) {
+        const double t0 = now_s();
         check(gg_memcpy_h2d(context(), b->dev, b->host, b->bytes), "upload");
+        Traffic &t = traffic();
+        t.up_s += now_s() - t0;
+        t.up_bytes += b->bytes;
+        t.ups++;
         b->state = Buffer::BOTH;
     }
 }
 
 // Every host phase may read and write any buffer.
+inline double &host_phase_t0()
+{
+    static double t0 = 0;
+    return t0;
+}
 inline void host_phase_begin()
 {
     for (auto &kv : registry()) to_host(kv.second);
+    host_phase_t0() = now_s();
 }
 inline void host_phase_end()
 {
+    traffic().host_s += now_s() - host_phase_t0();
     for (auto &kv : registry()) kv.second->state = Buffer::HOST;
+}
+
+// "gg_front: round R: ..." after a round when GG_FRONT_STATS is set: the
+// round's HBM<->host copies (count, MB, ms) and its time in host loops.
+inline void report_traffic(long round)
+{
+    Traffic &t = traffic();
+    if (getenv("GG_FRONT_STATS"))
+        std::printf("gg_front: round %ld: downloads %llu (%.1f MB, %.3f ms), uploads %llu (%.1f MB, %.3f ms), "
+                    "host loops %.3f ms\n",
+                    round, (unsigned long long)t.downs, t.down_bytes / 1e6, t.down_s * 1e3,
+                    (unsigned long long)t.ups, t.up_bytes / 1e6, t.up_s * 1e3, t.host_s * 1e3);
+    t = Traffic{};
 }
 
 // Device address of an app array (a host shadow pointer handed to a functor).
@@ -194,7 +241,7 @@ public:
     {
         buf_ = new gg_front::Buffer();
         buf_->bytes = std::max<size_t>(elements, 1) * sizeof(T);
-        buf_->host = ::operator new(buf_->bytes);
+        gg_front::check(gg_host_alloc(gg_front::context(), buf_->bytes, &buf_->host), "gg_host_alloc");
         gg_front::check(gg_malloc(gg_front::context(), buf_->bytes, &buf_->dev), "gg_malloc");
         buf_->state = gg_front::Buffer::HOST;
         gg_front::registry()[buf_->host] = buf_;
@@ -204,7 +251,7 @@ public:
         if (!buf_) return;
         gg_front::registry().erase(buf_->host);
         gg_free(gg_front::context(), buf_->dev);
-        ::operator delete(buf_->host);
+        gg_host_free(gg_front::context(), buf_->host);
         delete buf_;
         buf_ = nullptr;
     }
@@ -1186,9 +1233,11 @@ int parallel_main(int argc, char *argv[])
         // like the reference, 'start' (-r, default 100) is only checked by
         // the apps that use it: a frontier on a vertex >= n fails there
         for (long r = 0; r < rounds; r++) {
+            gg_front::traffic() = gg_front::Traffic{};
             startTime();
             Compute(PG, start);
             nextTime("Running");
+            gg_front::report_traffic(r);
             // an app that transposed the graph (BC.C) gets it back (1195)
             if (PG.transposed()) PG.transpose();
         }

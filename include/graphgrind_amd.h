@@ -65,6 +65,11 @@ int gg_free(gg_ctx_t ctx, void *dev);
 int gg_memcpy_h2d(gg_ctx_t ctx, void *dev, const void *host, uint64_t bytes);
 int gg_memcpy_d2h(gg_ctx_t ctx, void *host, const void *dev, uint64_t bytes);
 int gg_memcpy_d2d(gg_ctx_t ctx, void *dst, const void *src, uint64_t bytes);
+/* Page-locked host memory for the host side of an mmap_ptr (the shadow the
+ * apps' host loops read and write, part_allocate / Interleave_allocate,
+ * mm.h:60-130): copies to and from HBM then run at full link speed. */
+int gg_host_alloc(gg_ctx_t ctx, uint64_t bytes, void **host);
+int gg_host_free(gg_ctx_t ctx, void *host);
 /* map_vertexL init loops (app/PageRank.C:124-125, BFS.C:88,
  * Components.C:113): dev[i] = value for i < count. */
 int gg_fill(gg_ctx_t ctx, void *dev, int dtype, uint64_t count, double value);
