@@ -137,6 +137,7 @@ SIGNATURES = {
     "gg_bfs": (C.c_int, [_VP, C.c_uint32, C.c_int64, _VP, _VP, C.POINTER(RunInfo)]),
     "gg_cc": (C.c_int, [_VP, _VP, C.POINTER(RunInfo)]),
     "gg_spmv": (C.c_int, [_VP, C.c_int, _VP, _VP, C.POINTER(RunInfo)]),
+    "gg_spmv_noalloc": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "gg_last_kernel_time": (C.c_int, [_VP, C.POINTER(C.c_double), _IP]),
     "gg_graph_set_option": (C.c_int, [_VP, C.c_int, C.c_int64]),
     "gg_graph_get_option": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int64)]),

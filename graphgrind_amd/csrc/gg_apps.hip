@@ -644,3 +644,82 @@ extern "C" int gg_spmv(gg_graph_t g, int dtype, const void *x, void *y, gg_run_i
     }
     return rc;
 }
+
+// ---------------------------------------------------------------------------
+// gg_spmv_noalloc: gg_spmv without any device allocation, for callers that
+// cannot allocate (code under HIP graph capture, tight serving loops).  The
+// one scratch buffer gg_spmv takes from the pool, x permuted into a
+// renumbered plan's order, lives in a file-scope device array instead.
+// ---------------------------------------------------------------------------
+namespace gg {
+namespace {
+constexpr uint64_t kScratchBytes = 64ull << 20; // n * sizeof(value) at most
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+// per-device address of 'scratch' (each GPU holds its own copy); the host
+// name of the array is only a stand-in and is never passed to a kernel
+void *g_scratch_ptr[kScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_for(int device, void **out)
+{
+    if (device < 0 || device >= kScratchDevices)
+        return fail(GG_ERR_UNSUPPORTED, "gg_spmv_noalloc: device ordinal above " + std::to_string(kScratchDevices - 1));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        // set-up on this device's first call, with the device current
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GG_ERR_HIP, std::string("gg_spmv_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return GG_OK;
+}
+} // namespace
+} // namespace gg
+
+extern "C" int gg_spmv_noalloc(gg_graph_t g, int dtype, const void *x, void *y)
+{
+    if (!g || !x || !y) return fail(GG_ERR_INVALID, "gg_spmv_noalloc: NULL argument");
+    if (dtype != GG_F64 && dtype != GG_F32) return fail(GG_ERR_INVALID, "gg_spmv_noalloc: dtype must be F64 or F32");
+    Ctx *c = g->ctx;
+    GG_HIP(hipSetDevice(c->device));
+    const uint64_t vbytes = (dtype == GG_F64 ? 8ull : 4ull) * g->n;
+    const bool planned = g->plan.mode != 0 && g->m;
+    const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
+    CscView pv;
+    const uint32_t *inv = nullptr;
+    void *xp = nullptr;
+    if (planned) {
+        if (vbytes > kScratchBytes)
+            return fail(GG_ERR_SIZE, "gg_spmv_noalloc: n * value size = " + std::to_string(vbytes) +
+                                         " bytes exceeds the " + std::to_string(kScratchBytes) +
+                                         "-byte scratch array; use gg_spmv");
+        GG_TRY(scratch_for(c->device, &xp));
+        pv = plan_view(g);
+        if (g->weighted) tw = (const float *)g->plan.tw.p;
+        inv = (const uint32_t *)g->plan.inv.p;
+    } else {
+        // the row-block schedule is built by the first pull on the graph
+        // (an allocation): it must exist already
+        if (g->pull_schedule == 1 && !g->rows_ok && g->m)
+            return fail(GG_ERR_STATE, "gg_spmv_noalloc: the graph's row-block schedule is not built yet "
+                                      "(one gg_spmv call builds it)");
+        GG_TRY(pull_view(g, &pv));
+    }
+    int rc = GG_OK;
+    if (planned) {
+        rc = dtype == GG_F64 ? gather_idx(c, (const double *)x, inv, (double *)xp, g->n)
+                             : gather_idx(c, (const float *)x, inv, (float *)xp, g->n);
+        x = xp;
+    }
+    if (!rc)
+        rc = dtype == GG_F64
+                 ? launch_pull(c, pv, OpSpmvY<double>{(const double *)x, tw, (double *)y, inv}, nullptr, nullptr,
+                               nullptr)
+                 : launch_pull(c, pv, OpSpmvY<float>{(const float *)x, tw, (float *)y, inv}, nullptr, nullptr,
+                               nullptr);
+    return rc;
+}

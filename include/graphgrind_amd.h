@@ -300,6 +300,18 @@ int gg_cc(gg_graph_t g, uint32_t *labels_dev, gg_run_info *info);
 /* app/SPMV.C:64-92 generalised: y[d] = sum_{s->d} w(s->d) x[s] over the
  * in-edge twin (unit weights when the graph has none). */
 int gg_spmv(gg_graph_t g, int dtype, const void *x_dev, void *y_dev, gg_run_info *info);
+/* gg_spmv without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops).  Same kernels and result as gg_spmv.  The x vector
+ * permuted into a renumbered plan's order (gg_graph_reorder) goes to a
+ * 64 MiB device array the library holds per GPU until the process ends, so
+ * n * sizeof(value) must be at most 64 MiB with a plan (GG_ERR_SIZE
+ * otherwise: R-MAT scale 24 fits in FP32, scale 23 in FP64).  Without a plan
+ * the graph's row-block schedule must exist (one gg_spmv call builds it;
+ * GG_ERR_STATE otherwise).  Asynchronous on the context's stream: no
+ * synchronisation and no timing.  Calls that overlap on one GPU share the
+ * array and race. */
+int gg_spmv_noalloc(gg_graph_t g, int dtype, const void *x_dev, void *y_dev);
 
 /* ---- further apps on the engine --------------------------------------------- */
 /* app/BC.C:154-226 from 'root': forward pulls level by level summing
