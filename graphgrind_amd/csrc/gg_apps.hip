@@ -171,8 +171,17 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
         view.tids = (const uint32_t *)g->coo_tids.p;
     }
     const uint32_t *outdeg = use_plan ? (const uint32_t *)g->plan.outdeg.p : (const uint32_t *)g->outdeg.p;
+    // the source-segmented pull (GG_OPT_XCD_SEGMENTS): each XCD gathers from
+    // its own slice of the contributions, a combine pass finishes the rows
+    const bool xseg = use_plan && m && (g->xcd_segments == 2 || (g->xcd_segments == 1 && m >= (1ull << 24)));
+    if (xseg) GG_TRY(ensure_xseg(g));
+    XSeg &X = g->xseg;
+    const uint32_t xhot = kXHotBytes / sizeof(V); // sources the partial pass caches in LDS
+    if (xseg)
+        GG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xseg_part<V>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(xhot * sizeof(V))));
     const uint32_t nblk = view.blocks();
-    const uint32_t nfix = view.fix_blocks();
+    const uint32_t nfix = xseg ? std::max(view.fix_blocks(), X.comb_blocks) : view.fix_blocks();
     PRBuffers B(g);
     int rc = GG_OK;
     for (int i = 0; i < 3 && !rc; i++) rc = buf_ensure(c, B.R[i], sizeof(V) * (uint64_t)n);
@@ -214,6 +223,30 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
         op.damping = (V)damping;
         op.added = added;
         op.done = done;
+        if (xseg) {
+            // group partials (each group's items on one XCD, its hottest
+            // sources in LDS), long-row chunks, then the combine with the
+            // loop's epilogue
+            const uint32_t *meta = (const uint32_t *)X.meta.p;
+            OpSegPartial<V> sop{op.contrib_in, (V *)X.partial.p, -V(0), done};
+            GG_HIP(hipEventRecord(ev[2 * it], c->stream));
+            k_xseg_part<V><<<X.grid, kXPartThreads, xhot * sizeof(V), c->stream>>>(
+                op.contrib_in, (V *)X.partial.p, (V *)X.rows.slot_val.p, (const uint32_t *)X.vtids.p,
+                (const uint32_t *)X.rsb.p, (const RowWork *)X.rows.work.p, meta, n, xhot, done);
+            if (X.rows.nlong)
+                k_rows_fix<OpSegPartial<V>><<<blocks_for(X.rows.nlong, kBlock), kBlock, 0, c->stream>>>(
+                    sop, (const uint32_t *)X.rows.long_rows.p, X.rows.nlong, (const V *)X.rows.slot_val.p, 0,
+                    nullptr, nullptr, nullptr);
+            k_xseg_combine<OpPRFused<V>><<<X.comb_blocks, kBlock, 0, c->stream>>>(
+                op, (const V *)X.partial.p, (const uint32_t *)X.mask.p, (const uint32_t *)X.cursor.p,
+                meta + kXSegs + 1, X.vrows, n, cnt, (double *)B.l1f.p);
+            GG_HIP(hipEventRecord(ev[2 * it + 1], c->stream));
+            GG_HIP(hipGetLastError());
+            k_pr_check<<<1, kBlock, 0, c->stream>>>((const double *)B.l1p.p, 0, (const double *)B.l1f.p,
+                                                     X.comb_blocks, eps, it, done, done_iter, cnt,
+                                                     (unsigned long long *)B.stats.p, (double *)B.l1h.p);
+            continue;
+        }
         rc = launch_pull(c, view, op, cnt, (double *)B.l1p.p, (double *)B.l1f.p, ev[2 * it], ev[2 * it + 1]);
         if (rc) break;
         k_pr_check<<<1, kBlock, 0, c->stream>>>((const double *)B.l1p.p, nblk, (const double *)B.l1f.p, nfix,
@@ -336,24 +369,6 @@ static int pagerank_generic(Graph *g, int max_iter, double damping, double eps, 
     return rc;
 }
 
-// BFS start state (app/BFS.C:86-92): Parents[i] = -1 except the root's
-// (itself), levels -1 except the root's 0, the one-entry frontier list, and
-// the root's out-degree copied out for the frontier's statistics.
-__global__ void k_bfs_start(int32_t *__restrict__ parents, int32_t *__restrict__ levels, uint32_t n,
-                            uint32_t root, uint32_t *__restrict__ list, const uint32_t *__restrict__ outdeg,
-                            uint32_t *__restrict__ root_deg)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const bool r = i == root;
-    parents[i] = r ? (int32_t)root : -1;
-    if (levels) levels[i] = r ? 0 : -1;
-    if (r) {
-        list[0] = root;
-        *root_deg = outdeg[root];
-    }
-}
-
 } // namespace gg
 
 using namespace gg;
@@ -409,55 +424,15 @@ static int cc_loop(Graph *g, uint32_t *labels, uint32_t *prev, gg_run_info *info
     }
     return rc;
 }
-// BFS.C's Compute (77-101) on g from 'root': the start state, then edgeMap
-// levels until the frontier is empty.
-static int bfs_loop(Graph *g, uint32_t root, int64_t threshold, int32_t *parents, int32_t *levels,
-                    gg_run_info *info)
-{
-    Ctx *c = g->ctx;
-    // app/BFS.C:86-92 and partitioned_vertices::create(n, root,
-    // outdeg(root)) (ligra-numa.h:103-116) in one kernel; the list holds the
-    // root and, after it, the root's out-degree for the read-back
-    Frontier *F = new Frontier();
-    frontier_alloc(g, F);
-    uint32_t od = 0;
-    int rc0 = pool_take(c, F->list, 8);
-    if (!rc0) {
-        k_bfs_start<<<blocks_for(g->n, kBlock), kBlock, 0, c->stream>>>(
-            parents, levels, g->n, root, (uint32_t *)F->list.p, (const uint32_t *)g->outdeg.p,
-            (uint32_t *)F->list.p + 1);
-        if (hipGetLastError() != hipSuccess) rc0 = fail(GG_ERR_HIP, "gg_bfs: start kernel");
-    }
-    if (!rc0) rc0 = read_back(c, &od, (uint32_t *)F->list.p + 1, 4);
-    if (rc0) { frontier_delete(F); return rc0; }
-    F->d_m = 1;
-    F->out_edges = od;
-    F->deg_src = g->outdeg.p;
-    F->sparse_valid = true;
-    EdgeMapExtra ex;
-    ex.levels = levels;
-    int lv = 0, rc = GG_OK;
-    while (F->d_m != 0) {
-        ex.level = lv + 1;
-        gg_op_t op = {GG_OP_BFS, GG_I32, parents, nullptr};
-        Frontier *out = nullptr;
-        int path = 0;
-        if ((rc = edgemap_core(g, F, &op, threshold, 0, &out, &path, &ex))) break;
-        if (info && lv < 64) info->paths[lv] = path;
-        lv++;
-        frontier_delete(F);
-        F = out;
-    }
-    frontier_delete(F);
-    if (info) {
-        info->iters = lv;
-        info->edges_scanned = ex.scanned;
-    }
-    return rc;
-}
+} // namespace gg
 
-// Parents and levels of a BFS run in the plan's numbering, in the caller's:
-// parents[v] = inv[pp[perm[v]]] (or -1), levels[v] = lp[perm[v]].
+namespace gg {
+// Parents and levels of a search in the plan's numbering, in the caller's:
+// parents[v] = inv[pp[perm[v]]] (or -1), levels[v] = lp[perm[v]].  A gather
+// over the caller's order: the reads of pp and lp are random but the arrays
+// sit in the MALL, and every write is coalesced (scattering each parent to
+// the caller's position as it is found costs a read-modify-write of a line
+// in HBM per vertex instead).
 __global__ void k_bfs_unmap(const int32_t *__restrict__ pp, const int32_t *__restrict__ lp,
                             const uint32_t *__restrict__ perm, const uint32_t *__restrict__ inv,
                             int32_t *__restrict__ parents, int32_t *__restrict__ levels, uint32_t n)
@@ -479,26 +454,31 @@ extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *p
     Ctx *c = g->ctx;
     GG_HIP(hipSetDevice(c->device));
     if (info) std::memset(info, 0, sizeof(*info));
-    // With a renumbered plan the levels run on the plan's relabelled arrays
-    // (hot vertices first in every in-list, so bottom-up scans meet the
-    // frontier sooner) and parents and levels are mapped back at the end.
+    // app/BFS.C:86-101: Parents[root] = root, the root's frontier, edgeMap
+    // until the frontier is empty, each level's direction chosen on the
+    // device.  With a renumbered plan the levels run on the plan's
+    // relabelled arrays (every in-list starts with its largest out-degree,
+    // the likeliest frontier member, and vertices of like degree share
+    // waves); parents and levels are mapped back after the last level, on
+    // the device.
     const bool planned = g->plan.mode != 0 && g->m;
     Buf pp, lp;
     Graph t;
-    int rc = GG_OK;
-    if (planned) {
+    uint32_t r2 = root;
+    int rc = ensure_first_in(g);
+    if (!rc && planned) {
         rc = pool_take(c, pp, 4ull * g->n);
         if (!rc && levels) rc = pool_take(c, lp, 4ull * g->n);
         if (!rc) rc = plan_alias(g, t);
+        if (!rc) rc = read_back(c, &r2, (const uint32_t *)g->plan.perm.p + root, 4);
     }
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, c->stream);
     if (!rc && planned) {
-        uint32_t r2 = 0;
-        rc = read_back(c, &r2, (const uint32_t *)g->plan.perm.p + root, 4);
-        if (!rc) rc = bfs_loop(&t, r2, threshold, (int32_t *)pp.p, levels ? (int32_t *)lp.p : nullptr, info);
+        rc = bfs_device_loop(&t, r2, (const uint32_t *)g->plan.first_in.p, threshold, (int32_t *)pp.p,
+                             levels ? (int32_t *)lp.p : nullptr, info);
         if (!rc) {
             k_bfs_unmap<<<blocks_for(g->n, kBlock), kBlock, 0, c->stream>>>(
                 (const int32_t *)pp.p, (const int32_t *)lp.p, (const uint32_t *)g->plan.perm.p,
@@ -506,7 +486,7 @@ extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *p
             if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_bfs: map-back kernel");
         }
     } else if (!rc) {
-        rc = bfs_loop(g, root, threshold, parents, levels, info);
+        rc = bfs_device_loop(g, root, (const uint32_t *)g->first_in.p, threshold, parents, levels, info);
     }
     (void)hipEventRecord(e1, c->stream);
     if (!rc) rc = sync(c);

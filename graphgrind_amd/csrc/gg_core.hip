@@ -321,17 +321,19 @@ void rows_free(RowSched &s)
 // not hold up a block of short ones); rows above kRowEdges edges become
 // chunked long rows.  Lanes per row: the power of two G <= 64 that gives
 // each lane about 'edges_per_lane' of the block's average row.
-int rows_build(Ctx *c, RowSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges, int edges_per_lane)
+int rows_build(Ctx *c, RowSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges, int edges_per_lane,
+               uint32_t item_edges, uint32_t row_cap, const std::vector<uint64_t> *breaks)
 {
     rows_free(s);
     std::vector<uint32_t> off((size_t)rows + 1);
     GG_HIP(hipMemcpyAsync(off.data(), d_off, 4ull * (rows + 1), hipMemcpyDeviceToHost, c->stream));
     GG_TRY(sync(c));
     if ((uint64_t)off[rows] != edges) return fail(GG_ERR_SIZE, "rows_build: offsets do not end at the edge count");
-    const uint32_t kRowBig = 256;
+    const uint32_t kRowBig = std::min<uint32_t>(256, item_edges / 2);
     std::vector<RowWork> work;
     std::vector<uint32_t> longs;
-    work.reserve(edges / kRowEdges + rows / kRowCap + 16);
+    work.reserve(edges / item_edges + rows / row_cap + 16);
+    size_t bi = 0; // next forced break (items never span one)
     uint32_t slots = 0;
     uint32_t r0 = 0, nr = 0;
     uint64_t ne = 0;
@@ -352,17 +354,21 @@ int rows_build(Ctx *c, RowSched &s, const uint32_t *d_off, uint32_t rows, uint64
         ne = 0;
     };
     for (uint32_t r = 0; r < rows; r++) {
+        if (breaks) {
+            while (bi < breaks->size() && (*breaks)[bi] < r) bi++;
+            if (bi < breaks->size() && (*breaks)[bi] == r) close();
+        }
         const uint32_t d = off[r + 1] - off[r];
-        if (d > (uint32_t)kRowEdges) {
+        if (d > item_edges) {
             close();
             longs.push_back(r);
             longs.push_back(slots);
-            for (uint32_t e = off[r]; e < off[r + 1]; e += kRowEdges) {
+            for (uint32_t e = off[r]; e < off[r + 1]; e += item_edges) {
                 RowWork w{};
                 w.r0 = r;
                 w.r1 = r + 1;
                 w.e0 = e;
-                w.e1 = std::min<uint32_t>(e + kRowEdges, off[r + 1]);
+                w.e1 = std::min<uint32_t>(e + item_edges, off[r + 1]);
                 w.kind = kRowLong;
                 w.slot = slots++;
                 work.push_back(w);
@@ -371,7 +377,7 @@ int rows_build(Ctx *c, RowSched &s, const uint32_t *d_off, uint32_t rows, uint64
             continue;
         }
         const bool b = d > kRowBig;
-        if (nr && (ne + d > (uint64_t)kRowEdges || nr + 1 > (uint32_t)kRowCap || b != big)) close();
+        if (nr && (ne + d > (uint64_t)item_edges || nr + 1 > row_cap || b != big)) close();
         if (!nr) {
             r0 = r;
             big = b;
@@ -774,7 +780,7 @@ static void graph_free(Graph *g)
 {
     Buf *bufs[] = {&g->off, &g->ids, &g->w, &g->toff, &g->tids, &g->tw, &g->perm, &g->outdeg,
                    &g->indeg, &g->comp, &g->coo_src, &g->coo_dst, &g->coo_w, &g->coo_tids, &g->coo_tw,
-                   &g->pull.start_row,
+                   &g->first_in, &g->plan.first_in, &g->pull.start_row,
                    &g->pull.start_edge, &g->pull.carry_row, &g->pull.carry_val, &g->pull.head_val,
                    &g->plan.perm, &g->plan.inv, &g->plan.toff, &g->plan.tids, &g->plan.tw, &g->plan.outdeg,
                    &g->plan.coff, &g->plan.cids, &g->plan.cw, &g->plan.cdelta, &g->plan.indeg,
@@ -788,6 +794,7 @@ static void graph_free(Graph *g)
     for (Buf *b : bufs) buf_free(*b);
     rows_free(g->plan.rows);
     rows_free(g->plan.crows);
+    xseg_free(g->xseg);
     rows_free(g->rows);
     rows_free(g->csr_rows);
     g->rows_ok = false;
@@ -1630,11 +1637,12 @@ extern "C" int gg_graph_reorder(gg_graph_t g, int mode)
     PullPlan &P = g->plan;
     Buf *pb[] = {&P.perm, &P.inv, &P.toff, &P.tids, &P.tw, &P.outdeg, &P.sched.start_row, &P.sched.start_edge,
                  &P.sched.carry_row, &P.sched.carry_val, &P.sched.head_val, &P.coff, &P.cids, &P.cw, &P.cdelta, &P.indeg,
-                 &P.csched.start_row, &P.csched.start_edge, &P.csched.carry_row, &P.csched.carry_val,
+                 &P.first_in, &P.csched.start_row, &P.csched.start_edge, &P.csched.carry_row, &P.csched.carry_val,
                  &P.csched.head_val};
     for (Buf *b : pb) buf_free(*b);
     rows_free(P.rows);
     rows_free(P.crows);
+    xseg_free(g->xseg);
     P.mode = 0;
     if (mode == 0) return GG_OK;
     const uint32_t n = g->n;
@@ -1882,6 +1890,7 @@ extern "C" int gg_graph_transpose(gg_graph_t g)
     g->csr_pull_ok = true;
     // state tied to one direction
     g->wsum_ok = false;
+    buf_free(g->first_in);
     graph_exec_drop(g);
     // destination partitions and COO lists follow the new in-edges
     if (g->P) GG_TRY(gg_graph_partition(g, g->P, g->part_vertex, g->relabel, g->hilbert));
@@ -1908,6 +1917,7 @@ extern "C" int gg_graph_set_option(gg_graph_t g, int option, int64_t value)
             rows_free(g->rows);
             rows_free(g->csr_rows);
             rows_free(g->plan.crows); // rebuilt on first use
+            xseg_free(g->xseg);       // likewise
             g->rows_ok = false;
             g->csr_rows_ok = false;
         }
@@ -1919,6 +1929,11 @@ extern "C" int gg_graph_set_option(gg_graph_t g, int option, int64_t value)
     case GG_OPT_DENSE_COO:
         if (value != 0 && value != 1) return fail(GG_ERR_INVALID, "gg_graph_set_option: dense COO must be 0 or 1");
         g->dense_coo = (int)value;
+        return GG_OK;
+    case GG_OPT_XCD_SEGMENTS:
+        if (value < 0 || value > 2)
+            return fail(GG_ERR_INVALID, "gg_graph_set_option: XCD segments must be 0, 1 or 2");
+        g->xcd_segments = (int)value;
         return GG_OK;
     default:
         return fail(GG_ERR_INVALID, "gg_graph_set_option: unknown option " + std::to_string(option));
@@ -1932,6 +1947,7 @@ extern "C" int gg_graph_get_option(gg_graph_t g, int option, int64_t *value)
     case GG_OPT_PULL_SCHEDULE: *value = g->pull_schedule; return GG_OK;
     case GG_OPT_ROW_EDGES_PER_LANE: *value = g->edges_per_lane; return GG_OK;
     case GG_OPT_DENSE_COO: *value = g->dense_coo; return GG_OK;
+    case GG_OPT_XCD_SEGMENTS: *value = g->xcd_segments; return GG_OK;
     default: return fail(GG_ERR_INVALID, "gg_graph_get_option: unknown option " + std::to_string(option));
     }
 }

@@ -95,6 +95,37 @@ struct RowSched {
     Buf slot_val;        // 8 bytes [slots]
 };
 
+// ---- source-segmented pull (one segment per XCD) -------------------------------
+// MI355X has 8 XCDs, each with its own 4 MB L2; a plain pull has every XCD
+// gather from the whole source vector, so each L2 caches the same hot lines
+// and misses the rest.  Splitting the sources into kXSegs groups (source s in
+// group (s >> kXSegShift) % kXSegs, whole 128-byte lines of values) and
+// letting each XCD pull only its group's in-edges makes the 8 L2s hold 8
+// different slices.  Each (row, group) pair with at least one edge is a
+// "virtual row" of its own CSC; its partial sum is written once, and a
+// combine pass adds a row's partials in group order (a fixed order) and
+// finishes the row.
+constexpr int kXSegs = 8;
+constexpr int kXSegShift = 5;     // 32 sources: one 128-byte line of FP32, two of FP64
+constexpr int kXCombRows = 1024;  // rows per combine work-group (4 per thread)
+constexpr int kXLaneEdges = 16;                 // edges a lane takes in a short item of the partial pass
+constexpr int kXWaveEdges = 64 * kXLaneEdges;   // edges one wave covers in a step
+constexpr int kXShortEdges = kXWaveEdges - 4;   // short items: whole rows, at most this many edges (the
+                                                // wave's window starts up to 3 edges early, on a 16-byte line)
+constexpr int kXPartThreads = 1024;             // partial-pass work-group (one per CU)
+constexpr uint32_t kXHotBytes = 156u * 1024u;   // its LDS cache of the group's hottest sources
+static_assert(kXSegs == 8, "the local numbering below assumes 8 groups");
+// a source's index within its group (its chunk number there, then its place
+// in the chunk), and back
+__host__ __device__ inline uint32_t xseg_local(uint32_t s)
+{
+    return ((s >> (kXSegShift + 3)) << kXSegShift) | (s & ((1u << kXSegShift) - 1u));
+}
+__host__ __device__ inline uint32_t xseg_global(uint32_t loc, uint32_t grp)
+{
+    return ((loc >> kXSegShift) << (kXSegShift + 3)) | (grp << kXSegShift) | (loc & ((1u << kXSegShift) - 1u));
+}
+
 // ---- relabelled pull plan ----------------------------------------------------
 // An internal renumbering of the vertices (hot sources first) with its own
 // in-edge twin and schedule; kernels run in the new numbering and results
@@ -115,9 +146,28 @@ struct PullPlan {
     // (mod 2^32) maps it back.
     Buf coff, cids, cw, cdelta;
     Buf indeg; // uint32[n] in-degrees in the new numbering
+    Buf first_in; // uint32[n]: first source of each in-list (BFS pulls), built on first use
     PullSched csched;
     RowSched crows;
 };
+
+// The source-segmented form of a plan's in-edge twin (built on first use).
+struct XSeg {
+    bool ok = false;
+    uint64_t vrows = 0;                  // virtual rows: (row, group) pairs with an edge
+    uint64_t row_start[kXSegs + 1] = {}; // first virtual row of each group
+    uint32_t item_start[kXSegs + 1] = {};// first row-block work item of each group
+    uint32_t comb_blocks = 0;            // combine work-groups (kXCombRows rows each)
+    uint32_t grid = 0;                   // partial-pass work-groups: kXSegs x (CUs / kXSegs)
+    Buf vtoff, vtids;                    // virtual CSC: offsets [vrows+1], sources [m + kXWaveEdges] (zero-padded)
+    Buf rsb;                             // uint32 bitmap over the edges: bit e when a virtual row starts at e
+    Buf mask;                            // uint32[n]: bit k when the row has group-k in-edges
+    Buf cursor;                          // uint32[comb_blocks*kXSegs]: group-k virtual rows before block b
+    Buf partial;                         // 8 bytes [vrows]
+    Buf meta;                            // uint32[2*(kXSegs+1)]: item_start, then row_start (device)
+    RowSched rows;                       // wave items over the virtual CSC (no item spans two groups)
+};
+void xseg_free(XSeg &x);
 
 // In-edge arrays + schedule a pull runs over: the row-block schedule when
 // 'rows' is set, else the merge-path tiles.
@@ -135,7 +185,12 @@ struct CscView {
                     : blocks_for(sched->tiles, kBlock);
     }
 };
-int rows_build(Ctx *c, RowSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges, int edges_per_lane);
+// Row-block work items over CSR offsets: runs of whole rows with at most
+// item_edges edges and row_cap rows, or item_edges-edge chunks of a longer
+// row; no item spans a row listed in 'breaks' (ascending).
+int rows_build(Ctx *c, RowSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges, int edges_per_lane,
+               uint32_t item_edges = kRowEdges, uint32_t row_cap = kRowCap,
+               const std::vector<uint64_t> *breaks = nullptr);
 void rows_free(RowSched &s);
 
 // ---- captured fixed-iteration loops ---------------------------------------------
@@ -223,6 +278,8 @@ struct Graph {
     // order: sources and weights, built on first use by the COO traversal
     Buf coo_tids, coo_tw;
     bool coo_pull_ok = false;
+    // first source of each in-list (BFS pulls), built on first use
+    Buf first_in;
     // merge-path schedule over the in-edge twin
     PullSched pull;
     PullPlan plan;
@@ -238,6 +295,10 @@ struct Graph {
     bool rows_ok = false, csr_rows_ok = false;
     // captured fixed-iteration loops (weighted PageRank forward / adjoint)
     GraphExec fwd_exec, adj_exec;
+    // source-segmented form of the plan's twin, and whether the fused
+    // PageRank loop uses it (GG_OPT_XCD_SEGMENTS)
+    XSeg xseg;
+    int xcd_segments = 0;
     // workspace of the fused PageRank loop, kept between calls (grow-only)
     Buf pr_ws[12];
     bool transposed = false; // gg_graph_transpose flips the two directions
@@ -251,6 +312,9 @@ void graph_exec_drop(Graph *g);
 // The COO-ordered in-lists (coo_tids / coo_tw), built on first use from the
 // COO partitions; requires edge partitions.
 int ensure_coo_pull(Graph *g);
+// The source-segmented form of the plan's twin (g->xseg), built on first use;
+// requires a plan.
+int ensure_xseg(Graph *g);
 // Row blocks of the in-edge twin / of the out-edge CSR (built on first use).
 int ensure_rows(Graph *g);
 int ensure_csr_rows(Graph *g);

@@ -428,6 +428,13 @@ __device__ __forceinline__ A wave_combine(const Op &op, A v)
 //     slot_val[slot]; k_rows_fix combines a row's slots in order.
 // Each row is finished by exactly one thread, so results are deterministic.
 // ---------------------------------------------------------------------------
+// The XCD (accelerator complex) this work-group runs on: HW_REG_XCC_ID
+// (hardware register 20), bits [3:0].
+__device__ __forceinline__ uint32_t xcc_id()
+{
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;
+}
+
 template <class Op>
 __global__ __launch_bounds__(kBlock) void k_rows(Op op, const uint32_t *__restrict__ toff,
                                                  const uint32_t *__restrict__ tids,
@@ -582,6 +589,243 @@ __global__ __launch_bounds__(kBlock) void k_rows(Op op, const uint32_t *__restri
     }
 }
 
+// Partial sums of a source-segmented pull: virtual row v (a row's in-edges
+// from one source group) gathers 'vals' over its sources and writes its sum
+// to partial[v].  'none' is the identity of the combine that follows (the
+// fused PageRank loop's -0.0 for "no active source").
+template <class V>
+struct OpSegPartial {
+    using acc_t = V;
+    const V *vals;
+    V *partial;
+    V none;
+    const int *done;
+    __device__ bool skip() const { return done && *done != 0; }
+    __device__ V identity() const { return none; }
+    __device__ V combine(V a, V b) const { return a + b; }
+    __device__ V gather(uint32_t, uint32_t s) const { return vals[s]; }
+    __device__ void finish(uint32_t v, V acc, LocalStats &) const { __builtin_nontemporal_store(acc, &partial[v]); }
+};
+
+// Partial pass of a source-segmented pull (GG_OPT_XCD_SEGMENTS): one
+// persistent work-group of kXPartThreads per CU.  Work-groups are dispatched
+// to the XCDs round-robin by index, so work-group b serves group b % kXSegs
+// and every work-group of a group shares one XCD's L2.  Each work-group
+// first copies the group's 'hot' lowest-numbered sources (with a
+// hot-sources-first plan, its highest out-degrees) into LDS; its waves then
+// take the group's work items in turn, gathering from LDS where the source
+// is cached and from global memory otherwise.  Long-row chunks reduce
+// lane-strided and then in a fixed wave tree into slot_val (finished by
+// k_rows_fix); a short item (whole rows, at most kXShortEdges edges) gives
+// each lane kXLaneEdges consecutive edges, finds its rows from the row-start
+// bits 'rsb' (bit e set when a virtual row starts at edge e), adds within
+// the lane, and joins rows that span lanes with a segmented scan across the
+// wave: fixed orders throughout, so the partials are deterministic.  Which
+// work-group serves a group and which XCD it runs on change no value.
+template <class V>
+__global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const V *__restrict__ vals, V *__restrict__ partial,
+                                                             V *__restrict__ slot_val,
+                                                             const uint32_t *__restrict__ vtids,
+                                                             const uint32_t *__restrict__ rsb,
+                                                             const RowWork *__restrict__ work,
+                                                             const uint32_t *__restrict__ item_start, uint32_t n,
+                                                             uint32_t hot, const int *__restrict__ done)
+{
+    extern __shared__ unsigned char s_raw[];
+    V *s_hot = reinterpret_cast<V *>(s_raw);
+    if (done && *done) return;
+    constexpr uint32_t kWaves = kXPartThreads / 64;
+    const uint32_t grp = blockIdx.x % kXSegs, wig = blockIdx.x / kXSegs, G = gridDim.x / kXSegs;
+    for (uint32_t i = threadIdx.x; i < hot; i += kXPartThreads) {
+        const uint32_t s = xseg_global(i, grp);
+        s_hot[i] = s < n ? vals[s] : V(0);
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t i0 = item_start[grp], cnt = item_start[grp + 1] - i0;
+    const uint32_t filler = xseg_global(0, grp); // cached whenever the group has sources
+    // both loads are issued for every edge, from LDS and from global memory
+    // (a cached source's global load reads the group's first source, one
+    // line for the whole wave), and the value is picked after: a branch
+    // would be merged into one flat load through a generic pointer, whose
+    // completion the compiler can only wait for with full waits
+    auto gather = [&](uint32_t s) -> V {
+        const uint32_t loc = xseg_local(s);
+        const bool h = loc < hot;
+        const V a = s_hot[h ? loc : 0u];
+        const V b = vals[h ? filler : s];
+        return h ? a : b;
+    };
+    const uint32_t stride = G * kWaves;
+    uint32_t t = wig * kWaves + wv;
+    RowWork wn{};
+    if (t < cnt) wn = work[i0 + t];
+    for (; t < cnt; t += stride) {
+        const RowWork w = wn; // the next item's descriptor loads while this one runs
+        if (t + stride < cnt) wn = work[i0 + t + stride];
+        if (w.kind == kRowLong) {
+            const uint32_t last = w.e1 - 1;
+            uint32_t src[kXLaneEdges];
+#pragma unroll
+            for (int j = 0; j < kXLaneEdges; j++) src[j] = vtids[min(w.e0 + lane + 64u * j, last)];
+            V acc = -V(0);
+#pragma unroll
+            for (int j = 0; j < kXLaneEdges; j++) {
+                const bool in = w.e0 + lane + 64u * j <= last;
+                const V v = gather(in ? src[j] : filler);
+                if (in) acc = acc + v;
+            }
+            for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
+            if (lane == 0) slot_val[w.slot] = acc;
+            continue;
+        }
+        // the lanes cover [A, A + kXWaveEdges), A = e0 rounded down to 4
+        // edges, so each lane's edges are whole 16-byte vectors; positions
+        // outside [e0, e1) are masked off
+        static_assert(kXLaneEdges == 16, "a lane reads its edges as four 16-byte vectors");
+        const uint32_t A0 = w.e0 & ~3u;
+        const uint32_t base = A0 + lane * kXLaneEdges;
+        const uint32_t lo = w.e0 > base ? w.e0 - base : 0u;
+        const uint32_t hi = w.e1 > base ? min(w.e1 - base, (uint32_t)kXLaneEdges) : 0u;
+        const uint32_t vmask = hi > lo ? ((1u << hi) - 1u) & ~((1u << lo) - 1u) : 0u;
+        uint32_t src[kXLaneEdges];
+        {
+            const uint4 *p4 = reinterpret_cast<const uint4 *>(vtids + base); // vtids is padded past its end
+#pragma unroll
+            for (int q = 0; q < kXLaneEdges / 4; q++) {
+                const uint4 u = p4[q];
+                src[4 * q] = u.x;
+                src[4 * q + 1] = u.y;
+                src[4 * q + 2] = u.z;
+                src[4 * q + 3] = u.w;
+            }
+        }
+        // base is a multiple of 4, so the lane's 16 bits may straddle two words
+        const uint32_t wi = base >> 5;
+        const uint64_t pair = ((uint64_t)rsb[wi + 1] << 32) | rsb[wi];
+        const uint32_t bits = (uint32_t)(pair >> (base & 31)) & vmask;
+        V val[kXLaneEdges];
+#pragma unroll
+        for (int j = 0; j < kXLaneEdges; j++) val[j] = gather((vmask >> j) & 1u ? src[j] : filler);
+        // rows started in earlier lanes of the item
+        const uint32_t nst = __popc(bits);
+        uint32_t inc = nst;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t q = __shfl_up(inc, off, 64);
+            if ((int)lane >= off) inc += q;
+        }
+        // r: the row open before the lane's next edge (w.r0 - 1 before the first)
+        uint32_t r = w.r0 + (inc - nst) - 1;
+        V acc = -V(0), head = -V(0);
+        bool seen = false;
+#pragma unroll
+        for (int j = 0; j < kXLaneEdges; j++) {
+            if ((vmask >> j) & 1u) {
+                if ((bits >> j) & 1u) {
+                    if (seen) __builtin_nontemporal_store(acc, &partial[r]); // a row inside the lane ends
+                    else head = acc;
+                    seen = true;
+                    r++;
+                    acc = -V(0);
+                }
+                acc = acc + val[j];
+            }
+        }
+        // S: the open row's sum up to the lane's end (segmented inclusive scan)
+        V S = acc;
+        bool f = seen;
+        for (int off = 1; off < 64; off <<= 1) {
+            const V pv = __shfl_up(S, off, 64);
+            const int pf = __shfl_up((int)f, off, 64);
+            if ((int)lane >= off && !f) {
+                S = pv + S;
+                f = pf != 0;
+            }
+        }
+        V C = __shfl_up(S, 1, 64);
+        if (lane == 0) C = -V(0);
+        // the row open at the lane's start ends at its first row start
+        if (seen && lane > 0) __builtin_nontemporal_store(C + head, &partial[w.r0 + (inc - nst) - 1]);
+        // the item's last row ends with its last edge
+        if (lane == (w.e1 - 1 - A0) / kXLaneEdges) __builtin_nontemporal_store(S, &partial[r]);
+    }
+}
+
+// Combine pass of a source-segmented pull: row d adds the partials of its
+// virtual rows in group order 0..kXSegs-1 and is finished by Op (its
+// row_load / store_row, or finish).  Group k's virtual rows are in row
+// order, so the position of row d's is the number of group-k virtual rows
+// before it: the block's cursor plus the rows of the block before d with
+// bit k in their masks (wave ballots, then the earlier waves' counts).
+// l1_part receives each block's residual (and, for kAux ops, its aux sum in
+// l1_part[gridDim.x + b]).
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_xseg_combine(Op op, const typename Op::acc_t *__restrict__ partial,
+                                                         const uint32_t *__restrict__ mask,
+                                                         const uint32_t *__restrict__ cursor,
+                                                         const uint32_t *__restrict__ row_start, uint64_t vrows,
+                                                         uint32_t n, unsigned long long *__restrict__ counters,
+                                                         double *__restrict__ l1_part)
+{
+    using A = typename Op::acc_t;
+    using R = typename row_of<Op>::type;
+    constexpr int kPer = kXCombRows / kBlock;
+    constexpr int kW = kBlock / 64;
+    __shared__ uint32_t s_wcnt[kW][kXSegs];
+    __shared__ uint32_t s_base[kXSegs];
+    if (op.skip()) return;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    if (tid < kXSegs) s_base[tid] = row_start[tid] + cursor[(uint64_t)blockIdx.x * kXSegs + tid];
+    LocalStats st;
+    const uint64_t vlast = vrows ? vrows - 1 : 0;
+    for (int q = 0; q < kPer; q++) {
+        const uint32_t d = blockIdx.x * kXCombRows + q * kBlock + tid;
+        const bool in = d < n;
+        const uint32_t dd = in ? d : n - 1;
+        const uint32_t mk = in ? __builtin_nontemporal_load(&mask[dd]) : 0u;
+        uint32_t lrank[kXSegs];
+#pragma unroll
+        for (int k = 0; k < kXSegs; k++) {
+            const uint64_t bal = __ballot((mk >> k) & 1u);
+            lrank[k] = (uint32_t)__popcll(bal & lt);
+            if (lane == 0) s_wcnt[wv][k] = (uint32_t)__popcll(bal);
+        }
+        __syncthreads(); // s_wcnt and (first pass) s_base written
+        // every group's partial is loaded (clamped, unconditional: the
+        // neighbouring lanes read the same lines) and added when the bit is set
+        A part[kXSegs];
+#pragma unroll
+        for (int k = 0; k < kXSegs; k++) {
+            uint32_t pre = 0;
+            for (int w = 0; w < (int)wv; w++) pre += s_wcnt[w][k];
+            const uint64_t pos = min((uint64_t)s_base[k] + pre + lrank[k], vlast);
+            part[k] = partial[pos];
+        }
+        const R rin = row_load(op, dd);
+        A acc = op.identity();
+#pragma unroll
+        for (int k = 0; k < kXSegs; k++)
+            if ((mk >> k) & 1u) acc = op.combine(acc, part[k]);
+        __syncthreads(); // every lane has read s_base and s_wcnt
+        if (tid < kXSegs) {
+            uint32_t tot = 0;
+            for (int w = 0; w < kW; w++) tot += s_wcnt[w][tid];
+            s_base[tid] += tot;
+        }
+        if (in) row_store(op, d, rin, acc, st);
+        __syncthreads(); // s_base updated before the next rows read it
+    }
+    constexpr bool kAux = has_aux<Op>::value;
+    double aux = 0;
+    const double l1 = block_stats<kAux>(st, counters, &aux);
+    if (l1_part && tid == 0) {
+        l1_part[blockIdx.x] = l1;
+        if constexpr (kAux) l1_part[gridDim.x + blockIdx.x] = aux;
+    }
+}
+
 // Finishes the long rows (their chunk partials combined in chunk order) and
 // folds the per-block residuals l1_part[256b .. 256b+255] into l1_fix[b].
 template <class Op>
@@ -721,6 +965,15 @@ struct EdgeMapExtra {
 
 int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, int remdups,
                  Frontier **pout, int *path, EdgeMapExtra *ex);
+// BFS.C's loop with the direction choice made on the device (no read-back
+// per level) over g's arrays from 'root', parents and levels (nullable) in
+// g's numbering; 'first' holds the first entry of each of g's in-lists
+// (ensure_first_in).
+int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t threshold, int32_t *parents,
+                    int32_t *levels, gg_run_info *info);
+// The first entry of each in-list of the graph's traversal arrays (the
+// plan's when it has one; 0xffffffff for an empty list), built on first use.
+int ensure_first_in(Graph *g);
 int vertexmap_core(Graph *g, Frontier *f, const gg_vop_t *op);
 int reduce_sum_core(Ctx *c, const void *dev, int dtype, uint64_t count, double *dev_out);
 

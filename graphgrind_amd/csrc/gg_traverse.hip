@@ -1148,6 +1148,612 @@ int reduce_sum_core(Ctx *c, const void *dev, int dtype, uint64_t count, double *
     return GG_OK;
 }
 
+// ============================================================================
+// BFS.C's loop (app/BFS.C:77-101) with every edgeMap direction choice
+// (ligra-numa.h:929-957) made on the device.  The host enqueues levels in
+// batches without reading anything back; each level's kernels run only if
+// the level's action word selects them, and one read-back per batch tells
+// whether the frontier emptied.  Same branches and levels as the edgeMap
+// path.  The visited set is a bitmap (2 MiB at 2^24 vertices, held in L2):
+// a sparse level pushes and claims each new vertex with an atomic OR on its
+// bit (the first claimant is its parent, as BFS_F::updateAtomic's CAS on
+// Parents, app/BFS.C:36-40); a dense level pulls bottom-up, each unvisited
+// vertex taking its first in-neighbour in the frontier (BFS_F::update and
+// cond, app/BFS.C:31-47).  Frontiers are lists after a push and bitmaps
+// after a pull.
+// ============================================================================
+
+namespace {
+// state words (uint64)
+enum {
+    BD_ACT = 0,     // this level's actions (BA_* bits)
+    BD_MODE = 1,    // this level's branch: 0 sparse, 1 dense, 2 empty, 3 done
+    BD_DM = 2,      // frontier size entering this level
+    BD_OUT = 3,     // its out-degree sum
+    BD_LEVELS = 4,  // levels run
+    BD_SCANNED = 5, // edges read
+    BD_PATH = 8     // GG_PATH_* of level L at BD_PATH + L - 1 (first 64 levels)
+};
+constexpr int kBdWords = BD_PATH + 64;
+enum : unsigned long long {
+    BA_BM_LIST = 1, // dense input from a list: set its bits in the (cleared) input bitmap
+    BA_PACK = 4,    // sparse input from a bitmap: pack it into a list
+    BA_PUSH = 8,    // sparse: scan the degrees and push
+    BA_PULL = 16    // dense: bottom-up pull
+};
+enum { BM_SPARSE = 0, BM_DENSE = 1, BM_EMPTY = 2, BM_DONE = 3 };
+constexpr uint32_t kNoVertex = 0xffffffffu;
+
+// Arrays of one search.
+struct BfsdArgs {
+    const uint32_t *off, *ids;   // out-edge CSR (push)
+    const uint32_t *toff, *tids; // in-edge twin (pull)
+    const uint32_t *first;       // first entry of each in-list, kNoVertex if empty
+    const uint32_t *outdeg;
+    uint32_t *vis;               // visited bitmap
+    int32_t *parents, *levels;   // levels nullable
+    unsigned long long *st, *counters;
+    uint32_t n, nwords;
+};
+
+// Start state (app/BFS.C:86-92): Parents = -1 but the root's, the visited
+// bitmap and the first input bitmap hold the root alone, the root's list.
+// Each word has one writer.
+__global__ void k_bfsd_start(BfsdArgs A, uint32_t root, uint32_t *__restrict__ list, uint32_t *__restrict__ fbm0)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kBdWords)
+        A.st[i] = i == BD_DM ? 1ull : i == BD_OUT ? (unsigned long long)A.outdeg[root] : 0ull;
+    if (i == 0) list[0] = root;
+    if (i < A.nwords) {
+        const uint32_t b = (root >> 5) == i ? 1u << (root & 31) : 0u;
+        A.vis[i] = b;
+        fbm0[i] = b;
+    }
+    if (i < A.n) {
+        const bool r = i == root;
+        A.parents[i] = r ? (int32_t)root : -1;
+        if (A.levels) A.levels[i] = r ? 0 : -1;
+    }
+}
+
+// Level L's decision from the previous level's output (one wave): the
+// output statistics from the counter stripes (cleared here), the branch
+// (ligra-numa.h:929-957: empty when no out-edges, dense when d_m + out-
+// degree > threshold, COO label when also > m/2 on edge partitions), and
+// the conversion the input needs (the previous output is a list after a
+// push, a bitmap after a pull).
+__global__ void k_bfsd_begin(unsigned long long *__restrict__ st, unsigned long long *__restrict__ counters,
+                             int level, int64_t thr, int64_t dense_thr, int coo_ok)
+{
+    const int lane = threadIdx.x;
+    unsigned long long a = 0, o = 0, sc = 0;
+    if (lane < kStripes) {
+        unsigned long long *p = counters + kCntStripeBase + lane * kStripeStride;
+        a = p[CNT_ACTIVE];
+        o = p[CNT_OUT];
+        sc = p[CNT_SCANNED];
+        p[CNT_ACTIVE] = 0;
+        p[CNT_OUT] = 0;
+        p[CNT_SCANNED] = 0;
+    }
+    a = wave_sum_u64(a);
+    o = wave_sum_u64(o);
+    sc = wave_sum_u64(sc);
+    if (lane != 0) return;
+    const unsigned long long prev = st[BD_MODE];
+    if (prev == BM_DONE) {
+        st[BD_ACT] = 0;
+        return;
+    }
+    if (level > 1) {
+        st[BD_DM] = a;
+        st[BD_OUT] = prev == BM_EMPTY ? 0 : o;
+        st[BD_SCANNED] += sc;
+    }
+    counters[CNT_CURSOR] = 0;
+    const unsigned long long dm = st[BD_DM], out = st[BD_OUT];
+    if (dm == 0 || (level > 1 && prev == BM_EMPTY)) {
+        st[BD_MODE] = BM_DONE;
+        st[BD_ACT] = 0;
+        return;
+    }
+    st[BD_LEVELS] = (unsigned long long)level;
+    int path;
+    unsigned long long act = 0, mode;
+    if (out == 0) {
+        mode = BM_EMPTY; // partitioned_vertices::empty() (ligra-numa.h:947)
+        path = GG_PATH_EMPTY;
+    } else if ((int64_t)(dm + out) > thr) {
+        mode = BM_DENSE;
+        path = ((int64_t)(dm + out) > dense_thr && coo_ok) ? GG_PATH_COO : GG_PATH_CSC;
+        // the first level's input bitmap is set by the start kernel
+        act = BA_PULL | ((level > 1 && prev == BM_SPARSE) ? BA_BM_LIST : 0ull);
+    } else {
+        mode = BM_SPARSE;
+        path = GG_PATH_SPARSE;
+        act = BA_PUSH | (prev == BM_DENSE ? BA_PACK : 0ull);
+        st[BD_SCANNED] += out;
+    }
+    st[BD_MODE] = mode;
+    st[BD_ACT] = act;
+    if (level <= 64) st[BD_PATH + level - 1] = (unsigned long long)(long long)path;
+}
+
+// dense input after a push: the list's bits into the input bitmap (cleared
+// by that push's level)
+__global__ void k_bfsd_scatter(const uint32_t *__restrict__ list, uint32_t *__restrict__ fbm,
+                               const unsigned long long *__restrict__ st)
+{
+    if (!(st[BD_ACT] & BA_BM_LIST)) return;
+    const uint64_t k = st[BD_DM];
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < k; i += stride) {
+        const uint32_t v = list[i];
+        atomicOr(&fbm[v >> 5], 1u << (v & 31));
+    }
+}
+
+// sparse input after a pull: the bitmap packed into an ascending list (each
+// work-group kPackWords words, 4 a thread)
+constexpr uint32_t kPackWords = kBlock * 4;
+
+__global__ __launch_bounds__(kBlock) void k_bfsd_pack_count(const uint32_t *__restrict__ fbm, uint32_t nwords,
+                                                            uint32_t *__restrict__ pcnt,
+                                                            const unsigned long long *__restrict__ st)
+{
+    if (!(st[BD_ACT] & BA_PACK)) return;
+    const uint32_t w0 = blockIdx.x * kPackWords + threadIdx.x * 4;
+    uint32_t c = 0;
+    for (int j = 0; j < 4; j++)
+        if (w0 + j < nwords) c += __popc(fbm[w0 + j]);
+    __shared__ unsigned long long s_w[kBlock / 64];
+    const unsigned long long t = wave_sum_u64(c);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int k = 0; k < kBlock / 64; k++) s += s_w[k];
+        pcnt[blockIdx.x] = (uint32_t)s;
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane)
+{
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// exclusive prefix of a kBlock-thread block's values (and the block total)
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t *total)
+{
+    __shared__ uint32_t s_w[kBlock / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t inc = wave_incl_scan_u32(v, lane);
+    if (lane == 63) s_w[wv] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int k = 0; k < kBlock / 64; k++) {
+        if (k < wv) before += s_w[k];
+        all += s_w[k];
+    }
+    __syncthreads(); // s_w is written again by the next call
+    *total = all;
+    return before + inc - v;
+}
+
+// in-place exclusive scan of cnt[0, count) in one work-group; cnt[count]
+// gets the total when 'total_at' is set
+__global__ __launch_bounds__(kBlock) void k_bfsd_scan1(uint32_t *__restrict__ cnt, uint32_t count,
+                                                       const unsigned long long *__restrict__ st,
+                                                       unsigned long long gate_on, uint32_t *__restrict__ total_at)
+{
+    if (!(st[BD_ACT] & gate_on)) return;
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < count; b += kBlock) { // block-uniform bounds
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < count ? cnt[i] : 0u;
+        uint32_t all;
+        const uint32_t ex = block_excl_scan_u32(v, &all);
+        if (i < count) cnt[i] = carry + ex;
+        carry += all;
+    }
+    if (total_at && threadIdx.x == 0) total_at[st[BD_DM]] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bfsd_pack_write(const uint32_t *__restrict__ fbm, uint32_t nwords,
+                                                            const uint32_t *__restrict__ pbase,
+                                                            uint32_t *__restrict__ out,
+                                                            const unsigned long long *__restrict__ st)
+{
+    if (!(st[BD_ACT] & BA_PACK)) return;
+    const uint32_t w0 = blockIdx.x * kPackWords + threadIdx.x * 4;
+    uint32_t word[4], c = 0;
+    for (int j = 0; j < 4; j++) {
+        word[j] = w0 + j < nwords ? fbm[w0 + j] : 0u;
+        c += __popc(word[j]);
+    }
+    uint32_t all;
+    uint32_t pos = pbase[blockIdx.x] + block_excl_scan_u32(c, &all);
+    for (int j = 0; j < 4; j++) {
+        uint32_t x = word[j];
+        while (x) {
+            const uint32_t b = __builtin_ctz(x);
+            out[pos++] = (w0 + j) * 32 + b;
+            x &= x - 1;
+        }
+    }
+}
+
+// The push's degree scan, sized by the frontier on the device: kDegParts
+// parts of whole kBlock tiles each cover a slice of the list; the parts'
+// sums, their scan (k_bfsd_scan1, which also writes the total at
+// offs[d_m]), then each part's exclusive prefixes.  The parts' pass also
+// clears the bitmap a dense next level would take its input in.
+constexpr uint32_t kDegParts = 1024;
+
+__device__ __forceinline__ uint64_t deg_chunk(uint64_t dm)
+{
+    const uint64_t per = (dm + kDegParts - 1) / kDegParts;
+    return (per + kBlock - 1) / kBlock * kBlock;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bfsd_deg_part(const uint32_t *__restrict__ list,
+                                                          const uint32_t *__restrict__ outdeg,
+                                                          const unsigned long long *__restrict__ st,
+                                                          uint32_t *__restrict__ part, uint32_t *__restrict__ fclr,
+                                                          uint32_t nwords)
+{
+    if (!(st[BD_ACT] & BA_PUSH)) return;
+    for (uint32_t w = blockIdx.x * kBlock + threadIdx.x; w < nwords; w += kDegParts * kBlock) fclr[w] = 0;
+    const uint64_t dm = st[BD_DM], ch = deg_chunk(dm);
+    const uint64_t a = (uint64_t)blockIdx.x * ch, b = min(a + ch, dm);
+    unsigned long long s = 0;
+    for (uint64_t i = a + threadIdx.x; i < b; i += kBlock) s += outdeg[list[i]];
+    __shared__ unsigned long long s_w[kBlock / 64];
+    s = wave_sum_u64(s);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < kBlock / 64; k++) t += s_w[k];
+        part[blockIdx.x] = (uint32_t)t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bfsd_deg_write(const uint32_t *__restrict__ list,
+                                                           const uint32_t *__restrict__ outdeg,
+                                                           const uint32_t *__restrict__ part,
+                                                           uint32_t *__restrict__ offs,
+                                                           const unsigned long long *__restrict__ st)
+{
+    if (!(st[BD_ACT] & BA_PUSH)) return;
+    const uint64_t dm = st[BD_DM], ch = deg_chunk(dm);
+    const uint64_t a = (uint64_t)blockIdx.x * ch, b = min(a + ch, dm);
+    uint32_t carry = part[blockIdx.x];
+    for (uint64_t t0 = a; t0 < b; t0 += kBlock) { // block-uniform bounds
+        const uint64_t i = t0 + threadIdx.x;
+        const uint32_t d = i < b ? outdeg[list[i]] : 0u;
+        uint32_t all;
+        const uint32_t ex = block_excl_scan_u32(d, &all);
+        if (i < b) offs[i] = carry + ex;
+        carry += all;
+    }
+}
+
+// Sparse level: the frontier's out-edges in kPushTile tiles (grid-stride),
+// each edge's head claimed through its visited bit; claimed heads get their
+// parent and level and are appended to the output list, one cursor add per
+// tile (a block scan places them).  The bit is read first (a stale read only
+// sends the edge on to the atomic, which decides).
+__global__ __launch_bounds__(kBlock) void k_bfsd_push(BfsdArgs A, const uint32_t *__restrict__ list,
+                                                      const uint32_t *__restrict__ offs, uint32_t *__restrict__ out,
+                                                      uint64_t cap, int32_t level)
+{
+    constexpr int kIt = kPushTile / kBlock;
+    __shared__ uint32_t s_off[kPushLds];
+    __shared__ uint32_t s_lo, s_hi;
+    __shared__ unsigned long long s_base;
+    __shared__ unsigned long long s_od[kBlock / 64];
+    if (!(A.st[BD_ACT] & BA_PUSH)) return;
+    const uint32_t k = (uint32_t)A.st[BD_DM];
+    const uint64_t total = min(cap, (uint64_t)offs[k]);
+    for (uint64_t t0 = (uint64_t)blockIdx.x * kPushTile; t0 < total; t0 += (uint64_t)gridDim.x * kPushTile) {
+        const uint64_t t1 = min(t0 + (uint64_t)kPushTile, total);
+        if (threadIdx.x == 0) {
+            s_lo = upper_idx(offs, 0, k, t0);
+            s_hi = upper_idx(offs, 0, k, t1 - 1) + 1;
+        }
+        __syncthreads();
+        const uint32_t lo = s_lo, hi = s_hi;
+        const bool in_lds = hi - lo <= (uint32_t)kPushLds;
+        if (in_lds)
+            for (uint32_t i = threadIdx.x; i < hi - lo; i += kBlock) s_off[i] = offs[lo + i];
+        __syncthreads();
+        uint32_t got[kIt];
+        uint32_t c = 0;
+        unsigned long long od = 0;
+#pragma unroll
+        for (int it = 0; it < kIt; it++) {
+            const uint64_t e = t0 + (uint64_t)it * kBlock + threadIdx.x;
+            got[it] = kNoVertex;
+            if (e < t1) {
+                const uint32_t kk = in_lds ? lo + upper_idx(s_off, 0, hi - lo, e) : upper_idx(offs, lo, hi, e);
+                const uint32_t v = list[kk];
+                const uint32_t ngh = A.ids[A.off[v] + (uint32_t)(e - offs[kk])];
+                const uint32_t w = ngh >> 5, bit = 1u << (ngh & 31);
+                if (!(A.vis[w] & bit) && !(atomicOr(&A.vis[w], bit) & bit)) {
+                    A.parents[ngh] = (int32_t)v;
+                    if (A.levels) A.levels[ngh] = level;
+                    got[it] = ngh;
+                    c++;
+                    od += A.outdeg[ngh];
+                }
+            }
+        }
+        uint32_t all;
+        uint32_t pos = block_excl_scan_u32(c, &all);
+        od = wave_sum_u64(od);
+        if ((threadIdx.x & 63) == 0) s_od[threadIdx.x >> 6] = od;
+        if (threadIdx.x == 0 && all) s_base = atomicAdd(&A.counters[CNT_CURSOR], (unsigned long long)all);
+        __syncthreads();
+        if (threadIdx.x == 0 && all) {
+            unsigned long long t = 0;
+            for (int q = 0; q < kBlock / 64; q++) t += s_od[q];
+            cnt_add(A.counters, CNT_ACTIVE, (unsigned long long)all);
+            cnt_add(A.counters, CNT_OUT, t);
+        }
+        const unsigned long long base = s_base;
+#pragma unroll
+        for (int it = 0; it < kIt; it++)
+            if (got[it] != kNoVertex) out[base + pos++] = got[it];
+        __syncthreads(); // s_lo, s_hi, s_off, s_base and s_od are written again by the next tile
+    }
+}
+
+// Dense level: every unvisited vertex looks for its first in-neighbour in
+// the input bitmap: the in-list's first entry from a coalesced array, then
+// the rest of a short list kPullStep entries at a time (independent loads,
+// so a step costs two round trips whatever its length); a rest longer than
+// kPullShort is walked by the whole wave, 64 entries a step, one such lane
+// after another (the first hit in list order is the lowest lane of the
+// first step with a hit).  Each wave owns 64 consecutive vertices, so it
+// writes their two words of the visited and output bitmaps whole.
+constexpr uint32_t kPullShort = 16;
+constexpr int kPullStep = 8;
+
+__global__ __launch_bounds__(kBlock) void k_bfsd_pull(BfsdArgs A, const uint32_t *__restrict__ fin,
+                                                      uint32_t *__restrict__ fout, int32_t level)
+{
+    if (!(A.st[BD_ACT] & BA_PULL)) return;
+    LocalStats ls;
+    unsigned long long scanned = 0;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t base = blockIdx.x * kBlock; base < A.n; base += gridDim.x * kBlock) {
+        const uint32_t d = base + threadIdx.x;
+        bool was = false, longl = false;
+        uint32_t par = kNoVertex, b = 0, e = 0;
+        if (d < A.n) {
+            was = bit_test(A.vis, d);
+            const uint32_t s0 = was ? kNoVertex : A.first[d];
+            if (s0 != kNoVertex) {
+                scanned++;
+                if (bit_test(fin, s0)) {
+                    par = s0;
+                } else {
+                    b = A.toff[d] + 1;
+                    e = A.toff[d + 1];
+                    if (e - b > kPullShort) {
+                        longl = true;
+                    } else {
+                        for (uint32_t j = b; j < e && par == kNoVertex; j += kPullStep) {
+                            uint32_t src[kPullStep];
+#pragma unroll
+                            for (int k = 0; k < kPullStep; k++) src[k] = A.tids[min(j + k, e - 1)];
+                            bool hit[kPullStep];
+#pragma unroll
+                            for (int k = 0; k < kPullStep; k++) hit[k] = bit_test(fin, src[k]);
+#pragma unroll
+                            for (int k = kPullStep - 1; k >= 0; k--)
+                                if (j + k < e && hit[k]) par = src[k];
+                            // entries read: up to the hit, or the step
+                            uint32_t used = min((uint32_t)kPullStep, e - j);
+#pragma unroll
+                            for (int k = kPullStep - 1; k >= 0; k--)
+                                if (j + k < e && hit[k]) used = k + 1;
+                            scanned += used;
+                        }
+                    }
+                }
+            }
+        }
+        uint64_t lm = __ballot(longl);
+        while (lm) { // wave-uniform
+            const int L = __builtin_ctzll(lm);
+            lm &= lm - 1;
+            const uint32_t lb = __shfl(b, L, 64), le = __shfl(e, L, 64);
+            uint32_t found = kNoVertex, cnt = 0;
+            for (uint32_t j0 = lb; j0 < le; j0 += 64) {
+                const uint32_t j = j0 + lane;
+                const uint32_t s = j < le ? A.tids[j] : kNoVertex;
+                const bool hit = s != kNoVertex && bit_test(fin, s);
+                const uint64_t hm = __ballot(hit);
+                if (hm) {
+                    const int p = __builtin_ctzll(hm);
+                    found = __shfl(s, p, 64);
+                    cnt += (uint32_t)p + 1;
+                    break;
+                }
+                cnt += min(64u, le - j0);
+            }
+            if (lane == L) {
+                par = found;
+                scanned += cnt;
+            }
+        }
+        const bool now = par != kNoVertex;
+        if (now) {
+            A.parents[d] = (int32_t)par;
+            if (A.levels) A.levels[d] = level;
+            ls.active++;
+            ls.out_edges += A.outdeg[d];
+        }
+        const uint64_t nb = __ballot(now), vb = __ballot(was || now);
+        const uint32_t w0 = (base + (threadIdx.x & ~63u)) >> 5;
+        if (lane == 0 && w0 < A.nwords) {
+            fout[w0] = (uint32_t)nb;
+            A.vis[w0] = (uint32_t)vb;
+        }
+        if (lane == 32 && w0 + 1 < A.nwords) {
+            fout[w0 + 1] = (uint32_t)(nb >> 32);
+            A.vis[w0 + 1] = (uint32_t)(vb >> 32);
+        }
+    }
+    __shared__ unsigned long long s_scan[kBlock / 64];
+    const unsigned long long sc = wave_sum_u64(scanned);
+    if (lane == 0) s_scan[threadIdx.x >> 6] = sc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < kBlock / 64; k++) t += s_scan[k];
+        cnt_add(A.counters, CNT_SCANNED, t);
+    }
+    block_stats(ls, A.counters);
+}
+
+// first[d] = the first source of d's in-list, kNoVertex when it is empty
+__global__ void k_first_in(const uint32_t *__restrict__ toff, const uint32_t *__restrict__ tids, uint32_t n,
+                           uint32_t *__restrict__ first)
+{
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    const uint32_t b = toff[d];
+    first[d] = b < toff[d + 1] ? tids[b] : kNoVertex;
+}
+} // namespace
+
+int ensure_first_in(Graph *g)
+{
+    const bool planned = g->plan.mode != 0;
+    Buf &first = planned ? g->plan.first_in : g->first_in;
+    if (first.p) return GG_OK;
+    Ctx *c = g->ctx;
+    const uint32_t n = g->n;
+    GG_TRY(buf_alloc(c, first, 4ull * std::max<uint32_t>(n, 1)));
+    if (n)
+        k_first_in<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
+            (const uint32_t *)(planned ? g->plan.toff.p : g->toff.p),
+            (const uint32_t *)(planned ? g->plan.tids.p : g->tids.p), n, (uint32_t *)first.p);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+
+int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t threshold, int32_t *parents,
+                    int32_t *levels, gg_run_info *info)
+{
+    Ctx *c = g->ctx;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    if (threshold == -1) threshold = (int64_t)(m / 20);
+    const int64_t dense_thr = (int64_t)(m / 2);
+    const int coo_ok = g->P && !g->part_vertex;
+    // a push emits each vertex at most once, and the scan's edges are at
+    // most the threshold when the level pushes
+    const uint64_t lcap = (uint64_t)n + 1;
+    const uint64_t ecap = (uint64_t)std::max<int64_t>(threshold, 0);
+    const uint32_t nwords = (n + 31) / 32;
+    const uint32_t npack = (nwords + kPackWords - 1) / kPackWords;
+    Buf st, list[2], fbm[2], vis, offs, pcnt, dpart;
+    auto give = [&]() {
+        Buf *bs[] = {&st, &list[0], &list[1], &fbm[0], &fbm[1], &vis, &offs, &pcnt, &dpart};
+        for (Buf *b : bs) pool_give(c, *b);
+    };
+    int rc = pool_take(c, st, 8ull * kBdWords);
+    if (!rc) rc = pool_take(c, list[0], 4 * lcap);
+    if (!rc) rc = pool_take(c, list[1], 4 * lcap);
+    if (!rc) rc = pool_take(c, fbm[0], 4ull * (nwords + 1));
+    if (!rc) rc = pool_take(c, fbm[1], 4ull * (nwords + 1));
+    if (!rc) rc = pool_take(c, vis, 4ull * (nwords + 1));
+    if (!rc) rc = pool_take(c, offs, 4ull * ((uint64_t)n + 2));
+    if (!rc) rc = pool_take(c, pcnt, 4ull * (npack + 1));
+    if (!rc) rc = pool_take(c, dpart, 4ull * kDegParts);
+    if (rc) { give(); return rc; }
+    unsigned long long *S = (unsigned long long *)st.p;
+    BfsdArgs A{};
+    A.off = (const uint32_t *)g->off.p;
+    A.ids = (const uint32_t *)g->ids.p;
+    A.toff = (const uint32_t *)g->toff.p;
+    A.tids = (const uint32_t *)g->tids.p;
+    A.first = first;
+    A.outdeg = (const uint32_t *)g->outdeg.p;
+    A.vis = (uint32_t *)vis.p;
+    A.parents = parents;
+    A.levels = levels;
+    A.st = S;
+    A.counters = (unsigned long long *)c->counters.p;
+    A.n = n;
+    A.nwords = nwords;
+    if ((rc = counters_zero(c))) { give(); return rc; }
+    k_bfsd_start<<<blocks_for(std::max<uint32_t>(n, kBdWords), kBlock), kBlock, 0, c->stream>>>(
+        A, root, (uint32_t *)list[0].p, (uint32_t *)fbm[0].p);
+    const uint32_t push_grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((ecap + kPushTile - 1) / kPushTile, 1),
+                                                            4096);
+    const uint32_t pull_grid = std::max<uint32_t>(blocks_for(n, kBlock), 1);
+    int level = 0;
+    bool done = false;
+    int batch = 8;
+    while (!done && !rc) {
+        for (int b = 0; b < batch && !rc; b++) {
+            level++;
+            // level L reads list[(L+1)&1] / fbm[(L+1)&1] (what level L-1
+            // wrote; the root's for L = 1) and writes list[L&1] / fbm[L&1]
+            const int in = (level + 1) & 1, ou = level & 1;
+            uint32_t *lin = (uint32_t *)list[in].p, *lout = (uint32_t *)list[ou].p;
+            uint32_t *fin = (uint32_t *)fbm[in].p, *fout = (uint32_t *)fbm[ou].p;
+            k_bfsd_begin<<<1, 64, 0, c->stream>>>(S, A.counters, level, threshold, dense_thr, coo_ok);
+            // dense input from the previous push's list
+            k_bfsd_scatter<<<1024, kBlock, 0, c->stream>>>(lin, fin, S);
+            // sparse input from the previous pull's bitmap
+            k_bfsd_pack_count<<<npack, kBlock, 0, c->stream>>>(fin, nwords, (uint32_t *)pcnt.p, S);
+            k_bfsd_scan1<<<1, kBlock, 0, c->stream>>>((uint32_t *)pcnt.p, npack, S, BA_PACK, nullptr);
+            k_bfsd_pack_write<<<npack, kBlock, 0, c->stream>>>(fin, nwords, (const uint32_t *)pcnt.p, lin, S);
+            // sparse: degree scan and push (fout cleared for a dense next level)
+            k_bfsd_deg_part<<<kDegParts, kBlock, 0, c->stream>>>(lin, A.outdeg, S, (uint32_t *)dpart.p, fout,
+                                                                 nwords);
+            k_bfsd_scan1<<<1, kBlock, 0, c->stream>>>((uint32_t *)dpart.p, kDegParts, S, BA_PUSH,
+                                                      (uint32_t *)offs.p);
+            k_bfsd_deg_write<<<kDegParts, kBlock, 0, c->stream>>>(lin, A.outdeg, (const uint32_t *)dpart.p,
+                                                                  (uint32_t *)offs.p, S);
+            k_bfsd_push<<<push_grid, kBlock, 0, c->stream>>>(A, lin, (const uint32_t *)offs.p, lout, ecap, level);
+            // dense: bottom-up pull
+            k_bfsd_pull<<<pull_grid, kBlock, 0, c->stream>>>(A, fin, fout, level);
+            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_bfs: level kernels");
+        }
+        if (rc) break;
+        unsigned long long mode = 0;
+        if ((rc = read_back(c, &mode, S + BD_MODE, 8))) break;
+        done = mode == BM_DONE;
+        batch = std::min(batch * 2, 64);
+        // every level that runs reaches a new vertex, so a search still
+        // going after n + 2 enqueued levels has gone wrong
+        if (!done && level > (int)n + 2) rc = fail(GG_ERR_STATE, "gg_bfs: more levels than vertices");
+    }
+    if (!rc) {
+        unsigned long long h[kBdWords];
+        if ((rc = gg_memcpy_d2h(abi(c), h, S, sizeof h)) == GG_OK && info) {
+            info->iters = (int)h[BD_LEVELS];
+            info->edges_scanned = h[BD_SCANNED];
+            for (int L = 1; L <= info->iters && L <= 64; L++) info->paths[L - 1] = (int)(long long)h[BD_PATH + L - 1];
+        }
+    }
+    give();
+    return rc;
+}
+
 } // namespace gg
 
 using namespace gg;

@@ -257,11 +257,16 @@ public:
     }
     // Host view: refreshed from HBM when a device operator wrote last, and
     // considered written afterwards.
+    // (the apps index an mmap_ptr once per element inside parallel host
+    // loops: the state is written only when it changes, so those accesses
+    // only read the shared Buffer record)
     operator T *()
     {
         if (!buf_) return nullptr;
-        gg_front::to_host(buf_);
-        buf_->state = gg_front::Buffer::HOST;
+        if (buf_->state != gg_front::Buffer::HOST) {
+            gg_front::to_host(buf_);
+            buf_->state = gg_front::Buffer::HOST;
+        }
         return reinterpret_cast<T *>(buf_->host);
     }
     operator const T *() const
@@ -273,8 +278,10 @@ public:
     T *get() const
     {
         if (!buf_) return nullptr;
-        gg_front::to_host(buf_);
-        buf_->state = gg_front::Buffer::HOST;
+        if (buf_->state != gg_front::Buffer::HOST) {
+            gg_front::to_host(buf_);
+            buf_->state = gg_front::Buffer::HOST;
+        }
         return reinterpret_cast<T *>(buf_->host);
     }
     explicit operator bool() const { return buf_ != nullptr; }
@@ -1015,19 +1022,23 @@ inline void writeAdd(ET *a, ET b)
 
 namespace sequence {
 // plusReduce (utils.h:192): 1024-element serial blocks, then over the sums.
+// The blocks are summed in parallel, each in order, so the result does not
+// depend on the thread count.
 template <class OT, class IT>
 OT plusReduce(OT *A, IT n)
 {
     if (n <= 0) return OT(0);
-    std::vector<OT> sums;
-    for (IT s = 0; s < n; s += 1024) {
-        IT e = std::min<IT>(s + 1024, n);
+    const IT nb = (n + 1023) / 1024;
+    std::vector<OT> sums((size_t)nb);
+    parallel_for(IT b = 0; b < nb; b++)
+    {
+        const IT s = b * 1024, e = std::min<IT>(s + 1024, n);
         OT r = A[s];
         for (IT j = s + 1; j < e; j++) r = r + A[j];
-        sums.push_back(r);
+        sums[(size_t)b] = r;
     }
-    if (sums.size() == 1) return sums[0];
-    return plusReduce(sums.data(), (IT)sums.size());
+    if (nb == 1) return sums[0];
+    return plusReduce(sums.data(), nb);
 }
 } // namespace sequence
 

@@ -375,14 +375,23 @@ int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t *levels_de
  * choices of the device pull, set per graph; the autotuner sweeps them. */
 #define GG_OPT_PULL_SCHEDULE 0      /* 0 merge-path tiles, 1 row blocks (default) */
 #define GG_OPT_ROW_EDGES_PER_LANE 1 /* row blocks: target edges per lane of a row group (8) */
-/* The kernel of edgeMap's dense COO branch (ligra-numa.h:953-961, taken when
- * d_m + out-degree > m/2 on an edge-partitioned graph): 0 (default) the
- * destination-owned pull over the in-edge twin, which computes the same
- * values with a fixed summation order; 1 the COO traversal: the Hilbert-
- * ordered partition edge lists (edgeMapDense, ligra-numa.h:803-866) streamed
- * in LDS-staged blocks with the functors' atomic updates, so floating-point
- * sums are in no fixed order. */
+/* The in-edge order of edgeMap's dense COO branch (ligra-numa.h:953-961,
+ * taken when d_m + out-degree > m/2 on an edge-partitioned graph).  Both
+ * are destination-owned pulls (one owner per destination, no atomics, a
+ * fixed summation order): 0 (default) each destination's in-edges by source
+ * id (the in-edge twin); 1 each destination's in-edges in its COO
+ * partition's Hilbert order (edgeMapDense, ligra-numa.h:803-866), the order
+ * in which the reference's PART96 loop adds them.  Graphs with a renumbered
+ * plan run the plan's pull either way. */
 #define GG_OPT_DENSE_COO 2
+/* The fused PageRank loop on a graph with a renumbered plan: the source-
+ * segmented pull, in which each XCD gathers from one eighth of the sources
+ * (its own L2's slice, the slice's hottest sources cached in each CU's LDS)
+ * and a combine pass adds the per-group partial sums of each row in group
+ * order, or one pull over the whole twin.  0 (default) never, 1 when
+ * m >= 2^24 (below that the source vector fits the L2s anyway), 2 always.
+ * Both have a fixed summation order. */
+#define GG_OPT_XCD_SEGMENTS 3
 int gg_graph_set_option(gg_graph_t g, int option, int64_t value);
 int gg_graph_get_option(gg_graph_t g, int option, int64_t *value);
 /* Autotuner: times 'trials' pulls of a probe operator (one gathered value

@@ -247,7 +247,14 @@ def test_batched_arena_sample_vs_oracle(ctx):
         e_r = R.rel_l1(rh[sl], want)
         _, gw_w, gv_w, ait = oracle.wpagerank_adjoint(po, pi, w1, rh[sl].astype(np.float64),
                                                       u[sl].astype(np.float64), alpha, 0.0, iters)
-        assert ait == iters
+        # targets that all hold rank 0 (none reachable from the graph's
+        # seeds) give u = 0 on this graph: the oracle's residual is exactly 0
+        # after one step and it stops there; the gradients are 0 either way
+        assert ait == iters or not u[sl].any()
+        if not u[sl].any():
+            assert not gvh[sl].any() and not gwh[el].any()
+            _say(f"arena graph {i}: u = 0 (targets unreached), zero gradients")
+            continue
         e_v = R.rel_l1(gvh[sl], gv_w)
         e_w = R.rel_l1(gwh[el], gw_w[:len(pi)]) if len(pi) else 0.0
         _say(f"arena graph {i}: scale {scales[i]} ef {efs[i]} n={p.n} m={p.m}: "
