@@ -39,7 +39,7 @@ VOP_PAGERANK, VOP_RESET, VOP_CC_COPY, VOP_ABSDIFF = 0, 1, 2, 3
 FRONTIER_EMPTY, FRONTIER_SINGLE, FRONTIER_ALL = 0, 1, 2
 PATH_EMPTY, PATH_SPARSE, PATH_CSC, PATH_COO = -1, 0, 1, 2
 PATH_NAMES = {-1: "empty", 0: "sparse", 1: "csc", 2: "coo"}
-OPT_PULL_SCHEDULE, OPT_ROW_EDGES_PER_LANE, OPT_DENSE_COO = 0, 1, 2
+OPT_PULL_SCHEDULE, OPT_ROW_EDGES_PER_LANE, OPT_DENSE_COO, OPT_XCD_SEGMENTS = 0, 1, 2, 3
 
 
 class GGError(RuntimeError):

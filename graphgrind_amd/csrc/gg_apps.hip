@@ -173,7 +173,10 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
     const uint32_t *outdeg = use_plan ? (const uint32_t *)g->plan.outdeg.p : (const uint32_t *)g->outdeg.p;
     // the source-segmented pull (GG_OPT_XCD_SEGMENTS): each XCD gathers from
     // its own slice of the contributions, a combine pass finishes the rows
-    const bool xseg = use_plan && m && (g->xcd_segments == 2 || (g->xcd_segments == 1 && m >= (1ull << 24)));
+    // (sources below 2^31, and the gathered vector below 2^31 bytes for the
+    // partial pass's buffer loads)
+    const bool xseg = use_plan && m && (uint64_t)n * sizeof(V) < (1ull << 31) &&
+                      (g->xcd_segments == 2 || (g->xcd_segments == 1 && m >= (1ull << 24)));
     if (xseg) GG_TRY(ensure_xseg(g));
     XSeg &X = g->xseg;
     const uint32_t xhot = kXHotBytes / sizeof(V); // sources the partial pass caches in LDS
@@ -232,7 +235,7 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
             GG_HIP(hipEventRecord(ev[2 * it], c->stream));
             k_xseg_part<V><<<X.grid, kXPartThreads, xhot * sizeof(V), c->stream>>>(
                 op.contrib_in, (V *)X.partial.p, (V *)X.rows.slot_val.p, (const uint32_t *)X.vtids.p,
-                (const uint32_t *)X.rsb.p, (const RowWork *)X.rows.work.p, meta, n, xhot, done);
+                (const RowWork *)X.rows.work.p, meta, n, xhot, done);
             if (X.rows.nlong)
                 k_rows_fix<OpSegPartial<V>><<<blocks_for(X.rows.nlong, kBlock), kBlock, 0, c->stream>>>(
                     sop, (const uint32_t *)X.rows.long_rows.p, X.rows.nlong, (const V *)X.rows.slot_val.p, 0,

@@ -159,8 +159,8 @@ struct XSeg {
     uint32_t item_start[kXSegs + 1] = {};// first row-block work item of each group
     uint32_t comb_blocks = 0;            // combine work-groups (kXCombRows rows each)
     uint32_t grid = 0;                   // partial-pass work-groups: kXSegs x (CUs / kXSegs)
-    Buf vtoff, vtids;                    // virtual CSC: offsets [vrows+1], sources [m + kXWaveEdges] (zero-padded)
-    Buf rsb;                             // uint32 bitmap over the edges: bit e when a virtual row starts at e
+    Buf vtoff, vtids;                    // virtual CSC: offsets [vrows+1], sources [m + kXWaveEdges] (zero-padded;
+                                         // bit 31 set on each virtual row's first edge)
     Buf mask;                            // uint32[n]: bit k when the row has group-k in-edges
     Buf cursor;                          // uint32[comb_blocks*kXSegs]: group-k virtual rows before block b
     Buf partial;                         // 8 bytes [vrows]
@@ -298,7 +298,7 @@ struct Graph {
     // source-segmented form of the plan's twin, and whether the fused
     // PageRank loop uses it (GG_OPT_XCD_SEGMENTS)
     XSeg xseg;
-    int xcd_segments = 0;
+    int xcd_segments = 1;
     // workspace of the fused PageRank loop, kept between calls (grow-only)
     Buf pr_ws[12];
     bool transposed = false; // gg_graph_transpose flips the two directions

@@ -607,6 +607,51 @@ struct OpSegPartial {
     __device__ void finish(uint32_t v, V acc, LocalStats &) const { __builtin_nontemporal_store(acc, &partial[v]); }
 };
 
+// Cross-lane moves on the DPP path (no LDS traffic, so no wait on the
+// LDS queue): update_dpp returns 'old' in lanes whose source lane is out of
+// range or whose row is masked off.  Controls: row_shr:k = 0x110 + k,
+// row_bcast:15 = 0x142, row_bcast:31 = 0x143, wave_shr:1 = 0x138.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, kCtrl, kRowMask, 0xf, false);
+}
+template <int kCtrl, int kRowMask = 0xf, class T>
+__device__ __forceinline__ T dpp_val(T old, T v)
+{
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t o = __builtin_bit_cast(uint64_t, old), x = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = dpp_u32<kCtrl, kRowMask>((uint32_t)o, (uint32_t)x);
+        const uint32_t hi = dpp_u32<kCtrl, kRowMask>((uint32_t)(o >> 32), (uint32_t)(x >> 32));
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, dpp_u32<kCtrl, kRowMask>(__builtin_bit_cast(uint32_t, old),
+                                                              __builtin_bit_cast(uint32_t, v)));
+    }
+}
+// inclusive wave scan (sum): in-row shifts 1, 2, 4, 8, then row 0's total
+// into row 1 and row 2's into row 3, then the first half's into rows 2, 3
+__device__ __forceinline__ uint32_t wave_incl_sum_dpp(uint32_t x)
+{
+    x += dpp_u32<0x111>(0u, x);
+    x += dpp_u32<0x112>(0u, x);
+    x += dpp_u32<0x114>(0u, x);
+    x += dpp_u32<0x118>(0u, x);
+    x += dpp_u32<0x142, 0xa>(0u, x);
+    x += dpp_u32<0x143, 0xc>(0u, x);
+    return x;
+}
+// one step of the inclusive segmented sum (S, f): a lane whose segment
+// started in it keeps S, else adds the earlier lanes' S in front
+template <int kCtrl, int kRowMask, class V>
+__device__ __forceinline__ void seg_step(V &S, uint32_t &f)
+{
+    const V pv = dpp_val<kCtrl, kRowMask>(-V(0), S);
+    const uint32_t pf = dpp_u32<kCtrl, kRowMask>(0u, f);
+    if (!f) S = pv + S;
+    f |= pf;
+}
+
 // Partial pass of a source-segmented pull (GG_OPT_XCD_SEGMENTS): one
 // persistent work-group of kXPartThreads per CU.  Work-groups are dispatched
 // to the XCDs round-robin by index, so work-group b serves group b % kXSegs
@@ -614,23 +659,29 @@ struct OpSegPartial {
 // first copies the group's 'hot' lowest-numbered sources (with a
 // hot-sources-first plan, its highest out-degrees) into LDS; its waves then
 // take the group's work items in turn, gathering from LDS where the source
-// is cached and from global memory otherwise.  Long-row chunks reduce
-// lane-strided and then in a fixed wave tree into slot_val (finished by
-// k_rows_fix); a short item (whole rows, at most kXShortEdges edges) gives
-// each lane kXLaneEdges consecutive edges, finds its rows from the row-start
-// bits 'rsb' (bit e set when a virtual row starts at edge e), adds within
-// the lane, and joins rows that span lanes with a segmented scan across the
-// wave: fixed orders throughout, so the partials are deterministic.  Which
-// work-group serves a group and which XCD it runs on change no value.
+// is cached and from global memory (a buffer load, which for a cached
+// source is given an offset out of range and so reads nothing) otherwise.
+// A wave covers kXWaveEdges edges from its item's first edge rounded down
+// to 4, 16 consecutive ones a lane read as four 16-byte vectors, and masks
+// off the positions outside the item.  vtids carries each virtual row's
+// start in bit 31 of its first source (sources are below 2^31).  Long-row
+// chunks sum in the lane and then in a fixed wave tree into slot_val
+// (finished by k_rows_fix); a short item (whole rows, at most kXShortEdges
+// edges) adds within the lane by row and joins rows that span lanes with a
+// segmented scan across the wave: fixed orders throughout, so the partials
+// are deterministic.  The partials are plain (cached) stores: stores share
+// the load counter, and waiting for the next item's loads would otherwise
+// wait for the far acknowledgements of streaming stores.  Which work-group serves a group and which XCD it runs
+// on change no value.
 template <class V>
 __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const V *__restrict__ vals, V *__restrict__ partial,
                                                              V *__restrict__ slot_val,
                                                              const uint32_t *__restrict__ vtids,
-                                                             const uint32_t *__restrict__ rsb,
                                                              const RowWork *__restrict__ work,
                                                              const uint32_t *__restrict__ item_start, uint32_t n,
                                                              uint32_t hot, const int *__restrict__ done)
 {
+    static_assert(kXLaneEdges == 16, "a lane reads its edges as four 16-byte vectors");
     extern __shared__ unsigned char s_raw[];
     V *s_hot = reinterpret_cast<V *>(s_raw);
     if (done && *done) return;
@@ -644,46 +695,26 @@ __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const V *__restrict
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t i0 = item_start[grp], cnt = item_start[grp + 1] - i0;
-    const uint32_t filler = xseg_global(0, grp); // cached whenever the group has sources
-    // both loads are issued for every edge, from LDS and from global memory
-    // (a cached source's global load reads the group's first source, one
-    // line for the whole wave), and the value is picked after: a branch
-    // would be merged into one flat load through a generic pointer, whose
-    // completion the compiler can only wait for with full waits
-    auto gather = [&](uint32_t s) -> V {
+    // buffer view of vals: n * sizeof(V) bytes (below 2^31, checked by the
+    // host); an offset of 2^31 is out of range and returns 0 without a read
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<V *>(vals), 0, (int)(n * sizeof(V)), 0x00020000);
+    auto gather = [&](uint32_t s, bool in) -> V {
         const uint32_t loc = xseg_local(s);
-        const bool h = loc < hot;
+        const bool h = in && loc < hot;
         const V a = s_hot[h ? loc : 0u];
-        const V b = vals[h ? filler : s];
-        return h ? a : b;
+        V g;
+        const uint32_t off = (in && !h) ? s * (uint32_t)sizeof(V) : 0x80000000u;
+        if constexpr (sizeof(V) == 8)
+            g = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+        else
+            g = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0));
+        return h ? a : g;
     };
     const uint32_t stride = G * kWaves;
-    uint32_t t = wig * kWaves + wv;
-    RowWork wn{};
-    if (t < cnt) wn = work[i0 + t];
-    for (; t < cnt; t += stride) {
-        const RowWork w = wn; // the next item's descriptor loads while this one runs
-        if (t + stride < cnt) wn = work[i0 + t + stride];
-        if (w.kind == kRowLong) {
-            const uint32_t last = w.e1 - 1;
-            uint32_t src[kXLaneEdges];
-#pragma unroll
-            for (int j = 0; j < kXLaneEdges; j++) src[j] = vtids[min(w.e0 + lane + 64u * j, last)];
-            V acc = -V(0);
-#pragma unroll
-            for (int j = 0; j < kXLaneEdges; j++) {
-                const bool in = w.e0 + lane + 64u * j <= last;
-                const V v = gather(in ? src[j] : filler);
-                if (in) acc = acc + v;
-            }
-            for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
-            if (lane == 0) slot_val[w.slot] = acc;
-            continue;
-        }
-        // the lanes cover [A, A + kXWaveEdges), A = e0 rounded down to 4
-        // edges, so each lane's edges are whole 16-byte vectors; positions
-        // outside [e0, e1) are masked off
-        static_assert(kXLaneEdges == 16, "a lane reads its edges as four 16-byte vectors");
+    for (uint32_t t = wig * kWaves + wv; t < cnt; t += stride) {
+        const RowWork w = work[i0 + t];
+        // the wave's window [A0, A0 + kXWaveEdges) and this lane's part of it
         const uint32_t A0 = w.e0 & ~3u;
         const uint32_t base = A0 + lane * kXLaneEdges;
         const uint32_t lo = w.e0 > base ? w.e0 - base : 0u;
@@ -691,30 +722,39 @@ __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const V *__restrict
         const uint32_t vmask = hi > lo ? ((1u << hi) - 1u) & ~((1u << lo) - 1u) : 0u;
         uint32_t src[kXLaneEdges];
         {
-            const uint4 *p4 = reinterpret_cast<const uint4 *>(vtids + base); // vtids is padded past its end
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            const u32x4 *p4 = reinterpret_cast<const u32x4 *>(vtids + base); // vtids is padded past its end
 #pragma unroll
             for (int q = 0; q < kXLaneEdges / 4; q++) {
-                const uint4 u = p4[q];
+                const u32x4 u = p4[q]; // (a streaming load here measured slower)
                 src[4 * q] = u.x;
                 src[4 * q + 1] = u.y;
                 src[4 * q + 2] = u.z;
                 src[4 * q + 3] = u.w;
             }
         }
-        // base is a multiple of 4, so the lane's 16 bits may straddle two words
-        const uint32_t wi = base >> 5;
-        const uint64_t pair = ((uint64_t)rsb[wi + 1] << 32) | rsb[wi];
-        const uint32_t bits = (uint32_t)(pair >> (base & 31)) & vmask;
+        uint32_t bits = 0;
+#pragma unroll
+        for (int j = 0; j < kXLaneEdges; j++) {
+            bits |= (src[j] >> 31) << j;
+            src[j] &= 0x7fffffffu;
+        }
+        bits &= vmask;
         V val[kXLaneEdges];
 #pragma unroll
-        for (int j = 0; j < kXLaneEdges; j++) val[j] = gather((vmask >> j) & 1u ? src[j] : filler);
+        for (int j = 0; j < kXLaneEdges; j++) val[j] = gather(src[j], (vmask >> j) & 1u);
+        if (w.kind == kRowLong) {
+            V acc = -V(0);
+#pragma unroll
+            for (int j = 0; j < kXLaneEdges; j++)
+                if ((vmask >> j) & 1u) acc = acc + val[j];
+            for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
+            if (lane == 0) slot_val[w.slot] = acc;
+            continue;
+        }
         // rows started in earlier lanes of the item
         const uint32_t nst = __popc(bits);
-        uint32_t inc = nst;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t q = __shfl_up(inc, off, 64);
-            if ((int)lane >= off) inc += q;
-        }
+        const uint32_t inc = wave_incl_sum_dpp(nst);
         // r: the row open before the lane's next edge (w.r0 - 1 before the first)
         uint32_t r = w.r0 + (inc - nst) - 1;
         V acc = -V(0), head = -V(0);
@@ -723,7 +763,7 @@ __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const V *__restrict
         for (int j = 0; j < kXLaneEdges; j++) {
             if ((vmask >> j) & 1u) {
                 if ((bits >> j) & 1u) {
-                    if (seen) __builtin_nontemporal_store(acc, &partial[r]); // a row inside the lane ends
+                    if (seen) partial[r] = acc; // a row inside the lane ends
                     else head = acc;
                     seen = true;
                     r++;
@@ -734,21 +774,18 @@ __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const V *__restrict
         }
         // S: the open row's sum up to the lane's end (segmented inclusive scan)
         V S = acc;
-        bool f = seen;
-        for (int off = 1; off < 64; off <<= 1) {
-            const V pv = __shfl_up(S, off, 64);
-            const int pf = __shfl_up((int)f, off, 64);
-            if ((int)lane >= off && !f) {
-                S = pv + S;
-                f = pf != 0;
-            }
-        }
-        V C = __shfl_up(S, 1, 64);
-        if (lane == 0) C = -V(0);
+        uint32_t f = seen ? 1u : 0u;
+        seg_step<0x111, 0xf>(S, f);
+        seg_step<0x112, 0xf>(S, f);
+        seg_step<0x114, 0xf>(S, f);
+        seg_step<0x118, 0xf>(S, f);
+        seg_step<0x142, 0xa>(S, f);
+        seg_step<0x143, 0xc>(S, f);
+        const V C = dpp_val<0x138, 0xf>(-V(0), S); // lane 0: the identity
         // the row open at the lane's start ends at its first row start
-        if (seen && lane > 0) __builtin_nontemporal_store(C + head, &partial[w.r0 + (inc - nst) - 1]);
+        if (seen && lane > 0) partial[w.r0 + (inc - nst) - 1] = C + head;
         // the item's last row ends with its last edge
-        if (lane == (w.e1 - 1 - A0) / kXLaneEdges) __builtin_nontemporal_store(S, &partial[r]);
+        if (lane == (w.e1 - 1 - A0) / kXLaneEdges) partial[r] = S;
     }
 }
 

@@ -22,7 +22,7 @@ namespace gg {
 
 void xseg_free(XSeg &x)
 {
-    Buf *bs[] = {&x.vtoff, &x.vtids, &x.rsb, &x.mask, &x.cursor, &x.partial, &x.meta};
+    Buf *bs[] = {&x.vtoff, &x.vtids, &x.mask, &x.cursor, &x.partial, &x.meta};
     for (Buf *b : bs) buf_free(*b);
     rows_free(x.rows);
     x.ok = false;
@@ -58,11 +58,11 @@ struct RunHead {
     __host__ __device__ uint32_t operator()(uint64_t e) const { return (e == 0 || keys[e] != keys[e - 1]) ? 1u : 0u; }
 };
 
-// the first edge of each run opens virtual row vidx[e] (and sets its bit in
-// the row-start bitmap)
+// the first edge of each run opens virtual row vidx[e] (and is marked in
+// bit 31 of its source)
 __global__ void k_xseg_runs(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vidx, uint64_t m,
                             uint32_t *__restrict__ vtoff, uint64_t *__restrict__ vkey, uint32_t *__restrict__ mask,
-                            uint32_t *__restrict__ rsb)
+                            uint32_t *__restrict__ vtids)
 {
     uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -73,7 +73,7 @@ __global__ void k_xseg_runs(const uint64_t *__restrict__ keys, const uint32_t *_
             vtoff[v] = (uint32_t)e;
             vkey[v] = k;
             atomicOr(&mask[(uint32_t)k], 1u << (uint32_t)(k >> 32));
-            atomicOr(&rsb[e >> 5], 1u << (e & 31));
+            vtids[e] |= 0x80000000u;
         }
     }
 }
@@ -120,6 +120,7 @@ int ensure_xseg(Graph *g)
     const uint32_t n = g->n;
     const uint64_t m = g->m;
     if (!m || !n) return fail(GG_ERR_STATE, "source-segmented pull: empty graph");
+    if (n > 0x7fffffffu) return fail(GG_ERR_SIZE, "source-segmented pull: more than 2^31 vertices");
     xseg_free(x);
     Buf k0, k1, vidx, vkey, rs;
     auto cleanup = [&]() {
@@ -135,9 +136,7 @@ int ensure_xseg(Graph *g)
     if (!rc) rc = buf_alloc(c, k1, 8 * m);
     // the partial pass reads whole lanes' edges past an item's end
     const uint64_t pad_edges = m + kXWaveEdges;
-    const uint64_t rsb_words = pad_edges / 32 + 2;
     if (!rc) rc = buf_alloc(c, x.vtids, 4 * pad_edges);
-    if (!rc) rc = buf_alloc(c, x.rsb, 4 * rsb_words);
     if (!rc) rc = buf_alloc(c, vidx, 4 * m);
     if (!rc) rc = buf_alloc(c, rs, 8ull * (kXSegs + 1));
     if (!rc) rc = buf_alloc(c, x.mask, 4ull * n);
@@ -182,11 +181,10 @@ int ensure_xseg(Graph *g)
     if (!rc) rc = buf_alloc(c, x.partial, 8ull * std::max<uint64_t>(vrows, 1));
     if (rc) return bail(rc);
     if ((rc = fill_bytes(c, x.mask.p, 4ull * n, 0))) return bail(rc);
-    if ((rc = fill_bytes(c, x.rsb.p, 4 * rsb_words, 0))) return bail(rc);
     if ((rc = fill_bytes(c, (uint32_t *)x.vtids.p + m, 4ull * kXWaveEdges, 0))) return bail(rc);
     k_xseg_runs<<<grid, kBlock, 0, c->stream>>>((const uint64_t *)k1.p, (const uint32_t *)vidx.p, m,
                                                  (uint32_t *)x.vtoff.p, (uint64_t *)vkey.p, (uint32_t *)x.mask.p,
-                                                 (uint32_t *)x.rsb.p);
+                                                 (uint32_t *)x.vtids.p);
     GG_HIP(hipGetLastError());
     if ((rc = fill_u32(c, (uint32_t *)x.vtoff.p + vrows, 1, (uint32_t)m))) return bail(rc);
     k_xseg_groups<<<1, 64, 0, c->stream>>>((const uint64_t *)vkey.p, vrows, (uint64_t *)rs.p);

@@ -388,7 +388,7 @@ int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t *levels_de
  * segmented pull, in which each XCD gathers from one eighth of the sources
  * (its own L2's slice, the slice's hottest sources cached in each CU's LDS)
  * and a combine pass adds the per-group partial sums of each row in group
- * order, or one pull over the whole twin.  0 (default) never, 1 when
+ * order, or one pull over the whole twin.  0 never, 1 (default) when
  * m >= 2^24 (below that the source vector fits the L2s anyway), 2 always.
  * Both have a fixed summation order. */
 #define GG_OPT_XCD_SEGMENTS 3

@@ -88,7 +88,7 @@ def test_segmented_pagerank_tiny(ctx, edges):
 
 def test_segmented_option_values(ctx):
     g = gg.Graph.rmat(ctx, 8, 16, 1)
-    assert g.get_option(OPT_XCD) == 0
+    assert g.get_option(OPT_XCD) == 1
     for v in (0, 1, 2):
         g.set_option(OPT_XCD, v)
         assert g.get_option(OPT_XCD) == v
