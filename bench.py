@@ -511,22 +511,27 @@ def copy_calibration(gg, ctx, nbytes=1 << 30, reps=3):
 
 
 def traffic_from_profile(kernel, n, m):
-    """HBM bytes per launch of 'kernel' on this workload, as recorded from
-    the rocprofv3 counter runs in profiles/ (None when absent)."""
-    path = os.path.join(ROOT, "profiles", "k_rows_pagerank_rmat24_fp64.md")
-    if not os.path.exists(path):
+    """HBM bytes per launch (per iteration for a multi-kernel iteration) of
+    'kernel' on this workload, as recorded from the rocprofv3 counter runs
+    in the machine-readable block of a profiles/*.md summary (None when no
+    summary covers it)."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
         return None
-    rec = {}
-    for line in open(path):
-        if line.startswith("    ") and "=" in line:
-            k, v = line.strip().split("=", 1)
-            rec[k.strip()] = v.strip()
-    try:
-        if rec.get("kernel") != kernel or int(rec["n"]) != n or int(rec["m"]) != m:
-            return None
-        return float(rec["traffic_bytes_per_launch"])
-    except (KeyError, ValueError):
-        return None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".md"):
+            continue
+        rec = {}
+        for line in open(os.path.join(pdir, name)):
+            if line.startswith("    ") and "=" in line and not line.strip().startswith(("rocprofv3", "export")):
+                k, v = line.strip().split("=", 1)
+                rec[k.strip()] = v.strip()
+        try:
+            if rec.get("kernel") == kernel and int(rec["n"]) == n and int(rec["m"]) == m:
+                return float(rec["traffic_bytes_per_launch"])
+        except (KeyError, ValueError):
+            continue
+    return None
 
 
 def main():
