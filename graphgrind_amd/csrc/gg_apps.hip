@@ -181,7 +181,7 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
     XSeg &X = g->xseg;
     const uint32_t xhot = kXHotBytes / sizeof(V); // sources the partial pass caches in LDS
     if (xseg)
-        GG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xseg_part<V>),
+        GG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xseg_part<SegSum<V>>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)(xhot * sizeof(V))));
     const uint32_t nblk = view.blocks();
     const uint32_t nfix = xseg ? std::max(view.fix_blocks(), X.comb_blocks) : view.fix_blocks();
@@ -231,13 +231,13 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
             // sources in LDS), long-row chunks, then the combine with the
             // loop's epilogue
             const uint32_t *meta = (const uint32_t *)X.meta.p;
-            OpSegPartial<V> sop{op.contrib_in, (V *)X.partial.p, -V(0), done};
+            OpSegPartial<SegSum<V>> sop{(V *)X.partial.p, done};
             GG_HIP(hipEventRecord(ev[2 * it], c->stream));
-            k_xseg_part<V><<<X.grid, kXPartThreads, xhot * sizeof(V), c->stream>>>(
+            k_xseg_part<SegSum<V>><<<X.grid, kXPartThreads, xhot * sizeof(V), c->stream>>>(
                 op.contrib_in, (V *)X.partial.p, (V *)X.rows.slot_val.p, (const uint32_t *)X.vtids.p,
                 (const RowWork *)X.rows.work.p, meta, n, xhot, done);
             if (X.rows.nlong)
-                k_rows_fix<OpSegPartial<V>><<<blocks_for(X.rows.nlong, kBlock), kBlock, 0, c->stream>>>(
+                k_rows_fix<OpSegPartial<SegSum<V>>><<<blocks_for(X.rows.nlong, kBlock), kBlock, 0, c->stream>>>(
                     sop, (const uint32_t *)X.rows.long_rows.p, X.rows.nlong, (const V *)X.rows.slot_val.p, 0,
                     nullptr, nullptr, nullptr);
             k_xseg_combine<OpPRFused<V>><<<X.comb_blocks, kBlock, 0, c->stream>>>(
@@ -429,26 +429,6 @@ static int cc_loop(Graph *g, uint32_t *labels, uint32_t *prev, gg_run_info *info
 }
 } // namespace gg
 
-namespace gg {
-// Parents and levels of a search in the plan's numbering, in the caller's:
-// parents[v] = inv[pp[perm[v]]] (or -1), levels[v] = lp[perm[v]].  A gather
-// over the caller's order: the reads of pp and lp are random but the arrays
-// sit in the MALL, and every write is coalesced (scattering each parent to
-// the caller's position as it is found costs a read-modify-write of a line
-// in HBM per vertex instead).
-__global__ void k_bfs_unmap(const int32_t *__restrict__ pp, const int32_t *__restrict__ lp,
-                            const uint32_t *__restrict__ perm, const uint32_t *__restrict__ inv,
-                            int32_t *__restrict__ parents, int32_t *__restrict__ levels, uint32_t n)
-{
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const uint32_t k = perm[v];
-    const int32_t p = pp[k];
-    parents[v] = p < 0 ? -1 : (int32_t)inv[p];
-    if (levels) levels[v] = lp[k];
-}
-} // namespace gg
-
 extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *parents, int32_t *levels,
                       gg_run_info *info)
 {
@@ -462,8 +442,8 @@ extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *p
     // device.  With a renumbered plan the levels run on the plan's
     // relabelled arrays (every in-list starts with its largest out-degree,
     // the likeliest frontier member, and vertices of like degree share
-    // waves); parents and levels are mapped back after the last level, on
-    // the device.
+    // waves); the loop's first level past the end maps parents and levels
+    // back to the caller's numbering.
     const bool planned = g->plan.mode != 0 && g->m;
     Buf pp, lp;
     Graph t;
@@ -480,16 +460,12 @@ extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *p
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, c->stream);
     if (!rc && planned) {
+        const BfsMapBack mb{parents, levels, (const uint32_t *)g->plan.perm.p, (const uint32_t *)g->plan.inv.p};
         rc = bfs_device_loop(&t, r2, (const uint32_t *)g->plan.first_in.p, threshold, (int32_t *)pp.p,
-                             levels ? (int32_t *)lp.p : nullptr, info);
-        if (!rc) {
-            k_bfs_unmap<<<blocks_for(g->n, kBlock), kBlock, 0, c->stream>>>(
-                (const int32_t *)pp.p, (const int32_t *)lp.p, (const uint32_t *)g->plan.perm.p,
-                (const uint32_t *)g->plan.inv.p, parents, levels, g->n);
-            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_bfs: map-back kernel");
-        }
+                             levels ? (int32_t *)lp.p : nullptr, info, &mb, &g->bfs_levels_hint);
     } else if (!rc) {
-        rc = bfs_device_loop(g, root, (const uint32_t *)g->first_in.p, threshold, parents, levels, info);
+        rc = bfs_device_loop(g, root, (const uint32_t *)g->first_in.p, threshold, parents, levels, info, nullptr,
+                             &g->bfs_levels_hint);
     }
     (void)hipEventRecord(e1, c->stream);
     if (!rc) rc = sync(c);

@@ -302,6 +302,7 @@ struct Graph {
     // workspace of the fused PageRank loop, kept between calls (grow-only)
     Buf pr_ws[12];
     bool transposed = false; // gg_graph_transpose flips the two directions
+    int bfs_levels_hint = 0;  // depth of the last BFS on the graph (sizes the next one's first batch of levels)
     // tuning (gg_graph_set_option)
     int pull_schedule = 1;  // 0 merge-path tiles, 1 row blocks where built
     int edges_per_lane = 8; // row blocks: target edges per lane of a row group

@@ -589,22 +589,36 @@ __global__ __launch_bounds__(kBlock) void k_rows(Op op, const uint32_t *__restri
     }
 }
 
-// Partial sums of a source-segmented pull: virtual row v (a row's in-edges
-// from one source group) gathers 'vals' over its sources and writes its sum
-// to partial[v].  'none' is the identity of the combine that follows (the
-// fused PageRank loop's -0.0 for "no active source").
-template <class V>
+// Reductions of a source-segmented pull: the value type, the identity and
+// the combine (applied in a fixed order).  SegSum: the fused PageRank
+// loop's sums, whose identity -0.0 marks "no active source"; SegMin: a
+// minimum (measured for Components' dense rounds, which keep the live-label
+// row-block pull: reading round-start labels costs them an extra round).
+template <class T>
+struct SegSum {
+    using V = T;
+    static __device__ __forceinline__ T identity() { return -T(0); }
+    static __device__ __forceinline__ T combine(T a, T b) { return a + b; }
+};
+template <class T>
+struct SegMin {
+    using V = T;
+    static __device__ __forceinline__ T identity() { return ~T(0); }
+    static __device__ __forceinline__ T combine(T a, T b) { return a < b ? a : b; }
+};
+
+// Finishes a long virtual row for k_rows_fix: its chunks' reductions
+// combined in chunk order go to partial[v].
+template <class P>
 struct OpSegPartial {
-    using acc_t = V;
-    const V *vals;
-    V *partial;
-    V none;
+    using acc_t = typename P::V;
+    acc_t *partial;
     const int *done;
     __device__ bool skip() const { return done && *done != 0; }
-    __device__ V identity() const { return none; }
-    __device__ V combine(V a, V b) const { return a + b; }
-    __device__ V gather(uint32_t, uint32_t s) const { return vals[s]; }
-    __device__ void finish(uint32_t v, V acc, LocalStats &) const { __builtin_nontemporal_store(acc, &partial[v]); }
+    __device__ acc_t identity() const { return P::identity(); }
+    __device__ acc_t combine(acc_t a, acc_t b) const { return P::combine(a, b); }
+    __device__ acc_t gather(uint32_t, uint32_t) const { return P::identity(); }
+    __device__ void finish(uint32_t v, acc_t acc, LocalStats &) const { partial[v] = acc; }
 };
 
 // Cross-lane moves on the DPP path (no LDS traffic, so no wait on the
@@ -643,12 +657,12 @@ __device__ __forceinline__ uint32_t wave_incl_sum_dpp(uint32_t x)
 }
 // one step of the inclusive segmented sum (S, f): a lane whose segment
 // started in it keeps S, else adds the earlier lanes' S in front
-template <int kCtrl, int kRowMask, class V>
-__device__ __forceinline__ void seg_step(V &S, uint32_t &f)
+template <int kCtrl, int kRowMask, class P>
+__device__ __forceinline__ void seg_step(typename P::V &S, uint32_t &f)
 {
-    const V pv = dpp_val<kCtrl, kRowMask>(-V(0), S);
+    const typename P::V pv = dpp_val<kCtrl, kRowMask>(P::identity(), S);
     const uint32_t pf = dpp_u32<kCtrl, kRowMask>(0u, f);
-    if (!f) S = pv + S;
+    if (!f) S = P::combine(pv, S);
     f |= pf;
 }
 
@@ -660,7 +674,8 @@ __device__ __forceinline__ void seg_step(V &S, uint32_t &f)
 // hot-sources-first plan, its highest out-degrees) into LDS; its waves then
 // take the group's work items in turn, gathering from LDS where the source
 // is cached and from global memory (a buffer load, which for a cached
-// source is given an offset out of range and so reads nothing) otherwise.
+// source is given an offset out of range and so reads nothing) otherwise,
+// and reduce with P (SegSum, SegMin).
 // A wave covers kXWaveEdges edges from its item's first edge rounded down
 // to 4, 16 consecutive ones a lane read as four 16-byte vectors, and masks
 // off the positions outside the item.  vtids carries each virtual row's
@@ -673,15 +688,17 @@ __device__ __forceinline__ void seg_step(V &S, uint32_t &f)
 // the load counter, and waiting for the next item's loads would otherwise
 // wait for the far acknowledgements of streaming stores.  Which work-group serves a group and which XCD it runs
 // on change no value.
-template <class V>
-__global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const V *__restrict__ vals, V *__restrict__ partial,
-                                                             V *__restrict__ slot_val,
+template <class P>
+__global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const typename P::V *__restrict__ vals,
+                                                             typename P::V *__restrict__ partial,
+                                                             typename P::V *__restrict__ slot_val,
                                                              const uint32_t *__restrict__ vtids,
                                                              const RowWork *__restrict__ work,
                                                              const uint32_t *__restrict__ item_start, uint32_t n,
                                                              uint32_t hot, const int *__restrict__ done)
 {
     static_assert(kXLaneEdges == 16, "a lane reads its edges as four 16-byte vectors");
+    using V = typename P::V;
     extern __shared__ unsigned char s_raw[];
     V *s_hot = reinterpret_cast<V *>(s_raw);
     if (done && *done) return;
@@ -744,11 +761,11 @@ __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const V *__restrict
 #pragma unroll
         for (int j = 0; j < kXLaneEdges; j++) val[j] = gather(src[j], (vmask >> j) & 1u);
         if (w.kind == kRowLong) {
-            V acc = -V(0);
+            V acc = P::identity();
 #pragma unroll
             for (int j = 0; j < kXLaneEdges; j++)
-                if ((vmask >> j) & 1u) acc = acc + val[j];
-            for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
+                if ((vmask >> j) & 1u) acc = P::combine(acc, val[j]);
+            for (int off = 32; off > 0; off >>= 1) acc = P::combine(acc, __shfl_down(acc, off, 64));
             if (lane == 0) slot_val[w.slot] = acc;
             continue;
         }
@@ -757,7 +774,7 @@ __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const V *__restrict
         const uint32_t inc = wave_incl_sum_dpp(nst);
         // r: the row open before the lane's next edge (w.r0 - 1 before the first)
         uint32_t r = w.r0 + (inc - nst) - 1;
-        V acc = -V(0), head = -V(0);
+        V acc = P::identity(), head = P::identity();
         bool seen = false;
 #pragma unroll
         for (int j = 0; j < kXLaneEdges; j++) {
@@ -767,23 +784,23 @@ __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const V *__restrict
                     else head = acc;
                     seen = true;
                     r++;
-                    acc = -V(0);
+                    acc = P::identity();
                 }
-                acc = acc + val[j];
+                acc = P::combine(acc, val[j]);
             }
         }
         // S: the open row's sum up to the lane's end (segmented inclusive scan)
         V S = acc;
         uint32_t f = seen ? 1u : 0u;
-        seg_step<0x111, 0xf>(S, f);
-        seg_step<0x112, 0xf>(S, f);
-        seg_step<0x114, 0xf>(S, f);
-        seg_step<0x118, 0xf>(S, f);
-        seg_step<0x142, 0xa>(S, f);
-        seg_step<0x143, 0xc>(S, f);
-        const V C = dpp_val<0x138, 0xf>(-V(0), S); // lane 0: the identity
+        seg_step<0x111, 0xf, P>(S, f);
+        seg_step<0x112, 0xf, P>(S, f);
+        seg_step<0x114, 0xf, P>(S, f);
+        seg_step<0x118, 0xf, P>(S, f);
+        seg_step<0x142, 0xa, P>(S, f);
+        seg_step<0x143, 0xc, P>(S, f);
+        const V C = dpp_val<0x138, 0xf>(P::identity(), S); // lane 0: the identity
         // the row open at the lane's start ends at its first row start
-        if (seen && lane > 0) partial[w.r0 + (inc - nst) - 1] = C + head;
+        if (seen && lane > 0) partial[w.r0 + (inc - nst) - 1] = P::combine(C, head);
         // the item's last row ends with its last edge
         if (lane == (w.e1 - 1 - A0) / kXLaneEdges) partial[r] = S;
     }
@@ -1005,9 +1022,17 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
 // BFS.C's loop with the direction choice made on the device (no read-back
 // per level) over g's arrays from 'root', parents and levels (nullable) in
 // g's numbering; 'first' holds the first entry of each of g's in-lists
-// (ensure_first_in).
+// (ensure_first_in).  'level_hint' (nullable) holds the depth of the last
+// search on the graph, which sizes the first batch of levels, and receives
+// this search's.
+// With a renumbered plan: the caller's parents and levels (nullable) and
+// the plan's permutation and inverse, filled when the search ends.
+struct BfsMapBack {
+    int32_t *parents, *levels;
+    const uint32_t *perm, *inv;
+};
 int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t threshold, int32_t *parents,
-                    int32_t *levels, gg_run_info *info);
+                    int32_t *levels, gg_run_info *info, const BfsMapBack *mb = nullptr, int *level_hint = nullptr);
 // The first entry of each in-list of the graph's traversal arrays (the
 // plan's when it has one; 0xffffffff for an empty list), built on first use.
 int ensure_first_in(Graph *g);

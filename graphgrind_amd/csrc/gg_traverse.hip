@@ -1179,7 +1179,8 @@ enum : unsigned long long {
     BA_BM_LIST = 1, // dense input from a list: set its bits in the (cleared) input bitmap
     BA_PACK = 4,    // sparse input from a bitmap: pack it into a list
     BA_PUSH = 8,    // sparse: scan the degrees and push
-    BA_PULL = 16    // dense: bottom-up pull
+    BA_PULL = 16,   // dense: bottom-up pull
+    BA_UNMAP = 32   // the search has ended: parents and levels into the caller's numbering
 };
 enum { BM_SPARSE = 0, BM_DENSE = 1, BM_EMPTY = 2, BM_DONE = 3 };
 constexpr uint32_t kNoVertex = 0xffffffffu;
@@ -1194,6 +1195,10 @@ struct BfsdArgs {
     int32_t *parents, *levels;   // levels nullable
     unsigned long long *st, *counters;
     uint32_t n, nwords;
+    // with a renumbered plan: the caller's outputs and the permutation
+    // (caller id -> search id) and its inverse; NULL otherwise
+    int32_t *out_parents, *out_levels;
+    const uint32_t *perm, *inv;
 };
 
 // Start state (app/BFS.C:86-92): Parents = -1 but the root's, the visited
@@ -1224,7 +1229,7 @@ __global__ void k_bfsd_start(BfsdArgs A, uint32_t root, uint32_t *__restrict__ l
 // the conversion the input needs (the previous output is a list after a
 // push, a bitmap after a pull).
 __global__ void k_bfsd_begin(unsigned long long *__restrict__ st, unsigned long long *__restrict__ counters,
-                             int level, int64_t thr, int64_t dense_thr, int coo_ok)
+                             int level, int64_t thr, int64_t dense_thr, int coo_ok, int unmap)
 {
     const int lane = threadIdx.x;
     unsigned long long a = 0, o = 0, sc = 0;
@@ -1254,8 +1259,10 @@ __global__ void k_bfsd_begin(unsigned long long *__restrict__ st, unsigned long 
     counters[CNT_CURSOR] = 0;
     const unsigned long long dm = st[BD_DM], out = st[BD_OUT];
     if (dm == 0 || (level > 1 && prev == BM_EMPTY)) {
+        // the search has ended: this level's pull launch maps the results
+        // back to the caller's numbering (once)
         st[BD_MODE] = BM_DONE;
-        st[BD_ACT] = 0;
+        st[BD_ACT] = unmap ? BA_UNMAP : 0ull;
         return;
     }
     st[BD_LEVELS] = (unsigned long long)level;
@@ -1529,7 +1536,20 @@ constexpr int kPullStep = 8;
 __global__ __launch_bounds__(kBlock) void k_bfsd_pull(BfsdArgs A, const uint32_t *__restrict__ fin,
                                                       uint32_t *__restrict__ fout, int32_t level)
 {
-    if (!(A.st[BD_ACT] & BA_PULL)) return;
+    const unsigned long long act = A.st[BD_ACT];
+    if (act & BA_UNMAP) {
+        // parents[v] = inv[pp[perm[v]]] (or -1), levels[v] = lp[perm[v]]: a
+        // gather over the caller's order (the reads of the plan-numbered
+        // arrays are random but sit in the MALL; every write is coalesced)
+        for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < A.n; v += gridDim.x * kBlock) {
+            const uint32_t k = A.perm[v];
+            const int32_t p = A.parents[k];
+            A.out_parents[v] = p < 0 ? -1 : (int32_t)A.inv[p];
+            if (A.out_levels) A.out_levels[v] = A.levels[k];
+        }
+        return;
+    }
+    if (!(act & BA_PULL)) return;
     LocalStats ls;
     unsigned long long scanned = 0;
     const int lane = threadIdx.x & 63;
@@ -1653,7 +1673,7 @@ int ensure_first_in(Graph *g)
 }
 
 int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t threshold, int32_t *parents,
-                    int32_t *levels, gg_run_info *info)
+                    int32_t *levels, gg_run_info *info, const BfsMapBack *mb, int *level_hint)
 {
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
@@ -1697,6 +1717,12 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
     A.counters = (unsigned long long *)c->counters.p;
     A.n = n;
     A.nwords = nwords;
+    if (mb) {
+        A.out_parents = mb->parents;
+        A.out_levels = mb->levels;
+        A.perm = mb->perm;
+        A.inv = mb->inv;
+    }
     if ((rc = counters_zero(c))) { give(); return rc; }
     k_bfsd_start<<<blocks_for(std::max<uint32_t>(n, kBdWords), kBlock), kBlock, 0, c->stream>>>(
         A, root, (uint32_t *)list[0].p, (uint32_t *)fbm[0].p);
@@ -1705,7 +1731,9 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
     const uint32_t pull_grid = std::max<uint32_t>(blocks_for(n, kBlock), 1);
     int level = 0;
     bool done = false;
-    int batch = 8;
+    // the first batch: the last search's depth on this graph plus the level
+    // that finds the frontier empty, else 8; then 16, 32, 64
+    int batch = level_hint && *level_hint > 0 ? std::min(*level_hint + 1, 64) : 8;
     while (!done && !rc) {
         for (int b = 0; b < batch && !rc; b++) {
             level++;
@@ -1714,7 +1742,7 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
             const int in = (level + 1) & 1, ou = level & 1;
             uint32_t *lin = (uint32_t *)list[in].p, *lout = (uint32_t *)list[ou].p;
             uint32_t *fin = (uint32_t *)fbm[in].p, *fout = (uint32_t *)fbm[ou].p;
-            k_bfsd_begin<<<1, 64, 0, c->stream>>>(S, A.counters, level, threshold, dense_thr, coo_ok);
+            k_bfsd_begin<<<1, 64, 0, c->stream>>>(S, A.counters, level, threshold, dense_thr, coo_ok, mb != nullptr);
             // dense input from the previous push's list
             k_bfsd_scatter<<<1024, kBlock, 0, c->stream>>>(lin, fin, S);
             // sparse input from the previous pull's bitmap
@@ -1737,14 +1765,15 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
         unsigned long long mode = 0;
         if ((rc = read_back(c, &mode, S + BD_MODE, 8))) break;
         done = mode == BM_DONE;
-        batch = std::min(batch * 2, 64);
+        batch = std::min(std::max(batch, 8) * 2, 64);
         // every level that runs reaches a new vertex, so a search still
         // going after n + 2 enqueued levels has gone wrong
         if (!done && level > (int)n + 2) rc = fail(GG_ERR_STATE, "gg_bfs: more levels than vertices");
     }
     if (!rc) {
         unsigned long long h[kBdWords];
-        if ((rc = gg_memcpy_d2h(abi(c), h, S, sizeof h)) == GG_OK && info) {
+        if ((rc = gg_memcpy_d2h(abi(c), h, S, sizeof h)) == GG_OK && level_hint) *level_hint = (int)h[BD_LEVELS];
+        if (rc == GG_OK && info) {
             info->iters = (int)h[BD_LEVELS];
             info->edges_scanned = h[BD_SCANNED];
             for (int L = 1; L <= info->iters && L <= 64; L++) info->paths[L - 1] = (int)(long long)h[BD_PATH + L - 1];
