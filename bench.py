@@ -224,7 +224,8 @@ def batched_bench(gg, ctx, count=512, seed=5, iters=30, msbfs_scale=22, plan=Tru
     graph, and as sub-batches of 64 graphs; the request latencies of the
     three under one serving model (all requests at t = 0, served back to
     back).  Then 32-source BFS on symmetric R-MAT scale 22 against 32
-    single-source BFS runs."""
+    single-source BFS runs, and BC.C's forward phase for the same 32
+    sources in one pass beside 32 single-source BC.C runs."""
     rng = np.random.default_rng(seed)
     scales = rng.integers(10, 19, count)
     efs = rng.choice([8, 16, 32], count)
@@ -361,6 +362,21 @@ def batched_bench(gg, ctx, count=512, seed=5, iters=30, msbfs_scale=22, plan=Tru
     ts = time.perf_counter() - t1
     res["msbfs"] = {"scale": msbfs_scale, "sources": 32, "levels": info.iters, "ms": tm * 1e3,
                     "gteps": und / tm / 1e9, "single_source_loop_ms": ts * 1e3, "single_source_gteps": und / ts / 1e9}
+    # BC.C's forward phase (levels and NumPaths) for the same 32 sources in
+    # one pass (FP32, the reference's fType), beside 32 single-source BC.C
+    # runs (forward and backward) on the same graph
+    pa, _, _ = gg.msbc_forward(g, src, np.float32)  # warm-up
+    t1 = time.perf_counter()
+    pa, _, binfo = gg.msbc_forward(g, src, np.float32, paths=pa)
+    tb = time.perf_counter() - t1
+    sc = gg.DeviceArray(ctx, g.n, np.float32)
+    gg.bc(g, int(src[0]), np.float32, out=sc)  # warm-up
+    t1 = time.perf_counter()
+    for s in src:
+        gg.bc(g, int(s), np.float32, out=sc)
+    tbs = time.perf_counter() - t1
+    res["msbc_forward"] = {"scale": msbfs_scale, "sources": 32, "levels": binfo.iters, "ms": tb * 1e3,
+                           "gteps": und / tb / 1e9, "single_source_bc_loop_ms": tbs * 1e3}
     g.close()
     return res
 

@@ -144,6 +144,7 @@ SIGNATURES = {
     "gg_graph_set_weights_dev": (C.c_int, [_VP, _VP]),
     "gg_graph_create_arena": (C.c_int, [_VP, C.c_int, _VP, C.POINTER(C.c_void_p), _VP, _VP]),
     "gg_msbfs": (C.c_int, [_VP, C.c_int, _VP, _VP, C.POINTER(RunInfo)]),
+    "gg_msbc_forward": (C.c_int, [_VP, C.c_int, _VP, C.c_int, _VP, _VP, C.POINTER(RunInfo)]),
     "gg_autotune_pull": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(TuneInfo)]),
     "gg_bc": (C.c_int, [_VP, C.c_uint32, C.c_int, _VP, _VP, C.POINTER(RunInfo)]),
     "gg_sssp": (C.c_int, [_VP, C.c_uint32, _VP, C.POINTER(RunInfo)]),
@@ -724,6 +725,25 @@ def msbfs(g: Graph, sources, out=None):
     lvp, _ = _dev(lv, len(src) * g.n, np.int32, g.ctx, "out")
     check(lib().gg_msbfs(g.h, len(src), src.ctypes.data, lvp, C.byref(info)))
     return lv, info
+
+
+def msbc_forward(g: Graph, sources, dtype=np.float64, paths=None, levels=False):
+    """The forward phase of app/BC.C for up to 32 sources at once: returns
+    (paths DeviceArray [len(sources) * n] of dtype, NumPaths of every vertex
+    from each source, 0 unreached; levels int32 [len(sources) * n] or None;
+    RunInfo)."""
+    src = np.ascontiguousarray(sources, np.uint32)
+    dt = np.dtype(dtype)
+    if dt not in (np.dtype(np.float64), np.dtype(np.float32)):
+        raise ValueError("msbc_forward: dtype must be float64 or float32")
+    total = len(src) * g.n
+    pa = paths if paths is not None else DeviceArray(g.ctx, total, dt)
+    pp, _ = _dev(pa, total, dt, g.ctx, "paths")
+    lv = DeviceArray(g.ctx, total, np.int32) if levels else None
+    lvp = _dev(lv, total, np.int32, g.ctx, "levels")[0] if lv is not None else None
+    info = RunInfo()
+    check(lib().gg_msbc_forward(g.h, len(src), src.ctypes.data, DT_OF[dt], pp, lvp, C.byref(info)))
+    return pa, lv, info
 
 
 def autotune_pull(g: Graph, dtype=np.float64, trials=5):

@@ -78,34 +78,149 @@ __global__ void k_msbfs_unmap(const int32_t *__restrict__ lp, const uint32_t *__
     }
 }
 
+// BC_F's path counts (app/BC.C:28-47) for the sources of a multi-source
+// level: P is vertex-major, P[v*32 + k] = source k's NumPaths of v.  Half a
+// wave per vertex v with a new bit: lane k adds P[u*32 + k] over v's
+// in-neighbours u whose previous-level mask holds bit k (BC_F::update for
+// the frontier of source k, in in-list order: a fixed order), and writes it
+// when bit k is new at v.  The in-list is read kPathStep entries a step
+// (independent loads: ids, then masks, then the counts); the lanes of a
+// half-wave read the same id and mask and one line of counts.  In-lists
+// longer than kHubChunk are left to the chunked pass below.
+constexpr int kPathStep = 8;
+
+// lane k's sum of P[u*32 + k] over in-list entries [b, e) whose mask
+// fin[u] & nw holds bit k, in list order
+template <class V>
+__device__ __forceinline__ V msbc_sum(const uint32_t *__restrict__ tids, const uint32_t *__restrict__ fin,
+                                      const V *__restrict__ P, uint32_t b, uint32_t e, uint32_t nw, uint32_t k)
+{
+    V acc = V(0);
+    for (uint32_t j = b; j < e; j += kPathStep) {
+        uint32_t u[kPathStep], msk[kPathStep];
+#pragma unroll
+        for (int q = 0; q < kPathStep; q++) u[q] = tids[min(j + q, e - 1)];
+#pragma unroll
+        for (int q = 0; q < kPathStep; q++) msk[q] = (j + q < e) ? fin[u[q]] & nw : 0u;
+        V c[kPathStep];
+#pragma unroll
+        for (int q = 0; q < kPathStep; q++) c[q] = ((msk[q] >> k) & 1u) ? P[(uint64_t)u[q] * 32 + k] : V(0);
+#pragma unroll
+        for (int q = 0; q < kPathStep; q++) acc += c[q];
+    }
+    return acc;
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_msbc_paths(const uint32_t *__restrict__ toff,
+                                                       const uint32_t *__restrict__ tids,
+                                                       const uint32_t *__restrict__ fin,
+                                                       const uint32_t *__restrict__ fout, V *__restrict__ P,
+                                                       uint32_t n)
+{
+    const uint32_t k = threadIdx.x & 31;
+    const uint64_t h0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+    const uint64_t nh = ((uint64_t)gridDim.x * blockDim.x) >> 5;
+    for (uint64_t vv = h0; vv < n; vv += nh) {
+        const uint32_t v = (uint32_t)vv;
+        const uint32_t nw = fout[v];
+        const uint32_t b = toff[v], e = toff[v + 1];
+        if (!nw || e - b > kHubChunk) continue; // no new source, or a chunked hub
+        const V acc = msbc_sum(tids, fin, P, b, e, nw, k);
+        if ((nw >> k) & 1u) P[(uint64_t)v * 32 + k] = acc;
+    }
+}
+
+// The hubs' chunks: half a wave per chunk item (v, e0, e1), lane k's sum
+// into slot[item*32 + k]; then each hub's slots added in chunk order.
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_msbc_hub_items(const uint32_t *__restrict__ items, uint32_t nitems,
+                                                           const uint32_t *__restrict__ tids,
+                                                           const uint32_t *__restrict__ fin,
+                                                           const uint32_t *__restrict__ fout,
+                                                           const V *__restrict__ P, V *__restrict__ slot)
+{
+    const uint32_t k = threadIdx.x & 31;
+    const uint64_t h0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+    const uint64_t nh = ((uint64_t)gridDim.x * blockDim.x) >> 5;
+    for (uint64_t i = h0; i < nitems; i += nh) {
+        const uint32_t v = items[3 * i], b = items[3 * i + 1], e = items[3 * i + 2];
+        const uint32_t nw = fout[v];
+        if (!nw) continue;
+        slot[i * 32 + k] = msbc_sum(tids, fin, P, b, e, nw, k);
+    }
+}
+
+template <class V>
+__global__ void k_msbc_hub_fix(const uint32_t *__restrict__ hubs, uint32_t nhubs, const uint32_t *__restrict__ fout,
+                               const V *__restrict__ slot, V *__restrict__ P)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)nhubs * 32) return;
+    const uint32_t h = (uint32_t)(t >> 5), k = (uint32_t)(t & 31);
+    const uint32_t v = hubs[3 * h], i0 = hubs[3 * h + 1], i1 = hubs[3 * h + 2];
+    if (!((fout[v] >> k) & 1u)) return;
+    V acc = V(0);
+    for (uint32_t i = i0; i < i1; i++) acc += slot[(uint64_t)i * 32 + k];
+    P[(uint64_t)v * 32 + k] = acc;
+}
+
+// sources: P[src_k*32 + k] = 1 (BC.C:196 NumPaths[start] = 1)
+template <class V>
+__global__ void k_msbc_init(const uint32_t *__restrict__ src, int nsrc, const uint32_t *__restrict__ perm,
+                            V *__restrict__ P)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsrc) return;
+    const uint32_t v = perm ? perm[src[k]] : src[k];
+    P[(uint64_t)v * 32 + k] = V(1);
+}
+
+// paths[k][v] = P[perm[v]*32 + k] where source k reached v, else 0
+template <class V>
+__global__ void k_msbc_out(const V *__restrict__ P, const int32_t *__restrict__ lv, const uint32_t *__restrict__ perm,
+                           int nsrc, uint32_t n, V *__restrict__ paths)
+{
+    const uint64_t total = (uint64_t)nsrc * n;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint64_t k = i / n;
+        const uint32_t v = (uint32_t)(i - k * n);
+        const uint32_t w = perm ? perm[v] : v;
+        paths[i] = lv[k * n + w] >= 0 ? P[(uint64_t)w * 32 + k] : V(0);
+    }
+}
+
 } // namespace gg
 
 using namespace gg;
 
-extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t *levels, gg_run_info *info)
+namespace {
+// The multi-source level loop shared by gg_msbfs and gg_msbc_forward: levels
+// lv[k][v] in the traversal's numbering; with P, the path counts too.
+template <class V>
+int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *paths, gg_run_info *info)
 {
-    if (!g || !sources || !levels) return fail(GG_ERR_INVALID, "gg_msbfs: NULL argument");
-    if (nsrc < 1 || nsrc > 32) return fail(GG_ERR_INVALID, "gg_msbfs: 1 to 32 sources");
-    for (int k = 0; k < nsrc; k++)
-        if (sources[k] >= g->n) return fail(GG_ERR_INVALID, "gg_msbfs: source out of range");
     Ctx *c = g->ctx;
-    GG_HIP(hipSetDevice(c->device));
-    if (info) std::memset(info, 0, sizeof(*info));
     const uint32_t n = g->n;
-    // With a renumbered plan the levels run over the plan's relabelled twin
-    // and the per-source levels are mapped back at the end.
     const bool planned = g->plan.mode != 0 && g->m;
     CscView view;
     if (planned) view = plan_view(g);
     else GG_TRY(pull_view(g, &view));
     const uint32_t *perm = planned ? (const uint32_t *)g->plan.perm.p : nullptr;
-    Buf F[2], seen, src, lp;
+    Buf F[2], seen, src, lp, pb, hslot;
     int rc = pool_take(c, F[0], 4ull * n);
     if (!rc) rc = pool_take(c, F[1], 4ull * n);
     if (!rc) rc = pool_take(c, seen, 4ull * n);
     if (!rc) rc = pool_take(c, src, 4ull * nsrc);
-    if (!rc && planned) rc = pool_take(c, lp, 4ull * nsrc * n);
-    int32_t *lv = planned ? (int32_t *)lp.p : levels;
+    if (!rc && (planned || paths)) rc = pool_take(c, lp, 4ull * nsrc * n);
+    if (!rc && paths) rc = pool_take(c, pb, sizeof(V) * 32ull * n);
+    // the long in-lists' chunks (cached per graph or plan)
+    HubChunks *hub = planned ? &g->plan.hubs : &g->hubs;
+    if (!rc && paths) rc = ensure_hub_chunks(c, view.toff, n, *hub);
+    if (!rc && paths && hub->nitems) rc = pool_take(c, hslot, sizeof(V) * 32ull * hub->nitems);
+    int32_t *lv = (planned || paths) ? (int32_t *)lp.p : levels;
+    V *P = paths ? (V *)pb.p : nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -116,31 +231,54 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
         if (!rc) rc = fill_u32(c, seen.p, n, 0);
         if (!rc) rc = gg_fill(abi(c), lv, GG_I32, (uint64_t)nsrc * n, -1.0);
         if (!rc && hipMemcpyAsync(src.p, sources, 4ull * nsrc, hipMemcpyHostToDevice, c->stream) != hipSuccess)
-            rc = fail(GG_ERR_HIP, "gg_msbfs: upload sources");
+            rc = fail(GG_ERR_HIP, "multi-source BFS: upload sources");
         if (!rc) {
             k_msbfs_init<<<1, 64, 0, c->stream>>>((const uint32_t *)src.p, nsrc, n, (uint32_t *)F[0].p,
                                                   (uint32_t *)seen.p, lv, perm);
-            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_msbfs: init");
+            if (P) k_msbc_init<V><<<1, 64, 0, c->stream>>>((const uint32_t *)src.p, nsrc, perm, P);
+            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: init");
         }
         unsigned long long *cnt = (unsigned long long *)c->counters.p;
+        const uint32_t pgrid = (uint32_t)std::min<uint64_t>(((uint64_t)n * 32 + kBlock - 1) / kBlock, 1u << 16);
         // one pull per level until no vertex gains a bit (at most n levels)
         while (!rc && level < (int)n) {
             level++;
-            OpMSBFS op{(const uint32_t *)F[(level - 1) % 2].p, (uint32_t *)F[level % 2].p, (uint32_t *)seen.p,
-                       lv, n, level};
+            const uint32_t *fin = (const uint32_t *)F[(level - 1) % 2].p;
+            uint32_t *fout = (uint32_t *)F[level % 2].p;
+            OpMSBFS op{fin, fout, (uint32_t *)seen.p, lv, n, level};
             rc = counters_zero(c);
             if (!rc) rc = launch_pull(c, view, op, cnt, nullptr, nullptr);
+            if (!rc && P) {
+                k_msbc_paths<V><<<pgrid, kBlock, 0, c->stream>>>(view.toff, view.tids, fin, fout, P, n);
+                if (hub->nitems) {
+                    const uint32_t igrid =
+                        (uint32_t)std::min<uint64_t>(((uint64_t)hub->nitems * 32 + kBlock - 1) / kBlock, 1u << 16);
+                    k_msbc_hub_items<V><<<igrid, kBlock, 0, c->stream>>>(
+                        (const uint32_t *)hub->items.p, hub->nitems, view.tids, fin, fout, P, (V *)hslot.p);
+                    k_msbc_hub_fix<V><<<blocks_for((uint64_t)hub->nhubs * 32, kBlock), kBlock, 0, c->stream>>>(
+                        (const uint32_t *)hub->hubs.p, hub->nhubs, fout, (const V *)hslot.p, P);
+                }
+                if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BC: path counts");
+            }
             unsigned long long cn[4] = {0, 0, 0, 0};
             if (!rc) rc = counters_read(c, cn);
             const unsigned long long active = cn[CNT_ACTIVE];
             if (!rc && info && level <= 64) info->paths[level - 1] = GG_PATH_CSC;
             if (!rc && active == 0) break;
         }
-        if (!rc && planned) {
-            const uint64_t total = (uint64_t)nsrc * n;
-            const uint32_t grid = (uint32_t)std::min<uint64_t>((total + kBlock - 1) / kBlock, 1u << 20);
-            k_msbfs_unmap<<<grid, kBlock, 0, c->stream>>>((const int32_t *)lp.p, perm, nsrc, n, levels);
-            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_msbfs: map-back");
+        const uint64_t total = (uint64_t)nsrc * n;
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((total + kBlock - 1) / kBlock, 1u << 20);
+        if (!rc && P) {
+            k_msbc_out<V><<<grid, kBlock, 0, c->stream>>>(P, lv, perm, nsrc, n, paths);
+            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BC: output");
+        }
+        if (!rc && lv != levels && levels) {
+            if (planned) {
+                k_msbfs_unmap<<<grid, kBlock, 0, c->stream>>>((const int32_t *)lp.p, perm, nsrc, n, levels);
+                if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: map-back");
+            } else if (hipMemcpyAsync(levels, lp.p, 4ull * total, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+                rc = fail(GG_ERR_HIP, "multi-source BFS: levels");
+            }
         }
         (void)hipEventRecord(e1, c->stream);
         if (!rc) rc = sync(c);
@@ -150,7 +288,7 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
     if (info && !rc) {
         info->iters = level;
         info->ms = ms;
-        info->edges_scanned = (uint64_t)level * g->m;
+        info->edges_scanned = (uint64_t)level * g->m * (P ? 2 : 1);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -159,7 +297,37 @@ extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t
     pool_give(c, seen);
     pool_give(c, src);
     pool_give(c, lp);
+    pool_give(c, pb);
+    pool_give(c, hslot);
     return rc;
+}
+} // namespace
+
+extern "C" int gg_msbc_forward(gg_graph_t g, int nsrc, const uint32_t *sources, int dtype, void *paths_dev,
+                               int32_t *levels_dev, gg_run_info *info)
+{
+    if (!g || !sources || !paths_dev) return fail(GG_ERR_INVALID, "gg_msbc_forward: NULL argument");
+    if (nsrc < 1 || nsrc > 32) return fail(GG_ERR_INVALID, "gg_msbc_forward: 1 to 32 sources");
+    if (dtype != GG_F64 && dtype != GG_F32) return fail(GG_ERR_INVALID, "gg_msbc_forward: dtype must be F64 or F32");
+    for (int k = 0; k < nsrc; k++)
+        if (sources[k] >= g->n) return fail(GG_ERR_INVALID, "gg_msbc_forward: source out of range");
+    GG_HIP(hipSetDevice(g->ctx->device));
+    if (info) std::memset(info, 0, sizeof(*info));
+    return dtype == GG_F64 ? msbfs_run<double>(g, nsrc, sources, levels_dev, (double *)paths_dev, info)
+                           : msbfs_run<float>(g, nsrc, sources, levels_dev, (float *)paths_dev, info);
+}
+
+extern "C" int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t *levels, gg_run_info *info)
+{
+    if (!g || !sources || !levels) return fail(GG_ERR_INVALID, "gg_msbfs: NULL argument");
+    if (nsrc < 1 || nsrc > 32) return fail(GG_ERR_INVALID, "gg_msbfs: 1 to 32 sources");
+    for (int k = 0; k < nsrc; k++)
+        if (sources[k] >= g->n) return fail(GG_ERR_INVALID, "gg_msbfs: source out of range");
+    GG_HIP(hipSetDevice(g->ctx->device));
+    if (info) std::memset(info, 0, sizeof(*info));
+    // With a renumbered plan the levels run over the plan's relabelled twin
+    // and the per-source levels are mapped back at the end.
+    return msbfs_run<float>(g, nsrc, sources, levels, nullptr, info);
 }
 
 namespace gg {

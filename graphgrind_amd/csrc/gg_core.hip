@@ -321,6 +321,46 @@ void rows_free(RowSched &s)
 // not hold up a block of short ones); rows above kRowEdges edges become
 // chunked long rows.  Lanes per row: the power of two G <= 64 that gives
 // each lane about 'edges_per_lane' of the block's average row.
+void hubs_free(HubChunks &h)
+{
+    buf_free(h.items);
+    buf_free(h.hubs);
+    h.nitems = h.nhubs = 0;
+    h.ok = false;
+}
+
+int ensure_hub_chunks(Ctx *c, const uint32_t *toff_dev, uint32_t n, HubChunks &h)
+{
+    if (h.ok) return GG_OK;
+    hubs_free(h);
+    std::vector<uint32_t> off((size_t)n + 1);
+    GG_HIP(hipMemcpyAsync(off.data(), toff_dev, 4ull * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    GG_TRY(sync(c));
+    std::vector<uint32_t> items, hubs;
+    for (uint32_t v = 0; v < n; v++) {
+        if (off[v + 1] - off[v] <= kHubChunk) continue;
+        hubs.push_back(v);
+        hubs.push_back((uint32_t)(items.size() / 3));
+        for (uint32_t e = off[v]; e < off[v + 1]; e += kHubChunk) {
+            items.push_back(v);
+            items.push_back(e);
+            items.push_back(std::min<uint32_t>(e + kHubChunk, off[v + 1]));
+        }
+        hubs.push_back((uint32_t)(items.size() / 3));
+    }
+    h.nitems = (uint32_t)(items.size() / 3);
+    h.nhubs = (uint32_t)(hubs.size() / 3);
+    GG_TRY(buf_alloc(c, h.items, 4ull * std::max<size_t>(items.size(), 1)));
+    GG_TRY(buf_alloc(c, h.hubs, 4ull * std::max<size_t>(hubs.size(), 1)));
+    if (!items.empty())
+        GG_HIP(hipMemcpyAsync(h.items.p, items.data(), 4 * items.size(), hipMemcpyHostToDevice, c->stream));
+    if (!hubs.empty())
+        GG_HIP(hipMemcpyAsync(h.hubs.p, hubs.data(), 4 * hubs.size(), hipMemcpyHostToDevice, c->stream));
+    GG_TRY(sync(c)); // the host vectors leave scope
+    h.ok = true;
+    return GG_OK;
+}
+
 int rows_build(Ctx *c, RowSched &s, const uint32_t *d_off, uint32_t rows, uint64_t edges, int edges_per_lane,
                uint32_t item_edges, uint32_t row_cap, const std::vector<uint64_t> *breaks)
 {
@@ -792,6 +832,8 @@ static void graph_free(Graph *g)
                    &g->csr_pull.start_edge, &g->csr_pull.carry_row, &g->csr_pull.carry_val,
                    &g->csr_pull.head_val, &g->wsum};
     for (Buf *b : bufs) buf_free(*b);
+    hubs_free(g->hubs);
+    hubs_free(g->plan.hubs);
     rows_free(g->plan.rows);
     rows_free(g->plan.crows);
     xseg_free(g->xseg);
@@ -1640,6 +1682,7 @@ extern "C" int gg_graph_reorder(gg_graph_t g, int mode)
                  &P.first_in, &P.csched.start_row, &P.csched.start_edge, &P.csched.carry_row, &P.csched.carry_val,
                  &P.csched.head_val};
     for (Buf *b : pb) buf_free(*b);
+    hubs_free(P.hubs);
     rows_free(P.rows);
     rows_free(P.crows);
     xseg_free(g->xseg);
@@ -1891,6 +1934,7 @@ extern "C" int gg_graph_transpose(gg_graph_t g)
     // state tied to one direction
     g->wsum_ok = false;
     buf_free(g->first_in);
+    hubs_free(g->hubs);
     graph_exec_drop(g);
     // destination partitions and COO lists follow the new in-edges
     if (g->P) GG_TRY(gg_graph_partition(g, g->P, g->part_vertex, g->relabel, g->hilbert));

@@ -131,6 +131,19 @@ __host__ __device__ inline uint32_t xseg_global(uint32_t loc, uint32_t grp)
 // in-edge twin and schedule; kernels run in the new numbering and results
 // are mapped back.  Built by gg_graph_reorder, like the reference's offline
 // VEBO reordering but kept inside the library.
+// In-lists longer than kHubChunk edges cut into chunks of kHubChunk, so a
+// per-vertex pass can spread one vertex's in-edges over many waves: items
+// are (vertex, first edge, end edge) triplets, hubs (vertex, first item,
+// end item), both in vertex order.
+constexpr uint32_t kHubChunk = 1024;
+struct HubChunks {
+    bool ok = false;
+    uint32_t nitems = 0, nhubs = 0;
+    Buf items, hubs;
+};
+void hubs_free(HubChunks &h);
+int ensure_hub_chunks(Ctx *c, const uint32_t *toff_dev, uint32_t n, HubChunks &h);
+
 struct PullPlan {
     int mode = 0;        // 0 none, 1 out-degree descending
     Buf perm;            // uint32[n]: new id of old vertex
@@ -147,6 +160,7 @@ struct PullPlan {
     Buf coff, cids, cw, cdelta;
     Buf indeg; // uint32[n] in-degrees in the new numbering
     Buf first_in; // uint32[n]: first source of each in-list (BFS pulls), built on first use
+    HubChunks hubs; // long in-lists in chunks (multi-source path counts), built on first use
     PullSched csched;
     RowSched crows;
 };
@@ -280,6 +294,7 @@ struct Graph {
     bool coo_pull_ok = false;
     // first source of each in-list (BFS pulls), built on first use
     Buf first_in;
+    HubChunks hubs; // long in-lists in chunks (multi-source path counts), built on first use
     // merge-path schedule over the in-edge twin
     PullSched pull;
     PullPlan plan;

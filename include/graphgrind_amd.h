@@ -369,6 +369,15 @@ int gg_wpagerank_adjoint(gg_graph_t g, int dtype, double alpha, double eps, int 
  * [nsrc][n]: levels_dev[k*n + v] = hop distance of v from sources[k]
  * (host array), -1 when unreached.  info->iters = levels run. */
 int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t *levels_dev, gg_run_info *info);
+/* No counterpart in the reference: the forward phase of app/BC.C (BC_F and
+ * BC_Vertex_F, 28-58, with NumPaths[start] = 1 at 196) for up to 32 sources
+ * at once, the multi-source levels of gg_msbfs plus each source's path
+ * counts.  paths_dev (F64 or F32 per dtype) is [nsrc][n]: paths_dev[k*n + v]
+ * = NumPaths of v from sources[k] (host array), 0 when unreached; levels_dev
+ * (I32 [nsrc][n], may be NULL) as gg_msbfs.  Sums run in in-list order, a
+ * fixed order.  info->iters = levels run. */
+int gg_msbc_forward(gg_graph_t g, int nsrc, const uint32_t *sources, int dtype, void *paths_dev,
+                    int32_t *levels_dev, gg_run_info *info);
 
 /* ---- tuning ---------------------------------------------------------------
  * No counterpart in the reference (its only knobs are -c/-v/-o): schedule

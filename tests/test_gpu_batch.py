@@ -200,3 +200,71 @@ def test_autotune_picks_fastest_and_keeps_results(ctx):
     r, info = gg.wpagerank(g, np.float64, ALPHA, 1e-7, 100)
     assert info.iters == it and _rel(r.to_host(), want) < 1e-12
     g.close()
+
+
+@pytest.mark.parametrize("plan", [0, 1])
+@pytest.mark.parametrize("scale,sym", [(12, True), (13, False)])
+def test_msbc_forward_paths_exact(ctx, scale, sym, plan):
+    """BC.C's forward phase for 32 sources at once (a repeat and an isolated
+    vertex among them): every source's levels and NumPaths equal the
+    oracle's single-source BC (path counts are integers, exact in FP64;
+    FP32 within its rounding)."""
+    print(f"msbc scale={scale} sym={sym} plan={plan}", flush=True)
+    g = gg.Graph.rmat(ctx, scale, 16, 2, symmetric=sym).reorder(plan)
+    off, ids, _ = g.csr()
+    n = g.n
+    deg = np.diff(off.astype(np.int64))
+    rng = np.random.default_rng(11)
+    src = rng.choice(np.flatnonzero(deg > 0), 30, replace=False).tolist()
+    src += [src[0], int(np.flatnonzero(deg == 0)[0])]
+    pa, lv, info = gg.msbc_forward(g, src, np.float64, levels=True)
+    P = pa.to_host().reshape(32, n)
+    L = lv.to_host().reshape(32, n)
+    for k, s in enumerate(src):
+        _, level, sigma, _ = oracle.bc(off, ids, s)
+        assert np.array_equal(L[k], level), k
+        assert np.array_equal(P[k], np.where(level >= 0, sigma, 0.0)), k
+    p32, _, _ = gg.msbc_forward(g, src, np.float32)
+    P32 = p32.to_host().reshape(32, n).astype(np.float64)
+    assert np.all(np.abs(P32 - P) <= 1e-6 * np.maximum(np.abs(P), 1.0))
+    g.close()
+
+
+@pytest.mark.parametrize("plan", [0, 1])
+def test_msbc_forward_hubs(ctx, plan):
+    """Vertices with in-lists longer than one 1024-edge chunk (their path
+    counts are summed in chunks, then in chunk order): a symmetric R-MAT
+    graph with two added hubs of 3000 and 5000 neighbours."""
+    off, ids = oracle.rmat_csr(13, 8, 5, symmetric=True)
+    n = len(off) - 1
+    src = np.repeat(np.arange(n), np.diff(off.astype(np.int64)))
+    rng = np.random.default_rng(3)
+    extra = []
+    for hub, k in ((7, 3000), (1000, 5000)):
+        nb = rng.choice(n, k, replace=False)
+        nb = nb[nb != hub]
+        extra += [(hub, int(u)) for u in nb] + [(int(u), hub) for u in nb]
+    es, ed = zip(*extra)
+    off2, ids2 = oracle.csr_from_edges(n, np.concatenate([src, es]), np.concatenate([ids.astype(np.int64), ed]))
+    g = gg.Graph.from_csr(ctx, off2, ids2, symmetric=True).reorder(plan)
+    deg = np.diff(off2.astype(np.int64))
+    sources = [int(v) for v in rng.choice(np.flatnonzero(deg > 0), 20, replace=False)] + [7, 1000]
+    pa, lv, _ = gg.msbc_forward(g, sources, np.float64, levels=True)
+    P = pa.to_host().reshape(len(sources), n)
+    L = lv.to_host().reshape(len(sources), n)
+    for k, s in enumerate(sources):
+        _, level, sigma, _ = oracle.bc(off2, ids2, s)
+        assert np.array_equal(L[k], level), k
+        assert np.array_equal(P[k], np.where(level >= 0, sigma, 0.0)), k
+    g.close()
+
+
+def test_msbc_forward_limits(ctx):
+    g = gg.Graph.rmat(ctx, 10, 8, 3)
+    with pytest.raises(gg.GGError):
+        gg.msbc_forward(g, list(range(33)))
+    with pytest.raises(gg.GGError):
+        gg.msbc_forward(g, [g.n])
+    with pytest.raises(ValueError):
+        gg.msbc_forward(g, [0], np.int32)
+    g.close()
