@@ -774,20 +774,19 @@ __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const typename P::V
         const uint32_t inc = wave_incl_sum_dpp(nst);
         // r: the row open before the lane's next edge (w.r0 - 1 before the first)
         uint32_t r = w.r0 + (inc - nst) - 1;
+        // (selects rather than nested branches: only the store branches)
         V acc = P::identity(), head = P::identity();
         bool seen = false;
 #pragma unroll
         for (int j = 0; j < kXLaneEdges; j++) {
-            if ((vmask >> j) & 1u) {
-                if ((bits >> j) & 1u) {
-                    if (seen) partial[r] = acc; // a row inside the lane ends
-                    else head = acc;
-                    seen = true;
-                    r++;
-                    acc = P::identity();
-                }
-                acc = P::combine(acc, val[j]);
-            }
+            const bool valid = (vmask >> j) & 1u;
+            const bool start = (bits >> j) & 1u; // set only where valid
+            if (start && seen) partial[r] = acc; // a row inside the lane ends
+            head = (start && !seen) ? acc : head;
+            seen = seen || start;
+            r += start ? 1u : 0u;
+            acc = start ? P::identity() : acc;
+            acc = valid ? P::combine(acc, val[j]) : acc;
         }
         // S: the open row's sum up to the lane's end (segmented inclusive scan)
         V S = acc;
