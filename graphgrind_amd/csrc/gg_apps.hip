@@ -392,43 +392,6 @@ extern "C" int gg_pagerank(gg_graph_t g, int dtype, int max_iter, double damping
                            : pagerank_generic<float>(g, max_iter, damping, eps, ranks_dev, info);
 }
 
-namespace gg {
-// Components.C's loop (101-126) from the labels already in 'labels': copy
-// to prev, edgeMap with the minimum-label functor, until the frontier is
-// empty; the dense rounds read the live labels.
-static int cc_loop(Graph *g, uint32_t *labels, uint32_t *prev, gg_run_info *info)
-{
-    Frontier *F = new Frontier();
-    frontier_alloc(g, F);
-    F->bit = true;
-    F->has_dense = true;
-    F->d_m = g->n;
-    F->out_edges = g->m;
-    F->deg_src = g->outdeg.p;
-    EdgeMapExtra ex;
-    ex.cc_live = true;
-    int rounds = 0, rc = GG_OK;
-    while (F->d_m != 0) {
-        gg_vop_t cp = {GG_VOP_CC_COPY, GG_U32, labels, prev, 0, 0};
-        if ((rc = vertexmap_core(g, F, &cp))) break;
-        gg_op_t op = {GG_OP_CC, GG_U32, labels, prev};
-        Frontier *out = nullptr;
-        int path = 0;
-        if ((rc = edgemap_core(g, F, &op, -1, 0, &out, &path, &ex))) break;
-        if (info && rounds < 64) info->paths[rounds] = path;
-        rounds++;
-        frontier_delete(F);
-        F = out;
-    }
-    frontier_delete(F);
-    if (info) {
-        info->iters = rounds;
-        info->edges_scanned = ex.scanned;
-    }
-    return rc;
-}
-} // namespace gg
-
 extern "C" int gg_bfs(gg_graph_t g, uint32_t root, int64_t threshold, int32_t *parents, int32_t *levels,
                       gg_run_info *info)
 {
@@ -486,11 +449,12 @@ extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
     Ctx *c = g->ctx;
     GG_HIP(hipSetDevice(c->device));
     if (info) std::memset(info, 0, sizeof(*info));
-    // With a renumbered plan the loop runs on the plan's relabelled arrays
-    // (the dense rounds gather labels of hot vertices from a narrow range);
-    // label values stay the caller's vertex ids, so the minimum of each
-    // component, the fixed point, is the same, and positions are mapped
-    // back at the end.
+    // Components.C's loop (101-126) on the device, each round's branch
+    // chosen there.  With a renumbered plan the loop runs on the plan's
+    // relabelled arrays (the dense rounds gather labels of hot vertices
+    // from a narrow range); label values stay the caller's vertex ids, so
+    // the minimum of each component, the fixed point, is the same, and
+    // positions are mapped back in the round after the last.
     const bool planned = g->plan.mode != 0 && g->m;
     Buf prev, lp;
     GG_TRY(pool_take(c, prev, 4ull * g->n));
@@ -508,14 +472,12 @@ extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, c->stream);
     if (!rc) {
-        if (planned) { // Components.C:113 IDs[i] = i, in the new numbering
-            if (hipMemcpyAsync(lp.p, g->plan.inv.p, 4ull * g->n, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
-                rc = fail(GG_ERR_HIP, "gg_cc: initial labels");
-            if (!rc) rc = cc_loop(&t, (uint32_t *)lp.p, (uint32_t *)prev.p, info);
-            if (!rc) rc = gather_idx(c, (const uint32_t *)lp.p, (const uint32_t *)g->plan.perm.p, labels, g->n);
+        if (planned) { // Components.C:113 IDs[i] = i: the caller's ids in the new numbering
+            // the round after the last maps the labels back
+            rc = cc_device_loop(&t, (const uint32_t *)g->plan.inv.p, (uint32_t *)lp.p, (uint32_t *)prev.p, info,
+                                &g->cc_rounds_hint, (const uint32_t *)g->plan.perm.p, labels);
         } else {
-            rc = gg_iota(abi(c), labels, GG_U32, g->n); // Components.C:113
-            if (!rc) rc = cc_loop(g, labels, (uint32_t *)prev.p, info);
+            rc = cc_device_loop(g, nullptr, labels, (uint32_t *)prev.p, info, &g->cc_rounds_hint);
         }
     }
     (void)hipEventRecord(e1, c->stream);

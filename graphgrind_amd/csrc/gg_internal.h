@@ -241,7 +241,7 @@ int counters_zero(Ctx *c);                           // totals and stripes
 // Copies 'bytes' (<= kPinBytes) from device memory to host memory through
 // the context's pinned buffer and waits for the stream: a read-back into
 // pageable memory goes through a driver staging copy and takes longer.
-constexpr size_t kPinBytes = 256;
+constexpr size_t kPinBytes = 1024;
 int read_back(Ctx *c, void *host, const void *dev, size_t bytes);
 int counters_read(Ctx *c, unsigned long long out[4]); // folds the stripes first
 
@@ -318,6 +318,7 @@ struct Graph {
     Buf pr_ws[12];
     bool transposed = false; // gg_graph_transpose flips the two directions
     int bfs_levels_hint = 0;  // depth of the last BFS on the graph (sizes the next one's first batch of levels)
+    int cc_rounds_hint = 0;   // rounds of the last Components call on the graph (likewise)
     // tuning (gg_graph_set_option)
     int pull_schedule = 1;  // 0 merge-path tiles, 1 row blocks where built
     int edges_per_lane = 8; // row blocks: target edges per lane of a row group

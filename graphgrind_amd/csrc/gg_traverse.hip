@@ -921,23 +921,12 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             // minimum prevID over active in-neighbours); the destination-
             // owned pull reads 4 bytes per edge where the COO lists read 8
             // and needs no atomics, so both branches run it.
-            if (ex && ex->cc_live) {
-                // fused Components loop: every in-neighbour's current label
-                // (a vertex outside the frontier holds a label its
-                // neighbours already took, so reading it changes nothing);
-                // labels only decrease, so a label read while its owner
-                // rewrites it is an old or a new upper bound, both valid
-                rc = launch_pull(c, pv, OpCCMin{(const uint32_t *)op->a, (uint32_t *)op->a,
-                                               (const uint32_t *)op->b, next, outdeg},
-                                 cnt, nullptr, nullptr);
-            } else {
-                if ((rc = pool_take(c, tmpv, 4ull * n))) return bail(rc);
-                k_cc_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)op->b, F, bit,
-                                                                            (uint32_t *)tmpv.p, n);
-                rc = launch_pull(c, pv, OpCCMin{(const uint32_t *)tmpv.p, (uint32_t *)op->a,
-                                               (const uint32_t *)op->b, next, outdeg},
-                                 cnt, nullptr, nullptr);
-            }
+            if ((rc = pool_take(c, tmpv, 4ull * n))) return bail(rc);
+            k_cc_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)op->b, F, bit,
+                                                                        (uint32_t *)tmpv.p, n);
+            rc = launch_pull(c, pv, OpCCMin{(const uint32_t *)tmpv.p, (uint32_t *)op->a,
+                                           (const uint32_t *)op->b, next, outdeg},
+                             cnt, nullptr, nullptr);
             break;
         }
         case GG_OP_BFS: {
@@ -1229,7 +1218,8 @@ __global__ void k_bfsd_start(BfsdArgs A, uint32_t root, uint32_t *__restrict__ l
 // the conversion the input needs (the previous output is a list after a
 // push, a bitmap after a pull).
 __global__ void k_bfsd_begin(unsigned long long *__restrict__ st, unsigned long long *__restrict__ counters,
-                             int level, int64_t thr, int64_t dense_thr, int coo_ok, int unmap)
+                             int level, int64_t thr, int64_t dense_thr, int coo_ok, int unmap,
+                             unsigned long long dense_scan)
 {
     const int lane = threadIdx.x;
     unsigned long long a = 0, o = 0, sc = 0;
@@ -1276,6 +1266,7 @@ __global__ void k_bfsd_begin(unsigned long long *__restrict__ st, unsigned long 
         path = ((int64_t)(dm + out) > dense_thr && coo_ok) ? GG_PATH_COO : GG_PATH_CSC;
         // the first level's input bitmap is set by the start kernel
         act = BA_PULL | ((level > 1 && prev == BM_SPARSE) ? BA_BM_LIST : 0ull);
+        st[BD_SCANNED] += dense_scan; // a dense pass that does not count its own edges
     } else {
         mode = BM_SPARSE;
         path = GG_PATH_SPARSE;
@@ -1409,18 +1400,25 @@ __device__ __forceinline__ uint64_t deg_chunk(uint64_t dm)
     return (per + kBlock - 1) / kBlock * kBlock;
 }
 
+// With 'cp_dst' set the pass also copies cp_src[v] to cp_dst[v] for every
+// listed v (Components' vertexMap, CC_Vertex_F).
 __global__ __launch_bounds__(kBlock) void k_bfsd_deg_part(const uint32_t *__restrict__ list,
                                                           const uint32_t *__restrict__ outdeg,
                                                           const unsigned long long *__restrict__ st,
                                                           uint32_t *__restrict__ part, uint32_t *__restrict__ fclr,
-                                                          uint32_t nwords)
+                                                          uint32_t nwords, const uint32_t *__restrict__ cp_src,
+                                                          uint32_t *__restrict__ cp_dst)
 {
     if (!(st[BD_ACT] & BA_PUSH)) return;
     for (uint32_t w = blockIdx.x * kBlock + threadIdx.x; w < nwords; w += kDegParts * kBlock) fclr[w] = 0;
     const uint64_t dm = st[BD_DM], ch = deg_chunk(dm);
     const uint64_t a = (uint64_t)blockIdx.x * ch, b = min(a + ch, dm);
     unsigned long long s = 0;
-    for (uint64_t i = a + threadIdx.x; i < b; i += kBlock) s += outdeg[list[i]];
+    for (uint64_t i = a + threadIdx.x; i < b; i += kBlock) {
+        const uint32_t v = list[i];
+        s += outdeg[v];
+        if (cp_dst) cp_dst[v] = cp_src[v];
+    }
     __shared__ unsigned long long s_w[kBlock / 64];
     s = wave_sum_u64(s);
     if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
@@ -1452,22 +1450,46 @@ __global__ __launch_bounds__(kBlock) void k_bfsd_deg_write(const uint32_t *__res
     }
 }
 
+// Sparse level's update of edge (v -> ngh) for BFS.C (BFS_F::updateAtomic,
+// app/BFS.C:49-56): the head is claimed through its visited bit (read
+// first: a stale read only sends the edge on to the atomic, which decides)
+// and gets its parent and level; returns whether ngh joins the output.
+struct BfsVisit {
+    uint32_t *vis;
+    int32_t *parents, *levels;
+    int32_t level;
+    __device__ __forceinline__ bool operator()(uint32_t v, uint32_t ngh) const
+    {
+        const uint32_t w = ngh >> 5, bit = 1u << (ngh & 31);
+        if (!(vis[w] & bit) && !(atomicOr(&vis[w], bit) & bit)) {
+            parents[ngh] = (int32_t)v;
+            if (levels) levels[ngh] = level;
+            return true;
+        }
+        return false;
+    }
+};
+
 // Sparse level: the frontier's out-edges in kPushTile tiles (grid-stride),
-// each edge's head claimed through its visited bit; claimed heads get their
-// parent and level and are appended to the output list, one cursor add per
-// tile (a block scan places them).  The bit is read first (a stale read only
-// sends the edge on to the atomic, which decides).
-__global__ __launch_bounds__(kBlock) void k_bfsd_push(BfsdArgs A, const uint32_t *__restrict__ list,
-                                                      const uint32_t *__restrict__ offs, uint32_t *__restrict__ out,
-                                                      uint64_t cap, int32_t level)
+// each edge updated by U; the heads it returns true for are appended to the
+// output list, one cursor add per tile (a block scan places them).
+template <class U>
+__global__ __launch_bounds__(kBlock) void k_dpush(U upd, const uint32_t *__restrict__ off,
+                                                  const uint32_t *__restrict__ ids,
+                                                  const uint32_t *__restrict__ outdeg,
+                                                  const unsigned long long *__restrict__ st,
+                                                  unsigned long long *__restrict__ counters,
+                                                  const uint32_t *__restrict__ list,
+                                                  const uint32_t *__restrict__ offs, uint32_t *__restrict__ out,
+                                                  uint64_t cap)
 {
     constexpr int kIt = kPushTile / kBlock;
     __shared__ uint32_t s_off[kPushLds];
     __shared__ uint32_t s_lo, s_hi;
     __shared__ unsigned long long s_base;
     __shared__ unsigned long long s_od[kBlock / 64];
-    if (!(A.st[BD_ACT] & BA_PUSH)) return;
-    const uint32_t k = (uint32_t)A.st[BD_DM];
+    if (!(st[BD_ACT] & BA_PUSH)) return;
+    const uint32_t k = (uint32_t)st[BD_DM];
     const uint64_t total = min(cap, (uint64_t)offs[k]);
     for (uint64_t t0 = (uint64_t)blockIdx.x * kPushTile; t0 < total; t0 += (uint64_t)gridDim.x * kPushTile) {
         const uint64_t t1 = min(t0 + (uint64_t)kPushTile, total);
@@ -1491,14 +1513,11 @@ __global__ __launch_bounds__(kBlock) void k_bfsd_push(BfsdArgs A, const uint32_t
             if (e < t1) {
                 const uint32_t kk = in_lds ? lo + upper_idx(s_off, 0, hi - lo, e) : upper_idx(offs, lo, hi, e);
                 const uint32_t v = list[kk];
-                const uint32_t ngh = A.ids[A.off[v] + (uint32_t)(e - offs[kk])];
-                const uint32_t w = ngh >> 5, bit = 1u << (ngh & 31);
-                if (!(A.vis[w] & bit) && !(atomicOr(&A.vis[w], bit) & bit)) {
-                    A.parents[ngh] = (int32_t)v;
-                    if (A.levels) A.levels[ngh] = level;
+                const uint32_t ngh = ids[off[v] + (uint32_t)(e - offs[kk])];
+                if (upd(v, ngh)) {
                     got[it] = ngh;
                     c++;
-                    od += A.outdeg[ngh];
+                    od += outdeg[ngh];
                 }
             }
         }
@@ -1506,13 +1525,13 @@ __global__ __launch_bounds__(kBlock) void k_bfsd_push(BfsdArgs A, const uint32_t
         uint32_t pos = block_excl_scan_u32(c, &all);
         od = wave_sum_u64(od);
         if ((threadIdx.x & 63) == 0) s_od[threadIdx.x >> 6] = od;
-        if (threadIdx.x == 0 && all) s_base = atomicAdd(&A.counters[CNT_CURSOR], (unsigned long long)all);
+        if (threadIdx.x == 0 && all) s_base = atomicAdd(&counters[CNT_CURSOR], (unsigned long long)all);
         __syncthreads();
         if (threadIdx.x == 0 && all) {
             unsigned long long t = 0;
             for (int q = 0; q < kBlock / 64; q++) t += s_od[q];
-            cnt_add(A.counters, CNT_ACTIVE, (unsigned long long)all);
-            cnt_add(A.counters, CNT_OUT, t);
+            cnt_add(counters, CNT_ACTIVE, (unsigned long long)all);
+            cnt_add(counters, CNT_OUT, t);
         }
         const unsigned long long base = s_base;
 #pragma unroll
@@ -1734,6 +1753,7 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
     // the first batch: the last search's depth on this graph plus the level
     // that finds the frontier empty, else 8; then 16, 32, 64
     int batch = level_hint && *level_hint > 0 ? std::min(*level_hint + 1, 64) : 8;
+    unsigned long long h[kBdWords];
     while (!done && !rc) {
         for (int b = 0; b < batch && !rc; b++) {
             level++;
@@ -1742,7 +1762,8 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
             const int in = (level + 1) & 1, ou = level & 1;
             uint32_t *lin = (uint32_t *)list[in].p, *lout = (uint32_t *)list[ou].p;
             uint32_t *fin = (uint32_t *)fbm[in].p, *fout = (uint32_t *)fbm[ou].p;
-            k_bfsd_begin<<<1, 64, 0, c->stream>>>(S, A.counters, level, threshold, dense_thr, coo_ok, mb != nullptr);
+            k_bfsd_begin<<<1, 64, 0, c->stream>>>(S, A.counters, level, threshold, dense_thr, coo_ok, mb != nullptr,
+                                                  0ull);
             // dense input from the previous push's list
             k_bfsd_scatter<<<1024, kBlock, 0, c->stream>>>(lin, fin, S);
             // sparse input from the previous pull's bitmap
@@ -1751,29 +1772,320 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
             k_bfsd_pack_write<<<npack, kBlock, 0, c->stream>>>(fin, nwords, (const uint32_t *)pcnt.p, lin, S);
             // sparse: degree scan and push (fout cleared for a dense next level)
             k_bfsd_deg_part<<<kDegParts, kBlock, 0, c->stream>>>(lin, A.outdeg, S, (uint32_t *)dpart.p, fout,
-                                                                 nwords);
+                                                                 nwords, nullptr, nullptr);
             k_bfsd_scan1<<<1, kBlock, 0, c->stream>>>((uint32_t *)dpart.p, kDegParts, S, BA_PUSH,
                                                       (uint32_t *)offs.p);
             k_bfsd_deg_write<<<kDegParts, kBlock, 0, c->stream>>>(lin, A.outdeg, (const uint32_t *)dpart.p,
                                                                   (uint32_t *)offs.p, S);
-            k_bfsd_push<<<push_grid, kBlock, 0, c->stream>>>(A, lin, (const uint32_t *)offs.p, lout, ecap, level);
+            k_dpush<<<push_grid, kBlock, 0, c->stream>>>(BfsVisit{A.vis, A.parents, A.levels, level}, A.off, A.ids,
+                                                         A.outdeg, S, A.counters, lin, (const uint32_t *)offs.p,
+                                                         lout, ecap);
             // dense: bottom-up pull
             k_bfsd_pull<<<pull_grid, kBlock, 0, c->stream>>>(A, fin, fout, level);
             if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_bfs: level kernels");
         }
         if (rc) break;
-        unsigned long long mode = 0;
-        if ((rc = read_back(c, &mode, S + BD_MODE, 8))) break;
-        done = mode == BM_DONE;
+        static_assert(sizeof h <= kPinBytes, "the state words fit the pinned read-back buffer");
+        if ((rc = read_back(c, h, S, sizeof h))) break;
+        done = h[BD_MODE] == BM_DONE;
         batch = std::min(std::max(batch, 8) * 2, 64);
         // every level that runs reaches a new vertex, so a search still
         // going after n + 2 enqueued levels has gone wrong
         if (!done && level > (int)n + 2) rc = fail(GG_ERR_STATE, "gg_bfs: more levels than vertices");
     }
     if (!rc) {
-        unsigned long long h[kBdWords];
-        if ((rc = gg_memcpy_d2h(abi(c), h, S, sizeof h)) == GG_OK && level_hint) *level_hint = (int)h[BD_LEVELS];
-        if (rc == GG_OK && info) {
+        if (level_hint) *level_hint = (int)h[BD_LEVELS];
+        if (info) {
+            info->iters = (int)h[BD_LEVELS];
+            info->edges_scanned = h[BD_SCANNED];
+            for (int L = 1; L <= info->iters && L <= 64; L++) info->paths[L - 1] = (int)(long long)h[BD_PATH + L - 1];
+        }
+    }
+    give();
+    return rc;
+}
+
+// ============================================================================
+// Components.C's Compute() on the device (app/Components.C:101-126).  Each
+// round's edgeMap branch is chosen by k_bfsd_begin from the previous round's
+// striped counters, as for BFS, with no read-back per round.  A dense round
+// is the live-label pull: each row takes the minimum over all its in-
+// neighbours' current labels (a vertex outside the frontier holds a label
+// its neighbours already took, so reading it changes nothing; labels only
+// decrease, so a label read while its owner rewrites it is the old or the
+// new one, both upper bounds of the component's least).  A sparse
+// round pushes the frontier's round-start labels with atomicMin
+// (CC_F::updateAtomic, app/Components.C:37-46); the first lowering of a
+// vertex in the round appends it to the next frontier.  prevIDs holds the
+// round-start labels.  CC_Vertex_F's copy over the frontier
+// (app/Components.C:88-99) runs in three places: in the degree scan of a
+// sparse round, in a list pass before a dense round that follows a sparse
+// one, and in the dense round's own row writes.  The dense round copies
+// every row it lowers, so after it prevIDs equals IDs everywhere.
+// ============================================================================
+namespace {
+struct OpCCDev {
+    using acc_t = uint32_t;
+    uint32_t *ids, *prev;
+    uint8_t *next;
+    const uint32_t *outdeg;
+    const unsigned long long *st;
+    __device__ bool skip() const { return !(st[BD_ACT] & BA_PULL); }
+    __device__ uint32_t identity() const { return 0xffffffffu; }
+    __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a < b ? a : b; }
+    __device__ uint32_t gather(uint32_t, uint32_t s) const { return ids[s]; }
+    // label 0 is the least there is: such a row cannot change
+    __device__ bool row_done(uint32_t d) const { return ids[d] == 0u; }
+    __device__ void finish(uint32_t d, uint32_t acc, LocalStats &ls) const
+    {
+        // a row's owner is its only writer in a dense round, so ids[d] is
+        // still the round-start label
+        const bool dec = acc < ids[d];
+        if (dec) {
+            ids[d] = acc;
+            prev[d] = acc;
+            ls.active++;
+            ls.out_edges += outdeg[d];
+        }
+        next[d] = dec;
+    }
+};
+
+// CC_F::updateAtomic on edge (v -> ngh): the source's round-start label
+// into the head with atomicMin; the lowering that finds the head still at
+// its round-start label is its first in the round.
+struct CcVisit {
+    uint32_t *ids;
+    const uint32_t *prev;
+    __device__ __forceinline__ bool operator()(uint32_t v, uint32_t ngh) const
+    {
+        const uint32_t val = prev[v];
+        if (val < ids[ngh]) {
+            const uint32_t old = atomicMin(&ids[ngh], val);
+            return val < old && old == prev[ngh];
+        }
+        return false;
+    }
+};
+
+// Start state (app/Components.C:110-116): IDs = prevIDs = the vertex ids
+// ('init', the caller's ids of a renumbered plan, else the position), the
+// frontier all vertices with d_m = n and m out-edges.
+__global__ void k_ccd_start(unsigned long long *__restrict__ st, uint32_t n, uint64_t m,
+                            const uint32_t *__restrict__ init, uint32_t *__restrict__ ids,
+                            uint32_t *__restrict__ prev)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kBdWords) st[i] = i == BD_DM ? (unsigned long long)n : i == BD_OUT ? (unsigned long long)m : 0ull;
+    if (i < n) {
+        const uint32_t l = init ? init[i] : i;
+        ids[i] = l;
+        prev[i] = l;
+    }
+}
+
+// dense round after a sparse one: prevIDs of the input list.  The round
+// after the last (with a renumbered plan): the labels into the caller's
+// order, out[v] = ids[perm[v]].
+__global__ void k_ccd_copy(const uint32_t *__restrict__ list, const uint32_t *__restrict__ ids,
+                           uint32_t *__restrict__ prev, const unsigned long long *__restrict__ st,
+                           const uint32_t *__restrict__ perm, uint32_t *__restrict__ out, uint32_t n)
+{
+    const unsigned long long act = st[BD_ACT];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (act & BA_UNMAP) {
+        // 8 independent gathers a thread in flight (the grid is small)
+        constexpr int kU = 8;
+        for (uint64_t v0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v0 < n; v0 += stride * kU) {
+            uint32_t p[kU], x[kU];
+#pragma unroll
+            for (int k = 0; k < kU; k++) {
+                const uint64_t v = v0 + k * stride;
+                p[k] = v < n ? perm[v] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < kU; k++) x[k] = ids[p[k]];
+#pragma unroll
+            for (int k = 0; k < kU; k++) {
+                const uint64_t v = v0 + k * stride;
+                if (v < n) out[v] = x[k];
+            }
+        }
+        return;
+    }
+    if (!(act & BA_BM_LIST)) return;
+    const uint64_t k = st[BD_DM];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) {
+        const uint32_t v = list[i];
+        prev[v] = ids[v];
+    }
+}
+
+// sparse round after a dense one: the dense round's output flags (one byte
+// per vertex, 0 or 1) packed into an ascending list, 16 flags a thread;
+// the flag array is padded to whole 16-byte groups, the pad masked here
+constexpr uint32_t kCcPack = kBlock * 16;
+
+// bit 7 of each byte of the result is set where that flag is nonzero; the
+// bytes at and past n are cleared
+__device__ __forceinline__ uint4 cc_flags16(const uint8_t *__restrict__ F, uint32_t n, uint32_t v0)
+{
+    const uint4 q = *(const uint4 *)(F + v0);
+    const uint32_t have = n - v0; // > 0
+    uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t x = w[j];
+        uint32_t y = (((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u;
+        const uint32_t lo = 4u * j;
+        if (have <= lo) y = 0u;
+        else if (have < lo + 4) y &= (1u << (8 * (have - lo))) - 1u;
+        w[j] = y;
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ccd_pack_count(const uint8_t *__restrict__ F, uint32_t n,
+                                                           uint32_t *__restrict__ pcnt,
+                                                           const unsigned long long *__restrict__ st)
+{
+    if (!(st[BD_ACT] & BA_PACK)) return;
+    const uint32_t v0 = blockIdx.x * kCcPack + threadIdx.x * 16;
+    uint32_t c = 0;
+    if (v0 < n) {
+        const uint4 q = cc_flags16(F, n, v0);
+        c = __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
+    }
+    __shared__ unsigned long long s_w[kBlock / 64];
+    const unsigned long long t = wave_sum_u64(c);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0;
+        for (int k = 0; k < kBlock / 64; k++) a += s_w[k];
+        pcnt[blockIdx.x] = (uint32_t)a;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ccd_pack_write(const uint8_t *__restrict__ F, uint32_t n,
+                                                           const uint32_t *__restrict__ pbase,
+                                                           uint32_t *__restrict__ out,
+                                                           const unsigned long long *__restrict__ st)
+{
+    if (!(st[BD_ACT] & BA_PACK)) return;
+    const uint32_t v0 = blockIdx.x * kCcPack + threadIdx.x * 16;
+    uint4 q = make_uint4(0u, 0u, 0u, 0u);
+    if (v0 < n) q = cc_flags16(F, n, v0);
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+    const uint32_t c = __popc(w[0]) + __popc(w[1]) + __popc(w[2]) + __popc(w[3]);
+    uint32_t all;
+    uint32_t pos = pbase[blockIdx.x] + block_excl_scan_u32(c, &all);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        uint32_t x = w[j];
+        while (x) {
+            const uint32_t b = __builtin_ctz(x); // bit 7 of byte b/8
+            out[pos++] = v0 + 4u * j + (b >> 3);
+            x &= x - 1;
+        }
+    }
+}
+} // namespace
+
+int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *prev, gg_run_info *info,
+                   int *round_hint, const uint32_t *perm, uint32_t *out)
+{
+    Ctx *c = g->ctx;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    if (!n) return GG_OK;
+    // Components.C:123 passes m/20; the first round (all n vertices, m
+    // out-edges) is then dense, or empty without edges, never sparse
+    const int64_t threshold = (int64_t)(m / 20);
+    const int64_t dense_thr = (int64_t)(m / 2);
+    const int coo_ok = g->P && !g->part_vertex;
+    const uint64_t lcap = (uint64_t)n + 1;
+    const uint64_t ecap = (uint64_t)std::max<int64_t>(threshold, 0);
+    const uint32_t npack = (n + kCcPack - 1) / kCcPack;
+    CscView pv;
+    GG_TRY(pull_view(g, &pv));
+    Buf st, list[2], flags, offs, pcnt, dpart;
+    auto give = [&]() {
+        Buf *bs[] = {&st, &list[0], &list[1], &flags, &offs, &pcnt, &dpart};
+        for (Buf *b : bs) pool_give(c, *b);
+    };
+    int rc = pool_take(c, st, 8ull * kBdWords);
+    if (!rc) rc = pool_take(c, list[0], 4 * lcap);
+    if (!rc) rc = pool_take(c, list[1], 4 * lcap);
+    if (!rc) rc = pool_take(c, flags, (uint64_t)npack * kCcPack); // whole 16-byte groups
+    if (!rc) rc = pool_take(c, offs, 4ull * ((uint64_t)n + 2));
+    if (!rc) rc = pool_take(c, pcnt, 4ull * (npack + 1));
+    if (!rc) rc = pool_take(c, dpart, 4ull * kDegParts);
+    if (rc) { give(); return rc; }
+    unsigned long long *S = (unsigned long long *)st.p;
+    unsigned long long *cnt = (unsigned long long *)c->counters.p;
+    const uint32_t *outdeg = (const uint32_t *)g->outdeg.p;
+    uint8_t *next = (uint8_t *)flags.p;
+    if ((rc = counters_zero(c))) { give(); return rc; }
+    k_ccd_start<<<blocks_for(std::max<uint32_t>(n, kBdWords), kBlock), kBlock, 0, c->stream>>>(S, n, m, init,
+                                                                                                  labels, prev);
+    const uint32_t push_grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((ecap + kPushTile - 1) / kPushTile, 1),
+                                                            4096);
+    const OpCCDev op{labels, prev, next, outdeg, S};
+    int level = 0;
+    bool done = false;
+    // the first batch: the last call's rounds on this graph plus the round
+    // that finds the frontier empty, else 8; then 16, 32, 64
+    int batch = round_hint && *round_hint > 0 ? std::min(*round_hint + 1, 64) : 8;
+    const int unmap = out != nullptr;
+    unsigned long long h[kBdWords];
+    while (!done && !rc) {
+        for (int b = 0; b < batch && !rc; b++) {
+            level++;
+            k_bfsd_begin<<<1, 64, 0, c->stream>>>(S, cnt, level, threshold, dense_thr, coo_ok, unmap, m);
+            if (level == 1) {
+                // the whole vertex set with its m out-edges: dense (or empty
+                // without edges), one work-group per item
+                rc = launch_pull(c, pv, op, cnt, nullptr, nullptr);
+                continue;
+            }
+            // round L reads list[(L+1)&1] (what round L-1 pushed, or the
+            // pack of its flags) and pushes into list[L&1]
+            uint32_t *lin = (uint32_t *)list[(level + 1) & 1].p, *lout = (uint32_t *)list[level & 1].p;
+            // dense after sparse: prevIDs of the pushed list; after the last
+            // round: the labels into the caller's order
+            k_ccd_copy<<<1024, kBlock, 0, c->stream>>>(lin, labels, prev, S, perm, out, n);
+            // sparse after dense: the dense round's flags into a list
+            k_ccd_pack_count<<<npack, kBlock, 0, c->stream>>>(next, n, (uint32_t *)pcnt.p, S);
+            k_bfsd_scan1<<<1, kBlock, 0, c->stream>>>((uint32_t *)pcnt.p, npack, S, BA_PACK, nullptr);
+            k_ccd_pack_write<<<npack, kBlock, 0, c->stream>>>(next, n, (const uint32_t *)pcnt.p, lin, S);
+            // sparse: degree scan (with prevIDs of the list) and push
+            k_bfsd_deg_part<<<kDegParts, kBlock, 0, c->stream>>>(lin, outdeg, S, (uint32_t *)dpart.p, nullptr, 0u,
+                                                                 labels, prev);
+            k_bfsd_scan1<<<1, kBlock, 0, c->stream>>>((uint32_t *)dpart.p, kDegParts, S, BA_PUSH,
+                                                      (uint32_t *)offs.p);
+            k_bfsd_deg_write<<<kDegParts, kBlock, 0, c->stream>>>(lin, outdeg, (const uint32_t *)dpart.p,
+                                                                  (uint32_t *)offs.p, S);
+            k_dpush<<<push_grid, kBlock, 0, c->stream>>>(CcVisit{labels, prev}, (const uint32_t *)g->off.p,
+                                                         (const uint32_t *)g->ids.p, outdeg, S, cnt, lin,
+                                                         (const uint32_t *)offs.p, lout, ecap);
+            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_cc: round kernels");
+            // dense: the live-label pull
+            if (!rc) rc = launch_pull_strided(c, pv, op, cnt);
+        }
+        if (rc) break;
+        static_assert(sizeof h <= kPinBytes, "the state words fit the pinned read-back buffer");
+        if ((rc = read_back(c, h, S, sizeof h))) break;
+        done = h[BD_MODE] == BM_DONE;
+        batch = std::min(std::max(batch, 8) * 2, 64);
+        // a component's least label moves on by an edge at least every
+        // round, so a run still going after n + 2 enqueued rounds has gone
+        // wrong
+        if (!done && level > (int)n + 2) rc = fail(GG_ERR_STATE, "gg_cc: more rounds than vertices");
+    }
+    if (!rc) {
+        if (round_hint) *round_hint = (int)h[BD_LEVELS];
+        if (info) {
             info->iters = (int)h[BD_LEVELS];
             info->edges_scanned = h[BD_SCANNED];
             for (int L = 1; L <= info->iters && L <= 64; L++) info->paths[L - 1] = (int)(long long)h[BD_PATH + L - 1];
