@@ -1193,16 +1193,24 @@ struct BfsdArgs {
 // Start state (app/BFS.C:86-92): Parents = -1 but the root's, the visited
 // bitmap and the first input bitmap hold the root alone, the root's list.
 // Each word has one writer.
-__global__ void k_bfsd_start(BfsdArgs A, uint32_t root, uint32_t *__restrict__ list, uint32_t *__restrict__ fbm0)
+// Level 1 runs no conversion: the start also writes the root list's degree
+// offsets and clears level 1's output bitmap.
+__global__ void k_bfsd_start(BfsdArgs A, uint32_t root, uint32_t *__restrict__ list, uint32_t *__restrict__ fbm0,
+                             uint32_t *__restrict__ fbm1, uint32_t *__restrict__ offs)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < kBdWords)
         A.st[i] = i == BD_DM ? 1ull : i == BD_OUT ? (unsigned long long)A.outdeg[root] : 0ull;
-    if (i == 0) list[0] = root;
+    if (i == 0) {
+        list[0] = root;
+        offs[0] = 0;
+        offs[1] = A.outdeg[root];
+    }
     if (i < A.nwords) {
         const uint32_t b = (root >> 5) == i ? 1u << (root & 31) : 0u;
         A.vis[i] = b;
         fbm0[i] = b;
+        fbm1[i] = 0;
     }
     if (i < A.n) {
         const bool r = i == root;
@@ -1278,45 +1286,6 @@ __global__ void k_bfsd_begin(unsigned long long *__restrict__ st, unsigned long 
     if (level <= 64) st[BD_PATH + level - 1] = (unsigned long long)(long long)path;
 }
 
-// dense input after a push: the list's bits into the input bitmap (cleared
-// by that push's level)
-__global__ void k_bfsd_scatter(const uint32_t *__restrict__ list, uint32_t *__restrict__ fbm,
-                               const unsigned long long *__restrict__ st)
-{
-    if (!(st[BD_ACT] & BA_BM_LIST)) return;
-    const uint64_t k = st[BD_DM];
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (; i < k; i += stride) {
-        const uint32_t v = list[i];
-        atomicOr(&fbm[v >> 5], 1u << (v & 31));
-    }
-}
-
-// sparse input after a pull: the bitmap packed into an ascending list (each
-// work-group kPackWords words, 4 a thread)
-constexpr uint32_t kPackWords = kBlock * 4;
-
-__global__ __launch_bounds__(kBlock) void k_bfsd_pack_count(const uint32_t *__restrict__ fbm, uint32_t nwords,
-                                                            uint32_t *__restrict__ pcnt,
-                                                            const unsigned long long *__restrict__ st)
-{
-    if (!(st[BD_ACT] & BA_PACK)) return;
-    const uint32_t w0 = blockIdx.x * kPackWords + threadIdx.x * 4;
-    uint32_t c = 0;
-    for (int j = 0; j < 4; j++)
-        if (w0 + j < nwords) c += __popc(fbm[w0 + j]);
-    __shared__ unsigned long long s_w[kBlock / 64];
-    const unsigned long long t = wave_sum_u64(c);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long s = 0;
-        for (int k = 0; k < kBlock / 64; k++) s += s_w[k];
-        pcnt[blockIdx.x] = (uint32_t)s;
-    }
-}
-
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane)
 {
     for (int off = 1; off < 64; off <<= 1) {
@@ -1344,102 +1313,252 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t *to
     return before + inc - v;
 }
 
-// in-place exclusive scan of cnt[0, count) in one work-group; cnt[count]
-// gets the total when 'total_at' is set
-__global__ __launch_bounds__(kBlock) void k_bfsd_scan1(uint32_t *__restrict__ cnt, uint32_t count,
-                                                       const unsigned long long *__restrict__ st,
-                                                       unsigned long long gate_on, uint32_t *__restrict__ total_at)
-{
-    if (!(st[BD_ACT] & gate_on)) return;
-    uint32_t carry = 0;
-    for (uint32_t b = 0; b < count; b += kBlock) { // block-uniform bounds
-        const uint32_t i = b + threadIdx.x;
-        const uint32_t v = i < count ? cnt[i] : 0u;
-        uint32_t all;
-        const uint32_t ex = block_excl_scan_u32(v, &all);
-        if (i < count) cnt[i] = carry + ex;
-        carry += all;
-    }
-    if (total_at && threadIdx.x == 0) total_at[st[BD_DM]] = carry;
-}
+// ---------------------------------------------------------------------------
+// A sparse level's input list and the push's degree offsets, in three
+// launches gated on BA_PUSH (k_fl_parts, k_fl_scan, k_fl_write).  The input
+// is the previous level's list (after a push) or its flags (after a dense
+// level, BA_PACK: a bitmap for BFS, a byte per vertex for Components), from
+// which the list is written in vertex order.  Parts: kFlParts slices of a
+// list, or kBlock flag groups (one a thread) of the flags.  offs[i] is the
+// first out-edge slot of list entry i and offs[d_m] the total.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kFlParts = 1024;
 
-__global__ __launch_bounds__(kBlock) void k_bfsd_pack_write(const uint32_t *__restrict__ fbm, uint32_t nwords,
-                                                            const uint32_t *__restrict__ pbase,
-                                                            uint32_t *__restrict__ out,
-                                                            const unsigned long long *__restrict__ st)
+__device__ __forceinline__ uint64_t fl_chunk(uint64_t dm)
 {
-    if (!(st[BD_ACT] & BA_PACK)) return;
-    const uint32_t w0 = blockIdx.x * kPackWords + threadIdx.x * 4;
-    uint32_t word[4], c = 0;
-    for (int j = 0; j < 4; j++) {
-        word[j] = w0 + j < nwords ? fbm[w0 + j] : 0u;
-        c += __popc(word[j]);
-    }
-    uint32_t all;
-    uint32_t pos = pbase[blockIdx.x] + block_excl_scan_u32(c, &all);
-    for (int j = 0; j < 4; j++) {
-        uint32_t x = word[j];
-        while (x) {
-            const uint32_t b = __builtin_ctz(x);
-            out[pos++] = (w0 + j) * 32 + b;
-            x &= x - 1;
-        }
-    }
-}
-
-// The push's degree scan, sized by the frontier on the device: kDegParts
-// parts of whole kBlock tiles each cover a slice of the list; the parts'
-// sums, their scan (k_bfsd_scan1, which also writes the total at
-// offs[d_m]), then each part's exclusive prefixes.  The parts' pass also
-// clears the bitmap a dense next level would take its input in.
-constexpr uint32_t kDegParts = 1024;
-
-__device__ __forceinline__ uint64_t deg_chunk(uint64_t dm)
-{
-    const uint64_t per = (dm + kDegParts - 1) / kDegParts;
+    const uint64_t per = (dm + kFlParts - 1) / kFlParts;
     return (per + kBlock - 1) / kBlock * kBlock;
 }
 
-// With 'cp_dst' set the pass also copies cp_src[v] to cp_dst[v] for every
-// listed v (Components' vertexMap, CC_Vertex_F).
-__global__ __launch_bounds__(kBlock) void k_bfsd_deg_part(const uint32_t *__restrict__ list,
-                                                          const uint32_t *__restrict__ outdeg,
-                                                          const unsigned long long *__restrict__ st,
-                                                          uint32_t *__restrict__ part, uint32_t *__restrict__ fclr,
-                                                          uint32_t nwords, const uint32_t *__restrict__ cp_src,
-                                                          uint32_t *__restrict__ cp_dst)
-{
-    if (!(st[BD_ACT] & BA_PUSH)) return;
-    for (uint32_t w = blockIdx.x * kBlock + threadIdx.x; w < nwords; w += kDegParts * kBlock) fclr[w] = 0;
-    const uint64_t dm = st[BD_DM], ch = deg_chunk(dm);
-    const uint64_t a = (uint64_t)blockIdx.x * ch, b = min(a + ch, dm);
-    unsigned long long s = 0;
-    for (uint64_t i = a + threadIdx.x; i < b; i += kBlock) {
-        const uint32_t v = list[i];
-        s += outdeg[v];
-        if (cp_dst) cp_dst[v] = cp_src[v];
+// 32 vertices a group from a bitmap (bits at and past n are clear)
+struct BitFlags {
+    static constexpr uint32_t kPer = 32;
+    const uint32_t *bm;
+    uint32_t groups; // ceil(n / 32)
+    __device__ __forceinline__ uint32_t mask(uint32_t g) const { return bm[g]; }
+};
+
+// 16 vertices a group from one byte each (nonzero: set); the array is
+// padded to whole groups and the bytes at and past n are ignored
+struct ByteFlags {
+    static constexpr uint32_t kPer = 16;
+    const uint8_t *f;
+    uint32_t n, groups; // ceil(n / 16)
+    __device__ __forceinline__ uint32_t mask(uint32_t g) const
+    {
+        const uint4 q = *(const uint4 *)(f + 16ull * g);
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            // bit 7 of each byte: that byte is nonzero
+            const uint32_t y = (((w[j] & 0x7f7f7f7fu) + 0x7f7f7f7fu) | w[j]) & 0x80808080u;
+            m |= (((y >> 7) & 1u) | ((y >> 14) & 2u) | ((y >> 21) & 4u) | ((y >> 28) & 8u)) << (4 * j);
+        }
+        const uint32_t have = n - 16u * g; // > 0 for g < groups
+        return have < 16 ? m & ((1u << have) - 1u) : m;
     }
-    __shared__ unsigned long long s_w[kBlock / 64];
-    s = wave_sum_u64(s);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int k = 0; k < kBlock / 64; k++) t += s_w[k];
-        part[blockIdx.x] = (uint32_t)t;
+};
+
+// Out-degrees of a flag group's vertices base .. base + kPer - 1 (one
+// 64- or 128-byte line, read whole; the last group element by element up to
+// n), all zero for an empty group.
+template <uint32_t kPer>
+__device__ __forceinline__ void group_degs(uint32_t m, uint32_t base, uint32_t n,
+                                           const uint32_t *__restrict__ outdeg, uint32_t (&deg)[kPer])
+{
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; j++) deg[j] = 0;
+    if (!m) return;
+    if (base + kPer <= n) {
+        const uint4 *p = (const uint4 *)(outdeg + base); // base is a multiple of kPer
+#pragma unroll
+        for (uint32_t q = 0; q < kPer / 4; q++) {
+            const uint4 x = p[q];
+            deg[4 * q] = x.x;
+            deg[4 * q + 1] = x.y;
+            deg[4 * q + 2] = x.z;
+            deg[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; j++)
+            if (base + j < n) deg[j] = outdeg[base + j];
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bfsd_deg_write(const uint32_t *__restrict__ list,
-                                                           const uint32_t *__restrict__ outdeg,
-                                                           const uint32_t *__restrict__ part,
-                                                           uint32_t *__restrict__ offs,
-                                                           const unsigned long long *__restrict__ st)
+// sum of deg[j] over the set bits j of m
+template <uint32_t kPer>
+__device__ __forceinline__ uint32_t masked_sum(uint32_t m, const uint32_t (&deg)[kPer])
 {
-    if (!(st[BD_ACT] & BA_PUSH)) return;
-    const uint64_t dm = st[BD_DM], ch = deg_chunk(dm);
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; j++) s += ((m >> j) & 1u) ? deg[j] : 0u;
+    return s;
+}
+
+// The block's sums of c and d into pc[blockIdx.x], pd[blockIdx.x].
+__device__ __forceinline__ void block_sums2(uint32_t c, uint32_t d, uint32_t *__restrict__ pc,
+                                            uint32_t *__restrict__ pd)
+{
+    __shared__ unsigned long long s_c[kBlock / 64], s_d[kBlock / 64];
+    const unsigned long long wc = wave_sum_u64(c), wd = wave_sum_u64(d);
+    if ((threadIdx.x & 63) == 0) {
+        s_c[threadIdx.x >> 6] = wc;
+        s_d[threadIdx.x >> 6] = wd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tc = 0, td = 0;
+        for (int k = 0; k < kBlock / 64; k++) {
+            tc += s_c[k];
+            td += s_d[k];
+        }
+        pc[blockIdx.x] = (uint32_t)tc;
+        pd[blockIdx.x] = (uint32_t)td;
+    }
+}
+
+// Stage 1.  Besides, outside BA_PUSH: with BA_UNMAP, out[v] = cp_src[perm[v]]
+// (Components' labels back into the caller's order); with BA_BM_LIST (a
+// dense level after a push), the list's bits into 'fbm' (BFS's input
+// bitmap, cleared by that push's level) and prevIDs copied over it (cp_*,
+// Components).  With BA_PUSH: 'fclr' cleared (the bitmap a dense next level
+// takes its input in), and per part the frontier count (flags) and the
+// out-degree sum; over a list, prevIDs copied as well (cp_*).
+template <class Fl>
+__global__ __launch_bounds__(kBlock) void k_fl_parts(Fl fl, const uint32_t *__restrict__ list,
+                                                     const uint32_t *__restrict__ outdeg,
+                                                     const unsigned long long *__restrict__ st,
+                                                     uint32_t *__restrict__ pcnt, uint32_t *__restrict__ pdeg,
+                                                     uint32_t *__restrict__ fbm, uint32_t *__restrict__ fclr,
+                                                     uint32_t nwords, const uint32_t *__restrict__ cp_src,
+                                                     uint32_t *__restrict__ cp_dst,
+                                                     const uint32_t *__restrict__ perm, uint32_t *__restrict__ out,
+                                                     uint32_t n)
+{
+    const unsigned long long act = st[BD_ACT];
+    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    if (act & BA_UNMAP) {
+        if (!out) return;
+        // 8 independent gathers a thread in flight
+        constexpr int kU = 8;
+        for (uint64_t v0 = gid; v0 < n; v0 += stride * kU) {
+            uint32_t p[kU], x[kU];
+#pragma unroll
+            for (int k = 0; k < kU; k++) {
+                const uint64_t v = v0 + k * stride;
+                p[k] = v < n ? perm[v] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < kU; k++) x[k] = cp_src[p[k]];
+#pragma unroll
+            for (int k = 0; k < kU; k++) {
+                const uint64_t v = v0 + k * stride;
+                if (v < n) out[v] = x[k];
+            }
+        }
+        return;
+    }
+    if (act & BA_BM_LIST) {
+        const uint64_t k = st[BD_DM];
+        for (uint64_t i = gid; i < k; i += stride) {
+            const uint32_t v = list[i];
+            if (fbm) atomicOr(&fbm[v >> 5], 1u << (v & 31));
+            if (cp_dst) cp_dst[v] = cp_src[v];
+        }
+        return;
+    }
+    if (!(act & BA_PUSH)) return;
+    if (fclr)
+        for (uint64_t w = gid; w < nwords; w += stride) fclr[w] = 0;
+    uint32_t c = 0, d = 0;
+    if (act & BA_PACK) {
+        if ((uint64_t)blockIdx.x * kBlock >= fl.groups) return; // the whole work-group
+        if (gid < fl.groups) {
+            const uint32_t m = fl.mask((uint32_t)gid);
+            uint32_t deg[Fl::kPer];
+            group_degs<Fl::kPer>(m, (uint32_t)gid * Fl::kPer, n, outdeg, deg);
+            c = __popc(m);
+            d = masked_sum<Fl::kPer>(m, deg);
+        }
+    } else {
+        if (blockIdx.x >= kFlParts) return; // the whole work-group
+        const uint64_t dm = st[BD_DM], ch = fl_chunk(dm);
+        const uint64_t a = (uint64_t)blockIdx.x * ch, b = min(a + ch, dm);
+        for (uint64_t i = a + threadIdx.x; i < b; i += kBlock) {
+            const uint32_t v = list[i];
+            d += outdeg[v];
+            if (cp_dst) cp_dst[v] = cp_src[v];
+        }
+    }
+    block_sums2(c, d, pcnt, pdeg);
+}
+
+// Stage 2 (one work-group): exclusive scans of the parts' counts and
+// degree sums in place; offs[d_m] gets the total out-degree.
+__global__ __launch_bounds__(kBlock) void k_fl_scan(uint32_t *__restrict__ pcnt, uint32_t *__restrict__ pdeg,
+                                                    const unsigned long long *__restrict__ st, uint32_t flag_parts,
+                                                    uint32_t *__restrict__ offs)
+{
+    const unsigned long long act = st[BD_ACT];
+    if (!(act & BA_PUSH)) return;
+    const uint32_t parts = (act & BA_PACK) ? flag_parts : kFlParts;
+    uint32_t cc = 0, cd = 0;
+    for (uint32_t b = 0; b < parts; b += kBlock) { // block-uniform bounds
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t c = i < parts ? pcnt[i] : 0u, d = i < parts ? pdeg[i] : 0u;
+        uint32_t all_c, all_d;
+        const uint32_t ec = block_excl_scan_u32(c, &all_c);
+        const uint32_t ed = block_excl_scan_u32(d, &all_d);
+        if (i < parts) {
+            pcnt[i] = cc + ec;
+            pdeg[i] = cd + ed;
+        }
+        cc += all_c;
+        cd += all_d;
+    }
+    if (threadIdx.x == 0) offs[st[BD_DM]] = cd;
+}
+
+// Stage 3: the list (from flags) and the offsets.
+template <class Fl>
+__global__ __launch_bounds__(kBlock) void k_fl_write(Fl fl, uint32_t *__restrict__ list,
+                                                     const uint32_t *__restrict__ outdeg,
+                                                     const unsigned long long *__restrict__ st,
+                                                     const uint32_t *__restrict__ pcnt,
+                                                     const uint32_t *__restrict__ pdeg, uint32_t *__restrict__ offs,
+                                                     uint32_t n)
+{
+    const unsigned long long act = st[BD_ACT];
+    if (!(act & BA_PUSH)) return;
+    if (act & BA_PACK) {
+        if ((uint64_t)blockIdx.x * kBlock >= fl.groups) return; // the whole work-group
+        const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+        const uint32_t base = g * Fl::kPer;
+        const uint32_t m = g < fl.groups ? fl.mask(g) : 0u;
+        uint32_t deg[Fl::kPer];
+        group_degs<Fl::kPer>(m, base, n, outdeg, deg);
+        uint32_t all;
+        uint32_t pos = pcnt[blockIdx.x] + block_excl_scan_u32(__popc(m), &all);
+        uint32_t dp = pdeg[blockIdx.x] + block_excl_scan_u32(masked_sum<Fl::kPer>(m, deg), &all);
+        if (m) {
+#pragma unroll
+            for (uint32_t j = 0; j < Fl::kPer; j++)
+                if ((m >> j) & 1u) {
+                    list[pos] = base + j;
+                    offs[pos] = dp;
+                    dp += deg[j];
+                    pos++;
+                }
+        }
+        return;
+    }
+    if (blockIdx.x >= kFlParts) return; // the whole work-group
+    const uint64_t dm = st[BD_DM], ch = fl_chunk(dm);
     const uint64_t a = (uint64_t)blockIdx.x * ch, b = min(a + ch, dm);
-    uint32_t carry = part[blockIdx.x];
+    uint32_t carry = pdeg[blockIdx.x];
     for (uint64_t t0 = a; t0 < b; t0 += kBlock) { // block-uniform bounds
         const uint64_t i = t0 + threadIdx.x;
         const uint32_t d = i < b ? outdeg[list[i]] : 0u;
@@ -1448,6 +1567,13 @@ __global__ __launch_bounds__(kBlock) void k_bfsd_deg_write(const uint32_t *__res
         if (i < b) offs[i] = carry + ex;
         carry += all;
     }
+}
+
+// Launch size of the three stages: whichever is larger of the flag parts
+// and the list parts.
+inline uint32_t fl_grid(uint32_t groups)
+{
+    return std::max<uint32_t>((groups + kBlock - 1) / kBlock, kFlParts);
 }
 
 // Sparse level's update of edge (v -> ngh) for BFS.C (BFS_F::updateAtomic,
@@ -1705,7 +1831,7 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
     const uint64_t lcap = (uint64_t)n + 1;
     const uint64_t ecap = (uint64_t)std::max<int64_t>(threshold, 0);
     const uint32_t nwords = (n + 31) / 32;
-    const uint32_t npack = (nwords + kPackWords - 1) / kPackWords;
+    const uint32_t flag_parts = (nwords + kBlock - 1) / kBlock, flg = fl_grid(nwords);
     Buf st, list[2], fbm[2], vis, offs, pcnt, dpart;
     auto give = [&]() {
         Buf *bs[] = {&st, &list[0], &list[1], &fbm[0], &fbm[1], &vis, &offs, &pcnt, &dpart};
@@ -1718,8 +1844,8 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
     if (!rc) rc = pool_take(c, fbm[1], 4ull * (nwords + 1));
     if (!rc) rc = pool_take(c, vis, 4ull * (nwords + 1));
     if (!rc) rc = pool_take(c, offs, 4ull * ((uint64_t)n + 2));
-    if (!rc) rc = pool_take(c, pcnt, 4ull * (npack + 1));
-    if (!rc) rc = pool_take(c, dpart, 4ull * kDegParts);
+    if (!rc) rc = pool_take(c, pcnt, 4ull * flg);
+    if (!rc) rc = pool_take(c, dpart, 4ull * flg);
     if (rc) { give(); return rc; }
     unsigned long long *S = (unsigned long long *)st.p;
     BfsdArgs A{};
@@ -1744,10 +1870,14 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
     }
     if ((rc = counters_zero(c))) { give(); return rc; }
     k_bfsd_start<<<blocks_for(std::max<uint32_t>(n, kBdWords), kBlock), kBlock, 0, c->stream>>>(
-        A, root, (uint32_t *)list[0].p, (uint32_t *)fbm[0].p);
+        A, root, (uint32_t *)list[0].p, (uint32_t *)fbm[0].p, (uint32_t *)fbm[1].p, (uint32_t *)offs.p);
     const uint32_t push_grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((ecap + kPushTile - 1) / kPushTile, 1),
                                                             4096);
-    const uint32_t pull_grid = std::max<uint32_t>(blocks_for(n, kBlock), 1);
+    // the pull walks the vertices grid-stride: a few waves of work-groups
+    // per CU, so a level that skips it (or only maps back) costs little
+    int cus = 0;
+    GG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const uint32_t pull_grid = std::max<uint32_t>(std::min<uint32_t>(blocks_for(n, kBlock), (uint32_t)cus * 16u), 1);
     int level = 0;
     bool done = false;
     // the first batch: the last search's depth on this graph plus the level
@@ -1764,19 +1894,19 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
             uint32_t *fin = (uint32_t *)fbm[in].p, *fout = (uint32_t *)fbm[ou].p;
             k_bfsd_begin<<<1, 64, 0, c->stream>>>(S, A.counters, level, threshold, dense_thr, coo_ok, mb != nullptr,
                                                   0ull);
-            // dense input from the previous push's list
-            k_bfsd_scatter<<<1024, kBlock, 0, c->stream>>>(lin, fin, S);
-            // sparse input from the previous pull's bitmap
-            k_bfsd_pack_count<<<npack, kBlock, 0, c->stream>>>(fin, nwords, (uint32_t *)pcnt.p, S);
-            k_bfsd_scan1<<<1, kBlock, 0, c->stream>>>((uint32_t *)pcnt.p, npack, S, BA_PACK, nullptr);
-            k_bfsd_pack_write<<<npack, kBlock, 0, c->stream>>>(fin, nwords, (const uint32_t *)pcnt.p, lin, S);
-            // sparse: degree scan and push (fout cleared for a dense next level)
-            k_bfsd_deg_part<<<kDegParts, kBlock, 0, c->stream>>>(lin, A.outdeg, S, (uint32_t *)dpart.p, fout,
-                                                                 nwords, nullptr, nullptr);
-            k_bfsd_scan1<<<1, kBlock, 0, c->stream>>>((uint32_t *)dpart.p, kDegParts, S, BA_PUSH,
-                                                      (uint32_t *)offs.p);
-            k_bfsd_deg_write<<<kDegParts, kBlock, 0, c->stream>>>(lin, A.outdeg, (const uint32_t *)dpart.p,
-                                                                  (uint32_t *)offs.p, S);
+            if (level > 1) {
+                // dense input from the previous push's list; sparse input
+                // (from the previous pull's bitmap) and its degree offsets,
+                // fout cleared for a dense next level
+                const BitFlags fl{fin, nwords};
+                k_fl_parts<<<flg, kBlock, 0, c->stream>>>(fl, lin, A.outdeg, S, (uint32_t *)pcnt.p,
+                                                          (uint32_t *)dpart.p, fin, fout, nwords, nullptr, nullptr,
+                                                          nullptr, nullptr, n);
+                k_fl_scan<<<1, kBlock, 0, c->stream>>>((uint32_t *)pcnt.p, (uint32_t *)dpart.p, S, flag_parts,
+                                                       (uint32_t *)offs.p);
+                k_fl_write<<<flg, kBlock, 0, c->stream>>>(fl, lin, A.outdeg, S, (const uint32_t *)pcnt.p,
+                                                          (const uint32_t *)dpart.p, (uint32_t *)offs.p, n);
+            }
             k_dpush<<<push_grid, kBlock, 0, c->stream>>>(BfsVisit{A.vis, A.parents, A.levels, level}, A.off, A.ids,
                                                          A.outdeg, S, A.counters, lin, (const uint32_t *)offs.p,
                                                          lout, ecap);
@@ -1884,112 +2014,6 @@ __global__ void k_ccd_start(unsigned long long *__restrict__ st, uint32_t n, uin
     }
 }
 
-// dense round after a sparse one: prevIDs of the input list.  The round
-// after the last (with a renumbered plan): the labels into the caller's
-// order, out[v] = ids[perm[v]].
-__global__ void k_ccd_copy(const uint32_t *__restrict__ list, const uint32_t *__restrict__ ids,
-                           uint32_t *__restrict__ prev, const unsigned long long *__restrict__ st,
-                           const uint32_t *__restrict__ perm, uint32_t *__restrict__ out, uint32_t n)
-{
-    const unsigned long long act = st[BD_ACT];
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    if (act & BA_UNMAP) {
-        // 8 independent gathers a thread in flight (the grid is small)
-        constexpr int kU = 8;
-        for (uint64_t v0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v0 < n; v0 += stride * kU) {
-            uint32_t p[kU], x[kU];
-#pragma unroll
-            for (int k = 0; k < kU; k++) {
-                const uint64_t v = v0 + k * stride;
-                p[k] = v < n ? perm[v] : 0u;
-            }
-#pragma unroll
-            for (int k = 0; k < kU; k++) x[k] = ids[p[k]];
-#pragma unroll
-            for (int k = 0; k < kU; k++) {
-                const uint64_t v = v0 + k * stride;
-                if (v < n) out[v] = x[k];
-            }
-        }
-        return;
-    }
-    if (!(act & BA_BM_LIST)) return;
-    const uint64_t k = st[BD_DM];
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) {
-        const uint32_t v = list[i];
-        prev[v] = ids[v];
-    }
-}
-
-// sparse round after a dense one: the dense round's output flags (one byte
-// per vertex, 0 or 1) packed into an ascending list, 16 flags a thread;
-// the flag array is padded to whole 16-byte groups, the pad masked here
-constexpr uint32_t kCcPack = kBlock * 16;
-
-// bit 7 of each byte of the result is set where that flag is nonzero; the
-// bytes at and past n are cleared
-__device__ __forceinline__ uint4 cc_flags16(const uint8_t *__restrict__ F, uint32_t n, uint32_t v0)
-{
-    const uint4 q = *(const uint4 *)(F + v0);
-    const uint32_t have = n - v0; // > 0
-    uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const uint32_t x = w[j];
-        uint32_t y = (((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u;
-        const uint32_t lo = 4u * j;
-        if (have <= lo) y = 0u;
-        else if (have < lo + 4) y &= (1u << (8 * (have - lo))) - 1u;
-        w[j] = y;
-    }
-    return make_uint4(w[0], w[1], w[2], w[3]);
-}
-
-__global__ __launch_bounds__(kBlock) void k_ccd_pack_count(const uint8_t *__restrict__ F, uint32_t n,
-                                                           uint32_t *__restrict__ pcnt,
-                                                           const unsigned long long *__restrict__ st)
-{
-    if (!(st[BD_ACT] & BA_PACK)) return;
-    const uint32_t v0 = blockIdx.x * kCcPack + threadIdx.x * 16;
-    uint32_t c = 0;
-    if (v0 < n) {
-        const uint4 q = cc_flags16(F, n, v0);
-        c = __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
-    }
-    __shared__ unsigned long long s_w[kBlock / 64];
-    const unsigned long long t = wave_sum_u64(c);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long a = 0;
-        for (int k = 0; k < kBlock / 64; k++) a += s_w[k];
-        pcnt[blockIdx.x] = (uint32_t)a;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_ccd_pack_write(const uint8_t *__restrict__ F, uint32_t n,
-                                                           const uint32_t *__restrict__ pbase,
-                                                           uint32_t *__restrict__ out,
-                                                           const unsigned long long *__restrict__ st)
-{
-    if (!(st[BD_ACT] & BA_PACK)) return;
-    const uint32_t v0 = blockIdx.x * kCcPack + threadIdx.x * 16;
-    uint4 q = make_uint4(0u, 0u, 0u, 0u);
-    if (v0 < n) q = cc_flags16(F, n, v0);
-    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-    const uint32_t c = __popc(w[0]) + __popc(w[1]) + __popc(w[2]) + __popc(w[3]);
-    uint32_t all;
-    uint32_t pos = pbase[blockIdx.x] + block_excl_scan_u32(c, &all);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        uint32_t x = w[j];
-        while (x) {
-            const uint32_t b = __builtin_ctz(x); // bit 7 of byte b/8
-            out[pos++] = v0 + 4u * j + (b >> 3);
-            x &= x - 1;
-        }
-    }
-}
 } // namespace
 
 int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *prev, gg_run_info *info,
@@ -2006,7 +2030,8 @@ int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *p
     const int coo_ok = g->P && !g->part_vertex;
     const uint64_t lcap = (uint64_t)n + 1;
     const uint64_t ecap = (uint64_t)std::max<int64_t>(threshold, 0);
-    const uint32_t npack = (n + kCcPack - 1) / kCcPack;
+    const uint32_t groups = (n + 15) / 16; // 16-flag groups of the dense rounds' output
+    const uint32_t flag_parts = (groups + kBlock - 1) / kBlock, flg = fl_grid(groups);
     CscView pv;
     GG_TRY(pull_view(g, &pv));
     Buf st, list[2], flags, offs, pcnt, dpart;
@@ -2017,10 +2042,10 @@ int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *p
     int rc = pool_take(c, st, 8ull * kBdWords);
     if (!rc) rc = pool_take(c, list[0], 4 * lcap);
     if (!rc) rc = pool_take(c, list[1], 4 * lcap);
-    if (!rc) rc = pool_take(c, flags, (uint64_t)npack * kCcPack); // whole 16-byte groups
+    if (!rc) rc = pool_take(c, flags, 16ull * groups); // whole 16-byte groups
     if (!rc) rc = pool_take(c, offs, 4ull * ((uint64_t)n + 2));
-    if (!rc) rc = pool_take(c, pcnt, 4ull * (npack + 1));
-    if (!rc) rc = pool_take(c, dpart, 4ull * kDegParts);
+    if (!rc) rc = pool_take(c, pcnt, 4ull * flg);
+    if (!rc) rc = pool_take(c, dpart, 4ull * flg);
     if (rc) { give(); return rc; }
     unsigned long long *S = (unsigned long long *)st.p;
     unsigned long long *cnt = (unsigned long long *)c->counters.p;
@@ -2053,19 +2078,16 @@ int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *p
             // pack of its flags) and pushes into list[L&1]
             uint32_t *lin = (uint32_t *)list[(level + 1) & 1].p, *lout = (uint32_t *)list[level & 1].p;
             // dense after sparse: prevIDs of the pushed list; after the last
-            // round: the labels into the caller's order
-            k_ccd_copy<<<1024, kBlock, 0, c->stream>>>(lin, labels, prev, S, perm, out, n);
-            // sparse after dense: the dense round's flags into a list
-            k_ccd_pack_count<<<npack, kBlock, 0, c->stream>>>(next, n, (uint32_t *)pcnt.p, S);
-            k_bfsd_scan1<<<1, kBlock, 0, c->stream>>>((uint32_t *)pcnt.p, npack, S, BA_PACK, nullptr);
-            k_ccd_pack_write<<<npack, kBlock, 0, c->stream>>>(next, n, (const uint32_t *)pcnt.p, lin, S);
-            // sparse: degree scan (with prevIDs of the list) and push
-            k_bfsd_deg_part<<<kDegParts, kBlock, 0, c->stream>>>(lin, outdeg, S, (uint32_t *)dpart.p, nullptr, 0u,
-                                                                 labels, prev);
-            k_bfsd_scan1<<<1, kBlock, 0, c->stream>>>((uint32_t *)dpart.p, kDegParts, S, BA_PUSH,
-                                                      (uint32_t *)offs.p);
-            k_bfsd_deg_write<<<kDegParts, kBlock, 0, c->stream>>>(lin, outdeg, (const uint32_t *)dpart.p,
-                                                                  (uint32_t *)offs.p, S);
+            // round: the labels into the caller's order; sparse: the input
+            // list (from the dense round's flags after a dense round, with
+            // prevIDs copied over a pushed one) and its degree offsets
+            const ByteFlags fl{next, n, groups};
+            k_fl_parts<<<flg, kBlock, 0, c->stream>>>(fl, lin, outdeg, S, (uint32_t *)pcnt.p, (uint32_t *)dpart.p,
+                                                      nullptr, nullptr, 0u, labels, prev, perm, out, n);
+            k_fl_scan<<<1, kBlock, 0, c->stream>>>((uint32_t *)pcnt.p, (uint32_t *)dpart.p, S, flag_parts,
+                                                   (uint32_t *)offs.p);
+            k_fl_write<<<flg, kBlock, 0, c->stream>>>(fl, lin, outdeg, S, (const uint32_t *)pcnt.p,
+                                                      (const uint32_t *)dpart.p, (uint32_t *)offs.p, n);
             k_dpush<<<push_grid, kBlock, 0, c->stream>>>(CcVisit{labels, prev}, (const uint32_t *)g->off.p,
                                                          (const uint32_t *)g->ids.p, outdeg, S, cnt, lin,
                                                          (const uint32_t *)offs.p, lout, ecap);
