@@ -1531,6 +1531,8 @@ __global__ __launch_bounds__(kBlock) void k_fl_write(Fl fl, uint32_t *__restrict
                                                      const uint32_t *__restrict__ pdeg, uint32_t *__restrict__ offs,
                                                      uint32_t n)
 {
+    constexpr uint32_t kStage = 2048; // entries staged per pass
+    __shared__ uint32_t s_v[kStage], s_o[kStage];
     const unsigned long long act = st[BD_ACT];
     if (!(act & BA_PUSH)) return;
     if (act & BA_PACK) {
@@ -1540,18 +1542,31 @@ __global__ __launch_bounds__(kBlock) void k_fl_write(Fl fl, uint32_t *__restrict
         const uint32_t m = g < fl.groups ? fl.mask(g) : 0u;
         uint32_t deg[Fl::kPer];
         group_degs<Fl::kPer>(m, base, n, outdeg, deg);
-        uint32_t all;
-        uint32_t pos = pcnt[blockIdx.x] + block_excl_scan_u32(__popc(m), &all);
-        uint32_t dp = pdeg[blockIdx.x] + block_excl_scan_u32(masked_sum<Fl::kPer>(m, deg), &all);
-        if (m) {
+        uint32_t total, dall;
+        const uint32_t li0 = block_excl_scan_u32(__popc(m), &total);
+        const uint32_t dp0 = pdeg[blockIdx.x] + block_excl_scan_u32(masked_sum<Fl::kPer>(m, deg), &dall);
+        const uint32_t out0 = pcnt[blockIdx.x];
+        // the work-group's entries are one run of the list: they are placed
+        // in LDS kStage at a time and written out coalesced
+        for (uint32_t p0 = 0; p0 < total; p0 += kStage) { // block-uniform bounds
+            uint32_t li = li0, dp = dp0;
 #pragma unroll
             for (uint32_t j = 0; j < Fl::kPer; j++)
                 if ((m >> j) & 1u) {
-                    list[pos] = base + j;
-                    offs[pos] = dp;
+                    if (li >= p0 && li < p0 + kStage) {
+                        s_v[li - p0] = base + j;
+                        s_o[li - p0] = dp;
+                    }
+                    li++;
                     dp += deg[j];
-                    pos++;
                 }
+            __syncthreads();
+            const uint32_t cnt = min(kStage, total - p0);
+            for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) {
+                list[out0 + p0 + i] = s_v[i];
+                offs[out0 + p0 + i] = s_o[i];
+            }
+            __syncthreads(); // the stage is filled again by the next pass
         }
         return;
     }
@@ -1685,12 +1700,31 @@ __global__ __launch_bounds__(kBlock) void k_bfsd_pull(BfsdArgs A, const uint32_t
     if (act & BA_UNMAP) {
         // parents[v] = inv[pp[perm[v]]] (or -1), levels[v] = lp[perm[v]]: a
         // gather over the caller's order (the reads of the plan-numbered
-        // arrays are random but sit in the MALL; every write is coalesced)
-        for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < A.n; v += gridDim.x * kBlock) {
-            const uint32_t k = A.perm[v];
-            const int32_t p = A.parents[k];
-            A.out_parents[v] = p < 0 ? -1 : (int32_t)A.inv[p];
-            if (A.out_levels) A.out_levels[v] = A.levels[k];
+        // arrays are random but sit in the MALL; every write is coalesced),
+        // kU vertices a thread with their loads in flight together
+        constexpr int kU = 8;
+        const uint32_t stride = gridDim.x * kBlock;
+        for (uint32_t v0 = blockIdx.x * kBlock + threadIdx.x; v0 < A.n; v0 += stride * kU) {
+            uint32_t k[kU];
+            int32_t p[kU];
+#pragma unroll
+            for (int j = 0; j < kU; j++) {
+                const uint64_t v = v0 + (uint64_t)j * stride;
+                k[j] = v < A.n ? A.perm[v] : 0u;
+            }
+#pragma unroll
+            for (int j = 0; j < kU; j++) p[j] = A.parents[k[j]];
+            uint32_t q[kU];
+#pragma unroll
+            for (int j = 0; j < kU; j++) q[j] = A.inv[p[j] < 0 ? 0 : p[j]];
+#pragma unroll
+            for (int j = 0; j < kU; j++) {
+                const uint64_t v = v0 + (uint64_t)j * stride;
+                if (v < A.n) {
+                    A.out_parents[v] = p[j] < 0 ? -1 : (int32_t)q[j];
+                    if (A.out_levels) A.out_levels[v] = A.levels[k[j]];
+                }
+            }
         }
         return;
     }
