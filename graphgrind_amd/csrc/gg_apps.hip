@@ -466,7 +466,8 @@ extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
         }
     }
     Graph t;
-    int rc = planned ? plan_alias(g, t) : GG_OK;
+    int rc = ensure_first_in(g);
+    if (!rc && planned) rc = plan_alias(g, t);
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -474,10 +475,12 @@ extern "C" int gg_cc(gg_graph_t g, uint32_t *labels, gg_run_info *info)
     if (!rc) {
         if (planned) { // Components.C:113 IDs[i] = i: the caller's ids in the new numbering
             // the round after the last maps the labels back
-            rc = cc_device_loop(&t, (const uint32_t *)g->plan.inv.p, (uint32_t *)lp.p, (uint32_t *)prev.p, info,
-                                &g->cc_rounds_hint, (const uint32_t *)g->plan.perm.p, labels);
+            rc = cc_device_loop(&t, (const uint32_t *)g->plan.first_in.p, (const uint32_t *)g->plan.inv.p,
+                                (uint32_t *)lp.p, (uint32_t *)prev.p, info, &g->cc_rounds_hint,
+                                (const uint32_t *)g->plan.perm.p, labels);
         } else {
-            rc = cc_device_loop(g, nullptr, labels, (uint32_t *)prev.p, info, &g->cc_rounds_hint);
+            rc = cc_device_loop(g, (const uint32_t *)g->first_in.p, nullptr, labels, (uint32_t *)prev.p, info,
+                                &g->cc_rounds_hint);
         }
     }
     (void)hipEventRecord(e1, c->stream);

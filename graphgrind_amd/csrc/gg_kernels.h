@@ -435,14 +435,13 @@ __device__ __forceinline__ uint32_t xcc_id()
     return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;
 }
 
-// One work item of the row-block pull, its rows' results into 'st'.
-// Returns false for a long row's chunk (its partial goes to its slot; with
-// l1_part set, the work-group's residual partial is zeroed there).
 template <class Op>
-__device__ __forceinline__ bool rows_item(const Op &op, const uint32_t *__restrict__ toff,
-                                          const uint32_t *__restrict__ tids, const RowWork w,
-                                          typename Op::acc_t *__restrict__ slot_val, LocalStats &st,
-                                          double *__restrict__ l1_part)
+__global__ __launch_bounds__(kBlock) void k_rows(Op op, const uint32_t *__restrict__ toff,
+                                                 const uint32_t *__restrict__ tids,
+                                                 const RowWork *__restrict__ work,
+                                                 typename Op::acc_t *__restrict__ slot_val,
+                                                 unsigned long long *__restrict__ counters,
+                                                 double *__restrict__ l1_part)
 {
     using A = typename Op::acc_t;
     using R = typename row_of<Op>::type;
@@ -451,7 +450,9 @@ __device__ __forceinline__ bool rows_item(const Op &op, const uint32_t *__restri
     __shared__ A s_sum[kRowCap];
     __shared__ uint32_t s_rs[kRowCap + 1];
     __shared__ A s_w[kBlock / 64];
+    if (op.skip()) return;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const RowWork w = work[blockIdx.x];
     uint32_t ne = w.e1 - w.e0;
     if constexpr (has_row_done<Op>::value) {
         // every row of the item done: no gathers; rows finish with the
@@ -524,7 +525,7 @@ __device__ __forceinline__ bool rows_item(const Op &op, const uint32_t *__restri
                 if constexpr (has_aux<Op>::value) l1_part[gridDim.x + blockIdx.x] = 0.0;
             }
         }
-        return false;
+        return;
     }
     const uint32_t nr = w.r1 - w.r0;
     // row starts (relative to e0) in LDS; no per-row state is held in
@@ -571,6 +572,7 @@ __device__ __forceinline__ bool rows_item(const Op &op, const uint32_t *__restri
         }
     }
     // epilogue: the rows' own inputs are loaded now, all before any store
+    LocalStats st;
     R rin[kRPT];
     const uint32_t nrl = nr - 1;
 #pragma unroll
@@ -578,46 +580,13 @@ __device__ __forceinline__ bool rows_item(const Op &op, const uint32_t *__restri
 #pragma unroll
     for (int k = 0; k < kRPT; k++)
         if (tid + k * kBlock < nr) row_store(op, w.r0 + tid + k * kBlock, rin[k], acc[k], st);
-    return true;
-}
-
-template <class Op>
-__global__ __launch_bounds__(kBlock) void k_rows(Op op, const uint32_t *__restrict__ toff,
-                                                 const uint32_t *__restrict__ tids,
-                                                 const RowWork *__restrict__ work,
-                                                 typename Op::acc_t *__restrict__ slot_val,
-                                                 unsigned long long *__restrict__ counters,
-                                                 double *__restrict__ l1_part)
-{
-    if (op.skip()) return;
-    LocalStats st;
-    if (!rows_item(op, toff, tids, work[blockIdx.x], slot_val, st, l1_part)) return;
     constexpr bool kAux = has_aux<Op>::value;
     double aux = 0;
     double l1 = block_stats<kAux>(st, counters, &aux);
-    if (l1_part && threadIdx.x == 0) {
+    if (l1_part && tid == 0) {
         l1_part[blockIdx.x] = l1;
         if constexpr (kAux) l1_part[gridDim.x + blockIdx.x] = aux;
     }
-}
-
-// The row-block pull with a grid of a few work-groups per CU walking the
-// items (item b, b + grid, ...), so a launch that its operator skips costs
-// those work-groups instead of one per item.  No residual partials.
-template <class Op>
-__global__ __launch_bounds__(kBlock) void k_rows_strided(Op op, const uint32_t *__restrict__ toff,
-                                                         const uint32_t *__restrict__ tids,
-                                                         const RowWork *__restrict__ work,
-                                                         typename Op::acc_t *__restrict__ slot_val,
-                                                         unsigned long long *__restrict__ counters, uint32_t items)
-{
-    if (op.skip()) return;
-    LocalStats st;
-    for (uint32_t i = blockIdx.x; i < items; i += gridDim.x) {
-        __syncthreads(); // the previous item's LDS reads are done
-        rows_item(op, toff, tids, work[i], slot_val, st, nullptr);
-    }
-    block_stats(st, counters);
 }
 
 // Reductions of a source-segmented pull: the value type, the identity and
@@ -974,37 +943,6 @@ int launch_pull(Ctx *c, const CscView &v, const Op &op, unsigned long long *coun
     return GG_OK;
 }
 
-// launch_pull for an operator whose skip() may be true on the device: the
-// row-block pull runs with 'waves' times as many work-groups as fit on the
-// device at once, each walking every grid-th item (k_rows_strided), so a
-// skipped launch costs those work-groups rather than one per item, and the
-// dispatcher still evens out the work-groups' durations.  No residual
-// partials.
-template <class Op>
-int launch_pull_strided(Ctx *c, const CscView &v, const Op &op, unsigned long long *counters, int waves = 4)
-{
-    using A = typename Op::acc_t;
-    if (!v.rows) return launch_pull(c, v, op, counters, nullptr, nullptr);
-    RowSched &r = *v.rows;
-    static int per_cu = 0; // resident work-groups per CU (one instantiation, one architecture)
-    if (!per_cu) {
-        int k = 0;
-        GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, k_rows_strided<Op>, kBlock, 0));
-        per_cu = k > 0 ? k : 1;
-    }
-    int cus = 0;
-    GG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    const uint32_t grid =
-        (uint32_t)std::max<uint64_t>(std::min<uint64_t>(r.blocks, (uint64_t)per_cu * cus * std::max(waves, 1)), 1);
-    k_rows_strided<Op><<<grid, kBlock, 0, c->stream>>>(op, v.toff, v.tids, (const RowWork *)r.work.p,
-                                                       (A *)r.slot_val.p, counters, r.blocks);
-    k_rows_fix<Op><<<v.fix_blocks(), kBlock, 0, c->stream>>>(op, (const uint32_t *)r.long_rows.p, r.nlong,
-                                                             (const A *)r.slot_val.p, r.blocks, counters, nullptr,
-                                                             nullptr);
-    GG_HIP(hipGetLastError());
-    return GG_OK;
-}
-
 inline CscView graph_view(Graph *g)
 {
     CscView v;
@@ -1092,14 +1030,16 @@ struct BfsMapBack {
 int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t threshold, int32_t *parents,
                     int32_t *levels, gg_run_info *info, const BfsMapBack *mb = nullptr, int *level_hint = nullptr);
 // Components.C's loop with each round's branch chosen on the device, over
-// g's arrays: labels (and prev, n entries of scratch) start at init[v]
-// (the vertex's id in the caller's numbering; NULL: v) and end at each
-// component's least label.  'round_hint' (nullable) holds the rounds of the
+// g's arrays ('first': the first entry of each of g's in-lists, as
+// ensure_first_in builds): labels (and prev, n entries of scratch) start at
+// init[v] (the vertex's id in the caller's numbering; NULL: v) and end at
+// each component's least label.  'round_hint' (nullable) holds the rounds of the
 // last call on the graph, which size the first batch, and receives this
 // call's.  With 'out' set (a renumbered plan) the round after the last
 // writes out[v] = labels[perm[v]] (the caller's order).
-int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *prev, gg_run_info *info,
-                   int *round_hint = nullptr, const uint32_t *perm = nullptr, uint32_t *out = nullptr);
+int cc_device_loop(Graph *g, const uint32_t *first, const uint32_t *init, uint32_t *labels, uint32_t *prev,
+                   gg_run_info *info, int *round_hint = nullptr, const uint32_t *perm = nullptr,
+                   uint32_t *out = nullptr);
 // The first entry of each in-list of the graph's traversal arrays (the
 // plan's when it has one; 0xffffffff for an empty list), built on first use.
 int ensure_first_in(Graph *g);

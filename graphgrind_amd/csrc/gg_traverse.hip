@@ -1973,48 +1973,21 @@ int bfs_device_loop(Graph *g, uint32_t root, const uint32_t *first, int64_t thre
 // Components.C's Compute() on the device (app/Components.C:101-126).  Each
 // round's edgeMap branch is chosen by k_bfsd_begin from the previous round's
 // striped counters, as for BFS, with no read-back per round.  A dense round
-// is the live-label pull: each row takes the minimum over all its in-
-// neighbours' current labels (a vertex outside the frontier holds a label
-// its neighbours already took, so reading it changes nothing; labels only
-// decrease, so a label read while its owner rewrites it is the old or the
-// new one, both upper bounds of the component's least).  A sparse
-// round pushes the frontier's round-start labels with atomicMin
-// (CC_F::updateAtomic, app/Components.C:37-46); the first lowering of a
-// vertex in the round appends it to the next frontier.  prevIDs holds the
-// round-start labels.  CC_Vertex_F's copy over the frontier
-// (app/Components.C:88-99) runs in three places: in the degree scan of a
-// sparse round, in a list pass before a dense round that follows a sparse
-// one, and in the dense round's own row writes.  The dense round copies
-// every row it lowers, so after it prevIDs equals IDs everywhere.
+// (k_ccd_pull) gives each row the minimum over its in-neighbours' current
+// labels: a vertex outside the frontier holds a label its neighbours
+// already took, so reading it changes nothing, and labels only decrease, so
+// a label read while its owner rewrites it is the old or the new one, both
+// upper bounds of the component's least.  A sparse round pushes the
+// frontier's round-start labels with atomicMin (CC_F::updateAtomic,
+// app/Components.C:37-46); the first lowering of a vertex in the round
+// appends it to the next frontier.  prevIDs holds the round-start labels.
+// CC_Vertex_F's copy over the frontier (app/Components.C:88-99) runs in
+// three places: in the degree scan of a sparse round, in a list pass before
+// a dense round that follows a sparse one, and in the dense round's own row
+// writes.  The dense round copies every row it lowers, so after it prevIDs
+// equals IDs everywhere.
 // ============================================================================
 namespace {
-struct OpCCDev {
-    using acc_t = uint32_t;
-    uint32_t *ids, *prev;
-    uint8_t *next;
-    const uint32_t *outdeg;
-    const unsigned long long *st;
-    __device__ bool skip() const { return !(st[BD_ACT] & BA_PULL); }
-    __device__ uint32_t identity() const { return 0xffffffffu; }
-    __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a < b ? a : b; }
-    __device__ uint32_t gather(uint32_t, uint32_t s) const { return ids[s]; }
-    // label 0 is the least there is: such a row cannot change
-    __device__ bool row_done(uint32_t d) const { return ids[d] == 0u; }
-    __device__ void finish(uint32_t d, uint32_t acc, LocalStats &ls) const
-    {
-        // a row's owner is its only writer in a dense round, so ids[d] is
-        // still the round-start label
-        const bool dec = acc < ids[d];
-        if (dec) {
-            ids[d] = acc;
-            prev[d] = acc;
-            ls.active++;
-            ls.out_edges += outdeg[d];
-        }
-        next[d] = dec;
-    }
-};
-
 // CC_F::updateAtomic on edge (v -> ngh): the source's round-start label
 // into the head with atomicMin; the lowering that finds the head still at
 // its round-start label is its first in the round.
@@ -2031,6 +2004,102 @@ struct CcVisit {
         return false;
     }
 };
+
+// Dense round, bottom-up: each row not already at label 0 takes the
+// minimum over its in-neighbours' current labels, walking its in-list in
+// order and stopping at label 0, the least there is (the minimum is then
+// exact without the rest).  The in-list's first entry comes from a
+// coalesced array; a short rest is read kPullStep entries a step, a rest over
+// kPullShort entries by the whole wave, 64 a step.  With a renumbered plan
+// the lists start with their hottest sources, and those rows run first in
+// the grid-stride order, so most rows stop at their first entry.  A
+// row's owner writes its label (and prevIDs) and flag; the round's
+// statistics and scanned entries go to the counter stripes.
+__global__ __launch_bounds__(kBlock) void k_ccd_pull(const uint32_t *__restrict__ first,
+                                                     const uint32_t *__restrict__ toff,
+                                                     const uint32_t *__restrict__ tids, uint32_t *__restrict__ ids,
+                                                     uint32_t *__restrict__ prev, uint8_t *__restrict__ next,
+                                                     const uint32_t *__restrict__ outdeg,
+                                                     unsigned long long *__restrict__ counters,
+                                                     const unsigned long long *__restrict__ st, uint32_t n)
+{
+    if (!(st[BD_ACT] & BA_PULL)) return;
+    LocalStats ls;
+    unsigned long long scanned = 0;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t d = base + threadIdx.x;
+        uint32_t cur = 0, acc = 0xffffffffu, b = 0, e = 0;
+        bool longl = false;
+        if (d < n) {
+            cur = ids[d];
+            const uint32_t s0 = cur ? first[d] : kNoVertex;
+            if (s0 != kNoVertex) {
+                scanned++;
+                acc = ids[s0];
+                if (acc != 0) {
+                    b = toff[d] + 1;
+                    e = toff[d + 1];
+                    if (e - b > kPullShort) {
+                        longl = true;
+                    } else {
+                        for (uint32_t j = b; j < e && acc != 0; j += kPullStep) {
+                            uint32_t src[kPullStep];
+#pragma unroll
+                            for (int k = 0; k < kPullStep; k++) src[k] = tids[min(j + k, e - 1)];
+                            uint32_t lab[kPullStep];
+#pragma unroll
+                            for (int k = 0; k < kPullStep; k++) lab[k] = ids[src[k]];
+#pragma unroll
+                            for (int k = 0; k < kPullStep; k++) acc = min(acc, lab[k]); // clamped repeats are harmless
+                            scanned += min((uint32_t)kPullStep, e - j);
+                        }
+                    }
+                }
+            }
+        }
+        uint64_t lm = __ballot(longl);
+        while (lm) { // wave-uniform
+            const int L = __builtin_ctzll(lm);
+            lm &= lm - 1;
+            const uint32_t lb = __shfl(b, L, 64), le = __shfl(e, L, 64);
+            uint32_t wmin = 0xffffffffu, cnt = 0;
+            for (uint32_t j0 = lb; j0 < le; j0 += 64) {
+                const uint32_t j = j0 + lane;
+                const uint32_t l = j < le ? ids[tids[j]] : 0xffffffffu;
+                wmin = min(wmin, l);
+                cnt += min(64u, le - j0);
+                if (__ballot(l == 0)) break;
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) wmin = min(wmin, (uint32_t)__shfl_xor((int)wmin, off, 64));
+            if (lane == L) {
+                acc = min(acc, wmin);
+                scanned += cnt;
+            }
+        }
+        if (d < n) {
+            const bool dec = acc < cur;
+            if (dec) {
+                ids[d] = acc;
+                prev[d] = acc;
+                ls.active++;
+                ls.out_edges += outdeg[d];
+            }
+            next[d] = dec;
+        }
+    }
+    __shared__ unsigned long long s_scan[kBlock / 64];
+    const unsigned long long sc = wave_sum_u64(scanned);
+    if (lane == 0) s_scan[threadIdx.x >> 6] = sc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < kBlock / 64; k++) t += s_scan[k];
+        cnt_add(counters, CNT_SCANNED, t);
+    }
+    block_stats(ls, counters);
+}
 
 // Start state (app/Components.C:110-116): IDs = prevIDs = the vertex ids
 // ('init', the caller's ids of a renumbered plan, else the position), the
@@ -2050,8 +2119,8 @@ __global__ void k_ccd_start(unsigned long long *__restrict__ st, uint32_t n, uin
 
 } // namespace
 
-int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *prev, gg_run_info *info,
-                   int *round_hint, const uint32_t *perm, uint32_t *out)
+int cc_device_loop(Graph *g, const uint32_t *first, const uint32_t *init, uint32_t *labels, uint32_t *prev,
+                   gg_run_info *info, int *round_hint, const uint32_t *perm, uint32_t *out)
 {
     Ctx *c = g->ctx;
     const uint32_t n = g->n;
@@ -2066,8 +2135,6 @@ int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *p
     const uint64_t ecap = (uint64_t)std::max<int64_t>(threshold, 0);
     const uint32_t groups = (n + 15) / 16; // 16-flag groups of the dense rounds' output
     const uint32_t flag_parts = (groups + kBlock - 1) / kBlock, flg = fl_grid(groups);
-    CscView pv;
-    GG_TRY(pull_view(g, &pv));
     Buf st, list[2], flags, offs, pcnt, dpart;
     auto give = [&]() {
         Buf *bs[] = {&st, &list[0], &list[1], &flags, &offs, &pcnt, &dpart};
@@ -2090,7 +2157,10 @@ int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *p
                                                                                                   labels, prev);
     const uint32_t push_grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((ecap + kPushTile - 1) / kPushTile, 1),
                                                             4096);
-    const OpCCDev op{labels, prev, next, outdeg, S};
+    int cus = 0;
+    GG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const uint32_t pull_grid = std::max<uint32_t>(std::min<uint32_t>(blocks_for(n, kBlock), (uint32_t)cus * 16u), 1);
+    const uint32_t *toff = (const uint32_t *)g->toff.p, *tids = (const uint32_t *)g->tids.p;
     int level = 0;
     bool done = false;
     // the first batch: the last call's rounds on this graph plus the round
@@ -2101,11 +2171,13 @@ int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *p
     while (!done && !rc) {
         for (int b = 0; b < batch && !rc; b++) {
             level++;
-            k_bfsd_begin<<<1, 64, 0, c->stream>>>(S, cnt, level, threshold, dense_thr, coo_ok, unmap, m);
+            k_bfsd_begin<<<1, 64, 0, c->stream>>>(S, cnt, level, threshold, dense_thr, coo_ok, unmap, 0ull);
             if (level == 1) {
                 // the whole vertex set with its m out-edges: dense (or empty
-                // without edges), one work-group per item
-                rc = launch_pull(c, pv, op, cnt, nullptr, nullptr);
+                // without edges)
+                k_ccd_pull<<<pull_grid, kBlock, 0, c->stream>>>(first, toff, tids, labels, prev, next, outdeg, cnt, S,
+                                                                n);
+                if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_cc: round kernels");
                 continue;
             }
             // round L reads list[(L+1)&1] (what round L-1 pushed, or the
@@ -2125,9 +2197,9 @@ int cc_device_loop(Graph *g, const uint32_t *init, uint32_t *labels, uint32_t *p
             k_dpush<<<push_grid, kBlock, 0, c->stream>>>(CcVisit{labels, prev}, (const uint32_t *)g->off.p,
                                                          (const uint32_t *)g->ids.p, outdeg, S, cnt, lin,
                                                          (const uint32_t *)offs.p, lout, ecap);
-            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_cc: round kernels");
             // dense: the live-label pull
-            if (!rc) rc = launch_pull_strided(c, pv, op, cnt);
+            k_ccd_pull<<<pull_grid, kBlock, 0, c->stream>>>(first, toff, tids, labels, prev, next, outdeg, cnt, S, n);
+            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "gg_cc: round kernels");
         }
         if (rc) break;
         static_assert(sizeof h <= kPinBytes, "the state words fit the pinned read-back buffer");
