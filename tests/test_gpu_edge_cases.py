@@ -92,6 +92,32 @@ def test_hub_in_list_over_many_work_items(ctx):
     _check_all(ctx, off, ids, False, 64)
 
 
+_PATHS = {oracle.PATH_SPARSE: "sparse", oracle.PATH_CSC: "csc", oracle.PATH_COO: "coo"}
+
+
+@pytest.mark.parametrize("plan", [0, 1])
+@pytest.mark.parametrize("root", [0, 5])
+def test_bfs_first_level_either_branch(ctx, plan, root):
+    """Hub 0 with 20,000 out-edges.  From the hub, level 1 is dense (1 +
+    20,000 out-edges over m/20): a pull over the start's root bitmap with no
+    conversion before it.  From a leaf, level 1 is a push and level 2 dense
+    (the hub's out-degree), its input bitmap scattered from the pushed list.
+    Levels, parents and the branch trace against the oracle."""
+    n = 20001
+    src = np.concatenate([np.arange(1, n), np.zeros(n - 1, np.int64)])
+    dst = np.concatenate([np.zeros(n - 1, np.int64), np.arange(1, n)])
+    off, ids = _csr(n, src, dst)
+    print(f"bfs hub graph root={root} plan={plan}", flush=True)
+    _, lv, tr = oracle.Graph(off, ids, symmetric=False, P=64).bfs(root)
+    g = gg.Graph.from_csr(ctx, off, ids).partition(64).reorder(plan)
+    p, l, info = gg.bfs(g, root)
+    assert np.array_equal(l.to_host(), R.check_parents(off, ids, root, p.to_host()))
+    assert info.iters == lv
+    assert info.path_list() == [_PATHS[t.path] for t in tr]
+    assert (info.path_list()[0] == "sparse") == (root != 0)
+    g.close()
+
+
 def test_long_path(ctx):
     """A directed path of 3000 vertices: 2999 BFS levels, 3000 CC rounds."""
     n = 3000
