@@ -235,7 +235,7 @@ static int pagerank_fused(Graph *g, int max_iter, double damping, double eps, vo
             GG_HIP(hipEventRecord(ev[2 * it], c->stream));
             k_xseg_part<SegSum<V>><<<X.grid, kXPartThreads, xhot * sizeof(V), c->stream>>>(
                 op.contrib_in, (V *)X.partial.p, (V *)X.rows.slot_val.p, (const uint32_t *)X.vtids.p,
-                (const RowWork *)X.rows.work.p, meta, n, xhot, done);
+                (const RowWork *)X.rows.work.p, meta, n, xhot, done, nullptr);
             if (X.rows.nlong)
                 k_rows_fix<OpSegPartial<SegSum<V>>><<<blocks_for(X.rows.nlong, kBlock), kBlock, 0, c->stream>>>(
                     sop, (const uint32_t *)X.rows.long_rows.p, X.rows.nlong, (const V *)X.rows.slot_val.p, 0,
@@ -512,6 +512,40 @@ struct OpSpmvY {
 };
 } // namespace gg
 
+namespace gg {
+// SpMV over the source-segmented form of the plan's in-edges: the group
+// partials (each group's items on one XCD, its hottest sources of x in LDS,
+// each term x[s] times the edge's weight on a weighted graph), the long
+// rows' chunks, then the combine, which adds each row's partials in group
+// order and writes y in the caller's numbering.  x is in the plan's
+// numbering.
+template <class V>
+int spmv_xseg(Graph *g, const V *x, V *y)
+{
+    Ctx *c = g->ctx;
+    XSeg &X = g->xseg;
+    const uint32_t n = g->n;
+    const uint32_t xhot = kXHotBytes / sizeof(V);
+    const bool weighted = X.vw.p != nullptr;
+    auto part = weighted ? &k_xseg_part<SegSum<V>, true> : &k_xseg_part<SegSum<V>, false>;
+    GG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(part), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(xhot * sizeof(V))));
+    const uint32_t *meta = (const uint32_t *)X.meta.p;
+    part<<<X.grid, kXPartThreads, xhot * sizeof(V), c->stream>>>(
+        x, (V *)X.partial.p, (V *)X.rows.slot_val.p, (const uint32_t *)X.vtids.p, (const RowWork *)X.rows.work.p,
+        meta, n, xhot, nullptr, (const float *)X.vw.p);
+    if (X.rows.nlong)
+        k_rows_fix<OpSegPartial<SegSum<V>>><<<blocks_for(X.rows.nlong, kBlock), kBlock, 0, c->stream>>>(
+            OpSegPartial<SegSum<V>>{(V *)X.partial.p, nullptr}, (const uint32_t *)X.rows.long_rows.p, X.rows.nlong,
+            (const V *)X.rows.slot_val.p, 0, nullptr, nullptr, nullptr);
+    k_xseg_combine<OpSpmvY<V>><<<X.comb_blocks, kBlock, 0, c->stream>>>(
+        OpSpmvY<V>{x, nullptr, y, (const uint32_t *)g->plan.inv.p}, (const V *)X.partial.p,
+        (const uint32_t *)X.mask.p, (const uint32_t *)X.cursor.p, meta + kXSegs + 1, X.vrows, n, nullptr, nullptr);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+}
+} // namespace gg
+
 extern "C" int gg_spmv(gg_graph_t g, int dtype, const void *x, void *y, gg_run_info *info)
 {
     if (!g || !x || !y) return fail(GG_ERR_INVALID, "gg_spmv: NULL argument");
@@ -525,6 +559,11 @@ extern "C" int gg_spmv(gg_graph_t g, int dtype, const void *x, void *y, gg_run_i
     // (rows of similar in-degree together, hot sources together): x is
     // permuted into the new numbering first and y scattered back per row.
     const bool planned = g->plan.mode != 0 && g->m;
+    // the source-segmented pull, as PageRank.C's loop takes it (sources
+    // below 2^31, and x below 2^31 bytes for the partial pass's buffer loads)
+    const bool xseg = planned && (uint64_t)g->n * (dtype == GG_F64 ? 8ull : 4ull) < (1ull << 31) &&
+                      (g->xcd_segments == 2 || (g->xcd_segments == 1 && g->m >= (1ull << 24)));
+    if (xseg) GG_TRY(ensure_xseg(g));
     const uint32_t *inv = nullptr;
     Buf xp;
     if (planned) {
@@ -545,7 +584,10 @@ extern "C" int gg_spmv(gg_graph_t g, int dtype, const void *x, void *y, gg_run_i
                              : gather_idx(c, (const float *)x, inv, (float *)xp.p, g->n);
         x = xp.p;
     }
-    if (!rc)
+    if (!rc && xseg)
+        rc = dtype == GG_F64 ? spmv_xseg<double>(g, (const double *)x, (double *)y)
+                             : spmv_xseg<float>(g, (const float *)x, (float *)y);
+    else if (!rc)
         rc = dtype == GG_F64
                  ? launch_pull(c, pv, OpSpmvY<double>{(const double *)x, tw, (double *)y, inv}, nullptr, nullptr,
                                nullptr)

@@ -175,6 +175,8 @@ struct XSeg {
     uint32_t grid = 0;                   // partial-pass work-groups: kXSegs x (CUs / kXSegs)
     Buf vtoff, vtids;                    // virtual CSC: offsets [vrows+1], sources [m + kXWaveEdges] (zero-padded;
                                          // bit 31 set on each virtual row's first edge)
+    Buf vw;                              // f32 [m + kXWaveEdges]: the edges' weights in vtids order (weighted
+                                         // graphs; zero-padded)
     Buf mask;                            // uint32[n]: bit k when the row has group-k in-edges
     Buf cursor;                          // uint32[comb_blocks*kXSegs]: group-k virtual rows before block b
     Buf partial;                         // 8 bytes [vrows]

@@ -686,16 +686,19 @@ __device__ __forceinline__ void seg_step(typename P::V &S, uint32_t &f)
 // segmented scan across the wave: fixed orders throughout, so the partials
 // are deterministic.  The partials are plain (cached) stores: stores share
 // the load counter, and waiting for the next item's loads would otherwise
-// wait for the far acknowledgements of streaming stores.  Which work-group serves a group and which XCD it runs
-// on change no value.
-template <class P>
+// wait for the far acknowledgements of streaming stores.  Which work-group
+// serves a group and which XCD it runs on change no value.  kWeighted: each
+// edge's term is the gathered value times the edge's weight from vw (the
+// weights in vtids order), as SPMV_F's update (app/SPMV.C:17-21).
+template <class P, bool kWeighted = false>
 __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const typename P::V *__restrict__ vals,
                                                              typename P::V *__restrict__ partial,
                                                              typename P::V *__restrict__ slot_val,
                                                              const uint32_t *__restrict__ vtids,
                                                              const RowWork *__restrict__ work,
                                                              const uint32_t *__restrict__ item_start, uint32_t n,
-                                                             uint32_t hot, const int *__restrict__ done)
+                                                             uint32_t hot, const int *__restrict__ done,
+                                                             const float *__restrict__ vw)
 {
     static_assert(kXLaneEdges == 16, "a lane reads its edges as four 16-byte vectors");
     using V = typename P::V;
@@ -760,6 +763,19 @@ __global__ __launch_bounds__(kXPartThreads) void k_xseg_part(const typename P::V
         V val[kXLaneEdges];
 #pragma unroll
         for (int j = 0; j < kXLaneEdges; j++) val[j] = gather(src[j], (vmask >> j) & 1u);
+        if constexpr (kWeighted) {
+            // each edge's term is the gathered value times its weight, read
+            // from the weights in the same order as the ids
+            const float4 *w4 = reinterpret_cast<const float4 *>(vw + base); // padded like vtids
+#pragma unroll
+            for (int q = 0; q < kXLaneEdges / 4; q++) {
+                const float4 wq = w4[q];
+                val[4 * q] *= (V)wq.x;
+                val[4 * q + 1] *= (V)wq.y;
+                val[4 * q + 2] *= (V)wq.z;
+                val[4 * q + 3] *= (V)wq.w;
+            }
+        }
         if (w.kind == kRowLong) {
             V acc = P::identity();
 #pragma unroll

@@ -2,7 +2,8 @@
 // pull that gives each XCD one slice of the source vector (see XSeg in
 // gg_internal.h).  Built once per plan on first use:
 //   1. key each in-edge by (source group, row), stably radix-sorted, so each
-//      group's edges are grouped by row with the row's source order kept;
+//      group's edges are grouped by row with the row's source order kept
+//      (and, on a weighted graph, the same sort moves the weights);
 //   2. each run of equal keys is a virtual row: offsets, the row's group mask,
 //      the group boundaries;
 //   3. wave work items over the virtual rows (no item spans two groups),
@@ -22,7 +23,7 @@ namespace gg {
 
 void xseg_free(XSeg &x)
 {
-    Buf *bs[] = {&x.vtoff, &x.vtids, &x.mask, &x.cursor, &x.partial, &x.meta};
+    Buf *bs[] = {&x.vtoff, &x.vtids, &x.vw, &x.mask, &x.cursor, &x.partial, &x.meta};
     for (Buf *b : bs) buf_free(*b);
     rows_free(x.rows);
     x.ok = false;
@@ -155,6 +156,26 @@ int ensure_xseg(Graph *g)
                                       c->stream);
     if (rc) return bail(rc);
     if (e != hipSuccess) return bail(fail(GG_ERR_HIP, std::string("source-segmented pull: sort: ") + hipGetErrorString(e)));
+    if (g->weighted && g->plan.tw.p) {
+        // the weights in the same order: the same stable sort of the same
+        // keys (still in k0), the weights riding along
+        if ((rc = buf_alloc(c, x.vw, 4 * pad_edges))) return bail(rc);
+        if ((rc = fill_bytes(c, (float *)x.vw.p + m, 4ull * kXWaveEdges, 0))) return bail(rc);
+        Buf kout;
+        if ((rc = buf_alloc(c, kout, 8 * m))) return bail(rc);
+        tb = 0;
+        e = rocprim::radix_sort_pairs(nullptr, tb, (const uint64_t *)k0.p, (uint64_t *)kout.p,
+                                      (const float *)g->plan.tw.p, (float *)x.vw.p, (size_t)m, 0, 32 + 3, c->stream);
+        if (e == hipSuccess && !(rc = buf_ensure(c, c->tmp, tb + 16)))
+            e = rocprim::radix_sort_pairs(c->tmp.p, tb, (const uint64_t *)k0.p, (uint64_t *)kout.p,
+                                          (const float *)g->plan.tw.p, (float *)x.vw.p, (size_t)m, 0, 32 + 3,
+                                          c->stream);
+        if (!rc) rc = sync(c); // kout is freed next
+        buf_free(kout);
+        if (rc) return bail(rc);
+        if (e != hipSuccess)
+            return bail(fail(GG_ERR_HIP, std::string("source-segmented pull: weight sort: ") + hipGetErrorString(e)));
+    }
     // 2. virtual rows: runs of equal keys
     {
         auto it = rocprim::make_transform_iterator(rocprim::make_counting_iterator<uint64_t>(0),
