@@ -313,7 +313,7 @@ struct Graph {
     // captured fixed-iteration loops (weighted PageRank forward / adjoint)
     GraphExec fwd_exec, adj_exec;
     // source-segmented form of the plan's twin, and whether the fused
-    // PageRank loop uses it (GG_OPT_XCD_SEGMENTS)
+    // PageRank loop and SpMV use it (GG_OPT_XCD_SEGMENTS)
     XSeg xseg;
     int xcd_segments = 1;
     // workspace of the fused PageRank loop, kept between calls (grow-only)

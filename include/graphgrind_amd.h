@@ -393,13 +393,13 @@ int gg_msbc_forward(gg_graph_t g, int nsrc, const uint32_t *sources, int dtype, 
  * in which the reference's PART96 loop adds them.  Graphs with a renumbered
  * plan run the plan's pull either way. */
 #define GG_OPT_DENSE_COO 2
-/* The fused PageRank loop on a graph with a renumbered plan: the source-
- * segmented pull, in which each XCD gathers from one eighth of the sources
- * (its own L2's slice, the slice's hottest sources cached in each CU's LDS)
- * and a combine pass adds the per-group partial sums of each row in group
- * order, or one pull over the whole twin.  0 never, 1 (default) when
- * m >= 2^24 (below that the source vector fits the L2s anyway), 2 always.
- * Both have a fixed summation order. */
+/* The fused PageRank loop and gg_spmv on a graph with a renumbered plan:
+ * the source-segmented pull, in which each XCD gathers from one eighth of
+ * the sources (its own L2's slice, the slice's hottest sources cached in
+ * each CU's LDS) and a combine pass adds the per-group partial sums of each
+ * row in group order, or one pull over the whole twin.  0 never, 1
+ * (default) when m >= 2^24 (below that the source vector fits the L2s
+ * anyway), 2 always.  Both have a fixed summation order. */
 #define GG_OPT_XCD_SEGMENTS 3
 int gg_graph_set_option(gg_graph_t g, int option, int64_t value);
 int gg_graph_get_option(gg_graph_t g, int option, int64_t *value);
