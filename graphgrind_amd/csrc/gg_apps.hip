@@ -654,6 +654,9 @@ extern "C" int gg_spmv_noalloc(gg_graph_t g, int dtype, const void *x, void *y)
     GG_HIP(hipSetDevice(c->device));
     const uint64_t vbytes = (dtype == GG_F64 ? 8ull : 4ull) * g->n;
     const bool planned = g->plan.mode != 0 && g->m;
+    // the path gg_spmv takes (the same condition), so the result is its own
+    const bool xseg = planned && vbytes < (1ull << 31) &&
+                      (g->xcd_segments == 2 || (g->xcd_segments == 1 && g->m >= (1ull << 24)));
     const float *tw = g->weighted ? (const float *)g->tw.p : nullptr;
     CscView pv;
     const uint32_t *inv = nullptr;
@@ -663,6 +666,11 @@ extern "C" int gg_spmv_noalloc(gg_graph_t g, int dtype, const void *x, void *y)
             return fail(GG_ERR_SIZE, "gg_spmv_noalloc: n * value size = " + std::to_string(vbytes) +
                                          " bytes exceeds the " + std::to_string(kScratchBytes) +
                                          "-byte scratch array; use gg_spmv");
+        // the segmented layout is built by the first segmented pull on the
+        // plan (an allocation): it must exist already
+        if (xseg && !g->xseg.ok)
+            return fail(GG_ERR_STATE, "gg_spmv_noalloc: the plan's source-segmented layout is not built yet "
+                                      "(one gg_spmv call builds it)");
         GG_TRY(scratch_for(c->device, &xp));
         pv = plan_view(g);
         if (g->weighted) tw = (const float *)g->plan.tw.p;
@@ -681,7 +689,10 @@ extern "C" int gg_spmv_noalloc(gg_graph_t g, int dtype, const void *x, void *y)
                              : gather_idx(c, (const float *)x, inv, (float *)xp, g->n);
         x = xp;
     }
-    if (!rc)
+    if (!rc && xseg)
+        rc = dtype == GG_F64 ? spmv_xseg<double>(g, (const double *)x, (double *)y)
+                             : spmv_xseg<float>(g, (const float *)x, (float *)y);
+    else if (!rc)
         rc = dtype == GG_F64
                  ? launch_pull(c, pv, OpSpmvY<double>{(const double *)x, tw, (double *)y, inv}, nullptr, nullptr,
                                nullptr)

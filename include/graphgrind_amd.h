@@ -307,8 +307,9 @@ int gg_spmv(gg_graph_t g, int dtype, const void *x_dev, void *y_dev, gg_run_info
  * 64 MiB device array the library holds per GPU until the process ends, so
  * n * sizeof(value) must be at most 64 MiB with a plan (GG_ERR_SIZE
  * otherwise: R-MAT scale 24 fits in FP32, scale 23 in FP64).  Without a plan
- * the graph's row-block schedule must exist (one gg_spmv call builds it;
- * GG_ERR_STATE otherwise).  Asynchronous on the context's stream: no
+ * the graph's row-block schedule must exist, and where gg_spmv takes the
+ * source-segmented pull (GG_OPT_XCD_SEGMENTS) its layout must (one gg_spmv
+ * call builds either; GG_ERR_STATE otherwise).  Asynchronous on the context's stream: no
  * synchronisation and no timing.  Calls that overlap on one GPU share the
  * array and race. */
 int gg_spmv_noalloc(gg_graph_t g, int dtype, const void *x_dev, void *y_dev);

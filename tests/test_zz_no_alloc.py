@@ -1,9 +1,10 @@
 """gg_spmv_noalloc (the entry point that allocates nothing) against gg_spmv:
 the same kernels, so the results must be bitwise equal.  Two calls in a row,
 the second on a smaller graph (the shared scratch array keeps the first
-call's contents), in FP32 and FP64, with and without a renumbered plan; and
-the documented refusals (too large for the scratch array, schedule not
-built) come back as status codes before any kernel runs."""
+call's contents), in FP32 and FP64, with and without a renumbered plan, and
+on a plan that takes the source-segmented pull; and the documented refusals
+(too large for the scratch array, schedule or segmented layout not built)
+come back as status codes before any kernel runs."""
 import ctypes as C
 
 import numpy as np
@@ -50,6 +51,26 @@ def test_noalloc_matches_spmv_two_calls(ctx, dt, plan):
     print(f"no-alloc spmv {np.dtype(dt).name} plan={plan}", flush=True)
     _case(ctx, 18, 3, dt, plan)  # first call
     _case(ctx, 14, 4, dt, plan)  # second call, smaller graph
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_noalloc_segmented_pull(ctx, dt):
+    """On a plan whose gg_spmv takes the source-segmented pull (forced here
+    with GG_OPT_XCD_SEGMENTS = 2), the entry point takes it too: refused
+    until one gg_spmv call has built the layout, then equal to gg_spmv bit
+    for bit, on two graphs in a row, the second smaller."""
+    for scale, seed in ((16, 5), (13, 6)):
+        print(f"no-alloc segmented spmv {np.dtype(dt).name} scale={scale}", flush=True)
+        g = gg.Graph.rmat(ctx, scale, 16, seed, weighted=True)
+        g.reorder(1)
+        g.set_option(3, 2)  # GG_OPT_XCD_SEGMENTS: always
+        x = gg.DeviceArray.from_host(ctx, np.random.default_rng(seed).uniform(-1, 1, g.n).astype(dt))
+        y = gg.DeviceArray(ctx, g.n, dt).fill(7.0)
+        assert _noalloc(g, dt, x, y) == -7  # GG_ERR_STATE: no layout yet
+        want, _ = gg.spmv(g, x)
+        assert _noalloc(g, dt, x, y) == 0, gg.lib().gg_last_error().decode()
+        assert np.array_equal(y.to_host(), want.to_host()), (scale, np.dtype(dt).name)
+        g.close()
 
 
 def test_noalloc_refusals(ctx):
