@@ -262,10 +262,19 @@ struct OpPathSum {
     const uint8_t *vis;
     uint8_t *next;
     const uint32_t *outdeg;
+    const uint32_t *bm; // the frontier's bitmap (unread when 'bit')
+    int bit;
     __device__ bool skip() const { return false; }
     __device__ V identity() const { return path_none_value<V>(); }
     __device__ V combine(V x, V y) const { return path_none(x) ? y : path_none(y) ? x : x + y; }
-    __device__ V gather(uint32_t, uint32_t s) const { return gate[s]; }
+    // the frontier bit first (a 2 MiB bitmap that stays in L2): only the
+    // frontier's sources read their gated value
+    __device__ V gather(uint32_t, uint32_t s) const
+    {
+        return (bit || bit_test(bm, s)) ? gate[s] : path_none_value<V>();
+    }
+    // a visited row cannot change: a work item of visited rows only skips
+    __device__ bool row_done(uint32_t d) const { return vis[d] != 0; }
     __device__ void finish(uint32_t d, V acc, LocalStats &st) const
     {
         bool emit = false;
@@ -954,13 +963,13 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
                 k_pathsum_gate<float><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const float *)op->a, F, bit,
                                                                                        (float *)tmpv.p, n);
                 rc = launch_pull(c, pv, OpPathSum<float>{(const float *)tmpv.p, (float *)op->a,
-                                                         (const uint8_t *)op->b, next, outdeg},
+                                                         (const uint8_t *)op->b, next, outdeg, bm, bit},
                                  cnt, nullptr, nullptr);
             } else {
                 k_pathsum_gate<double><<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(
                     (const double *)op->a, F, bit, (double *)tmpv.p, n);
                 rc = launch_pull(c, pv, OpPathSum<double>{(const double *)tmpv.p, (double *)op->a,
-                                                          (const uint8_t *)op->b, next, outdeg},
+                                                          (const uint8_t *)op->b, next, outdeg, bm, bit},
                                  cnt, nullptr, nullptr);
             }
             break;
