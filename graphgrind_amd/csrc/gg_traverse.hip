@@ -153,10 +153,17 @@ struct OpCCMin {
     const uint32_t *prev;
     uint8_t *next;
     const uint32_t *outdeg;
+    const uint32_t *bm; // the frontier's bitmap (unread when 'bit')
+    int bit;
     __device__ bool skip() const { return false; }
     __device__ uint32_t identity() const { return 0xffffffffu; }
     __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a < b ? a : b; }
-    __device__ uint32_t gather(uint32_t, uint32_t s) const { return g[s]; }
+    // the frontier bit first (a bitmap that stays in L2): only the
+    // frontier's sources read their label
+    __device__ uint32_t gather(uint32_t, uint32_t s) const
+    {
+        return (bit || bit_test(bm, s)) ? g[s] : 0xffffffffu;
+    }
     // label 0 is the least there is: such a row cannot change
     __device__ bool row_done(uint32_t d) const { return ids[d] == 0u; }
     __device__ void finish(uint32_t d, uint32_t acc, LocalStats &st) const
@@ -309,11 +316,16 @@ struct OpSPRelax {
     int32_t *vis;
     uint8_t *next;
     const uint32_t *outdeg;
+    const uint32_t *bm; // the frontier's bitmap (unread when 'bit')
+    int bit;
     __device__ bool skip() const { return false; }
     __device__ int32_t identity() const { return INT32_MAX; }
     __device__ int32_t combine(int32_t a, int32_t b) const { return a < b ? a : b; }
+    // the frontier bit first (a bitmap that stays in L2): only the
+    // frontier's sources read their distance and the edge's length
     __device__ int32_t gather(uint32_t e, uint32_t s) const
     {
+        if (!bit && !bit_test(bm, s)) return INT32_MAX;
         const int32_t g = gate[s];
         const int32_t w = tw ? (int32_t)__builtin_nontemporal_load(&tw[e]) : 1;
         return g == INT32_MIN ? INT32_MAX : g + w;
@@ -934,7 +946,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
             k_cc_gate<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>((const uint32_t *)op->b, F, bit,
                                                                         (uint32_t *)tmpv.p, n);
             rc = launch_pull(c, pv, OpCCMin{(const uint32_t *)tmpv.p, (uint32_t *)op->a,
-                                           (const uint32_t *)op->b, next, outdeg},
+                                           (const uint32_t *)op->b, next, outdeg, bm, bit},
                              cnt, nullptr, nullptr);
             break;
         }
@@ -980,7 +992,7 @@ int edgemap_core(Graph *g, Frontier *in, const gg_op_t *op, int64_t threshold, i
                                                                         (int32_t *)tmpv.p, n);
             const float *tw = g->weighted ? tw_v : nullptr;
             rc = launch_pull(c, pv, OpSPRelax{(const int32_t *)tmpv.p, tw, (int32_t *)op->a, (int32_t *)op->b, next,
-                                              outdeg},
+                                              outdeg, bm, bit},
                              cnt, nullptr, nullptr);
             break;
         }
