@@ -398,6 +398,24 @@ __device__ __forceinline__ uint32_t upper_idx(const uint32_t *a, uint32_t lo, ui
     return lo;
 }
 
+// The same search by a whole wave (every lane calls it with the same
+// arguments; a[lo] <= e): 64 probes a step, so a list of 2^22 entries takes
+// four dependent loads instead of 22.
+__device__ __forceinline__ uint32_t wave_upper_idx(const uint32_t *a, uint32_t lo, uint32_t hi, uint64_t e)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t idx = lo + lane * step;
+        const bool ok = idx < hi && (uint64_t)a[idx] <= e;
+        const uint64_t m = __ballot(ok); // lane 0 (idx = lo) is always set
+        const uint32_t last = 63u - (uint32_t)__builtin_clzll(m);
+        lo += last * step;
+        hi = min(lo + step, hi);
+    }
+    return lo;
+}
+
 // One update of edge (v -> ngh) by the functor of A.kind (the apps'
 // updateAtomic: BFS_F, CC_F, PR_F, SPMV_F, BF_F, BC_F), w[pos] its weight;
 // returns whether ngh joins the output frontier.  Shared by the sparse push
@@ -477,9 +495,14 @@ __global__ __launch_bounds__(kBlock) void k_push(PushArgs A)
     const uint64_t t0 = (uint64_t)blockIdx.x * kPushTile;
     if (t0 >= total) return; // whole work-group: no barrier is skipped by part of it
     const uint64_t t1 = min(t0 + (uint64_t)kPushTile, total);
-    if (threadIdx.x == 0) {
-        s_lo = upper_idx(A.offs, 0, A.k, t0);
-        s_hi = upper_idx(A.offs, 0, A.k, t1 - 1) + 1;
+    // the tile's first and last entries: two waves search at once
+    const uint32_t wv = threadIdx.x >> 6;
+    if (wv == 0) {
+        const uint32_t r = wave_upper_idx(A.offs, 0, A.k, t0);
+        if (threadIdx.x == 0) s_lo = r;
+    } else if (wv == 1) {
+        const uint32_t r = wave_upper_idx(A.offs, 0, A.k, t1 - 1) + 1;
+        if (threadIdx.x == 64) s_hi = r;
     }
     __syncthreads();
     const uint32_t lo = s_lo, hi = s_hi;
@@ -1655,9 +1678,14 @@ __global__ __launch_bounds__(kBlock) void k_dpush(U upd, const uint32_t *__restr
     const uint64_t total = min(cap, (uint64_t)offs[k]);
     for (uint64_t t0 = (uint64_t)blockIdx.x * kPushTile; t0 < total; t0 += (uint64_t)gridDim.x * kPushTile) {
         const uint64_t t1 = min(t0 + (uint64_t)kPushTile, total);
-        if (threadIdx.x == 0) {
-            s_lo = upper_idx(offs, 0, k, t0);
-            s_hi = upper_idx(offs, 0, k, t1 - 1) + 1;
+        // the tile's first and last entries: two waves search at once
+        const uint32_t wv = threadIdx.x >> 6;
+        if (wv == 0) {
+            const uint32_t r = wave_upper_idx(offs, 0, k, t0);
+            if (threadIdx.x == 0) s_lo = r;
+        } else if (wv == 1) {
+            const uint32_t r = wave_upper_idx(offs, 0, k, t1 - 1) + 1;
+            if (threadIdx.x == 64) s_hi = r;
         }
         __syncthreads();
         const uint32_t lo = s_lo, hi = s_hi;
