@@ -84,6 +84,15 @@ def test_segmented_pagerank_tiny(ctx, edges):
     a, info = _run(g, np.float64, 2)
     assert info.iters == it and R.rel_l1(a, want) < 1e-12
     g.close()
+    # the segmented SpMV on the same graph, weighted
+    w = (np.arange(len(ids), dtype=np.float32) + 1) / 4
+    x = np.arange(1, n + 1, dtype=np.float64)
+    g = gg.Graph.from_csr(ctx, off, ids, w=w).partition(1)
+    g.reorder(1)
+    g.set_option(OPT_XCD, 2)
+    y = gg.spmv(g, gg.DeviceArray.from_host(ctx, x))[0].to_host()
+    assert R.rel_l1(y, oracle.spmv_csr(off, ids, w, x)) < 1e-12
+    g.close()
 
 
 @pytest.mark.parametrize("weighted", [True, False])
