@@ -22,10 +22,17 @@ struct OpMSBFS {
     const uint32_t *fin;
     uint32_t *fout;
     uint32_t *seen;
-    int32_t *levels; // [nsrc][n]
+    int32_t *levels; // vmajor: [n][nsrc] (a vertex's levels in one run), else [nsrc][n]
     uint32_t n;
     int32_t level;
-    __device__ bool skip() const { return false; }
+    uint32_t nsrc;
+    int vmajor;
+    const int *done; // set once a level gains no bit: later levels return at once
+    uint32_t full;   // every source's bit
+    __device__ bool skip() const { return *done != 0; }
+    // a vertex every source has reached gains nothing: the row-block pull
+    // skips the gathers of a work item whose rows all have
+    __device__ bool row_done(uint32_t v) const { return seen[v] == full; }
     __device__ uint32_t identity() const { return 0u; }
     __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a | b; }
     __device__ uint32_t gather(uint32_t, uint32_t s) const { return fin[s]; }
@@ -44,7 +51,7 @@ struct OpMSBFS {
             while (b) {
                 const int k = __builtin_ctz(b);
                 b &= b - 1;
-                levels[(uint64_t)k * n + v] = level;
+                levels[vmajor ? (uint64_t)v * nsrc + k : (uint64_t)k * n + v] = level;
             }
         }
     }
@@ -53,7 +60,7 @@ struct OpMSBFS {
 // perm (may be NULL): the renumbered plan's new id of each caller's vertex
 __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_t n, uint32_t *__restrict__ fin,
                              uint32_t *__restrict__ seen, int32_t *__restrict__ levels,
-                             const uint32_t *__restrict__ perm)
+                             const uint32_t *__restrict__ perm, int vmajor)
 {
     // one thread per source; sources may repeat, so bits are ORed atomically
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -61,20 +68,29 @@ __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_
     const uint32_t v = perm ? perm[src[k]] : src[k];
     atomicOr(&fin[v], 1u << k);
     atomicOr(&seen[v], 1u << k);
-    levels[(uint64_t)k * n + v] = 0;
+    levels[vmajor ? (uint64_t)v * nsrc + k : (uint64_t)k * n + v] = 0;
 }
 
-// levels[k][v] = lp[k][perm[v]]: per-source levels from the plan's
-// numbering into the caller's
-__global__ void k_msbfs_unmap(const int32_t *__restrict__ lp, const uint32_t *__restrict__ perm, int nsrc,
-                              uint32_t n, int32_t *__restrict__ levels)
+// After each level, one wave: the vertices that gained a bit (the counter
+// stripes, one per lane, cleared for the next level).  None: flags[0] = 1
+// and flags[1] = this level, which ends the loop (every later level's
+// kernels return at once).
+__global__ __launch_bounds__(64) void k_msbfs_check(unsigned long long *__restrict__ counters, int level,
+                                                    int *__restrict__ flags)
 {
-    const uint64_t total = (uint64_t)nsrc * n;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const uint64_t k = i / n;
-        const uint32_t v = (uint32_t)(i - k * n);
-        levels[i] = lp[k * n + perm[v]];
+    if (flags[0]) return;
+    unsigned long long a = 0;
+    if (threadIdx.x < kStripes) {
+        unsigned long long *st = counters + kCntStripeBase + threadIdx.x * kStripeStride;
+        a = st[CNT_ACTIVE];
+        st[CNT_ACTIVE] = 0;
+        st[CNT_OUT] = 0;
+        st[CNT_SCANNED] = 0;
+    }
+    a = wave_sum_u64(a);
+    if (threadIdx.x == 0 && a == 0) {
+        flags[0] = 1;
+        flags[1] = level;
     }
 }
 
@@ -116,8 +132,9 @@ __global__ __launch_bounds__(kBlock) void k_msbc_paths(const uint32_t *__restric
                                                        const uint32_t *__restrict__ tids,
                                                        const uint32_t *__restrict__ fin,
                                                        const uint32_t *__restrict__ fout, V *__restrict__ P,
-                                                       uint32_t n)
+                                                       uint32_t n, const int *__restrict__ done)
 {
+    if (*done) return;
     const uint32_t k = threadIdx.x & 31;
     const uint64_t h0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
     const uint64_t nh = ((uint64_t)gridDim.x * blockDim.x) >> 5;
@@ -138,8 +155,10 @@ __global__ __launch_bounds__(kBlock) void k_msbc_hub_items(const uint32_t *__res
                                                            const uint32_t *__restrict__ tids,
                                                            const uint32_t *__restrict__ fin,
                                                            const uint32_t *__restrict__ fout,
-                                                           const V *__restrict__ P, V *__restrict__ slot)
+                                                           const V *__restrict__ P, V *__restrict__ slot,
+                                                           const int *__restrict__ done)
 {
+    if (*done) return;
     const uint32_t k = threadIdx.x & 31;
     const uint64_t h0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
     const uint64_t nh = ((uint64_t)gridDim.x * blockDim.x) >> 5;
@@ -153,10 +172,10 @@ __global__ __launch_bounds__(kBlock) void k_msbc_hub_items(const uint32_t *__res
 
 template <class V>
 __global__ void k_msbc_hub_fix(const uint32_t *__restrict__ hubs, uint32_t nhubs, const uint32_t *__restrict__ fout,
-                               const V *__restrict__ slot, V *__restrict__ P)
+                               const V *__restrict__ slot, V *__restrict__ P, const int *__restrict__ done)
 {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)nhubs * 32) return;
+    if (*done || t >= (uint64_t)nhubs * 32) return;
     const uint32_t h = (uint32_t)(t >> 5), k = (uint32_t)(t & 31);
     const uint32_t v = hubs[3 * h], i0 = hubs[3 * h + 1], i1 = hubs[3 * h + 2];
     if (!((fout[v] >> k) & 1u)) return;
@@ -176,18 +195,81 @@ __global__ void k_msbc_init(const uint32_t *__restrict__ src, int nsrc, const ui
     P[(uint64_t)v * 32 + k] = V(1);
 }
 
-// paths[k][v] = P[perm[v]*32 + k] where source k reached v, else 0
-template <class V>
-__global__ void k_msbc_out(const V *__restrict__ P, const int32_t *__restrict__ lv, const uint32_t *__restrict__ perm,
-                           int nsrc, uint32_t n, V *__restrict__ paths)
+// The output of a run whose levels are vertex-major in the traversal's
+// numbering (lp[w*nsrc + k], set where seen[w] holds bit k): levels[k][v]
+// (-1 unreached) and, with P, paths[k][v] (0 unreached) in the caller's
+// numbering, w = perm[v] (v without perm).  A work-group takes kOutTile
+// vertices: it reads each one's nsrc levels and 32 counts as runs (16-byte
+// loads for 32 sources, all of a thread's loads issued together),
+// transposes them in LDS and writes each source's kOutTile entries
+// coalesced.  kLv / kPa: levels / paths wanted (LDS only for those).
+constexpr uint32_t kOutTile = 128;
+
+template <class T>
+__device__ __forceinline__ void out_stage(T (*s)[kOutTile + 1], const T *__restrict__ src, const uint32_t *s_w,
+                                          const uint32_t *s_s, uint32_t stride, T none)
 {
-    const uint64_t total = (uint64_t)nsrc * n;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const uint64_t k = i / n;
-        const uint32_t v = (uint32_t)(i - k * n);
-        const uint32_t w = perm ? perm[v] : v;
-        paths[i] = lv[k * n + w] >= 0 ? P[(uint64_t)w * 32 + k] : V(0);
+    // 32 values of a vertex as 32 / E loads of 16 bytes
+    constexpr uint32_t E = 16 / sizeof(T), Q = 32 / E, R = kOutTile * Q / kBlock;
+    struct alignas(16) Vec {
+        T x[E];
+    };
+    Vec v[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; r++) {
+        const uint32_t j = threadIdx.x + r * kBlock, vi = j / Q;
+        if (s_s[vi]) v[r] = *(const Vec *)(src + (uint64_t)s_w[vi] * stride + (j % Q) * E);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < R; r++) {
+        const uint32_t j = threadIdx.x + r * kBlock, vi = j / Q, k0 = (j % Q) * E;
+#pragma unroll
+        for (uint32_t i = 0; i < E; i++) s[k0 + i][vi] = (s_s[vi] >> (k0 + i)) & 1u ? v[r].x[i] : none;
+    }
+}
+
+template <class V, bool kLv, bool kPa>
+__global__ __launch_bounds__(kBlock) void k_msbfs_out(const int32_t *__restrict__ lp,
+                                                      const uint32_t *__restrict__ seen, const V *__restrict__ P,
+                                                      const uint32_t *__restrict__ perm, uint32_t nsrc, uint32_t n,
+                                                      int32_t *__restrict__ levels, V *__restrict__ paths)
+{
+    static_assert(kBlock % kOutTile == 0, "whole tiles of threads");
+    __shared__ int32_t s_l[kLv ? 32 : 1][kOutTile + 1];
+    __shared__ V s_p[kPa ? 32 : 1][kOutTile + 1];
+    __shared__ uint32_t s_w[kOutTile], s_s[kOutTile];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t v0 = (uint64_t)blockIdx.x * kOutTile;
+    if (tid < kOutTile) {
+        const uint64_t v = v0 + tid;
+        uint32_t w = 0, sn = 0; // past n: no bit, nothing read
+        if (v < n) {
+            w = perm ? perm[v] : (uint32_t)v;
+            sn = seen[w];
+        }
+        s_w[tid] = w;
+        s_s[tid] = sn;
+    }
+    __syncthreads();
+    if (nsrc == 32) {
+        if constexpr (kLv) out_stage<int32_t>(s_l, lp, s_w, s_s, 32, -1);
+        if constexpr (kPa) out_stage<V>(s_p, P, s_w, s_s, 32, V(0));
+    } else {
+        for (uint32_t j = tid; j < kOutTile * nsrc; j += kBlock) {
+            const uint32_t vi = j / nsrc, k = j - vi * nsrc;
+            const uint64_t w = s_w[vi];
+            const bool hit = (s_s[vi] >> k) & 1u;
+            if constexpr (kLv) s_l[k][vi] = hit ? lp[w * nsrc + k] : -1;
+            if constexpr (kPa) s_p[k][vi] = hit ? P[w * 32 + k] : V(0);
+        }
+    }
+    __syncthreads();
+    const uint32_t vi = tid % kOutTile;
+    const uint64_t v = v0 + vi;
+    if (v >= n) return;
+    for (uint32_t k = tid / kOutTile; k < nsrc; k += kBlock / kOutTile) {
+        if constexpr (kLv) levels[(uint64_t)k * n + v] = s_l[k][vi];
+        if constexpr (kPa) paths[(uint64_t)k * n + v] = s_p[k][vi];
     }
 }
 
@@ -208,8 +290,9 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     if (planned) view = plan_view(g);
     else GG_TRY(pull_view(g, &view));
     const uint32_t *perm = planned ? (const uint32_t *)g->plan.perm.p : nullptr;
-    Buf F[2], seen, src, lp, pb, hslot;
+    Buf F[2], seen, src, lp, pb, hslot, fb;
     int rc = pool_take(c, F[0], 4ull * n);
+    if (!rc) rc = pool_take(c, fb, 64);
     if (!rc) rc = pool_take(c, F[1], 4ull * n);
     if (!rc) rc = pool_take(c, seen, 4ull * n);
     if (!rc) rc = pool_take(c, src, 4ull * nsrc);
@@ -219,7 +302,11 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     HubChunks *hub = planned ? &g->plan.hubs : &g->hubs;
     if (!rc && paths) rc = ensure_hub_chunks(c, view.toff, n, *hub);
     if (!rc && paths && hub->nitems) rc = pool_take(c, hslot, sizeof(V) * 32ull * hub->nitems);
-    int32_t *lv = (planned || paths) ? (int32_t *)lp.p : levels;
+    // levels in the traversal's numbering: with a plan or path counts,
+    // vertex-major in lp (a vertex's levels in one run, mapped out by
+    // k_msbfs_out with the counts); else straight into the caller's array
+    const int vmajor = planned || paths;
+    int32_t *lv = vmajor ? (int32_t *)lp.p : levels;
     V *P = paths ? (V *)pb.p : nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
@@ -229,56 +316,70 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
         (void)hipEventRecord(e0, c->stream);
         rc = fill_u32(c, F[0].p, n, 0);
         if (!rc) rc = fill_u32(c, seen.p, n, 0);
-        if (!rc) rc = gg_fill(abi(c), lv, GG_I32, (uint64_t)nsrc * n, -1.0);
+        if (!rc && !vmajor) rc = gg_fill(abi(c), lv, GG_I32, (uint64_t)nsrc * n, -1.0);
         if (!rc && hipMemcpyAsync(src.p, sources, 4ull * nsrc, hipMemcpyHostToDevice, c->stream) != hipSuccess)
             rc = fail(GG_ERR_HIP, "multi-source BFS: upload sources");
         if (!rc) {
             k_msbfs_init<<<1, 64, 0, c->stream>>>((const uint32_t *)src.p, nsrc, n, (uint32_t *)F[0].p,
-                                                  (uint32_t *)seen.p, lv, perm);
+                                                  (uint32_t *)seen.p, lv, perm, vmajor);
             if (P) k_msbc_init<V><<<1, 64, 0, c->stream>>>((const uint32_t *)src.p, nsrc, perm, P);
             if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: init");
         }
         unsigned long long *cnt = (unsigned long long *)c->counters.p;
         const uint32_t pgrid = (uint32_t)std::min<uint64_t>(((uint64_t)n * 32 + kBlock - 1) / kBlock, 1u << 16);
-        // one pull per level until no vertex gains a bit (at most n levels)
-        while (!rc && level < (int)n) {
-            level++;
-            const uint32_t *fin = (const uint32_t *)F[(level - 1) % 2].p;
-            uint32_t *fout = (uint32_t *)F[level % 2].p;
-            OpMSBFS op{fin, fout, (uint32_t *)seen.p, lv, n, level};
-            rc = counters_zero(c);
-            if (!rc) rc = launch_pull(c, view, op, cnt, nullptr, nullptr);
-            if (!rc && P) {
-                k_msbc_paths<V><<<pgrid, kBlock, 0, c->stream>>>(view.toff, view.tids, fin, fout, P, n);
-                if (hub->nitems) {
-                    const uint32_t igrid =
-                        (uint32_t)std::min<uint64_t>(((uint64_t)hub->nitems * 32 + kBlock - 1) / kBlock, 1u << 16);
-                    k_msbc_hub_items<V><<<igrid, kBlock, 0, c->stream>>>(
-                        (const uint32_t *)hub->items.p, hub->nitems, view.tids, fin, fout, P, (V *)hslot.p);
-                    k_msbc_hub_fix<V><<<blocks_for((uint64_t)hub->nhubs * 32, kBlock), kBlock, 0, c->stream>>>(
-                        (const uint32_t *)hub->hubs.p, hub->nhubs, fout, (const V *)hslot.p, P);
+        int *flags = (int *)fb.p;
+        const uint32_t full = nsrc == 32 ? ~0u : (1u << nsrc) - 1;
+        if (!rc) rc = counters_zero(c);
+        if (!rc) rc = fill_bytes(c, flags, 64, 0);
+        // Levels are enqueued in batches, each level a pull (with BC, the
+        // path counts) and k_msbfs_check; the flags are read back once per
+        // batch.  The first batch is the last call's depth, else 8, then 16,
+        // 32 and 64; at most n levels.
+        int batch = g->msbfs_levels_hint > 0 ? std::min(g->msbfs_levels_hint, 64) : 8;
+        int enq = 0, hf[2] = {0, 0};
+        while (!rc && enq < (int)n) {
+            const int upto = (int)std::min<int64_t>((int64_t)enq + batch, (int64_t)n);
+            for (int lvl = enq + 1; !rc && lvl <= upto; lvl++) {
+                const uint32_t *fin = (const uint32_t *)F[(lvl - 1) % 2].p;
+                uint32_t *fout = (uint32_t *)F[lvl % 2].p;
+                OpMSBFS op{fin, fout, (uint32_t *)seen.p, lv, n, lvl, (uint32_t)nsrc, vmajor, flags, full};
+                rc = launch_pull(c, view, op, cnt, nullptr, nullptr);
+                if (!rc && P) {
+                    k_msbc_paths<V><<<pgrid, kBlock, 0, c->stream>>>(view.toff, view.tids, fin, fout, P, n, flags);
+                    if (hub->nitems) {
+                        const uint32_t igrid =
+                            (uint32_t)std::min<uint64_t>(((uint64_t)hub->nitems * 32 + kBlock - 1) / kBlock, 1u << 16);
+                        k_msbc_hub_items<V><<<igrid, kBlock, 0, c->stream>>>(
+                            (const uint32_t *)hub->items.p, hub->nitems, view.tids, fin, fout, P, (V *)hslot.p, flags);
+                        k_msbc_hub_fix<V><<<blocks_for((uint64_t)hub->nhubs * 32, kBlock), kBlock, 0, c->stream>>>(
+                            (const uint32_t *)hub->hubs.p, hub->nhubs, fout, (const V *)hslot.p, P, flags);
+                    }
+                    if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BC: path counts");
                 }
-                if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BC: path counts");
+                if (!rc) {
+                    k_msbfs_check<<<1, 64, 0, c->stream>>>(cnt, lvl, flags);
+                    if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: level check");
+                }
             }
-            unsigned long long cn[4] = {0, 0, 0, 0};
-            if (!rc) rc = counters_read(c, cn);
-            const unsigned long long active = cn[CNT_ACTIVE];
-            if (!rc && info && level <= 64) info->paths[level - 1] = GG_PATH_CSC;
-            if (!rc && active == 0) break;
+            enq = upto;
+            if (!rc) rc = read_back(c, hf, flags, 8);
+            if (rc || hf[0]) break;
+            batch = std::min(batch * 2, 64);
         }
-        const uint64_t total = (uint64_t)nsrc * n;
-        const uint32_t grid = (uint32_t)std::min<uint64_t>((total + kBlock - 1) / kBlock, 1u << 20);
-        if (!rc && P) {
-            k_msbc_out<V><<<grid, kBlock, 0, c->stream>>>(P, lv, perm, nsrc, n, paths);
-            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BC: output");
-        }
-        if (!rc && lv != levels && levels) {
-            if (planned) {
-                k_msbfs_unmap<<<grid, kBlock, 0, c->stream>>>((const int32_t *)lp.p, perm, nsrc, n, levels);
-                if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: map-back");
-            } else if (hipMemcpyAsync(levels, lp.p, 4ull * total, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
-                rc = fail(GG_ERR_HIP, "multi-source BFS: levels");
-            }
+        level = hf[0] ? hf[1] : enq;
+        if (!rc) g->msbfs_levels_hint = level;
+        if (!rc && info)
+            for (int l = 1; l <= level && l <= 64; l++) info->paths[l - 1] = GG_PATH_CSC;
+        if (!rc && vmajor && (levels || P)) {
+            const uint32_t ob = (uint32_t)((n + kOutTile - 1) / kOutTile);
+            const uint32_t *sn = (const uint32_t *)seen.p;
+            if (levels && P)
+                k_msbfs_out<V, true, true><<<ob, kBlock, 0, c->stream>>>(lv, sn, P, perm, nsrc, n, levels, paths);
+            else if (levels)
+                k_msbfs_out<V, true, false><<<ob, kBlock, 0, c->stream>>>(lv, sn, P, perm, nsrc, n, levels, paths);
+            else
+                k_msbfs_out<V, false, true><<<ob, kBlock, 0, c->stream>>>(lv, sn, P, perm, nsrc, n, levels, paths);
+            if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: output");
         }
         (void)hipEventRecord(e1, c->stream);
         if (!rc) rc = sync(c);
@@ -299,6 +400,7 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     pool_give(c, lp);
     pool_give(c, pb);
     pool_give(c, hslot);
+    pool_give(c, fb);
     return rc;
 }
 } // namespace

@@ -321,6 +321,7 @@ struct Graph {
     bool transposed = false; // gg_graph_transpose flips the two directions
     int bfs_levels_hint = 0;  // depth of the last BFS on the graph (sizes the next one's first batch of levels)
     int cc_rounds_hint = 0;   // rounds of the last Components call on the graph (likewise)
+    int msbfs_levels_hint = 0; // levels of the last multi-source BFS or BC forward on the graph (likewise)
     // tuning (gg_graph_set_option)
     int pull_schedule = 1;  // 0 merge-path tiles, 1 row blocks where built
     int edges_per_lane = 8; // row blocks: target edges per lane of a row group

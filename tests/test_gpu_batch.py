@@ -187,6 +187,38 @@ def test_msbfs_scale22_against_single_source(ctx):
     g.close()
 
 
+@pytest.mark.parametrize("plan", [0, 1])
+def test_msbfs_many_levels_and_depth_changes(ctx, plan):
+    """A symmetric path with shuffled ids: levels are enqueued in batches
+    (the first sized by the previous call's depth), so calls whose depth
+    needs several batches, and calls shallower than the previous one (the
+    levels past the end enqueued and skipped), must give the same levels and
+    path counts as the textbook BFS."""
+    n = 230
+    order = np.random.default_rng(11).permutation(n)
+    off, ids = oracle.csr_from_edges(n, order[:-1].astype(np.uint32), order[1:].astype(np.uint32),
+                                     symmetric=True)
+    g = gg.Graph.from_csr(ctx, off, ids, symmetric=True).partition(8).reorder(plan)
+    pos = np.empty(n, np.int64)
+    pos[order] = np.arange(n)
+    # path positions of the sources: an end (depth n), the middle, near the middle
+    for where in ([0], [0, 115], [115], [100, 130, 115], [229, 0]):
+        src = [int(order[p]) for p in where]
+        print(f"msbfs path n={n} plan={plan} sources at {where}", flush=True)
+        depth = max(max(p, n - 1 - p) for p in where)
+        lv, info = gg.msbfs(g, src)
+        L = lv.to_host().reshape(len(src), n)
+        for k, s in enumerate(src):
+            assert np.array_equal(L[k], R.bfs_levels(off, ids, s)), (where, k)
+        assert info.iters == depth + 1
+        pa, lv2, binfo = gg.msbc_forward(g, src, np.float64, levels=True)
+        assert np.array_equal(lv2.to_host().reshape(len(src), n), L)
+        assert binfo.iters == depth + 1
+        # one shortest path to every vertex of a path
+        assert np.array_equal(pa.to_host().reshape(len(src), n), np.ones((len(src), n)))
+    g.close()
+
+
 def test_autotune_picks_fastest_and_keeps_results(ctx):
     g = gg.Graph.rmat(ctx, 16, 16, 4, weighted=True)
     t = gg.autotune_pull(g, np.float64, trials=3)
