@@ -29,13 +29,21 @@ struct OpMSBFS {
     int vmajor;
     const int *done; // set once a level gains no bit: later levels return at once
     uint32_t full;   // every source's bit
+    // a sparse frontier (flags[2] clear): the gathers read fin only where
+    // bit s of fbm (fin[s] != 0) is set, a 2 MiB bitmap at n = 2^24 that
+    // stays in L2, instead of a gather from the n-word array per edge
+    const uint32_t *fbm;
+    const int *dense;
     __device__ bool skip() const { return *done != 0; }
     // a vertex every source has reached gains nothing: the row-block pull
     // skips the gathers of a work item whose rows all have
     __device__ bool row_done(uint32_t v) const { return seen[v] == full; }
     __device__ uint32_t identity() const { return 0u; }
     __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a | b; }
-    __device__ uint32_t gather(uint32_t, uint32_t s) const { return fin[s]; }
+    __device__ uint32_t gather(uint32_t, uint32_t s) const
+    {
+        return *dense || bit_test(fbm, s) ? fin[s] : 0u;
+    }
     struct row_t {
         uint32_t seen;
     };
@@ -60,7 +68,7 @@ struct OpMSBFS {
 // perm (may be NULL): the renumbered plan's new id of each caller's vertex
 __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_t n, uint32_t *__restrict__ fin,
                              uint32_t *__restrict__ seen, int32_t *__restrict__ levels,
-                             const uint32_t *__restrict__ perm, int vmajor)
+                             const uint32_t *__restrict__ perm, int vmajor, uint32_t *__restrict__ fbm)
 {
     // one thread per source; sources may repeat, so bits are ORed atomically
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -68,15 +76,18 @@ __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_
     const uint32_t v = perm ? perm[src[k]] : src[k];
     atomicOr(&fin[v], 1u << k);
     atomicOr(&seen[v], 1u << k);
+    atomicOr(&fbm[v >> 5], 1u << (v & 31));
     levels[vmajor ? (uint64_t)v * nsrc + k : (uint64_t)k * n + v] = 0;
 }
 
 // After each level, one wave: the vertices that gained a bit (the counter
 // stripes, one per lane, cleared for the next level).  None: flags[0] = 1
 // and flags[1] = this level, which ends the loop (every later level's
-// kernels return at once).
+// kernels return at once).  flags[2] = 1 when they are n/32 or more: the
+// next level's gathers then skip the bitmap test (at that density most
+// tests pass and only add a dependent load per edge).
 __global__ __launch_bounds__(64) void k_msbfs_check(unsigned long long *__restrict__ counters, int level,
-                                                    int *__restrict__ flags)
+                                                    uint32_t n, int *__restrict__ flags)
 {
     if (flags[0]) return;
     unsigned long long a = 0;
@@ -88,10 +99,26 @@ __global__ __launch_bounds__(64) void k_msbfs_check(unsigned long long *__restri
         st[CNT_SCANNED] = 0;
     }
     a = wave_sum_u64(a);
-    if (threadIdx.x == 0 && a == 0) {
-        flags[0] = 1;
-        flags[1] = level;
+    if (threadIdx.x == 0) {
+        if (a == 0) {
+            flags[0] = 1;
+            flags[1] = level;
+        }
+        flags[2] = a * 32 >= n;
     }
+}
+
+// bm bit v = (f[v] != 0), a wave ballot per 64 vertices: after a level
+// whose new vertices are a sparse frontier (flags[2] clear), for the next
+// level's gathers
+__global__ __launch_bounds__(kBlock) void k_msbfs_bits(const uint32_t *__restrict__ f, uint32_t n,
+                                                       uint32_t *__restrict__ bm, const int *__restrict__ flags)
+{
+    if (flags[0] || flags[2]) return;
+    const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const unsigned long long b = __ballot(v < n && f[v] != 0);
+    const uint32_t lane = threadIdx.x & 63, w = (uint32_t)((v - lane) >> 5) + lane;
+    if (lane < 2 && (uint64_t)w * 32 < n) bm[w] = (uint32_t)(b >> (32 * lane));
 }
 
 // BC_F's path counts (app/BC.C:28-47) for the sources of a multi-source
@@ -99,28 +126,32 @@ __global__ __launch_bounds__(64) void k_msbfs_check(unsigned long long *__restri
 // wave per vertex v with a new bit: lane k adds P[u*32 + k] over v's
 // in-neighbours u whose previous-level mask holds bit k (BC_F::update for
 // the frontier of source k, in in-list order: a fixed order), and writes it
-// when bit k is new at v.  The in-list is read kPathStep entries a step
-// (independent loads: ids, then masks, then the counts); the lanes of a
-// half-wave read the same id and mask and one line of counts.  In-lists
+// when bit k is new at v.  The in-list is read kPathStep entries a step:
+// lane j of the half-wave loads entry j's id and then its mask, so a step
+// costs two dependent loads whatever its length; every lane then takes the
+// step's ids and masks by shuffles and issues its kPathStep count loads
+// together (one line of counts per entry for the half-wave).  In-lists
 // longer than kHubChunk are left to the chunked pass below.
-constexpr int kPathStep = 8;
+constexpr int kPathStep = 32;
 
 // lane k's sum of P[u*32 + k] over in-list entries [b, e) whose mask
-// fin[u] & nw holds bit k, in list order
+// fin[u] & nw holds bit k, in list order (k: the lane in its half-wave)
 template <class V>
 __device__ __forceinline__ V msbc_sum(const uint32_t *__restrict__ tids, const uint32_t *__restrict__ fin,
                                       const V *__restrict__ P, uint32_t b, uint32_t e, uint32_t nw, uint32_t k)
 {
+    static_assert(kPathStep == 32, "one entry per lane of a half-wave");
     V acc = V(0);
     for (uint32_t j = b; j < e; j += kPathStep) {
-        uint32_t u[kPathStep], msk[kPathStep];
-#pragma unroll
-        for (int q = 0; q < kPathStep; q++) u[q] = tids[min(j + q, e - 1)];
-#pragma unroll
-        for (int q = 0; q < kPathStep; q++) msk[q] = (j + q < e) ? fin[u[q]] & nw : 0u;
+        const bool in = j + k < e;
+        const uint32_t u = in ? tids[j + k] : 0u;
+        const uint32_t m = in ? fin[u] & nw : 0u;
         V c[kPathStep];
 #pragma unroll
-        for (int q = 0; q < kPathStep; q++) c[q] = ((msk[q] >> k) & 1u) ? P[(uint64_t)u[q] * 32 + k] : V(0);
+        for (int q = 0; q < kPathStep; q++) {
+            const uint32_t uq = __shfl(u, q, 32), mq = __shfl(m, q, 32);
+            c[q] = ((mq >> k) & 1u) ? P[(uint64_t)uq * 32 + k] : V(0);
+        }
 #pragma unroll
         for (int q = 0; q < kPathStep; q++) acc += c[q];
     }
@@ -135,16 +166,41 @@ __global__ __launch_bounds__(kBlock) void k_msbc_paths(const uint32_t *__restric
                                                        uint32_t n, const int *__restrict__ done)
 {
     if (*done) return;
-    const uint32_t k = threadIdx.x & 31;
-    const uint64_t h0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
-    const uint64_t nh = ((uint64_t)gridDim.x * blockDim.x) >> 5;
-    for (uint64_t vv = h0; vv < n; vv += nh) {
-        const uint32_t v = (uint32_t)vv;
-        const uint32_t nw = fout[v];
-        const uint32_t b = toff[v], e = toff[v + 1];
-        if (!nw || e - b > kHubChunk) continue; // no new source, or a chunked hub
-        const V acc = msbc_sum(tids, fin, P, b, e, nw, k);
-        if ((nw >> k) & 1u) P[(uint64_t)v * 32 + k] = acc;
+    // a wave reads 64 vertices' new masks (and their in-list bounds) at
+    // once, one per lane; its two halves then take the vertices with a new
+    // source two at a time.  Lane l of wave w takes vertex first + l*W (W
+    // waves): consecutive ids have alike degrees under a hot-first
+    // numbering, so each wave gets a spread of them.
+    const uint32_t lane = threadIdx.x & 63, k = lane & 31, half = lane >> 5;
+    const uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t W = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t first = w0; first < n; first += 64 * W) {
+        const uint64_t vl = first + lane * W;
+        uint32_t f = 0, lb = 0, le = 0;
+        if (vl < n) {
+            f = fout[vl];
+            if (f) {
+                lb = toff[vl];
+                le = toff[vl + 1];
+            }
+        }
+        unsigned long long bal = __ballot(f != 0 && le - lb <= kHubChunk); // longer: the chunked pass
+        while (bal) {
+            const int i0 = __builtin_ctzll(bal);
+            bal &= bal - 1;
+            int i1 = -1;
+            if (bal) {
+                i1 = __builtin_ctzll(bal);
+                bal &= bal - 1;
+            }
+            const int mine = half ? i1 : i0;
+            const int from = mine < 0 ? 0 : mine;
+            const uint32_t nw = __shfl(f, from, 64), b = __shfl(lb, from, 64), e = __shfl(le, from, 64);
+            if (mine >= 0) {
+                const V acc = msbc_sum(tids, fin, P, b, e, nw, k);
+                if ((nw >> k) & 1u) P[(first + mine * W) * 32 + k] = acc;
+            }
+        }
     }
 }
 
@@ -290,8 +346,11 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     if (planned) view = plan_view(g);
     else GG_TRY(pull_view(g, &view));
     const uint32_t *perm = planned ? (const uint32_t *)g->plan.perm.p : nullptr;
-    Buf F[2], seen, src, lp, pb, hslot, fb;
+    Buf F[2], seen, src, lp, pb, hslot, fb, B[2];
+    const uint64_t bmb = 4ull * ((n + 31) / 32);
     int rc = pool_take(c, F[0], 4ull * n);
+    if (!rc) rc = pool_take(c, B[0], bmb);
+    if (!rc) rc = pool_take(c, B[1], bmb);
     if (!rc) rc = pool_take(c, fb, 64);
     if (!rc) rc = pool_take(c, F[1], 4ull * n);
     if (!rc) rc = pool_take(c, seen, 4ull * n);
@@ -316,17 +375,18 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
         (void)hipEventRecord(e0, c->stream);
         rc = fill_u32(c, F[0].p, n, 0);
         if (!rc) rc = fill_u32(c, seen.p, n, 0);
+        if (!rc) rc = fill_bytes(c, B[0].p, bmb, 0);
         if (!rc && !vmajor) rc = gg_fill(abi(c), lv, GG_I32, (uint64_t)nsrc * n, -1.0);
         if (!rc && hipMemcpyAsync(src.p, sources, 4ull * nsrc, hipMemcpyHostToDevice, c->stream) != hipSuccess)
             rc = fail(GG_ERR_HIP, "multi-source BFS: upload sources");
         if (!rc) {
             k_msbfs_init<<<1, 64, 0, c->stream>>>((const uint32_t *)src.p, nsrc, n, (uint32_t *)F[0].p,
-                                                  (uint32_t *)seen.p, lv, perm, vmajor);
+                                                  (uint32_t *)seen.p, lv, perm, vmajor, (uint32_t *)B[0].p);
             if (P) k_msbc_init<V><<<1, 64, 0, c->stream>>>((const uint32_t *)src.p, nsrc, perm, P);
             if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: init");
         }
         unsigned long long *cnt = (unsigned long long *)c->counters.p;
-        const uint32_t pgrid = (uint32_t)std::min<uint64_t>(((uint64_t)n * 32 + kBlock - 1) / kBlock, 1u << 16);
+        const uint32_t pgrid = (uint32_t)std::min<uint64_t>(blocks_for(n, 4 * kBlock), 1u << 16);
         int *flags = (int *)fb.p;
         const uint32_t full = nsrc == 32 ? ~0u : (1u << nsrc) - 1;
         if (!rc) rc = counters_zero(c);
@@ -342,7 +402,8 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
             for (int lvl = enq + 1; !rc && lvl <= upto; lvl++) {
                 const uint32_t *fin = (const uint32_t *)F[(lvl - 1) % 2].p;
                 uint32_t *fout = (uint32_t *)F[lvl % 2].p;
-                OpMSBFS op{fin, fout, (uint32_t *)seen.p, lv, n, lvl, (uint32_t)nsrc, vmajor, flags, full};
+                OpMSBFS op{fin, fout, (uint32_t *)seen.p, lv, n, lvl, (uint32_t)nsrc, vmajor, flags, full,
+                           (const uint32_t *)B[(lvl - 1) % 2].p, flags + 2};
                 rc = launch_pull(c, view, op, cnt, nullptr, nullptr);
                 if (!rc && P) {
                     k_msbc_paths<V><<<pgrid, kBlock, 0, c->stream>>>(view.toff, view.tids, fin, fout, P, n, flags);
@@ -357,7 +418,9 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
                     if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BC: path counts");
                 }
                 if (!rc) {
-                    k_msbfs_check<<<1, 64, 0, c->stream>>>(cnt, lvl, flags);
+                    k_msbfs_check<<<1, 64, 0, c->stream>>>(cnt, lvl, n, flags);
+                    k_msbfs_bits<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(fout, n, (uint32_t *)B[lvl % 2].p,
+                                                                                flags);
                     if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: level check");
                 }
             }
@@ -401,6 +464,8 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     pool_give(c, pb);
     pool_give(c, hslot);
     pool_give(c, fb);
+    pool_give(c, B[0]);
+    pool_give(c, B[1]);
     return rc;
 }
 } // namespace
