@@ -341,8 +341,18 @@ def batched_bench(gg, ctx, count=512, seed=5, iters=30, msbfs_scale=22, plan=Tru
     arena.close()
     for p in parts:
         p.close()
-    # 32-source BFS on the BC shape
-    g = gg.Graph.rmat(ctx, msbfs_scale, 16, seed, symmetric=True)
+    res.update(multisource_bench(gg, ctx, msbfs_scale, seed, plan))
+    return res
+
+
+def multisource_bench(gg, ctx, scale=22, seed=5, plan=True, calls=5):
+    """32-source BFS on symmetric R-MAT 'scale' (edge factor 16) against 32
+    single-source BFS runs, and BC.C's forward phase (levels and NumPaths,
+    FP32) for the same 32 sources in one pass beside 32 single-source BC.C
+    runs.  The multi-source times are the mean of 'calls' calls after one
+    warm-up."""
+    res = {}
+    g = gg.Graph.rmat(ctx, scale, 16, seed, symmetric=True)
     g.partition(384)
     if plan:
         g.reorder(1)  # hot-sources-first numbering for both legs (untimed preprocessing)
@@ -350,8 +360,9 @@ def batched_bench(gg, ctx, count=512, seed=5, iters=30, msbfs_scale=22, plan=Tru
     src = np.random.default_rng(seed).choice(np.flatnonzero(deg > 0), 32, replace=False)
     lv, _ = gg.msbfs(g, src)  # warm-up (row blocks)
     t1 = time.perf_counter()
-    lv, info = gg.msbfs(g, src, out=lv)
-    tm = time.perf_counter() - t1
+    for _ in range(calls):
+        lv, info = gg.msbfs(g, src, out=lv)
+    tm = (time.perf_counter() - t1) / calls
     L = lv.to_host().reshape(32, g.n)
     und = sum(int(deg[L[k] >= 0].astype(np.int64).sum()) // 2 for k in range(32))
     par = gg.DeviceArray(ctx, g.n, np.int32)
@@ -360,22 +371,23 @@ def batched_bench(gg, ctx, count=512, seed=5, iters=30, msbfs_scale=22, plan=Tru
     for s in src:
         gg.bfs(g, int(s), parents=par, levels=False)
     ts = time.perf_counter() - t1
-    res["msbfs"] = {"scale": msbfs_scale, "sources": 32, "levels": info.iters, "ms": tm * 1e3,
+    res["msbfs"] = {"scale": scale, "sources": 32, "levels": info.iters, "ms": tm * 1e3,
                     "gteps": und / tm / 1e9, "single_source_loop_ms": ts * 1e3, "single_source_gteps": und / ts / 1e9}
     # BC.C's forward phase (levels and NumPaths) for the same 32 sources in
     # one pass (FP32, the reference's fType), beside 32 single-source BC.C
     # runs (forward and backward) on the same graph
     pa, _, _ = gg.msbc_forward(g, src, np.float32)  # warm-up
     t1 = time.perf_counter()
-    pa, _, binfo = gg.msbc_forward(g, src, np.float32, paths=pa)
-    tb = time.perf_counter() - t1
+    for _ in range(calls):
+        pa, _, binfo = gg.msbc_forward(g, src, np.float32, paths=pa)
+    tb = (time.perf_counter() - t1) / calls
     sc = gg.DeviceArray(ctx, g.n, np.float32)
     gg.bc(g, int(src[0]), np.float32, out=sc)  # warm-up
     t1 = time.perf_counter()
     for s in src:
         gg.bc(g, int(s), np.float32, out=sc)
     tbs = time.perf_counter() - t1
-    res["msbc_forward"] = {"scale": msbfs_scale, "sources": 32, "levels": binfo.iters, "ms": tb * 1e3,
+    res["msbc_forward"] = {"scale": scale, "sources": 32, "levels": binfo.iters, "ms": tb * 1e3,
                            "gteps": und / tb / 1e9, "single_source_bc_loop_ms": tbs * 1e3}
     g.close()
     return res
@@ -567,6 +579,8 @@ def main():
                     help="only run the BFS + Components leg (for kernel traces of it)")
     ap.add_argument("--only-adjoints", action="store_true",
                     help="only run the SpMV and PageRank adjoint legs (for kernel traces of them)")
+    ap.add_argument("--only-msbfs", action="store_true",
+                    help="only run the 32-source BFS and BC forward leg (for kernel traces of it)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU (--gpus 1): the engine runs on a single MI355X; "
@@ -581,6 +595,10 @@ def main():
         return
     if a.only_bfs_cc:
         log(f"[bench] bfs/cc: {bfs_cc_bench(gg, ctx, a.scale, a.steps)}")
+        ctx.close()
+        return
+    if a.only_msbfs:
+        log(f"[bench] multi-source: {multisource_bench(gg, ctx, a.scale - 2)}")
         ctx.close()
         return
     if a.only_adjoints:
