@@ -27,14 +27,14 @@ struct OpMSBFS {
     int32_t level;
     uint32_t nsrc;
     int vmajor;
-    const int *done; // set once a level gains no bit: later levels return at once
+    const int *done; // the run's flags: [0] set once a level gains no bit, [3] a push level
     uint32_t full;   // every source's bit
     // a sparse frontier (flags[2] clear): the gathers read fin only where
     // bit s of fbm (fin[s] != 0) is set, a 2 MiB bitmap at n = 2^24 that
     // stays in L2, instead of a gather from the n-word array per edge
     const uint32_t *fbm;
     const int *dense;
-    __device__ bool skip() const { return *done != 0; }
+    __device__ bool skip() const { return done[0] != 0 || done[3] != 0; } // ended, or a push level
     // a vertex every source has reached gains nothing: the row-block pull
     // skips the gathers of a work item whose rows all have
     __device__ bool row_done(uint32_t v) const { return seen[v] == full; }
@@ -68,7 +68,8 @@ struct OpMSBFS {
 // perm (may be NULL): the renumbered plan's new id of each caller's vertex
 __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_t n, uint32_t *__restrict__ fin,
                              uint32_t *__restrict__ seen, int32_t *__restrict__ levels,
-                             const uint32_t *__restrict__ perm, int vmajor, uint32_t *__restrict__ fbm)
+                             const uint32_t *__restrict__ perm, int vmajor, uint32_t *__restrict__ fbm,
+                             uint32_t *__restrict__ list, int *__restrict__ flags, int push_ok)
 {
     // one thread per source; sources may repeat, so bits are ORed atomically
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,7 +78,12 @@ __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_
     atomicOr(&fin[v], 1u << k);
     atomicOr(&seen[v], 1u << k);
     atomicOr(&fbm[v >> 5], 1u << (v & 31));
+    list[k] = v; // a repeated source is pushed twice: OR is idempotent
     levels[vmajor ? (uint64_t)v * nsrc + k : (uint64_t)k * n + v] = 0;
+    if (k == 0) {
+        flags[3] = push_ok; // level 1 pushes from the sources on a symmetric graph
+        flags[4] = nsrc;
+    }
 }
 
 // After each level, one wave: the vertices that gained a bit (the counter
@@ -85,9 +91,14 @@ __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_
 // and flags[1] = this level, which ends the loop (every later level's
 // kernels return at once).  flags[2] = 1 when they are n/32 or more: the
 // next level's gathers then skip the bitmap test (at that density most
-// tests pass and only add a dependent load per edge).
+// tests pass and only add a dependent load per edge).  flags[3] = 1 when
+// they are at most kPushCap and the graph is symmetric: the next level
+// pushes from their list (flags[4] entries, rebuilt by k_msbfs_bits)
+// instead of pulling over every in-edge.
+constexpr uint32_t kPushCap = 4096;
+
 __global__ __launch_bounds__(64) void k_msbfs_check(unsigned long long *__restrict__ counters, int level,
-                                                    uint32_t n, int *__restrict__ flags)
+                                                    uint32_t n, int push_ok, int *__restrict__ flags)
 {
     if (flags[0]) return;
     unsigned long long a = 0;
@@ -105,20 +116,97 @@ __global__ __launch_bounds__(64) void k_msbfs_check(unsigned long long *__restri
             flags[1] = level;
         }
         flags[2] = a * 32 >= n;
+        flags[3] = push_ok && a <= kPushCap;
+        flags[4] = 0;
     }
 }
 
 // bm bit v = (f[v] != 0), a wave ballot per 64 vertices: after a level
 // whose new vertices are a sparse frontier (flags[2] clear), for the next
-// level's gathers
+// level's gathers.  Before a push level (flags[3]) the same vertices are
+// appended to list, flags[4] counting them (a list in no fixed order: the
+// push ORs).
 __global__ __launch_bounds__(kBlock) void k_msbfs_bits(const uint32_t *__restrict__ f, uint32_t n,
-                                                       uint32_t *__restrict__ bm, const int *__restrict__ flags)
+                                                       uint32_t *__restrict__ bm, uint32_t *__restrict__ list,
+                                                       int *__restrict__ flags)
 {
-    if (flags[0] || flags[2]) return;
+    if (flags[0] || (flags[2] && !flags[3])) return;
     const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const unsigned long long b = __ballot(v < n && f[v] != 0);
+    const bool in = v < n && f[v] != 0;
+    const unsigned long long b = __ballot(in);
     const uint32_t lane = threadIdx.x & 63, w = (uint32_t)((v - lane) >> 5) + lane;
-    if (lane < 2 && (uint64_t)w * 32 < n) bm[w] = (uint32_t)(b >> (32 * lane));
+    if (!flags[2] && lane < 2 && (uint64_t)w * 32 < n) bm[w] = (uint32_t)(b >> (32 * lane));
+    if (flags[3] && b) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd((unsigned int *)&flags[4], (unsigned int)__popcll(b));
+        base = __shfl(base, 0, 64);
+        if (in) list[base + __popcll(b & ((1ull << lane) - 1))] = (uint32_t)v;
+    }
+}
+
+// A push level (flags[3]): every frontier vertex u of the list ORs its mask
+// into nxt[v] of its neighbours v (the in-list of a symmetric graph is its
+// out-list), bits v has already seen left out.  Each list entry's edges are
+// cut into kPushParts equal runs, so a hub is spread over work-groups.
+constexpr uint32_t kPushParts = 16;
+constexpr uint32_t kMsGrid = 2048; // work-groups of the push and finish passes (grid-stride)
+
+__global__ __launch_bounds__(kBlock) void k_msbfs_push(const uint32_t *__restrict__ toff,
+                                                       const uint32_t *__restrict__ tids,
+                                                       const uint32_t *__restrict__ list,
+                                                       const uint32_t *__restrict__ fin,
+                                                       const uint32_t *__restrict__ seen, uint32_t *__restrict__ nxt,
+                                                       const int *__restrict__ flags)
+{
+    if (flags[0] || !flags[3]) return;
+    const uint32_t parts = (uint32_t)flags[4] * kPushParts;
+    for (uint32_t p = blockIdx.x; p < parts; p += gridDim.x) {
+        const uint32_t u = list[p / kPushParts], part = p % kPushParts;
+        const uint32_t mask = fin[u];
+        const uint32_t b = toff[u];
+        const uint64_t len = toff[u + 1] - b;
+        const uint32_t lo = b + (uint32_t)(len * part / kPushParts);
+        const uint32_t hi = b + (uint32_t)(len * (part + 1) / kPushParts);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+            const uint32_t v = tids[i];
+            const uint32_t add = mask & ~seen[v];
+            if (add) atomicOr(&nxt[v], add);
+        }
+    }
+}
+
+// The rest of a push level, over every vertex: the new bits (nxt, cleared
+// for the next push), seen, levels, fout and the count of vertices with a
+// new bit, as the pull's rows would have written them.
+__global__ __launch_bounds__(kBlock) void k_msbfs_finish(uint32_t *__restrict__ nxt, uint32_t *__restrict__ seen,
+                                                         uint32_t *__restrict__ fout, int32_t *__restrict__ levels,
+                                                         uint32_t n, int32_t level, uint32_t nsrc, int vmajor,
+                                                         unsigned long long *__restrict__ counters,
+                                                         const int *__restrict__ flags)
+{
+    if (flags[0] || !flags[3]) return;
+    LocalStats st;
+    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t x = nxt[v];
+        uint32_t nw = 0;
+        if (x) {
+            nxt[v] = 0;
+            const uint32_t sn = seen[v];
+            nw = x & ~sn;
+            if (nw) {
+                seen[v] = sn | nw;
+                st.active++;
+                uint32_t bb = nw;
+                while (bb) {
+                    const uint32_t k = __builtin_ctz(bb);
+                    bb &= bb - 1;
+                    levels[vmajor ? v * nsrc + k : (uint64_t)k * n + v] = level;
+                }
+            }
+        }
+        fout[v] = nw;
+    }
+    (void)block_stats(st, counters);
 }
 
 // BC_F's path counts (app/BC.C:28-47) for the sources of a multi-source
@@ -346,9 +434,11 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     if (planned) view = plan_view(g);
     else GG_TRY(pull_view(g, &view));
     const uint32_t *perm = planned ? (const uint32_t *)g->plan.perm.p : nullptr;
-    Buf F[2], seen, src, lp, pb, hslot, fb, B[2];
+    Buf F[2], seen, src, lp, pb, hslot, fb, B[2], fl, nx;
     const uint64_t bmb = 4ull * ((n + 31) / 32);
     int rc = pool_take(c, F[0], 4ull * n);
+    if (!rc) rc = pool_take(c, fl, 4ull * std::max<uint32_t>(n, 32)); // the frontier list (the sources first)
+    if (!rc) rc = pool_take(c, nx, 4ull * n);
     if (!rc) rc = pool_take(c, B[0], bmb);
     if (!rc) rc = pool_take(c, B[1], bmb);
     if (!rc) rc = pool_take(c, fb, 64);
@@ -365,6 +455,7 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     // vertex-major in lp (a vertex's levels in one run, mapped out by
     // k_msbfs_out with the counts); else straight into the caller's array
     const int vmajor = planned || paths;
+    const int push_ok = g->symmetric; // in-lists are the out-lists
     int32_t *lv = vmajor ? (int32_t *)lp.p : levels;
     V *P = paths ? (V *)pb.p : nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -376,12 +467,15 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
         rc = fill_u32(c, F[0].p, n, 0);
         if (!rc) rc = fill_u32(c, seen.p, n, 0);
         if (!rc) rc = fill_bytes(c, B[0].p, bmb, 0);
+        if (!rc) rc = fill_u32(c, nx.p, n, 0);
+        if (!rc) rc = fill_bytes(c, fb.p, 64, 0);
         if (!rc && !vmajor) rc = gg_fill(abi(c), lv, GG_I32, (uint64_t)nsrc * n, -1.0);
         if (!rc && hipMemcpyAsync(src.p, sources, 4ull * nsrc, hipMemcpyHostToDevice, c->stream) != hipSuccess)
             rc = fail(GG_ERR_HIP, "multi-source BFS: upload sources");
         if (!rc) {
             k_msbfs_init<<<1, 64, 0, c->stream>>>((const uint32_t *)src.p, nsrc, n, (uint32_t *)F[0].p,
-                                                  (uint32_t *)seen.p, lv, perm, vmajor, (uint32_t *)B[0].p);
+                                                  (uint32_t *)seen.p, lv, perm, vmajor, (uint32_t *)B[0].p,
+                                                  (uint32_t *)fl.p, (int *)fb.p, push_ok);
             if (P) k_msbc_init<V><<<1, 64, 0, c->stream>>>((const uint32_t *)src.p, nsrc, perm, P);
             if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: init");
         }
@@ -389,8 +483,8 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
         const uint32_t pgrid = (uint32_t)std::min<uint64_t>(blocks_for(n, 4 * kBlock), 1u << 16);
         int *flags = (int *)fb.p;
         const uint32_t full = nsrc == 32 ? ~0u : (1u << nsrc) - 1;
+        const uint32_t fgrid = std::min<uint32_t>(blocks_for(n, kBlock), kMsGrid);
         if (!rc) rc = counters_zero(c);
-        if (!rc) rc = fill_bytes(c, flags, 64, 0);
         // Levels are enqueued in batches, each level a pull (with BC, the
         // path counts) and k_msbfs_check; the flags are read back once per
         // batch.  The first batch is the last call's depth, else 8, then 16,
@@ -404,7 +498,14 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
                 uint32_t *fout = (uint32_t *)F[lvl % 2].p;
                 OpMSBFS op{fin, fout, (uint32_t *)seen.p, lv, n, lvl, (uint32_t)nsrc, vmajor, flags, full,
                            (const uint32_t *)B[(lvl - 1) % 2].p, flags + 2};
-                rc = launch_pull(c, view, op, cnt, nullptr, nullptr);
+                // a push level (small frontier, symmetric graph) or the pull:
+                // the kernels of the branch not taken return at once
+                k_msbfs_push<<<kMsGrid, kBlock, 0, c->stream>>>(view.toff, view.tids, (const uint32_t *)fl.p, fin,
+                                                                 (const uint32_t *)seen.p, (uint32_t *)nx.p, flags);
+                k_msbfs_finish<<<fgrid, kBlock, 0, c->stream>>>((uint32_t *)nx.p, (uint32_t *)seen.p, fout, lv, n,
+                                                                lvl, (uint32_t)nsrc, vmajor, cnt, flags);
+                if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: push");
+                if (!rc) rc = launch_pull(c, view, op, cnt, nullptr, nullptr);
                 if (!rc && P) {
                     k_msbc_paths<V><<<pgrid, kBlock, 0, c->stream>>>(view.toff, view.tids, fin, fout, P, n, flags);
                     if (hub->nitems) {
@@ -418,9 +519,9 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
                     if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BC: path counts");
                 }
                 if (!rc) {
-                    k_msbfs_check<<<1, 64, 0, c->stream>>>(cnt, lvl, n, flags);
+                    k_msbfs_check<<<1, 64, 0, c->stream>>>(cnt, lvl, n, push_ok, flags);
                     k_msbfs_bits<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(fout, n, (uint32_t *)B[lvl % 2].p,
-                                                                                flags);
+                                                                                (uint32_t *)fl.p, flags);
                     if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: level check");
                 }
             }
@@ -466,6 +567,8 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     pool_give(c, fb);
     pool_give(c, B[0]);
     pool_give(c, B[1]);
+    pool_give(c, fl);
+    pool_give(c, nx);
     return rc;
 }
 } // namespace

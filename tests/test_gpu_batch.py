@@ -219,6 +219,28 @@ def test_msbfs_many_levels_and_depth_changes(ctx, plan):
     g.close()
 
 
+@pytest.mark.parametrize("plan", [0, 1])
+def test_msbfs_fewer_vertices_than_sources(ctx, plan):
+    """32 sources on a 6-vertex symmetric graph (every source repeated, an
+    isolated vertex among them): the levels and path counts of each."""
+    off, ids = oracle.csr_from_edges(6, np.array([0, 1, 2, 0], np.uint32), np.array([1, 2, 3, 2], np.uint32),
+                                     symmetric=True)
+    g = gg.Graph.from_csr(ctx, off, ids, symmetric=True).reorder(plan)
+    src = [k % 6 for k in range(32)]
+    print(f"msbfs n=6 plan={plan}", flush=True)
+    lv, _ = gg.msbfs(g, src)
+    L = lv.to_host().reshape(32, 6)
+    pa, lv2, _ = gg.msbc_forward(g, src, np.float64, levels=True)
+    P = pa.to_host().reshape(32, 6)
+    for k, s in enumerate(src):
+        _, level, sigma, _ = oracle.bc(off, ids, s)
+        assert np.array_equal(L[k], R.bfs_levels(off, ids, s)), k
+        assert np.array_equal(L[k], level), k
+        assert np.array_equal(P[k], np.where(level >= 0, sigma, 0.0)), k
+    assert np.array_equal(lv2.to_host().reshape(32, 6), L)
+    g.close()
+
+
 def test_autotune_picks_fastest_and_keeps_results(ctx):
     g = gg.Graph.rmat(ctx, 16, 16, 4, weighted=True)
     t = gg.autotune_pull(g, np.float64, trials=3)
