@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_bits(const uint32_t *__restric
 // into nxt[v] of its neighbours v (the in-list of a symmetric graph is its
 // out-list), bits v has already seen left out.  Each list entry's edges are
 // cut into kPushParts equal runs, so a hub is spread over work-groups.
-constexpr uint32_t kPushParts = 16;
+constexpr uint32_t kPushParts = 16, kPushStep = 8;
 constexpr uint32_t kMsGrid = 2048; // work-groups of the push and finish passes (grid-stride)
 
 __global__ __launch_bounds__(kBlock) void k_msbfs_push(const uint32_t *__restrict__ toff,
@@ -167,10 +167,21 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_push(const uint32_t *__restric
         const uint64_t len = toff[u + 1] - b;
         const uint32_t lo = b + (uint32_t)(len * part / kPushParts);
         const uint32_t hi = b + (uint32_t)(len * (part + 1) / kPushParts);
-        for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-            const uint32_t v = tids[i];
-            const uint32_t add = mask & ~seen[v];
-            if (add) atomicOr(&nxt[v], add);
+        // 8 edges a thread per step: their ids, then their seen words, then
+        // the ORs, so a hub's run is not one chain of dependent loads per edge
+        for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += kBlock * kPushStep) {
+            uint32_t v[kPushStep], add[kPushStep];
+#pragma unroll
+            for (uint32_t q = 0; q < kPushStep; q++) {
+                const uint32_t i = i0 + q * kBlock;
+                v[q] = i < hi ? tids[i] : 0u;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < kPushStep; q++)
+                add[q] = i0 + q * kBlock < hi ? mask & ~seen[v[q]] : 0u;
+#pragma unroll
+            for (uint32_t q = 0; q < kPushStep; q++)
+                if (add[q]) atomicOr(&nxt[v[q]], add[q]);
         }
     }
 }
