@@ -94,8 +94,11 @@ __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_
 // tests pass and only add a dependent load per edge).  flags[3] = 1 when
 // they are at most kPushCap and the graph is symmetric: the next level
 // pushes from their list (flags[4] entries, rebuilt by k_msbfs_bits)
-// instead of pulling over every in-edge.
+// instead of pulling over every in-edge.  flags[kBranchLog + l - 1] records
+// whether level l (up to 64) pushed.
 constexpr uint32_t kPushCap = 4096;
+constexpr int kBranchLog = 16, kFlagWords = kBranchLog + 64;
+static_assert(4 * kFlagWords <= kPinBytes, "the flags are read back through the pinned buffer");
 
 __global__ __launch_bounds__(64) void k_msbfs_check(unsigned long long *__restrict__ counters, int level,
                                                     uint32_t n, int push_ok, int *__restrict__ flags)
@@ -111,6 +114,7 @@ __global__ __launch_bounds__(64) void k_msbfs_check(unsigned long long *__restri
     }
     a = wave_sum_u64(a);
     if (threadIdx.x == 0) {
+        if (level <= 64) flags[kBranchLog + level - 1] = flags[3];
         if (a == 0) {
             flags[0] = 1;
             flags[1] = level;
@@ -452,7 +456,7 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     if (!rc) rc = pool_take(c, nx, 4ull * n);
     if (!rc) rc = pool_take(c, B[0], bmb);
     if (!rc) rc = pool_take(c, B[1], bmb);
-    if (!rc) rc = pool_take(c, fb, 64);
+    if (!rc) rc = pool_take(c, fb, 4 * kFlagWords);
     if (!rc) rc = pool_take(c, F[1], 4ull * n);
     if (!rc) rc = pool_take(c, seen, 4ull * n);
     if (!rc) rc = pool_take(c, src, 4ull * nsrc);
@@ -479,7 +483,7 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
         if (!rc) rc = fill_u32(c, seen.p, n, 0);
         if (!rc) rc = fill_bytes(c, B[0].p, bmb, 0);
         if (!rc) rc = fill_u32(c, nx.p, n, 0);
-        if (!rc) rc = fill_bytes(c, fb.p, 64, 0);
+        if (!rc) rc = fill_bytes(c, fb.p, 4 * kFlagWords, 0);
         if (!rc && !vmajor) rc = gg_fill(abi(c), lv, GG_I32, (uint64_t)nsrc * n, -1.0);
         if (!rc && hipMemcpyAsync(src.p, sources, 4ull * nsrc, hipMemcpyHostToDevice, c->stream) != hipSuccess)
             rc = fail(GG_ERR_HIP, "multi-source BFS: upload sources");
@@ -501,7 +505,7 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
         // batch.  The first batch is the last call's depth, else 8, then 16,
         // 32 and 64; at most n levels.
         int batch = g->msbfs_levels_hint > 0 ? std::min(g->msbfs_levels_hint, 64) : 8;
-        int enq = 0, hf[2] = {0, 0};
+        int enq = 0, hf[kFlagWords] = {};
         while (!rc && enq < (int)n) {
             const int upto = (int)std::min<int64_t>((int64_t)enq + batch, (int64_t)n);
             for (int lvl = enq + 1; !rc && lvl <= upto; lvl++) {
@@ -537,14 +541,15 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
                 }
             }
             enq = upto;
-            if (!rc) rc = read_back(c, hf, flags, 8);
+            if (!rc) rc = read_back(c, hf, flags, sizeof(hf));
             if (rc || hf[0]) break;
             batch = std::min(batch * 2, 64);
         }
         level = hf[0] ? hf[1] : enq;
         if (!rc) g->msbfs_levels_hint = level;
         if (!rc && info)
-            for (int l = 1; l <= level && l <= 64; l++) info->paths[l - 1] = GG_PATH_CSC;
+            for (int l = 1; l <= level && l <= 64; l++)
+                info->paths[l - 1] = hf[kBranchLog + l - 1] ? GG_PATH_SPARSE : GG_PATH_CSC;
         if (!rc && vmajor && (levels || P)) {
             const uint32_t ob = (uint32_t)((n + kOutTile - 1) / kOutTile);
             const uint32_t *sn = (const uint32_t *)seen.p;

@@ -141,6 +141,9 @@ def test_msbfs_levels_exact(ctx, plan):
     for k, s in enumerate(src):
         assert np.array_equal(L[k], R.bfs_levels(off, ids, s)), k
     assert info.iters == int(L.max()) + 1  # the last level finds nothing new
+    # level 1 pushes from the 32 sources (a symmetric graph)
+    names = info.path_list()
+    assert len(names) == info.iters and names[0] == "sparse" and set(names) <= {"sparse", "csc"}, names
     g.close()
 
 
@@ -149,7 +152,8 @@ def test_msbfs_directed_and_limits(ctx, plan):
     g = gg.Graph.rmat(ctx, 12, 8, 3).reorder(plan)
     off, ids, _ = g.csr()
     src = [0, 5, 17]
-    lv, _ = gg.msbfs(g, src)
+    lv, info = gg.msbfs(g, src)
+    assert set(info.path_list()) == {"csc"}  # a directed graph always pulls
     L = lv.to_host().reshape(3, g.n)
     for k, s in enumerate(src):
         assert np.array_equal(L[k], R.bfs_levels(off, ids, s))
@@ -172,6 +176,8 @@ def test_msbfs_scale22_against_single_source(ctx):
     deg = np.diff(off.astype(np.int64))
     src = np.random.default_rng(22).choice(np.flatnonzero(deg > 0), 32, replace=False).tolist()
     lv, info = gg.msbfs(g, src)
+    names = info.path_list()
+    assert "sparse" in names and "csc" in names, names  # both branches at this size
     L = lv.to_host().reshape(32, n)
     for k in (0, 13, 31):
         assert np.array_equal(L[k], R.bfs_levels(off, ids, src[k])), k
@@ -211,6 +217,8 @@ def test_msbfs_many_levels_and_depth_changes(ctx, plan):
         for k, s in enumerate(src):
             assert np.array_equal(L[k], R.bfs_levels(off, ids, s)), (where, k)
         assert info.iters == depth + 1
+        # frontiers of one or two vertices push (the first 64 levels are logged)
+        assert info.path_list() == ["sparse"] * min(info.iters, 64)
         pa, lv2, binfo = gg.msbc_forward(g, src, np.float64, levels=True)
         assert np.array_equal(lv2.to_host().reshape(len(src), n), L)
         assert binfo.iters == depth + 1
