@@ -368,7 +368,9 @@ int gg_wpagerank_adjoint(gg_graph_t g, int dtype, double alpha, double eps, int 
  * No counterpart in the reference: the forward (BFS) phase of app/BC.C for
  * up to 32 sources at once, one 32-bit mask per vertex.  levels_dev is I32
  * [nsrc][n]: levels_dev[k*n + v] = hop distance of v from sources[k]
- * (host array), -1 when unreached.  info->iters = levels run. */
+ * (host array), -1 when unreached.  info->iters = levels run; info->paths
+ * (first 64 levels): GG_PATH_SPARSE where the level pushed from a frontier of
+ * at most 4096 vertices (symmetric graphs only), GG_PATH_CSC where it pulled. */
 int gg_msbfs(gg_graph_t g, int nsrc, const uint32_t *sources, int32_t *levels_dev, gg_run_info *info);
 /* No counterpart in the reference: the forward phase of app/BC.C (BC_F and
  * BC_Vertex_F, 28-58, with NumPaths[start] = 1 at 196) for up to 32 sources
