@@ -34,7 +34,8 @@ struct OpMSBFS {
     // stays in L2, instead of a gather from the n-word array per edge
     const uint32_t *fbm;
     const int *dense;
-    __device__ bool skip() const { return done[0] != 0 || done[3] != 0; } // ended, or a push level
+    int gate;        // flags[gate] set: the other pull runs this level
+    __device__ bool skip() const { return done[0] != 0 || done[3] != 0 || done[gate] != 0; }
     // a vertex every source has reached gains nothing: the row-block pull
     // skips the gathers of a work item whose rows all have
     __device__ bool row_done(uint32_t v) const { return seen[v] == full; }
@@ -55,6 +56,7 @@ struct OpMSBFS {
         if (nw) {
             seen[v] = q.seen | nw;
             st.active++;
+            st.out_edges += (q.seen | nw) == full; // reached by every source now
             uint32_t b = nw;
             while (b) {
                 const int k = __builtin_ctz(b);
@@ -64,6 +66,10 @@ struct OpMSBFS {
         }
     }
 };
+
+// flag words of a multi-source run (k_msbfs_check): which pull a dense
+// level takes
+constexpr int kSegRun = 6, kRowsRun = 7, kSegSkip = 8;
 
 // perm (may be NULL): the renumbered plan's new id of each caller's vertex
 __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_t n, uint32_t *__restrict__ fin,
@@ -83,6 +89,8 @@ __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_
     if (k == 0) {
         flags[3] = push_ok; // level 1 pushes from the sources on a symmetric graph
         flags[4] = nsrc;
+        flags[kRowsRun] = 1; // and pulls, if it does, by row blocks
+        flags[kSegSkip] = 1;
     }
 }
 
@@ -94,34 +102,49 @@ __global__ void k_msbfs_init(const uint32_t *__restrict__ src, int nsrc, uint32_
 // tests pass and only add a dependent load per edge).  flags[3] = 1 when
 // they are at most kPushCap and the graph is symmetric: the next level
 // pushes from their list (flags[4] entries, rebuilt by k_msbfs_bits)
-// instead of pulling over every in-edge.  flags[kBranchLog + l - 1] records
-// whether level l (up to 64) pushed.
+// instead of pulling over every in-edge.  The first dense level after
+// sparse ones pulls through the source-segmented layout (flags[kSegRun];
+// where it is built) when fewer than a quarter of this level's new vertices
+// were reached by every source: most rows then still gather, and the
+// row-block pull (flags[kRowsRun]), which skips work items whose rows all
+// have been, gains little.  Later dense levels skip enough items that the
+// row blocks are as fast.
+// flags[kSegSkip]: the segmented partial pass does not run (the run ended,
+// a push, or row blocks).  flags[kBranchLog + l - 1] records the branch of
+// level l (up to 64): 0 row blocks, 1 push, 2 segmented.
 constexpr uint32_t kPushCap = 4096;
 constexpr int kBranchLog = 16, kFlagWords = kBranchLog + 64;
 static_assert(4 * kFlagWords <= kPinBytes, "the flags are read back through the pinned buffer");
 
 __global__ __launch_bounds__(64) void k_msbfs_check(unsigned long long *__restrict__ counters, int level,
-                                                    uint32_t n, int push_ok, int *__restrict__ flags)
+                                                    uint32_t n, int push_ok, int seg_ok, int *__restrict__ flags)
 {
     if (flags[0]) return;
-    unsigned long long a = 0;
+    unsigned long long a = 0, full = 0;
     if (threadIdx.x < kStripes) {
         unsigned long long *st = counters + kCntStripeBase + threadIdx.x * kStripeStride;
         a = st[CNT_ACTIVE];
+        full = st[CNT_OUT]; // the new vertices now reached by every source
         st[CNT_ACTIVE] = 0;
         st[CNT_OUT] = 0;
         st[CNT_SCANNED] = 0;
     }
     a = wave_sum_u64(a);
+    full = wave_sum_u64(full);
     if (threadIdx.x == 0) {
-        if (level <= 64) flags[kBranchLog + level - 1] = flags[3];
+        if (level <= 64) flags[kBranchLog + level - 1] = flags[3] ? 1 : flags[kSegRun] ? 2 : 0;
         if (a == 0) {
             flags[0] = 1;
             flags[1] = level;
         }
+        const int push = push_ok && a <= kPushCap;
+        const int seg = seg_ok && !push && a * 32 >= n && !flags[2] && full * 4 < a;
         flags[2] = a * 32 >= n;
-        flags[3] = push_ok && a <= kPushCap;
+        flags[3] = push;
         flags[4] = 0;
+        flags[kSegRun] = seg;
+        flags[kRowsRun] = !seg;
+        flags[kSegSkip] = a == 0 || !seg;
     }
 }
 
@@ -196,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_push(const uint32_t *__restric
 __global__ __launch_bounds__(kBlock) void k_msbfs_finish(uint32_t *__restrict__ nxt, uint32_t *__restrict__ seen,
                                                          uint32_t *__restrict__ fout, int32_t *__restrict__ levels,
                                                          uint32_t n, int32_t level, uint32_t nsrc, int vmajor,
-                                                         unsigned long long *__restrict__ counters,
+                                                         uint32_t full, unsigned long long *__restrict__ counters,
                                                          const int *__restrict__ flags)
 {
     if (flags[0] || !flags[3]) return;
@@ -211,6 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_finish(uint32_t *__restrict__ 
             if (nw) {
                 seen[v] = sn | nw;
                 st.active++;
+                st.out_edges += (sn | nw) == full;
                 uint32_t bb = nw;
                 while (bb) {
                     const uint32_t k = __builtin_ctz(bb);
@@ -471,6 +495,16 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     // k_msbfs_out with the counts); else straight into the caller's array
     const int vmajor = planned || paths;
     const int push_ok = g->symmetric; // in-lists are the out-lists
+    // the source-segmented layout of the plan's in-edges, as PageRank.C's
+    // loop takes it (masks below 2^31 bytes for the partial pass's loads)
+    const int seg_ok = planned && (uint64_t)n * 4 < (1ull << 31) &&
+                       (g->xcd_segments == 2 || (g->xcd_segments == 1 && g->m >= (1ull << 24)));
+    const uint32_t xhot = kXHotBytes / 4;
+    if (seg_ok) {
+        GG_TRY(ensure_xseg(g));
+        GG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xseg_part<SegOr<uint32_t>>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(xhot * 4)));
+    }
     int32_t *lv = vmajor ? (int32_t *)lp.p : levels;
     V *P = paths ? (V *)pb.p : nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -512,15 +546,37 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
                 const uint32_t *fin = (const uint32_t *)F[(lvl - 1) % 2].p;
                 uint32_t *fout = (uint32_t *)F[lvl % 2].p;
                 OpMSBFS op{fin, fout, (uint32_t *)seen.p, lv, n, lvl, (uint32_t)nsrc, vmajor, flags, full,
-                           (const uint32_t *)B[(lvl - 1) % 2].p, flags + 2};
+                           (const uint32_t *)B[(lvl - 1) % 2].p, flags + 2, kSegRun};
                 // a push level (small frontier, symmetric graph) or the pull:
                 // the kernels of the branch not taken return at once
                 k_msbfs_push<<<kMsGrid, kBlock, 0, c->stream>>>(view.toff, view.tids, (const uint32_t *)fl.p, fin,
                                                                  (const uint32_t *)seen.p, (uint32_t *)nx.p, flags);
                 k_msbfs_finish<<<fgrid, kBlock, 0, c->stream>>>((uint32_t *)nx.p, (uint32_t *)seen.p, fout, lv, n,
-                                                                lvl, (uint32_t)nsrc, vmajor, cnt, flags);
+                                                                lvl, (uint32_t)nsrc, vmajor, full, cnt, flags);
                 if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: push");
                 if (!rc) rc = launch_pull(c, view, op, cnt, nullptr, nullptr);
+                if (!rc && seg_ok) {
+                    // the segmented pull: partials of fin's ORs per (row,
+                    // source group), long rows' chunks, then each row's
+                    // partials through OpMSBFS's store
+                    XSeg &X = g->xseg;
+                    const uint32_t *meta = (const uint32_t *)X.meta.p;
+                    k_xseg_part<SegOr<uint32_t>><<<X.grid, kXPartThreads, xhot * 4, c->stream>>>(
+                        fin, (uint32_t *)X.partial.p, (uint32_t *)X.rows.slot_val.p, (const uint32_t *)X.vtids.p,
+                        (const RowWork *)X.rows.work.p, meta, n, xhot, flags + kSegSkip, nullptr);
+                    if (X.rows.nlong)
+                        k_rows_fix<OpSegPartial<SegOr<uint32_t>>>
+                            <<<blocks_for(X.rows.nlong, kBlock), kBlock, 0, c->stream>>>(
+                                OpSegPartial<SegOr<uint32_t>>{(uint32_t *)X.partial.p, flags + kSegSkip},
+                                (const uint32_t *)X.rows.long_rows.p, X.rows.nlong, (const uint32_t *)X.rows.slot_val.p,
+                                0, nullptr, nullptr, nullptr);
+                    OpMSBFS sop = op;
+                    sop.gate = kRowsRun;
+                    k_xseg_combine<OpMSBFS><<<X.comb_blocks, kBlock, 0, c->stream>>>(
+                        sop, (const uint32_t *)X.partial.p, (const uint32_t *)X.mask.p, (const uint32_t *)X.cursor.p,
+                        meta + kXSegs + 1, X.vrows, n, cnt, nullptr);
+                    if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: segmented pull");
+                }
                 if (!rc && P) {
                     k_msbc_paths<V><<<pgrid, kBlock, 0, c->stream>>>(view.toff, view.tids, fin, fout, P, n, flags);
                     if (hub->nitems) {
@@ -534,7 +590,7 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
                     if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BC: path counts");
                 }
                 if (!rc) {
-                    k_msbfs_check<<<1, 64, 0, c->stream>>>(cnt, lvl, n, push_ok, flags);
+                    k_msbfs_check<<<1, 64, 0, c->stream>>>(cnt, lvl, n, push_ok, seg_ok, flags);
                     k_msbfs_bits<<<blocks_for(n, kBlock), kBlock, 0, c->stream>>>(fout, n, (uint32_t *)B[lvl % 2].p,
                                                                                 (uint32_t *)fl.p, flags);
                     if (hipGetLastError() != hipSuccess) rc = fail(GG_ERR_HIP, "multi-source BFS: level check");
@@ -549,7 +605,7 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
         if (!rc) g->msbfs_levels_hint = level;
         if (!rc && info)
             for (int l = 1; l <= level && l <= 64; l++)
-                info->paths[l - 1] = hf[kBranchLog + l - 1] ? GG_PATH_SPARSE : GG_PATH_CSC;
+                info->paths[l - 1] = hf[kBranchLog + l - 1] == 1 ? GG_PATH_SPARSE : GG_PATH_CSC;
         if (!rc && vmajor && (levels || P)) {
             const uint32_t ob = (uint32_t)((n + kOutTile - 1) / kOutTile);
             const uint32_t *sn = (const uint32_t *)seen.p;

@@ -607,6 +607,13 @@ struct SegMin {
     static __device__ __forceinline__ T combine(T a, T b) { return a < b ? a : b; }
 };
 
+template <class T>
+struct SegOr {
+    using V = T;
+    static __device__ __forceinline__ T identity() { return T(0); }
+    static __device__ __forceinline__ T combine(T a, T b) { return a | b; }
+};
+
 // Finishes a long virtual row for k_rows_fix: its chunks' reductions
 // combined in chunk order go to partial[v].
 template <class P>

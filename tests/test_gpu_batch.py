@@ -249,6 +249,33 @@ def test_msbfs_fewer_vertices_than_sources(ctx, plan):
     g.close()
 
 
+@pytest.mark.parametrize("scale,ef", [(13, 16), (15, 8)])
+def test_msbfs_segmented_pull(ctx, scale, ef):
+    """The source-segmented layout forced on (it is taken from 2^24 edges by
+    default): dense levels where most rows still gather pull through it.
+    Levels and FP64 path counts of all 32 sources against the oracle, and the
+    same results with the layout off."""
+    g = gg.Graph.rmat(ctx, scale, ef, 9, symmetric=True).reorder(1)
+    g.set_option(3, 2)  # GG_OPT_XCD_SEGMENTS: always
+    off, ids, _ = g.csr()
+    n = g.n
+    deg = np.diff(off.astype(np.int64))
+    src = np.random.default_rng(scale).choice(np.flatnonzero(deg > 0), 32, replace=False).tolist()
+    print(f"msbfs segmented scale={scale} ef={ef}", flush=True)
+    lv, _ = gg.msbfs(g, src)
+    pa, lv2, _ = gg.msbc_forward(g, src, np.float64, levels=True)
+    L, P = lv.to_host().reshape(32, n), pa.to_host().reshape(32, n)
+    for k, s in enumerate(src):
+        _, level, sigma, _ = oracle.bc(off, ids, s)
+        assert np.array_equal(L[k], level), k
+        assert np.array_equal(P[k], np.where(level >= 0, sigma, 0.0)), k
+    assert np.array_equal(lv2.to_host().reshape(32, n), L)
+    g.set_option(3, 0)
+    lv3, _ = gg.msbfs(g, src)
+    assert np.array_equal(lv3.to_host().reshape(32, n), L)
+    g.close()
+
+
 def test_autotune_picks_fastest_and_keeps_results(ctx):
     g = gg.Graph.rmat(ctx, 16, 16, 4, weighted=True)
     t = gg.autotune_pull(g, np.float64, trials=3)
