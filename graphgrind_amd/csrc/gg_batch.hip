@@ -473,6 +473,16 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     if (planned) view = plan_view(g);
     else GG_TRY(pull_view(g, &view));
     const uint32_t *perm = planned ? (const uint32_t *)g->plan.perm.p : nullptr;
+    // the source-segmented layout of the plan's in-edges, as PageRank.C's
+    // loop takes it (masks below 2^31 bytes for the partial pass's loads)
+    const int seg_ok = planned && (uint64_t)n * 4 < (1ull << 31) &&
+                       (g->xcd_segments == 2 || (g->xcd_segments == 1 && g->m >= (1ull << 24)));
+    const uint32_t xhot = kXHotBytes / 4;
+    if (seg_ok) {
+        GG_TRY(ensure_xseg(g));
+        GG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xseg_part<SegOr<uint32_t>>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(xhot * 4)));
+    }
     Buf F[2], seen, src, lp, pb, hslot, fb, B[2], fl, nx;
     const uint64_t bmb = 4ull * ((n + 31) / 32);
     int rc = pool_take(c, F[0], 4ull * n);
@@ -495,16 +505,6 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
     // k_msbfs_out with the counts); else straight into the caller's array
     const int vmajor = planned || paths;
     const int push_ok = g->symmetric; // in-lists are the out-lists
-    // the source-segmented layout of the plan's in-edges, as PageRank.C's
-    // loop takes it (masks below 2^31 bytes for the partial pass's loads)
-    const int seg_ok = planned && (uint64_t)n * 4 < (1ull << 31) &&
-                       (g->xcd_segments == 2 || (g->xcd_segments == 1 && g->m >= (1ull << 24)));
-    const uint32_t xhot = kXHotBytes / 4;
-    if (seg_ok) {
-        GG_TRY(ensure_xseg(g));
-        GG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xseg_part<SegOr<uint32_t>>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(xhot * 4)));
-    }
     int32_t *lv = vmajor ? (int32_t *)lp.p : levels;
     V *P = paths ? (V *)pb.p : nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -534,10 +534,12 @@ int msbfs_run(Graph *g, int nsrc, const uint32_t *sources, int32_t *levels, V *p
         const uint32_t full = nsrc == 32 ? ~0u : (1u << nsrc) - 1;
         const uint32_t fgrid = std::min<uint32_t>(blocks_for(n, kBlock), kMsGrid);
         if (!rc) rc = counters_zero(c);
-        // Levels are enqueued in batches, each level a pull (with BC, the
-        // path counts) and k_msbfs_check; the flags are read back once per
-        // batch.  The first batch is the last call's depth, else 8, then 16,
-        // 32 and 64; at most n levels.
+        // Levels are enqueued in batches.  A level is the push and the two
+        // pulls, of which the flags let one run (k_msbfs_check chose it at
+        // the level before), with BC the path counts, then k_msbfs_check and
+        // k_msbfs_bits; the flags are read back once per batch.  The first
+        // batch is the last call's depth, else 8, then 16, 32 and 64; at
+        // most n levels.
         int batch = g->msbfs_levels_hint > 0 ? std::min(g->msbfs_levels_hint, 64) : 8;
         int enq = 0, hf[kFlagWords] = {};
         while (!rc && enq < (int)n) {
